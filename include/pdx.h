@@ -1,0 +1,192 @@
+/*
+ * pdx.h -- C-ABI of the MI355X PDLSM-FEM hot-path library (libpdx.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (PDX_OK = 0, negative on error) instead of calling exit() as the reference
+ * does; pdx_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * Reference seams (QibangLiu/PDLSM-FEM_mpi) each entry point replaces:
+ *   pdx_model_create / pdx_model_read
+ *       datModel::readdata            SourceFiles/Solver/datModel.cpp:58-326
+ *       fioFiles::excuteCMD           SourceFiles/Solver/fioFile.cpp:69-462
+ *       pdsolve::setDatModel          SourceFiles/Solver/pdsolve.cpp:114-151
+ *       pdsolve::setCSRIndexes_gloStiffMat  pdsolve.cpp:959-1121
+ *   pdx_assemble
+ *       pdsolve::assembleInterWorkPD_CSRformat  pdsolve.cpp:2524-2730 (HeaderFiles/Solver/pdsolve.h:100)
+ *       pdsolve::assemblePDBEwork_CSRformat     pdsolve.cpp:3674-4273 (pdsolve.h:107)
+ *       pdsolve::assembleSEDbyFEM_CSRformat     pdsolve.cpp:4882-5007 (pdsolve.h:122)
+ *       pdsolve::calExternalForce_CSRformat     pdsolve.cpp:5009-5071 (pdsolve.h:123)
+ *   pdx_lumped_mass
+ *       pdsolve::assembleLumpedMass             pdsolve.cpp:4618-4669
+ *   pdx_explicit_run / pdx_explicit_steps
+ *       pdsolve::calinternalForce_CSRformat     pdsolve.cpp:5819-5840 (pdsolve.h:128)
+ *       pdsolve::timeIntegration_LM             pdsolve.cpp:5113-5132 (pdsolve.h:145)
+ *       driver loop of pdfemDynamicSolver_CSRformat  pdsolve.cpp:5842-6090
+ *   pdx_static_solve
+ *       pdsolve::pdfemStaticSolver_CSRformat    pdsolve.cpp:5557-5654
+ *   pdx_newmark_run
+ *       pdsolve::pdfemDynamicNewmarkSolver_CSRformat pdsolve.cpp:6092-6290,
+ *       pdfemAssembleKNFR_CSRformat :6327-6397, updateDispVelo :6399-6422
+ *   pdx_solve_csr
+ *       calMatrixOperations::cluster_PARDISO_64Solver
+ *       SourceFiles/Solver/calMatrixOperations.cpp:378-498
+ *       (HeaderFiles/Solver/calMatrixOperations.h:47-48)
+ */
+#ifndef PDX_H
+#define PDX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDX_OK 0
+#define PDX_ERR_ARG (-1)
+#define PDX_ERR_SINGULAR (-2)
+#define PDX_ERR_PATTERN (-3)
+#define PDX_ERR_MESH (-4)
+#define PDX_ERR_IO (-5)
+#define PDX_ERR_HIP (-6)
+#define PDX_ERR_NOCONV (-7)
+#define PDX_ERR_LIMIT (-8)
+
+/* assembly modes: the reference's ci_solvFlag / cb_Newmark (datModel.h:122, :132) */
+#define PDX_MODE_EXPLICIT 0 /* solvFlag 0, Newmark off: F = F_ext - K u          */
+#define PDX_MODE_STATIC 1   /* solvFlag 1: K_ff, F = F_ext - K_fp u_p            */
+#define PDX_MODE_QUASI 2    /* solvFlag 2: K_ff, F = F_ext - K u (all columns)   */
+#define PDX_MODE_NEWMARK 3  /* solvFlag 0, Newmark on: K_ff, F = F_ext - K U_N   */
+
+/* precision of the explicit internal force / Krylov inner iterations */
+#define PDX_FP64 0
+#define PDX_FP32 1 /* FP32 K values, FP32 accumulation, FP64 state */
+
+/* Krylov methods and preconditioners for pdx_static_solve / pdx_solve_csr */
+#define PDX_KRYLOV_BICGSTAB 0
+#define PDX_KRYLOV_CG 1
+#define PDX_PREC_NONE 0
+#define PDX_PREC_JACOBI 1
+#define PDX_PREC_BLOCK_JACOBI 2 /* node d x d blocks */
+#define PDX_PREC_LEARNED 3      /* graph-network sparse approximate inverse */
+
+typedef struct pdx_model pdx_model; /* host-side data model (families, pattern) */
+typedef struct pdx_ctx pdx_ctx;     /* device-resident state on one GPU         */
+
+/* Raw mesh + solver settings: the content of a .lis mesh file after
+ * datModel::readdata plus the SETSOLVING/EBC/VEBC/NBC/VNBC commands.
+ * Arrays are caller-owned and copied by pdx_model_create. */
+typedef struct pdx_mesh_desc {
+    int dim;      /* 2 or 3 */
+    int pro_type; /* 2D: 1 plane stress, 2 plane strain */
+    double E, nu, rho;
+    int n_node;
+    const double *xyz; /* [n_node*3] */
+    int n_ele;
+    const int *ele_nodes; /* [n_ele*8] 1-based node ids */
+    const int *ele_nen;   /* [n_ele] 3/4 (2D), 4/8 (3D) */
+    const int *ele_algo;  /* [n_ele] 1 = PD element, 2 = FE element */
+    int n_ebc;
+    const int *ebc_off;   /* [n_ebc+1] */
+    const int *ebc_nid;   /* [ebc_off[n_ebc]] 1-based */
+    const int *ebc_dof;   /* [n_ebc] 0 UX, 1 UY, 2 UZ */
+    const double *ebc_val, *ebc_vel;
+    const int *ebc_vary;
+    int n_nbc;
+    const int *nbc_off;   /* [n_nbc+1] elements per set */
+    const int *nbc_con;   /* [nbc_off[n_nbc]*(dim==3?4:2)] 1-based */
+    const double *nbc_val, *nbc_vel;
+    const int *nbc_vary;
+    int n_crack;
+    const double *crack;  /* [n_crack*9] three points (2D: two) */
+    double horizon_factor; /* m of SETSOLVING */
+    double nlf;            /* non-local factor a */
+    double me;             /* enrichment radius factor */
+    double dt, beta, gamma;
+} pdx_mesh_desc;
+
+typedef struct pdx_solve_opts {
+    int krylov;        /* PDX_KRYLOV_* */
+    int precond;       /* PDX_PREC_* */
+    int precision;     /* PDX_FP64: plain FP64 Krylov; PDX_FP32: FP32 inner + FP64 refinement */
+    double tol;        /* true relative residual target, e.g. 1e-8 */
+    int max_iter;      /* total inner iterations */
+    double inner_tol;  /* FP32 inner relative tolerance (refinement), e.g. 1e-4 */
+    int warm_start;    /* use x as the initial guess */
+} pdx_solve_opts;
+
+typedef struct pdx_solve_stats {
+    int iterations;    /* inner Krylov iterations */
+    int refinements;   /* FP64 refinement sweeps */
+    double true_relres; /* ||b - A x|| / ||b|| in FP64 */
+    double seconds;
+    int converged;
+} pdx_solve_stats;
+
+const char *pdx_last_error(void);
+const char *pdx_version(void);
+
+/* ---- host data model -------------------------------------------------- */
+int pdx_model_create(const pdx_mesh_desc *desc, pdx_model **out);
+int pdx_model_read(const char *cmd_file, pdx_model **out, int *solver_flag, int *num_steps);
+void pdx_model_destroy(pdx_model *m);
+/* out[0..15]: n_node, n_ele, dim, neq, nnz, nfam, n_members, npdbe, npde, nfe,
+ *             n_adj (node-graph entries), n_presc, nb0, nb1, nb2, max_family */
+int pdx_model_sizes(const pdx_model *m, long long *out);
+int pdx_model_csr(const pdx_model *m, long long *ia, long long *ja);
+/* families: node (0-based), offsets, members (1-based, self first), bond state, cd_v, horizon */
+int pdx_model_families(const pdx_model *m, int *fam_node, int *fam_off, int *fam_mem, int *bond,
+                       double *cd_v, double *delta);
+/* eq [n_node*3], node type [n_node], node volume [n_node], element volume [n_ele],
+ * boundary entities [npdbe*4] (1-based), element algorithm after crack update [n_ele] */
+int pdx_model_nodes(const pdx_model *m, int *eq, int *ntype, double *nv, double *ev, int *pdbe, int *algo);
+/* initial free-DOF displacement (from DOF values) and prescribed values, [n_node*dim] */
+int pdx_model_values(const pdx_model *m, double *u_all);
+
+/* ---- device context --------------------------------------------------- */
+int pdx_ctx_create(const pdx_model *m, int device, pdx_ctx **out);
+void pdx_ctx_destroy(pdx_ctx *c);
+/* the HIP stream every kernel of this context runs on (hipStream_t) */
+void *pdx_ctx_stream(pdx_ctx *c);
+
+/* Assemble K and F on the GPU in the reference's CSR layout.
+ * Ku: [nnz] K_ff values (NULL in PDX_MODE_EXPLICIT); F: [neq].
+ * UN: [n_node*dim] predictor displacement for PDX_MODE_NEWMARK, else NULL.
+ * with_ext: add the natural-BC force at the current NBC values. */
+int pdx_assemble(pdx_ctx *c, int mode, const double *UN, double *Ku, double *F, int with_ext);
+int pdx_lumped_mass(pdx_ctx *c, double *m);
+/* Re-assemble K in device memory only (no host copies); ms: kernel time of the
+ * family + element kernels measured with HIP events on the context stream. */
+int pdx_assemble_timed(pdx_ctx *c, double *ms);
+/* consistent FE mass + PD lumped mass on the K pattern, [nnz] */
+int pdx_mass_on_K(pdx_ctx *c, double *Mv);
+
+/* DOF values (free and prescribed, [n_node*dim]) used by PDX_MODE_EXPLICIT and
+ * PDX_MODE_QUASI assembly and by pdx_static_solve for the prescribed part;
+ * the reference keeps them in pdDof::cd_u (pdDof.h:18). */
+int pdx_set_dof_values(pdx_ctx *c, const double *u_all);
+
+/* Explicit central difference with lumped mass (the reference driver's
+ * start-up step u1 = u0 + dt^2/2 a0 on the first call, then nsteps loop
+ * steps).  u_out/u_prev: [neq] host (may be NULL). */
+int pdx_explicit_run(pdx_ctx *c, int nsteps, int precision, double *u_out, double *u_prev);
+/* Same loop on device-resident state only, no host copies (for timing). */
+int pdx_explicit_steps(pdx_ctx *c, int nsteps, int precision, int use_graph);
+/* Largest eigenvalue of M^-1 K (power iteration) -> critical dt = 2/sqrt(lmax) */
+int pdx_max_eig(pdx_ctx *c, int iters, double *lmax);
+int pdx_set_dt(pdx_ctx *c, double dt);
+
+int pdx_static_solve(pdx_ctx *c, const pdx_solve_opts *o, double *u, pdx_solve_stats *st);
+int pdx_newmark_run(pdx_ctx *c, int nsteps, const pdx_solve_opts *o, double *u, double *v, double *a,
+                    pdx_solve_stats *st);
+
+/* PARDISO seam: solve A x = b for a zero-based int64 CSR (n, ia, ja, a). */
+int pdx_solve_csr(int device, long long n, const long long *ia, const long long *ja, const double *a,
+                  const double *b, double *x, const pdx_solve_opts *o, pdx_solve_stats *st);
+
+/* timing of the last pdx_explicit_steps call measured with HIP events on the
+ * context stream: total ms and per-step ms */
+int pdx_last_timing(pdx_ctx *c, double *total_ms, double *per_step_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

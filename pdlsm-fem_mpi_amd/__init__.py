@@ -1,0 +1,320 @@
+"""Python binding of libpdx.so, the MI355X PDLSM-FEM hot-path library.
+
+The package directory name contains a hyphen, so load it with
+``pdx = load_package()`` from ``pdlsm_fem_loader`` or import it by path:
+
+    import importlib.util, sys
+    spec = importlib.util.spec_from_file_location(
+        "pdlsm_fem_mpi_amd", "pdlsm-fem_mpi_amd/__init__.py",
+        submodule_search_locations=["pdlsm-fem_mpi_amd"])
+
+Everything here is a thin ctypes layer over the C-ABI declared in
+include/pdx.h; all compute runs in the HIP library.  A missing library raises
+at import of the first object instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpdx.so")
+
+PDX_MODE_EXPLICIT, PDX_MODE_STATIC, PDX_MODE_QUASI, PDX_MODE_NEWMARK = 0, 1, 2, 3
+PDX_FP64, PDX_FP32 = 0, 1
+PDX_KRYLOV_BICGSTAB, PDX_KRYLOV_CG = 0, 1
+PDX_PREC_NONE, PDX_PREC_JACOBI, PDX_PREC_BLOCK_JACOBI, PDX_PREC_LEARNED = 0, 1, 2, 3
+
+_P, _I, _D, _LL = C.c_void_p, C.c_int, C.c_double, C.c_longlong
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("dim", _I), ("pro_type", _I), ("E", _D), ("nu", _D), ("rho", _D),
+                ("n_node", _I), ("xyz", _P), ("n_ele", _I), ("ele_nodes", _P), ("ele_nen", _P),
+                ("ele_algo", _P), ("n_ebc", _I), ("ebc_off", _P), ("ebc_nid", _P), ("ebc_dof", _P),
+                ("ebc_val", _P), ("ebc_vel", _P), ("ebc_vary", _P), ("n_nbc", _I), ("nbc_off", _P),
+                ("nbc_con", _P), ("nbc_val", _P), ("nbc_vel", _P), ("nbc_vary", _P), ("n_crack", _I),
+                ("crack", _P), ("horizon_factor", _D), ("nlf", _D), ("me", _D), ("dt", _D), ("beta", _D),
+                ("gamma", _D)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("krylov", _I), ("precond", _I), ("precision", _I), ("tol", _D), ("max_iter", _I),
+                ("inner_tol", _D), ("warm_start", _I)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", _I), ("refinements", _I), ("true_relres", _D), ("seconds", _D),
+                ("converged", _I)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+EXPORTS = {
+    "pdx_last_error": (C.c_char_p, []),
+    "pdx_version": (C.c_char_p, []),
+    "pdx_model_create": (_I, [_P, _P]),
+    "pdx_model_read": (_I, [C.c_char_p, _P, _P, _P]),
+    "pdx_model_destroy": (None, [_P]),
+    "pdx_model_sizes": (_I, [_P, _P]),
+    "pdx_model_csr": (_I, [_P, _P, _P]),
+    "pdx_model_families": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "pdx_model_nodes": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "pdx_model_values": (_I, [_P, _P]),
+    "pdx_ctx_create": (_I, [_P, _I, _P]),
+    "pdx_ctx_destroy": (None, [_P]),
+    "pdx_ctx_stream": (_P, [_P]),
+    "pdx_assemble": (_I, [_P, _I, _P, _P, _P, _I]),
+    "pdx_lumped_mass": (_I, [_P, _P]),
+    "pdx_assemble_timed": (_I, [_P, _P]),
+    "pdx_mass_on_K": (_I, [_P, _P]),
+    "pdx_set_dof_values": (_I, [_P, _P]),
+    "pdx_explicit_run": (_I, [_P, _I, _I, _P, _P]),
+    "pdx_explicit_steps": (_I, [_P, _I, _I, _I]),
+    "pdx_max_eig": (_I, [_P, _I, _P]),
+    "pdx_set_dt": (_I, [_P, _D]),
+    "pdx_static_solve": (_I, [_P, _P, _P, _P]),
+    "pdx_newmark_run": (_I, [_P, _I, _P, _P, _P, _P, _P]),
+    "pdx_solve_csr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
+    "pdx_last_timing": (_I, [_P, _P, _P]),
+}
+
+
+def lib():
+    """Load libpdx.so; raises if it was not built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python pdlsm-fem_mpi_amd/build.py` first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class PdxError(RuntimeError):
+    pass
+
+
+def _chk(rc):
+    if rc != 0:
+        raise PdxError(f"pdx error {rc}: {lib().pdx_last_error().decode()}")
+
+
+def mesh_desc(mesh: dict):
+    """Build a pdx_mesh_desc from a raw-mesh dict (see workloads.py); returns
+    (desc, keepalive list)."""
+    keep = []
+
+    def arr(x, dt):
+        a = np.ascontiguousarray(np.asarray(x, dtype=dt))
+        keep.append(a)
+        return a
+
+    dim = int(mesh["dim"])
+    x = arr(mesh["x"], np.float64).reshape(-1, 3)
+    econ = arr(mesh["econ"], np.int32).reshape(-1, 8)
+    nen = arr(mesh["nen"], np.int32)
+    algo = arr(mesh["algo"], np.int32)
+    ebc = mesh.get("ebc", [])
+    nbc = mesh.get("nbc", [])
+    tn = 4 if dim == 3 else 2
+    ebc_off = arr(np.concatenate([[0], np.cumsum([len(s["nid"]) for s in ebc])]), np.int32)
+    ebc_nid = arr(np.concatenate([np.asarray(s["nid"]) for s in ebc]) if ebc else np.zeros(1), np.int32)
+    nbc_off = arr(np.concatenate([[0], np.cumsum([len(s["con"]) for s in nbc])]), np.int32)
+    nbc_con = arr(np.concatenate([np.asarray(s["con"]).reshape(-1, tn) for s in nbc]).ravel()
+                  if nbc else np.zeros(1), np.int32)
+    cracks = arr(np.asarray(mesh.get("cracks", np.zeros((0, 3, 3)))).reshape(-1, 9), np.float64)
+    d = MeshDesc()
+    d.dim, d.pro_type = dim, int(mesh.get("proType", 1))
+    d.E, d.nu, d.rho = float(mesh["E"]), float(mesh["nu"]), float(mesh["rho"])
+    d.n_node, d.xyz = len(x), _p(x)
+    d.n_ele, d.ele_nodes, d.ele_nen, d.ele_algo = len(econ), _p(econ), _p(nen), _p(algo)
+    d.n_ebc, d.ebc_off, d.ebc_nid = len(ebc), _p(ebc_off), _p(ebc_nid)
+    d.ebc_dof = _p(arr([s["dof"] for s in ebc] or [0], np.int32))
+    d.ebc_val = _p(arr([s["val"] for s in ebc] or [0.0], np.float64))
+    d.ebc_vel = _p(arr([s.get("vel", 0.0) for s in ebc] or [0.0], np.float64))
+    d.ebc_vary = _p(arr([int(s.get("vary", False)) for s in ebc] or [0], np.int32))
+    d.n_nbc, d.nbc_off, d.nbc_con = len(nbc), _p(nbc_off), _p(nbc_con)
+    d.nbc_val = _p(arr([s["val"] for s in nbc] or [0.0], np.float64))
+    d.nbc_vel = _p(arr([s.get("vel", 0.0) for s in nbc] or [0.0], np.float64))
+    d.nbc_vary = _p(arr([int(s.get("vary", False)) for s in nbc] or [0], np.int32))
+    d.n_crack, d.crack = len(cracks), _p(cracks)
+    d.horizon_factor, d.nlf, d.me = float(mesh["fac"]), float(mesh["nlf"]), float(mesh.get("me", 0.0))
+    d.dt, d.beta, d.gamma = float(mesh.get("dt", 1.0)), float(mesh.get("beta", 0.25)), float(mesh.get("gamma", 0.5))
+    return d, keep
+
+
+class Model:
+    """Host data model (set-up of pdsolve::setDatModel and the CSR pattern)."""
+
+    def __init__(self, mesh: dict | None = None, cmd_file: str | None = None):
+        L = lib()
+        h = C.c_void_p()
+        if cmd_file is not None:
+            sf, ns = C.c_int(-1), C.c_int(1)
+            _chk(L.pdx_model_read(cmd_file.encode(), C.byref(h), C.byref(sf), C.byref(ns)))
+            self.solver_flag, self.num_steps = sf.value, ns.value
+        else:
+            desc, keep = mesh_desc(mesh)
+            _chk(L.pdx_model_create(C.byref(desc), C.byref(h)))
+        self.h = h
+        s = np.zeros(16, np.int64)
+        _chk(L.pdx_model_sizes(self.h, _p(s)))
+        (self.nn, self.ne, self.dim, self.neq, self.nnz, self.nfam, self.nmem, self.npdbe, self.npde, self.nfe,
+         self.nadj, self.npresc, nb0, nb1, nb2, self.max_family) = (int(v) for v in s)
+        self.nblocks = (nb0, nb1, nb2)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pdx_model_destroy(self.h)
+            self.h = None
+
+    def csr(self):
+        ia = np.zeros(self.neq + 1, np.int64)
+        ja = np.zeros(max(self.nnz, 1), np.int64)
+        _chk(lib().pdx_model_csr(self.h, _p(ia), _p(ja)))
+        return ia, ja[: self.nnz]
+
+    def families(self):
+        fn = np.zeros(max(self.nfam, 1), np.int32)
+        fo = np.zeros(self.nfam + 1, np.int32)
+        fm = np.zeros(max(self.nmem, 1), np.int32)
+        bd = np.zeros(max(self.nmem, 1), np.int32)
+        fv = np.zeros(max(self.nfam, 1))
+        fd = np.zeros(max(self.nfam, 1))
+        _chk(lib().pdx_model_families(self.h, _p(fn), _p(fo), _p(fm), _p(bd), _p(fv), _p(fd)))
+        n, m = self.nfam, self.nmem
+        return dict(node=fn[:n], off=fo, mem=fm[:m], bond=bd[:m], cd_v=fv[:n], delta=fd[:n])
+
+    def nodes(self):
+        eq = np.zeros(self.nn * 3, np.int32)
+        nt = np.zeros(self.nn, np.int32)
+        nv = np.zeros(self.nn)
+        ev = np.zeros(self.ne)
+        pb = np.zeros(max(self.npdbe, 1) * 4, np.int32)
+        al = np.zeros(self.ne, np.int32)
+        _chk(lib().pdx_model_nodes(self.h, _p(eq), _p(nt), _p(nv), _p(ev), _p(pb), _p(al)))
+        return dict(eq=eq.reshape(-1, 3), ntype=nt, nv=nv, ev=ev, pdbe=pb[: self.npdbe * 4].reshape(-1, 4), algo=al)
+
+    def values(self):
+        u = np.zeros(self.nn * self.dim)
+        _chk(lib().pdx_model_values(self.h, _p(u)))
+        return u
+
+
+def solve_opts(krylov=PDX_KRYLOV_BICGSTAB, precond=PDX_PREC_JACOBI, precision=PDX_FP64, tol=1e-8, max_iter=20000,
+               inner_tol=1e-4, warm_start=False):
+    o = SolveOpts()
+    o.krylov, o.precond, o.precision, o.tol, o.max_iter = krylov, precond, precision, tol, max_iter
+    o.inner_tol, o.warm_start = inner_tol, int(warm_start)
+    return o
+
+
+class Context:
+    """Device-resident state of one model on one GPU."""
+
+    def __init__(self, model: Model, device: int = 0):
+        self.model = model
+        h = C.c_void_p()
+        _chk(lib().pdx_ctx_create(model.h, device, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pdx_ctx_destroy(self.h)
+            self.h = None
+
+    def stream(self):
+        return lib().pdx_ctx_stream(self.h)
+
+    def set_dof_values(self, u_all):
+        _chk(lib().pdx_set_dof_values(self.h, _p(np.ascontiguousarray(u_all, np.float64))))
+
+    def assemble(self, mode=PDX_MODE_STATIC, UN=None, with_ext=True, want_K=True):
+        m = self.model
+        F = np.zeros(max(m.neq, 1))
+        K = np.zeros(max(m.nnz, 1)) if (want_K and mode != PDX_MODE_EXPLICIT) else None
+        un = None if UN is None else np.ascontiguousarray(UN, np.float64)
+        _chk(lib().pdx_assemble(self.h, mode, _p(un), _p(K), _p(F), int(with_ext)))
+        return (None if K is None else K[: m.nnz]), F[: m.neq]
+
+    def assemble_timed(self):
+        """Re-assemble K on the device only; returns the kernel time in ms."""
+        ms = C.c_double(0)
+        _chk(lib().pdx_assemble_timed(self.h, C.byref(ms)))
+        return ms.value
+
+    def lumped_mass(self):
+        out = np.zeros(max(self.model.neq, 1))
+        _chk(lib().pdx_lumped_mass(self.h, _p(out)))
+        return out[: self.model.neq]
+
+    def mass_on_K(self):
+        out = np.zeros(max(self.model.nnz, 1))
+        _chk(lib().pdx_mass_on_K(self.h, _p(out)))
+        return out[: self.model.nnz]
+
+    def explicit_run(self, nsteps, precision=PDX_FP64):
+        u = np.zeros(max(self.model.neq, 1))
+        up = np.zeros(max(self.model.neq, 1))
+        _chk(lib().pdx_explicit_run(self.h, nsteps, precision, _p(u), _p(up)))
+        return u[: self.model.neq], up[: self.model.neq]
+
+    def explicit_steps(self, nsteps, precision=PDX_FP64, use_graph=True):
+        _chk(lib().pdx_explicit_steps(self.h, nsteps, precision, int(use_graph)))
+
+    def last_timing(self):
+        t, s = C.c_double(0), C.c_double(0)
+        _chk(lib().pdx_last_timing(self.h, C.byref(t), C.byref(s)))
+        return t.value, s.value
+
+    def max_eig(self, iters=100):
+        v = C.c_double(0)
+        _chk(lib().pdx_max_eig(self.h, iters, C.byref(v)))
+        return v.value
+
+    def set_dt(self, dt):
+        _chk(lib().pdx_set_dt(self.h, float(dt)))
+
+    def static_solve(self, opts=None, u0=None):
+        opts = opts or solve_opts()
+        u = np.zeros(max(self.model.neq, 1)) if u0 is None else np.array(u0, np.float64)
+        st = SolveStats()
+        rc = lib().pdx_static_solve(self.h, C.byref(opts), _p(u), C.byref(st))
+        _chk(rc)
+        return u[: self.model.neq], st.as_dict()
+
+    def newmark_run(self, nsteps, opts=None):
+        opts = opts or solve_opts()
+        n = max(self.model.neq, 1)
+        u, v, a = np.zeros(n), np.zeros(n), np.zeros(n)
+        st = SolveStats()
+        _chk(lib().pdx_newmark_run(self.h, nsteps, C.byref(opts), _p(u), _p(v), _p(a), C.byref(st)))
+        k = self.model.neq
+        return u[:k], v[:k], a[:k], st.as_dict()
+
+
+def solve_csr(ia, ja, a, b, opts=None, device=0, x0=None):
+    """PARDISO seam: solve A x = b for a zero-based int64 CSR matrix."""
+    opts = opts or solve_opts()
+    ia = np.ascontiguousarray(ia, np.int64)
+    ja = np.ascontiguousarray(ja, np.int64)
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(len(b)) if x0 is None else np.array(x0, np.float64)
+    st = SolveStats()
+    _chk(lib().pdx_solve_csr(device, len(b), _p(ia), _p(ja), _p(a), _p(b), _p(x), C.byref(opts), C.byref(st)))
+    return x, st.as_dict()

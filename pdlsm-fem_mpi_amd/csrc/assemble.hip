@@ -1,0 +1,849 @@
+// Assembly kernels of the PDLSM-FEM hot path for CDNA4 (gfx950).
+//
+//  k_family   one wavefront per PD family: member staging in LDS, shape tensor
+//             (5x5 / 9x9) accumulated in the reference member order, one
+//             partial-pivot LU per family, one triangular solve per bond
+//             (lane-parallel), bond stiffness G, row block H, and the
+//             PD-region boundary traction terms N.D.C of the same family, all
+//             written to the family node's block row through the scatter map
+//             (plain stores: each family owns its row).
+//  k_fe_rows  one thread per (node, dof) row: element stiffness rows of every
+//             FE element around the node, added in element order.
+//  k_lumped   lumped mass rho * sum V_e / n_e, node-centric.
+//  k_nbc_*    equivalent nodal forces of natural BCs for a unit traction.
+//  k_spmv_*   node-block SpMV, short rows on 8-lane groups, long rows on a wave.
+//  k_csr_extract  block rows -> the reference CSR (K_ff values).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "device.hpp"
+
+namespace pdx {
+
+template <int D>
+struct Dm;
+template <>
+struct Dm<2> {
+    static constexpr int P = 5, NV = 3, DD = 4;
+};
+template <>
+struct Dm<3> {
+    static constexpr int P = 9, NV = 6, DD = 9;
+};
+
+// exponents of the basis p = [x,y,z,x^2,y^2,z^2,xy,yz,zx] (3D) / [x,y,x^2,y^2,xy] (2D)
+__constant__ int c_pe3[9][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {0, 2, 0},
+                                {0, 0, 2}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}};
+__constant__ int c_pe2[5][2] = {{1, 0}, {0, 1}, {2, 0}, {0, 2}, {1, 1}};
+
+// Shape-tensor term of entry e for one bond, in the reference's expression order
+// (pdsolve.cpp:1938-2013 for 2D, :2015-2064 for 3D).
+template <int D>
+__device__ inline double a_term(int e, double x, double y, double z, double om, double vm);
+
+template <>
+__device__ inline double a_term<3>(int e, double x, double y, double z, double om, double vm) {
+    int r = e / 9, c = e % 9;
+    int ex = c_pe3[r][0] + c_pe3[c][0], ey = c_pe3[r][1] + c_pe3[c][1], ez = c_pe3[r][2] + c_pe3[c][2];
+    double t = 1.0;
+    for (int i = 0; i < ex; i++) t = t * x;
+    for (int i = 0; i < ey; i++) t = t * y;
+    for (int i = 0; i < ez; i++) t = t * z;
+    if (c >= 3 && c < 6) t = t * 0.5;
+    return (om * vm) * t;
+}
+
+template <>
+__device__ inline double a_term<2>(int e, double x, double y, double, double om, double vm) {
+    if (e == 0) return ((om * x) * x) * vm;
+    int r = e / 5, c = e % 5;
+    int ex = c_pe2[r][0] + c_pe2[c][0], ey = c_pe2[r][1] + c_pe2[c][1];
+    double t = om * vm;
+    for (int i = 0; i < ex; i++) t = t * x;
+    for (int i = 0; i < ey; i++) t = t * y;
+    if (c == 2 || c == 3) t = t * 0.5;
+    return t;
+}
+
+struct FamArgs {
+    const double *x, *nv;
+    const int *fam_node, *fam_off, *fam_mem, *bond, *fam_blk;
+    const double *fam_v, *fam_delta;
+    const int *fp_off, *fp_list, *pdbe;
+    const double *Dm;
+    double lam, mu, nlf;
+    double *K;
+    long long nblk;
+    int *status;
+    int mf;  // LDS capacity in members
+};
+
+// normMatArea (pdsolve.cpp:2917-2967): area-weighted face normal matrix 3x6
+__device__ inline bool norm_mat_area(const double *x, const int *con, double Nm[18]) {
+    double xN[4][3];
+    for (int a = 0; a < 4; a++)
+        for (int c = 0; c < 3; c++) xN[a][c] = x[(size_t)con[a] * 3 + c];
+    double v1[3], v2[3], w = 1.0 / 3;
+    for (int i = 0; i < 3; i++) { v1[i] = xN[1][i] - xN[0][i]; v2[i] = xN[2][i] - xN[0][i]; }
+    double N1 = v1[1] * v2[2] - v1[2] * v2[1];
+    double N2 = v2[0] * v1[2] - v1[0] * v2[2];
+    double N3 = v1[0] * v2[1] - v2[0] * v1[1];
+    double A1 = sqrt(N1 * N1 + N2 * N2 + N3 * N3), A2 = 0;
+    if (A1 == 0) return false;
+    if (con[3] != con[2]) {
+        for (int i = 0; i < 3; i++) { v1[i] = xN[3][i] - xN[0][i]; v2[i] = xN[2][i] - xN[0][i]; }
+        double t1 = v1[1] * v2[2] - v1[2] * v2[1];
+        double t2 = v2[0] * v1[2] - v1[0] * v2[2];
+        double t3 = v1[0] * v2[1] - v2[0] * v1[1];
+        A2 = sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+        w = 0.25;
+    }
+    double area = (A1 + A2) * 0.5;
+    N1 = N1 / A1 * area * w;
+    N2 = N2 / A1 * area * w;
+    N3 = N3 / A1 * area * w;
+    for (int i = 0; i < 18; i++) Nm[i] = 0;
+    Nm[0] = N1; Nm[7] = N2; Nm[14] = N3;
+    Nm[3] = N2; Nm[5] = N3; Nm[9] = N1;
+    Nm[10] = N3; Nm[16] = N2; Nm[17] = N1;
+    return true;
+}
+
+// C-block of one bond (strain operator columns of member m, matC2D/3D
+// pdsolve.cpp:3212-3252, :3324-3398): NV x D
+template <int D>
+__device__ inline void c_block(const double c[D], double Cb[Dm<D>::NV * D]) {
+    if constexpr (D == 2) {
+        Cb[0] = c[0]; Cb[1] = 0;
+        Cb[2] = 0; Cb[3] = c[1];
+        Cb[4] = c[1]; Cb[5] = c[0];
+    } else {
+        for (int i = 0; i < 18; i++) Cb[i] = 0;
+        Cb[0 * 3 + 0] = c[0]; Cb[1 * 3 + 1] = c[1]; Cb[2 * 3 + 2] = c[2];
+        Cb[3 * 3 + 0] = c[1]; Cb[3 * 3 + 1] = c[0];
+        Cb[4 * 3 + 1] = c[2]; Cb[4 * 3 + 2] = c[1];
+        Cb[5 * 3 + 0] = c[2]; Cb[5 * 3 + 2] = c[0];
+    }
+}
+
+// boundary traction term of one PD-boundary entity for one C block, added to val (D x D)
+// 2D: 1/2 * N . (D . C)  (pdsolve.cpp:4049-4253); 3D: (Nm . D) . C  (pdsolve.cpp:2785-2915)
+template <int D>
+__device__ inline void add_pdbe_term(const FamArgs &A, int p, const double *Cb, double *val) {
+    constexpr int NV = Dm<D>::NV;
+    const double *Dg = A.Dm;
+    if constexpr (D == 2) {
+        int a = A.pdbe[p * 4], b = A.pdbe[p * 4 + 1];
+        double nx = A.x[(size_t)b * 3 + 1] - A.x[(size_t)a * 3 + 1];
+        double ny = A.x[(size_t)a * 3 + 0] - A.x[(size_t)b * 3 + 0];
+        double N[6] = {nx, 0, ny, 0, ny, nx};
+        double DC[6];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 2; j++) {
+                double t = 0.0;
+                for (int k = 0; k < 3; k++) t = Dg[i * 3 + k] * Cb[k * 2 + j] + t;
+                DC[i * 2 + j] = t;
+            }
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) {
+                double t = 0.0;
+                for (int k = 0; k < 3; k++) t = N[i * 3 + k] * DC[k * 2 + j] + t;
+                val[i * 2 + j] = val[i * 2 + j] + 1.0 / 2 * t;
+            }
+    } else {
+        double Nm[18], ND[18];
+        if (!norm_mat_area(A.x, &A.pdbe[p * 4], Nm)) {
+            atomicOr(A.status, 4);
+            return;
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 6; j++) {
+                double t = 0.0;
+                for (int k = 0; k < 6; k++) t = Nm[i * 6 + k] * Dg[k * 6 + j] + t;
+                ND[i * 6 + j] = t;
+            }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double t = 0.0;
+                for (int k = 0; k < NV; k++) t = ND[i * 6 + k] * Cb[k * 3 + j] + t;
+                val[i * 3 + j] = val[i * 3 + j] + t;
+            }
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(64) k_family(FamArgs A) {
+    constexpr int P = Dm<D>::P, DD = Dm<D>::DD;
+    extern __shared__ double lds[];
+    const int f = blockIdx.x, lane = threadIdx.x;
+    const int off = A.fam_off[f], n = A.fam_off[f + 1] - off;
+    const int mf = A.mf;
+    double *sx = lds;               // [mf]
+    double *sy = sx + mf;           // [mf]
+    double *sz = sy + mf;           // [mf]
+    double *som = sz + mf;          // [mf] influence function
+    double *svm = som + mf;         // [mf] member volume
+    double *sgv = svm + mf;         // [mf*DD] G_m * V_m
+    double *sc = sgv + (size_t)mf * DD;  // [mf*D]  mu_km * om * g * V_m
+    double *sA = sc + (size_t)mf * D;    // [P*P]
+    int *spiv = (int *)(sA + P * P);     // [P] + flag
+    if (n > mf) {
+        if (lane == 0) atomicOr(A.status, 8);
+        return;
+    }
+    const int k = A.fam_node[f];
+    const double xk0 = A.x[(size_t)k * 3], xk1 = A.x[(size_t)k * 3 + 1], xk2 = A.x[(size_t)k * 3 + 2];
+    const double delt = A.fam_delta[f];
+    const double Aa = A.nlf * A.nlf;
+    // stage members (inflFunc pdsolve.cpp:1929-1935)
+    for (int m = lane; m < n; m += 64) {
+        int j = A.fam_mem[off + m];
+        double x0 = A.x[(size_t)j * 3] - xk0, x1 = A.x[(size_t)j * 3 + 1] - xk1, x2 = A.x[(size_t)j * 3 + 2] - xk2;
+        sx[m] = x0; sy[m] = x1; sz[m] = x2;
+        som[m] = exp(-(x0 * x0 + x1 * x1 + x2 * x2) / Aa / delt / delt);
+        svm[m] = A.nv[j];
+    }
+    __syncthreads();
+    // shape tensor, one entry per lane, members in reference order
+    for (int e = lane; e < P * P; e += 64) {
+        double acc = 0.0;
+        for (int m = 1; m < n; m++) acc = a_term<D>(e, sx[m], sy[m], sz[m], som[m], svm[m]) + acc;
+        sA[e] = acc;
+    }
+    __syncthreads();
+    // partial-pivot LU (first maximum pivot), one lane, in LDS
+    if (lane == 0) {
+        int sing = 0;
+        for (int kk = 0; kk < P; kk++) {
+            int p = kk;
+            double mx = fabs(sA[kk * P + kk]);
+            for (int i = kk + 1; i < P; i++) {
+                double v = fabs(sA[i * P + kk]);
+                if (v > mx) { mx = v; p = i; }
+            }
+            spiv[kk] = p;
+            if (sA[p * P + kk] == 0.0) { sing = 1; break; }
+            if (p != kk)
+                for (int c = 0; c < P; c++) {
+                    double t = sA[kk * P + c];
+                    sA[kk * P + c] = sA[p * P + c];
+                    sA[p * P + c] = t;
+                }
+            for (int i = kk + 1; i < P; i++) {
+                double l = sA[i * P + kk] / sA[kk * P + kk];
+                sA[i * P + kk] = l;
+                for (int c = kk + 1; c < P; c++) sA[i * P + c] = sA[i * P + c] - l * sA[kk * P + c];
+            }
+        }
+        spiv[P] = sing;
+    }
+    __syncthreads();
+    if (spiv[P]) {
+        if (lane == 0) atomicOr(A.status, 1);
+        return;
+    }
+    const double cdv = A.fam_v[f];
+    const double lam = A.lam, mu = A.mu;
+    const int p0 = A.fp_off[f], p1 = A.fp_off[f + 1];
+    const size_t nb = (size_t)A.nblk;
+    for (int m = 1 + lane; m < n; m += 64) {
+        const double x0 = sx[m], x1 = sy[m], x2 = sz[m];
+        double b[P];
+        if constexpr (D == 2) {
+            b[0] = x0; b[1] = x1; b[2] = x0 * x0; b[3] = x1 * x1; b[4] = x0 * x1;
+        } else {
+            b[0] = x0; b[1] = x1; b[2] = x2;
+            b[3] = x0 * x0; b[4] = x1 * x1; b[5] = x2 * x2;
+            b[6] = x0 * x1; b[7] = x1 * x2; b[8] = x2 * x0;
+        }
+#pragma unroll
+        for (int kk = 0; kk < P; kk++) {
+            const int p = spiv[kk];
+#pragma unroll
+            for (int i = kk + 1; i < P; i++)
+                if (p == i) { double t = b[kk]; b[kk] = b[i]; b[i] = t; }
+#pragma unroll
+            for (int i = kk + 1; i < P; i++) b[i] = b[i] - sA[i * P + kk] * b[kk];
+        }
+        double s[P];
+#pragma unroll
+        for (int i = P - 1; i >= 0; i--) {
+            double t = b[i];
+#pragma unroll
+            for (int c = i + 1; c < P; c++) t = t - sA[i * P + c] * s[c];
+            s[i] = t / sA[i * P + i];
+        }
+        const double om = som[m], vm = svm[m];
+        const double mukm = (double)A.bond[off + m];
+        double G[DD];
+        // matG2D / matG3D (pdsolve.cpp:2277-2347)
+        if constexpr (D == 2) {
+            const double *dd = s + 2;
+            G[0] = mukm * om * ((2 * mu + lam) * dd[0] + mu * dd[1]);
+            G[1] = G[2] = mukm * om * (lam + mu) * dd[2];
+            G[3] = mukm * om * ((2 * mu + lam) * dd[1] + mu * dd[0]);
+        } else {
+            const double *dd = s + 3;
+            double tr = dd[0] + dd[1] + dd[2];
+            G[0] = mukm * om * ((mu + lam) * dd[0] + mu * tr);
+            G[1] = G[3] = mukm * om * (lam + mu) * dd[3];
+            G[2] = G[6] = mukm * om * (lam + mu) * dd[5];
+            G[4] = mukm * om * ((mu + lam) * dd[1] + mu * tr);
+            G[5] = G[7] = mukm * om * (lam + mu) * dd[4];
+            G[8] = mukm * om * ((mu + lam) * dd[2] + mu * tr);
+        }
+        double val[DD];
+#pragma unroll
+        for (int e = 0; e < DD; e++) {
+            double h = G[e] * vm;  // H[i][j + d*m]
+            sgv[(size_t)m * DD + e] = h;
+            val[e] = -(h * cdv);   // -(H * dv_k), pdsolve.cpp:2562
+        }
+        double c[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            c[i] = mukm * om * s[i] * vm;
+            sc[(size_t)m * D + i] = c[i];
+        }
+        if (p1 > p0) {
+            double Cb[Dm<D>::NV * D];
+            c_block<D>(c, Cb);
+            for (int q = p0; q < p1; q++) add_pdbe_term<D>(A, A.fp_list[q], Cb, val);
+        }
+        const size_t blk = (size_t)A.fam_blk[off + m];
+#pragma unroll
+        for (int e = 0; e < DD; e++) A.K[e * nb + blk] = val[e];
+    }
+    __syncthreads();
+    // self block: sequential sums over members in reference order
+    if (lane < DD) {
+        const int e = lane;
+        double hs = 0.0;
+        for (int m = 1; m < n; m++) hs = -sgv[(size_t)m * DD + e] + hs;
+        double val[DD];
+        for (int q = 0; q < DD; q++) val[q] = 0;
+        val[e] = -(hs * cdv);
+        if (p1 > p0) {
+            double cs[D];
+            for (int i = 0; i < D; i++) {
+                double t = 0.0;
+                for (int m = 1; m < n; m++) t = -sc[(size_t)m * D + i] + t;
+                cs[i] = t;
+            }
+            double Cb[Dm<D>::NV * D];
+            c_block<D>(cs, Cb);
+            for (int q = p0; q < p1; q++) add_pdbe_term<D>(A, A.fp_list[q], Cb, val);
+        }
+        A.K[e * nb + (size_t)A.fam_blk[off]] = val[e];
+    }
+}
+
+// ------------------------------------------------------------------------
+// FE element rows (pdfemEleQuad4N.cpp:252-359, pdfemEleTri3N.cpp:33-47,
+// pdfemEleBrick8N.cpp:324-552, pdfemEleTetra4N.cpp:38-170), node-centric
+// ------------------------------------------------------------------------
+
+struct FeArgs {
+    const double *x, *Dm;
+    const int *econ, *nen, *fe_elem, *fe_blk, *nfe_off, *nfe_item;
+    double *K;
+    long long nblk;
+    int nn;
+    int *status;
+};
+
+__device__ inline double det3(const double J[9]) {
+    // cofactor form of pdfemEles::detMat33 (pdfemEles.cpp:155-172)
+    double det = 0.0;
+    for (int i = 0; i < 3; i++) {
+        double a1 = 1., a2 = 1.;
+        for (int j = 0; j < 3; j++) {
+            a1 *= J[j * 3 + (i + j) % 3];
+            a2 *= J[j * 3 + (i - j + 3) % 3];
+        }
+        det = det + a1 - a2;
+    }
+    return det;
+}
+
+// 3x3 partial-pivot LU solve for 8 right-hand sides (PLUSolve per shape function,
+// pdfemEleBrick8N.cpp:380)
+__device__ inline bool lu3_solve8(double J[9], double dN[8][3], double out[8][3]) {
+    int piv[3];
+    for (int k = 0; k < 3; k++) {
+        int p = k;
+        double mx = fabs(J[k * 3 + k]);
+        for (int i = k + 1; i < 3; i++)
+            if (fabs(J[i * 3 + k]) > mx) { mx = fabs(J[i * 3 + k]); p = i; }
+        piv[k] = p;
+        if (J[p * 3 + k] == 0.0) return false;
+        if (p != k)
+            for (int c = 0; c < 3; c++) { double t = J[k * 3 + c]; J[k * 3 + c] = J[p * 3 + c]; J[p * 3 + c] = t; }
+        for (int i = k + 1; i < 3; i++) {
+            double l = J[i * 3 + k] / J[k * 3 + k];
+            J[i * 3 + k] = l;
+            for (int c = k + 1; c < 3; c++) J[i * 3 + c] = J[i * 3 + c] - l * J[k * 3 + c];
+        }
+    }
+    for (int a = 0; a < 8; a++) {
+        double b0 = dN[a][0], b1 = dN[a][1], b2 = dN[a][2];
+        double bb[3] = {b0, b1, b2};
+        for (int k = 0; k < 3; k++) {
+            int p = piv[k];
+            if (p != k) { double t = bb[k]; bb[k] = bb[p]; bb[p] = t; }
+            for (int i = k + 1; i < 3; i++) bb[i] = bb[i] - J[i * 3 + k] * bb[k];
+        }
+        for (int i = 2; i >= 0; i--) {
+            double t = bb[i];
+            for (int c = i + 1; c < 3; c++) t = t - J[i * 3 + c] * out[a][c];
+            out[a][i] = t / J[i * 3 + i];
+        }
+    }
+    return true;
+}
+
+__constant__ double c_hp[8] = {-1., 1., 1., -1., -1., 1., 1., -1.};
+__constant__ double c_hq[8] = {-1., -1., 1., 1., -1., -1., 1., 1.};
+__constant__ double c_hr[8] = {-1., -1., -1., -1., 1., 1., 1., 1.};
+
+// row `row` of B^T D B for a 3D element whose B columns are given by dNdX (nn nodes)
+// BtD[row][k] = sum_l B[l][row] D[l][k];  (BtD B)[row][c] = sum_k BtD[row][k] B[k][c]
+__device__ inline void btdb_row3(const double (*g)[3], int nnod, int la, int ii, const double *Dg, double w,
+                                 double *Krow) {
+    double Bc[6];  // column `row` of B
+    const double *ga = g[la];
+    for (int l = 0; l < 6; l++) Bc[l] = 0.0;
+    if (ii == 0) { Bc[0] = ga[0]; Bc[3] = ga[1]; Bc[5] = ga[2]; }
+    else if (ii == 1) { Bc[1] = ga[1]; Bc[3] = ga[0]; Bc[4] = ga[2]; }
+    else { Bc[2] = ga[2]; Bc[4] = ga[1]; Bc[5] = ga[0]; }
+    double btd[6];
+    for (int k = 0; k < 6; k++) {
+        double t = 0.0;
+        for (int l = 0; l < 6; l++) t = Bc[l] * Dg[l * 6 + k] + t;
+        btd[k] = t;
+    }
+    for (int b = 0; b < nnod; b++) {
+        const double *gb = g[b];
+        // B[:, 3b+0] = (gx,0,0,gy,0,gz); B[:,3b+1] = (0,gy,0,gx,gz,0); B[:,3b+2] = (0,0,gz,0,gy,gx)
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        t0 = btd[0] * gb[0] + t0; t0 = btd[3] * gb[1] + t0; t0 = btd[5] * gb[2] + t0;
+        t1 = btd[1] * gb[1] + t1; t1 = btd[3] * gb[0] + t1; t1 = btd[4] * gb[2] + t1;
+        t2 = btd[2] * gb[2] + t2; t2 = btd[4] * gb[1] + t2; t2 = btd[5] * gb[0] + t2;
+        Krow[3 * b + 0] = Krow[3 * b + 0] + t0 * w;
+        Krow[3 * b + 1] = Krow[3 * b + 1] + t1 * w;
+        Krow[3 * b + 2] = Krow[3 * b + 2] + t2 * w;
+    }
+}
+
+__device__ inline void btdb_row2(const double (*g)[2], int nnod, int la, int ii, const double *Dg, double w,
+                                 double *Krow) {
+    double Bc[3];
+    const double *ga = g[la];
+    if (ii == 0) { Bc[0] = ga[0]; Bc[1] = 0; Bc[2] = ga[1]; }
+    else { Bc[0] = 0; Bc[1] = ga[1]; Bc[2] = ga[0]; }
+    double btd[3];
+    for (int k = 0; k < 3; k++) {
+        double t = 0.0;
+        for (int l = 0; l < 3; l++) t = Bc[l] * Dg[l * 3 + k] + t;
+        btd[k] = t;
+    }
+    for (int b = 0; b < nnod; b++) {
+        const double *gb = g[b];
+        double t0 = 0.0, t1 = 0.0;
+        t0 = btd[0] * gb[0] + t0; t0 = btd[2] * gb[1] + t0;
+        t1 = btd[1] * gb[1] + t1; t1 = btd[2] * gb[0] + t1;
+        Krow[2 * b + 0] = Krow[2 * b + 0] + t0 * w;
+        Krow[2 * b + 1] = Krow[2 * b + 1] + t1 * w;
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(128) k_fe_rows(FeArgs A) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nn * D) return;
+    const int a = t / D, ii = t % D;
+    const double gp = 0.57735026918962573;
+    const double gpt[2] = {-gp, gp};
+    const size_t nb = (size_t)A.nblk;
+    for (int q = A.nfe_off[a]; q < A.nfe_off[a + 1]; q++) {
+        const int item = A.nfe_item[q];
+        const int fp = item >> 3, la = item & 7;
+        const int e = A.fe_elem[fp];
+        const int n = A.nen[e];
+        double xN[8][3];
+        for (int b = 0; b < n; b++)
+            for (int c = 0; c < 3; c++) xN[b][c] = A.x[(size_t)A.econ[(size_t)e * 8 + b] * 3 + c];
+        double Krow[24];
+        for (int c = 0; c < 24; c++) Krow[c] = 0.0;
+        if constexpr (D == 3) {
+            if (n == 8) {
+                for (int mr = 0; mr < 2; mr++)
+                    for (int mq = 0; mq < 2; mq++)
+                        for (int mp = 0; mp < 2; mp++) {
+                            const double p = gpt[mp], qq = gpt[mq], r = gpt[mr];
+                            double dN[8][3], J[9];
+                            for (int i = 0; i < 8; i++) {
+                                dN[i][0] = 1.0 / 8 * c_hp[i] * (1 + qq * c_hq[i]) * (1 + r * c_hr[i]);
+                                dN[i][1] = 1.0 / 8 * c_hq[i] * (1 + p * c_hp[i]) * (1 + r * c_hr[i]);
+                                dN[i][2] = 1.0 / 8 * c_hr[i] * (1 + p * c_hp[i]) * (1 + qq * c_hq[i]);
+                            }
+                            for (int i = 0; i < 9; i++) J[i] = 0.0;
+                            for (int cc = 0; cc < 3; cc++)
+                                for (int i = 0; i < 8; i++) {
+                                    J[0 * 3 + cc] = dN[i][0] * xN[i][cc] + J[0 * 3 + cc];
+                                    J[1 * 3 + cc] = dN[i][1] * xN[i][cc] + J[1 * 3 + cc];
+                                    J[2 * 3 + cc] = dN[i][2] * xN[i][cc] + J[2 * 3 + cc];
+                                }
+                            const double detJ = det3(J);
+                            double g[8][3];
+                            if (!lu3_solve8(J, dN, g)) { atomicOr(A.status, 2); return; }
+                            btdb_row3(g, 8, la, ii, A.Dm, 1.0 * 1.0 * 1.0 * detJ, Krow);
+                        }
+            } else {
+                // tet4 closed form (pdfemEleTetra4N.cpp:84-170): B = [b c d]/6V, Ke = Bt D B V
+                double m3[9];
+                auto det_rows = [&](int r0, int r1, int r2, int c0, int c1) {
+                    m3[0] = 1; m3[1] = xN[r0][c0]; m3[2] = xN[r0][c1];
+                    m3[3] = 1; m3[4] = xN[r1][c0]; m3[5] = xN[r1][c1];
+                    m3[6] = 1; m3[7] = xN[r2][c0]; m3[8] = xN[r2][c1];
+                    return det3(m3);
+                };
+                double bb[4], cc[4], dd[4];
+                bb[0] = -det_rows(1, 2, 3, 1, 2); bb[1] = det_rows(0, 2, 3, 1, 2);
+                bb[2] = -det_rows(0, 1, 3, 1, 2); bb[3] = det_rows(0, 1, 2, 1, 2);
+                cc[0] = det_rows(1, 2, 3, 0, 2); cc[1] = -det_rows(0, 2, 3, 0, 2);
+                cc[2] = det_rows(0, 1, 3, 0, 2); cc[3] = -det_rows(0, 1, 2, 0, 2);
+                dd[0] = -det_rows(1, 2, 3, 0, 1); dd[1] = det_rows(0, 2, 3, 0, 1);
+                dd[2] = -det_rows(0, 1, 3, 0, 1); dd[3] = det_rows(0, 1, 2, 0, 1);
+                double Jv[9];
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 3; c++) Jv[r * 3 + c] = xN[r + 1][c] - xN[0][c];
+                const double V = fabs(det3(Jv)) / 6.0, V6 = 6.0 * (fabs(det3(Jv)) / 6.0);
+                double g[8][3];
+                for (int i = 0; i < 4; i++) { g[i][0] = bb[i] / V6; g[i][1] = cc[i] / V6; g[i][2] = dd[i] / V6; }
+                // Ke = (BtD B) * V: accumulate with unit weight then scale (product order kept)
+                double tmp[24];
+                for (int c = 0; c < 24; c++) tmp[c] = 0.0;
+                btdb_row3(g, 4, la, ii, A.Dm, 1.0, tmp);
+                for (int c = 0; c < 12; c++) Krow[c] = tmp[c] * V;
+            }
+        } else {
+            if (n == 4) {
+                for (int mm = 0; mm < 2; mm++)
+                    for (int nn2 = 0; nn2 < 2; nn2++) {
+                        const double p = gpt[mm], qq = gpt[nn2];
+                        double dNdp[4] = {0.25 * (-1 + qq), 0.25 * (1 - qq), 0.25 * (1 + qq), 0.25 * (-1 - qq)};
+                        double dNdq[4] = {0.25 * (-1 + p), 0.25 * (-1 - p), 0.25 * (1 + p), 0.25 * (1 - p)};
+                        double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+                        for (int i = 0; i < 4; i++) {
+                            J00 = dNdp[i] * xN[i][0] + J00;
+                            J01 = dNdp[i] * xN[i][1] + J01;
+                            J10 = dNdq[i] * xN[i][0] + J10;
+                            J11 = dNdq[i] * xN[i][1] + J11;
+                        }
+                        const double detJ = J00 * J11 - J01 * J10;
+                        const double i00 = J11 / detJ, i01 = -J01 / detJ, i10 = -J10 / detJ, i11 = J00 / detJ;
+                        double g[4][2];
+                        for (int i = 0; i < 4; i++) {
+                            g[i][0] = i00 * dNdp[i] + i01 * dNdq[i];
+                            g[i][1] = i10 * dNdp[i] + i11 * dNdq[i];
+                        }
+                        btdb_row2(g, 4, la, ii, A.Dm, 1.0 * 1.0 * detJ, Krow);
+                    }
+            } else {
+                // tri3 closed form (pdfemEleTri3N.cpp:33-47, :185-210)
+                double m3[9] = {1, xN[0][0], xN[0][1], 1, xN[1][0], xN[1][1], 1, xN[2][0], xN[2][1]};
+                const double V = 0.5 * fabs(det3(m3)), V2 = 2.0 * (0.5 * fabs(det3(m3)));
+                double bb[3] = {xN[1][1] - xN[2][1], xN[2][1] - xN[0][1], xN[0][1] - xN[1][1]};
+                double cc[3] = {-xN[1][0] + xN[2][0], -xN[2][0] + xN[0][0], -xN[0][0] + xN[1][0]};
+                double g[4][2];
+                for (int i = 0; i < 3; i++) { g[i][0] = bb[i] / V2; g[i][1] = cc[i] / V2; }
+                double tmp[24];
+                for (int c = 0; c < 24; c++) tmp[c] = 0.0;
+                btdb_row2(g, 3, la, ii, A.Dm, 1.0, tmp);
+                for (int c = 0; c < 6; c++) Krow[c] = tmp[c] * V;
+            }
+        }
+        // add the element row into the node's block row (element order = reference order)
+        const int *blk = &A.fe_blk[(size_t)fp * 64 + la * 8];
+        for (int b = 0; b < n; b++) {
+            const size_t bi = (size_t)blk[b];
+            for (int jj = 0; jj < D; jj++) {
+                const size_t idx = (size_t)(ii * D + jj) * nb + bi;
+                A.K[idx] = A.K[idx] + Krow[D * b + jj];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// lumped mass (pdsolve.cpp:4618-4669), node-centric over all elements
+// ------------------------------------------------------------------------
+__global__ void k_lumped(const int *nall_off, const int *nall_ele, const double *ev, const int *nen,
+                         const int *eq, double rho, int nn, int D, double *m) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn * D) return;
+    const int a = t / D, i = t % D;
+    double s = 0.0;
+    for (int q = nall_off[a]; q < nall_off[a + 1]; q++) {
+        const int e = nall_ele[q];
+        const double dv = ev[e] / nen[e];
+        s = s + dv * rho;
+    }
+    m[t] = eq[(size_t)a * 3 + i] >= 0 ? s : 1.0;
+}
+
+// ------------------------------------------------------------------------
+// natural BCs: element forces for a unit traction
+// (pdfemEleLine2N.cpp:45-80; pdfemEleQuad4N.cpp:389-456)
+// ------------------------------------------------------------------------
+__global__ void k_nbc_elements(const double *x, const int *con, int n_ele, int D, double *fe) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_ele) return;
+    const double gp = 0.57735026918962573;
+    const double gpt[2] = {-gp, gp};
+    double *F = &fe[(size_t)e * 12];
+    for (int k = 0; k < 12; k++) F[k] = 0.0;
+    if (D == 2) {
+        const int a = con[e * 2], b = con[e * 2 + 1];
+        const double xN[2][2] = {{x[(size_t)a * 3], x[(size_t)a * 3 + 1]}, {x[(size_t)b * 3], x[(size_t)b * 3 + 1]}};
+        for (int m = 0; m < 2; m++) {
+            const double p = gpt[m], wp = 1.0;
+            const double N[2] = {0.5 * (1 - p), 0.5 * (1 + p)}, dNdp[2] = {-0.5, 0.5};
+            for (int ii = 0; ii < 2; ii++) {
+                double t1 = 0, t2 = 0;
+                for (int jj = 0; jj < 2; jj++) { t1 = t1 + dNdp[jj] * xN[jj][1]; t2 = t2 + dNdp[jj] * xN[jj][0]; }
+                F[ii * 2] = wp * N[ii] * (1.0 * t1) + F[ii * 2];
+                F[ii * 2 + 1] = wp * N[ii] * (-1.0 * t2) + F[ii * 2 + 1];
+            }
+        }
+    } else {
+        double xN[4][3];
+        for (int a = 0; a < 4; a++)
+            for (int c = 0; c < 3; c++) xN[a][c] = x[(size_t)con[e * 4 + a] * 3 + c];
+        for (int mp = 0; mp < 2; mp++)
+            for (int mq = 0; mq < 2; mq++) {
+                const double p = gpt[mp], q = gpt[mq];
+                double N[4] = {0.25 * (1 - p) * (1 - q), 0.25 * (1 + p) * (1 - q), 0.25 * (1 + p) * (1 + q),
+                               0.25 * (1 - p) * (1 + q)};
+                double dNdp[4] = {0.25 * (-1 + q), 0.25 * (1 - q), 0.25 * (1 + q), 0.25 * (-1 - q)};
+                double dNdq[4] = {0.25 * (-1 + p), 0.25 * (-1 - p), 0.25 * (1 + p), 0.25 * (1 - p)};
+                double dXdp[3] = {0, 0, 0}, dXdq[3] = {0, 0, 0};
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 4; j++) {
+                        dXdp[i] = dXdp[i] + dNdp[j] * xN[j][i];
+                        dXdq[i] = dXdq[i] + dNdq[j] * xN[j][i];
+                    }
+                const double nr[3] = {dXdp[1] * dXdq[2] - dXdp[2] * dXdq[1], dXdq[0] * dXdp[2] - dXdp[0] * dXdq[2],
+                                      dXdp[0] * dXdq[1] - dXdq[0] * dXdp[1]};
+                for (int k = 0; k < 4; k++)
+                    for (int i = 0; i < 3; i++) F[3 * k + i] = F[3 * k + i] + N[k] * nr[i];
+            }
+    }
+}
+
+// node-centric gather of unit element forces into f_unit[set][node*D+i] (element order)
+__global__ void k_nbc_gather(const int *off, const int *item, const int *nbc_set, const double *fe, int nn, int D,
+                             int n_set, double *f_unit) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn * D) return;
+    const int a = t / D, i = t % D;
+    const int tn = D == 3 ? 4 : 2;
+    for (int s = 0; s < n_set; s++) f_unit[(size_t)s * nn * D + t] = 0.0;
+    for (int q = off[a]; q < off[a + 1]; q++) {
+        const int it = item[q];
+        const int e = it / tn, la = it % tn;
+        const int s = nbc_set[e];
+        f_unit[(size_t)s * nn * D + t] = f_unit[(size_t)s * nn * D + t] + fe[(size_t)e * 12 + D * la + i];
+    }
+}
+
+// ------------------------------------------------------------------------
+// node-block SpMV: y = K x (full DOF vectors), optional mask of prescribed rows
+// ------------------------------------------------------------------------
+template <int D, typename T, int G>
+__global__ void __launch_bounds__(256) k_spmv(const int *rows, int nrows, const int *adj_off, const int *adj,
+                                              const T *K, long long nblk, const double *x, double *y,
+                                              const int *eq, int mask) {
+    const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    if (gid >= nrows) return;
+    const int r = rows[gid];
+    const size_t nb = (size_t)nblk;
+    T acc[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) acc[i] = T(0);
+    for (int b = adj_off[r] + lane; b < adj_off[r + 1]; b += G) {
+        const int c = adj[b];
+        T xv[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) xv[j] = T(x[(size_t)c * D + j]);
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) acc[i] += K[(size_t)(i * D + j) * nb + b] * xv[j];
+    }
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1)
+#pragma unroll
+        for (int i = 0; i < D; i++) acc[i] += __shfl_xor(acc[i], s, G);
+    if (lane < D) {
+        T v = acc[0];
+#pragma unroll
+        for (int i = 1; i < D; i++)
+            if (lane == i) v = acc[i];
+        const size_t o = (size_t)r * D + lane;
+        y[o] = (mask && eq[(size_t)r * 3 + lane] < 0) ? 0.0 : double(v);
+    }
+}
+
+// ------------------------------------------------------------------------
+// block rows -> reference CSR values of K_ff
+// ------------------------------------------------------------------------
+__global__ void k_csr_extract(const int *row_node, int neq, const int64_t *ia, const int *adj_off, const int *adj,
+                              const int *eq, const double *K, long long nblk, int D, double *Ku) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= neq) return;
+    const int code = row_node[r];
+    const int k = code / 3, ii = code % 3;
+    int64_t p = ia[r];
+    const size_t nb = (size_t)nblk;
+    for (int b = adj_off[k]; b < adj_off[k + 1]; b++) {
+        const int c = adj[b];
+        for (int jj = 0; jj < D; jj++)
+            if (eq[(size_t)c * 3 + jj] >= 0) Ku[p++] = K[(size_t)(ii * D + jj) * nb + b];
+    }
+}
+
+__global__ void k_fill(double *p, double v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------
+int dev_fill(Ctx &c, double *p, double v, size_t n) {
+    if (!n) return PDX_OK;
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    k_fill<<<blocks, 256, 0, c.stream>>>(p, v, n);
+    return hipGetLastError() == hipSuccess ? PDX_OK : PDX_ERR_HIP;
+}
+
+static size_t family_lds_bytes(int D, int mf) {
+    const int P = D == 2 ? 5 : 9, DD = D * D;
+    return (size_t)mf * (5 + DD + D) * sizeof(double) + (size_t)P * P * sizeof(double) + (P + 2) * sizeof(int);
+}
+
+int dev_assemble_K(Ctx &c, std::string &err) {
+    int *status = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&status, sizeof(int), c.stream));
+    PDX_HIP(hipMemsetAsync(status, 0, sizeof(int), c.stream));
+    if (dev_fill(c, c.K, 0.0, (size_t)c.dd * c.nblk)) { err = "fill failed"; return PDX_ERR_HIP; }
+    if (c.nfam > 0) {
+        FamArgs A;
+        A.x = c.x; A.nv = c.nv; A.fam_node = c.fam_node; A.fam_off = c.fam_off; A.fam_mem = c.fam_mem;
+        A.bond = c.bond; A.fam_blk = c.fam_blk; A.fam_v = c.fam_v; A.fam_delta = c.fam_delta;
+        A.fp_off = c.fp_off; A.fp_list = c.fp_list; A.pdbe = c.pdbe; A.Dm = c.Dm;
+        A.lam = c.lambda; A.mu = c.mu; A.nlf = c.nlf; A.K = c.K; A.nblk = c.nblk; A.status = status;
+        A.mf = c.max_family;
+        size_t lds = family_lds_bytes(c.dim, A.mf);
+        if (lds > 160 * 1024) {
+            err = "family too large for LDS (" + std::to_string(c.max_family) + " members)";
+            hipFreeAsync(status, c.stream);
+            return PDX_ERR_LIMIT;
+        }
+        if (c.dim == 3) {
+            PDX_HIP(hipFuncSetAttribute((const void *)k_family<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            k_family<3><<<c.nfam, 64, lds, c.stream>>>(A);
+        } else {
+            PDX_HIP(hipFuncSetAttribute((const void *)k_family<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            k_family<2><<<c.nfam, 64, lds, c.stream>>>(A);
+        }
+        PDX_HIP(hipGetLastError());
+    }
+    if (c.nfe > 0) {
+        FeArgs F;
+        F.x = c.x; F.Dm = c.Dm; F.econ = c.econ; F.nen = c.nen; F.fe_elem = c.fe_elem; F.fe_blk = c.fe_blk;
+        F.nfe_off = c.nfe_off; F.nfe_item = c.nfe_item; F.K = c.K; F.nblk = c.nblk; F.nn = c.nn; F.status = status;
+        int th = c.nn * c.dim;
+        if (c.dim == 3) k_fe_rows<3><<<(th + 127) / 128, 128, 0, c.stream>>>(F);
+        else k_fe_rows<2><<<(th + 127) / 128, 128, 0, c.stream>>>(F);
+        PDX_HIP(hipGetLastError());
+    }
+    int hs = 0;
+    PDX_HIP(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PDX_HIP(hipStreamSynchronize(c.stream));
+    PDX_HIP(hipFree(status));
+    if (hs & 1) { err = "singular shape tensor (family with too few non-collinear members)"; return PDX_ERR_SINGULAR; }
+    if (hs & 2) { err = "singular element Jacobian"; return PDX_ERR_SINGULAR; }
+    if (hs & 4) { err = "Area is zero (PD boundary face)"; return PDX_ERR_MESH; }
+    if (hs & 8) { err = "family exceeds the LDS capacity"; return PDX_ERR_LIMIT; }
+    c.K_valid = true;
+    c.K32_valid = false;
+    return PDX_OK;
+}
+
+int dev_lumped_mass(Ctx &c, std::string &err) {
+    int th = c.nn * c.dim;
+    k_lumped<<<(th + 255) / 256, 256, 0, c.stream>>>(c.nall_off, c.nall_ele, c.ev, c.nen, c.eq, c.rho, c.nn, c.dim,
+                                                      c.mlump);
+    PDX_HIP(hipGetLastError());
+    c.mlump_valid = true;
+    return PDX_OK;
+}
+
+int dev_nbc_unit(Ctx &c, std::string &err) {
+    if (c.n_nbc == 0) return PDX_OK;
+    if (c.n_nbc_ele > 0) {
+        k_nbc_elements<<<(c.n_nbc_ele + 127) / 128, 128, 0, c.stream>>>(c.x, c.nbc_con, c.n_nbc_ele, c.dim, c.nbc_fe);
+        PDX_HIP(hipGetLastError());
+    }
+    int th = c.nn * c.dim;
+    k_nbc_gather<<<(th + 255) / 256, 256, 0, c.stream>>>(c.nbcn_off, c.nbcn_item, c.nbc_set, c.nbc_fe, c.nn, c.dim,
+                                                          c.n_nbc, c.f_unit);
+    PDX_HIP(hipGetLastError());
+    return PDX_OK;
+}
+
+int dev_extract_csr(Ctx &c, const double *Kblk, double *Ku_dev) {
+    if (c.neq == 0) return PDX_OK;
+    k_csr_extract<<<(c.neq + 255) / 256, 256, 0, c.stream>>>(c.row_node, c.neq, c.ia, c.adj_off, c.adj, c.eq, Kblk,
+                                                             c.nblk, c.dim, Ku_dev);
+    return hipGetLastError() == hipSuccess ? PDX_OK : PDX_ERR_HIP;
+}
+
+template <int D, typename T>
+static void launch_spmv(Ctx &c, const T *K, const double *x, double *y, bool mask) {
+    if (c.n_short) {
+        const int th = c.n_short * 8;
+        k_spmv<D, T, 8><<<(th + 255) / 256, 256, 0, c.stream>>>(c.rows_short, c.n_short, c.adj_off, c.adj, K, c.nblk,
+                                                                 x, y, c.eq, mask);
+    }
+    if (c.n_long) {
+        const int th = c.n_long * 64;
+        k_spmv<D, T, 64><<<(th + 255) / 256, 256, 0, c.stream>>>(c.rows_long, c.n_long, c.adj_off, c.adj, K, c.nblk,
+                                                                 x, y, c.eq, mask);
+    }
+}
+
+int dev_spmv_K(Ctx &c, const double *K64, const double *x, double *y) {
+    if (c.dim == 3) launch_spmv<3, double>(c, K64, x, y, true);
+    else launch_spmv<2, double>(c, K64, x, y, true);
+    return hipGetLastError() == hipSuccess ? PDX_OK : PDX_ERR_HIP;
+}
+
+int dev_spmv(Ctx &c, const double *x, double *y, int prec, bool mask) {
+    if (prec == PDX_FP32) {
+        if (c.dim == 3) launch_spmv<3, float>(c, c.K32, x, y, mask);
+        else launch_spmv<2, float>(c, c.K32, x, y, mask);
+    } else {
+        if (c.dim == 3) launch_spmv<3, double>(c, c.K, x, y, mask);
+        else launch_spmv<2, double>(c, c.K, x, y, mask);
+    }
+    return hipGetLastError() == hipSuccess ? PDX_OK : PDX_ERR_HIP;
+}
+
+}  // namespace pdx

@@ -1,0 +1,568 @@
+// Device context construction (upload of the host model in the HBM layout of
+// device.hpp) and the extern "C" entry points of libpdx.so (include/pdx.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "../../include/pdx.h"
+#include "device.hpp"
+
+namespace pdx {
+
+int explicit_prepare(Ctx &c, int prec, std::string &err);
+int dev_explicit_steps(Ctx &c, int nsteps, int prec, int use_graph, std::string &err);
+int dev_explicit_state(Ctx &c, double *u_out, double *u_prev, std::string &err);
+int dev_max_eig(Ctx &c, int iters, double *lmax, std::string &err);
+int dev_solve(Ctx &c, const pdx_solve_opts &o, const double *b_dev, double *x_dev, pdx_solve_stats &st,
+              std::string &err);
+
+template <class T>
+static int up(Ctx &c, T *&dst, const std::vector<T> &src, std::string &err) {
+    size_t n = std::max<size_t>(src.size(), 1);
+    PDX_HIP(hipMalloc((void **)&dst, n * sizeof(T)));
+    c.owned.push_back(dst);
+    if (!src.empty()) PDX_HIP(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PDX_OK;
+}
+
+template <class T>
+static int alloc(Ctx &c, T *&dst, size_t n, std::string &err) {
+    PDX_HIP(hipMalloc((void **)&dst, std::max<size_t>(n, 1) * sizeof(T)));
+    c.owned.push_back(dst);
+    PDX_HIP(hipMemset(dst, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    return PDX_OK;
+}
+
+#define UP(dst, src) \
+    if ((rc = up(*c, dst, src, err))) return rc
+#define AL(dst, n) \
+    if ((rc = alloc(*c, dst, n, err))) return rc
+
+static int find_block(const Model &m, int row_node, int col_node) {
+    auto b = m.adj.begin() + m.adj_off[row_node], e = m.adj.begin() + m.adj_off[row_node + 1];
+    auto it = std::lower_bound(b, e, col_node);
+    if (it == e || *it != col_node) return -1;
+    return int(it - m.adj.begin());
+}
+
+int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
+    int rc = PDX_OK;
+    PDX_HIP(hipSetDevice(device));
+    Ctx *c = new Ctx();
+    out = c;
+    c->device = device;
+    c->host = &m;
+    PDX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const int D = m.dim;
+    c->dim = D;
+    c->dd = D * D;
+    c->nn = m.nn;
+    c->ne = m.ne;
+    c->neq = m.neq;
+    c->nfam = m.nfam;
+    c->max_family = std::max(m.max_family, 1);
+    c->npdbe = m.npdbe;
+    c->nblk = m.adj_off[m.nn];
+    c->nnz = m.nnz;
+    c->lambda = m.lambda;
+    c->mu = m.mu;
+    c->nlf = m.nlf;
+    c->rho = m.rho;
+    c->dt = m.dt;
+    c->beta = m.beta;
+    c->gamma = m.gamma;
+
+    UP(c->x, m.x);
+    UP(c->nv, m.nv);
+    UP(c->ev, m.ev);
+    UP(c->Dm, std::vector<double>(m.D, m.D + 36));
+    UP(c->econ, m.econ);
+    UP(c->nen, m.nen);
+    UP(c->eq, m.eq);
+    UP(c->presc_set, m.presc_set);
+    UP(c->adj_off, m.adj_off);
+    UP(c->adj, m.adj);
+    UP(c->ia, m.ia);
+
+    // families + scatter map (block index of every member in the family node's row)
+    std::vector<int> fam_blk(m.fam_mem.size());
+    for (int f = 0; f < m.nfam; f++)
+        for (int q = m.fam_off[f]; q < m.fam_off[f + 1]; q++) {
+            int b = find_block(m, m.fam_node[f], m.fam_mem[q]);
+            if (b < 0) { err = "Error: can not build matrix in CSR format (family member)"; return PDX_ERR_PATTERN; }
+            fam_blk[q] = b;
+        }
+    UP(c->fam_node, m.fam_node);
+    UP(c->fam_off, m.fam_off);
+    UP(c->fam_mem, m.fam_mem);
+    UP(c->bond, m.bond);
+    UP(c->fam_blk, fam_blk);
+    UP(c->fam_v, m.fam_v);
+    UP(c->fam_delta, m.fam_delta);
+    // PD-boundary entities incident to each family's node, in entity order
+    {
+        std::vector<std::vector<int>> inc(m.nfam);
+        const int per = D == 2 ? 2 : 4;
+        for (int p = 0; p < m.npdbe; p++) {
+            const int *con = &m.pdbe[p * 4];
+            int cnt = per;
+            if (D == 3 && con[3] == con[2]) cnt = 3;
+            for (int a = 0; a < cnt; a++) {
+                int f = m.famid[con[a]];
+                if (f < 0) { err = "Warning: This node of PD domain boundary is not PD node!"; return PDX_ERR_MESH; }
+                inc[f].push_back(p);
+            }
+        }
+        std::vector<int> off(m.nfam + 1, 0), lst;
+        for (int f = 0; f < m.nfam; f++) {
+            off[f + 1] = off[f] + int(inc[f].size());
+            lst.insert(lst.end(), inc[f].begin(), inc[f].end());
+        }
+        UP(c->fp_off, off);
+        UP(c->fp_list, lst);
+        std::vector<int> pd(m.pdbe);
+        UP(c->pdbe, pd);
+    }
+    // FE element rows: node -> (fe position, local index) in element order; block map
+    {
+        c->nfe = int(m.fe.size());
+        std::vector<int> fe_blk((size_t)c->nfe * 64, -1), cnt(m.nn + 1, 0);
+        for (int t = 0; t < c->nfe; t++) {
+            int e = m.fe[t];
+            for (int a = 0; a < m.nen[e]; a++) {
+                cnt[m.econ[(size_t)e * 8 + a] + 1]++;
+                for (int b = 0; b < m.nen[e]; b++) {
+                    int bi = find_block(m, m.econ[(size_t)e * 8 + a], m.econ[(size_t)e * 8 + b]);
+                    if (bi < 0) { err = "Error: can not build matrix in CSR format (element)"; return PDX_ERR_PATTERN; }
+                    fe_blk[(size_t)t * 64 + a * 8 + b] = bi;
+                }
+            }
+        }
+        for (int k = 0; k < m.nn; k++) cnt[k + 1] += cnt[k];
+        std::vector<int> item(cnt[m.nn]), fill(cnt.begin(), cnt.end() - 1);
+        for (int t = 0; t < c->nfe; t++) {
+            int e = m.fe[t];
+            for (int a = 0; a < m.nen[e]; a++) item[fill[m.econ[(size_t)e * 8 + a]]++] = t * 8 + a;
+        }
+        UP(c->fe_elem, m.fe);
+        UP(c->fe_blk, fe_blk);
+        UP(c->nfe_off, cnt);
+        UP(c->nfe_item, item);
+        // all elements (lumped mass)
+        std::vector<int> acnt(m.nn + 1, 0);
+        for (int e = 0; e < m.ne; e++)
+            for (int a = 0; a < m.nen[e]; a++) acnt[m.econ[(size_t)e * 8 + a] + 1]++;
+        for (int k = 0; k < m.nn; k++) acnt[k + 1] += acnt[k];
+        std::vector<int> aitem(acnt[m.nn]), afill(acnt.begin(), acnt.end() - 1);
+        for (int e = 0; e < m.ne; e++)
+            for (int a = 0; a < m.nen[e]; a++) aitem[afill[m.econ[(size_t)e * 8 + a]]++] = e;
+        UP(c->nall_off, acnt);
+        UP(c->nall_ele, aitem);
+    }
+    // rows: equation -> node*3+dof, and node rows binned by length
+    {
+        std::vector<int> rn(std::max(m.neq, 1), 0);
+        for (int k = 0; k < m.nn; k++)
+            for (int i = 0; i < D; i++)
+                if (m.eq[(size_t)k * 3 + i] >= 0) rn[m.eq[(size_t)k * 3 + i]] = k * 3 + i;
+        UP(c->row_node, rn);
+        std::vector<int> rs, rl;
+        for (int k = 0; k < m.nn; k++) (m.adj_off[k + 1] - m.adj_off[k] > 48 ? rl : rs).push_back(k);
+        c->n_short = int(rs.size());
+        c->n_long = int(rl.size());
+        UP(c->rows_short, rs);
+        UP(c->rows_long, rl);
+    }
+    // natural BCs
+    {
+        c->n_nbc = int(m.nbc.size());
+        const int tn = D == 3 ? 4 : 2;
+        std::vector<int> con, set;
+        for (int s = 0; s < c->n_nbc; s++) {
+            con.insert(con.end(), m.nbc[s].con.begin(), m.nbc[s].con.end());
+            for (size_t e = 0; e < m.nbc[s].con.size() / tn; e++) set.push_back(s);
+            c->nbc_val.push_back(m.nbc[s].val);
+            c->nbc_vel.push_back(m.nbc[s].vel);
+            c->nbc_vary.push_back(m.nbc[s].vary);
+        }
+        c->n_nbc_ele = int(set.size());
+        std::vector<int> cnt(m.nn + 1, 0);
+        for (int e = 0; e < c->n_nbc_ele; e++)
+            for (int a = 0; a < tn; a++) cnt[con[(size_t)e * tn + a] + 1]++;
+        for (int k = 0; k < m.nn; k++) cnt[k + 1] += cnt[k];
+        std::vector<int> item(cnt[m.nn]), fill(cnt.begin(), cnt.end() - 1);
+        for (int e = 0; e < c->n_nbc_ele; e++)
+            for (int a = 0; a < tn; a++) item[fill[con[(size_t)e * tn + a]]++] = e * tn + a;
+        UP(c->nbc_con, con);
+        UP(c->nbc_set, set);
+        UP(c->nbcn_off, cnt);
+        UP(c->nbcn_item, item);
+        AL(c->nbc_fe, (size_t)c->n_nbc_ele * 12);
+        AL(c->f_unit, (size_t)std::max(c->n_nbc, 1) * m.nn * D);
+        AL(c->nbcv, 2 * (size_t)std::max(c->n_nbc, 1));
+        AL(c->d_nbc_vel, (size_t)std::max(c->n_nbc, 1));
+        for (auto &s : m.ebc) {
+            c->ebc_val.push_back(s.val);
+            c->ebc_vel.push_back(s.vel);
+            c->ebc_vary.push_back(s.vary);
+        }
+        AL(c->d_ebc_vel, std::max<size_t>(m.ebc.size(), 1));
+    }
+    AL(c->K, (size_t)c->dd * c->nblk);
+    AL(c->mlump, (size_t)m.nn * D);
+    for (int i = 0; i < 3; i++) AL(c->U[i], (size_t)m.nn * D);
+    if ((rc = dev_nbc_unit(*c, err))) return rc;
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+void ctx_free(Ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &g : c->graph)
+        if (g) hipGraphExecDestroy(g);
+    for (void *p : c->owned) hipFree(p);
+    if (c->K32) hipFree(c->K32);
+    if (c->Mk) hipFree(c->Mk);
+    if (c->red) hipFree(c->red);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // namespace pdx
+
+// ==========================================================================
+// C-ABI
+// ==========================================================================
+using namespace pdx;
+
+static thread_local std::string g_err;
+
+namespace pdx {
+int set_error(int rc, const std::string &msg) {
+    g_err = msg;
+    return rc;
+}
+}  // namespace pdx
+
+static int fail(int rc, const std::string &msg) { return set_error(rc, msg); }
+
+extern "C" {
+
+const char *pdx_last_error(void) { return g_err.c_str(); }
+const char *pdx_version(void) { return "pdx 0.1 (gfx950)"; }
+
+int pdx_model_create(const pdx_mesh_desc *desc, pdx_model **out) {
+    if (!desc || !out) return fail(PDX_ERR_ARG, "null argument");
+    auto *pm = new pdx_model();
+    std::string err;
+    int rc = model_from_desc(*desc, pm->m, err);
+    if (rc) { delete pm; return fail(rc, err); }
+    *out = pm;
+    return PDX_OK;
+}
+
+int pdx_model_read(const char *cmd_file, pdx_model **out, int *solver_flag, int *num_steps) {
+    if (!cmd_file || !out) return fail(PDX_ERR_ARG, "null argument");
+    auto *pm = new pdx_model();
+    std::string err;
+    int sf = -1, ns = 1;
+    int rc = model_read_files(cmd_file, pm->m, sf, ns, err);
+    if (rc) { delete pm; return fail(rc, err); }
+    if (solver_flag) *solver_flag = sf;
+    if (num_steps) *num_steps = ns;
+    *out = pm;
+    return PDX_OK;
+}
+
+void pdx_model_destroy(pdx_model *m) { delete m; }
+
+int pdx_model_sizes(const pdx_model *pm, long long *out) {
+    if (!pm || !out) return fail(PDX_ERR_ARG, "null argument");
+    const Model &m = pm->m;
+    int npresc = 0;
+    for (int k = 0; k < m.nn; k++)
+        for (int i = 0; i < m.dim; i++) npresc += m.eq[(size_t)k * 3 + i] < 0;
+    long long v[16] = {m.nn, m.ne, m.dim, m.neq, (long long)m.nnz, m.nfam, (long long)m.fam_mem.size(), m.npdbe,
+                       (long long)m.pde.size(), (long long)m.fe.size(), (long long)m.adj.size(), npresc,
+                       m.nb[0], m.nb[1], m.nb[2], m.max_family};
+    std::memcpy(out, v, sizeof(v));
+    return PDX_OK;
+}
+
+int pdx_model_csr(const pdx_model *pm, long long *ia, long long *ja) {
+    if (!pm) return fail(PDX_ERR_ARG, "null argument");
+    const Model &m = pm->m;
+    if (ia) std::copy(m.ia.begin(), m.ia.end(), ia);
+    if (ja) std::copy(m.ja.begin(), m.ja.end(), ja);
+    return PDX_OK;
+}
+
+int pdx_model_families(const pdx_model *pm, int *fam_node, int *fam_off, int *fam_mem, int *bond, double *cd_v,
+                       double *delta) {
+    if (!pm) return fail(PDX_ERR_ARG, "null argument");
+    const Model &m = pm->m;
+    if (fam_node) std::copy(m.fam_node.begin(), m.fam_node.end(), fam_node);
+    if (fam_off) std::copy(m.fam_off.begin(), m.fam_off.end(), fam_off);
+    if (fam_mem)
+        for (size_t i = 0; i < m.fam_mem.size(); i++) fam_mem[i] = m.fam_mem[i] + 1;
+    if (bond) std::copy(m.bond.begin(), m.bond.end(), bond);
+    if (cd_v) std::copy(m.fam_v.begin(), m.fam_v.end(), cd_v);
+    if (delta) std::copy(m.fam_delta.begin(), m.fam_delta.end(), delta);
+    return PDX_OK;
+}
+
+int pdx_model_nodes(const pdx_model *pm, int *eq, int *ntype, double *nv, double *ev, int *pdbe, int *algo) {
+    if (!pm) return fail(PDX_ERR_ARG, "null argument");
+    const Model &m = pm->m;
+    if (eq) std::copy(m.eq.begin(), m.eq.end(), eq);
+    if (ntype) std::copy(m.ntype.begin(), m.ntype.end(), ntype);
+    if (nv) std::copy(m.nv.begin(), m.nv.end(), nv);
+    if (ev) std::copy(m.ev.begin(), m.ev.end(), ev);
+    if (pdbe)
+        for (size_t i = 0; i < m.pdbe.size(); i++) pdbe[i] = m.pdbe[i] + 1;
+    if (algo) std::copy(m.algo.begin(), m.algo.end(), algo);
+    return PDX_OK;
+}
+
+int pdx_model_values(const pdx_model *pm, double *u_all) {
+    if (!pm || !u_all) return fail(PDX_ERR_ARG, "null argument");
+    const Model &m = pm->m;
+    for (int k = 0; k < m.nn; k++)
+        for (int i = 0; i < m.dim; i++) u_all[(size_t)k * m.dim + i] = m.dofval[(size_t)k * 3 + i];
+    return PDX_OK;
+}
+
+int pdx_ctx_create(const pdx_model *pm, int device, pdx_ctx **out) {
+    if (!pm || !out) return fail(PDX_ERR_ARG, "null argument");
+    auto *pc = new pdx_ctx();
+    std::string err;
+    int rc = ctx_build(pm->m, device, pc->c, err);
+    if (rc) {
+        ctx_free(pc->c);
+        delete pc;
+        return fail(rc, err);
+    }
+    pc->model = pm;
+    const Model &m = pm->m;
+    pc->dofval.resize((size_t)m.nn * m.dim);
+    pdx_model_values(pm, pc->dofval.data());
+    *out = pc;
+    return PDX_OK;
+}
+
+void pdx_ctx_destroy(pdx_ctx *pc) {
+    if (!pc) return;
+    ctx_free(pc->c);
+    delete pc;
+}
+
+void *pdx_ctx_stream(pdx_ctx *pc) { return pc && pc->c ? (void *)pc->c->stream : nullptr; }
+
+int pdx_set_dof_values(pdx_ctx *pc, const double *u_all) {
+    if (!pc || !u_all) return fail(PDX_ERR_ARG, "null argument");
+    std::copy(u_all, u_all + pc->dofval.size(), pc->dofval.begin());
+    return PDX_OK;
+}
+
+int pdx_set_dt(pdx_ctx *pc, double dt) {
+    if (!pc || !(dt > 0)) return fail(PDX_ERR_ARG, "bad dt");
+    pc->c->dt = dt;
+    for (auto &g : pc->c->graph)
+        if (g) { hipGraphExecDestroy(g); g = nullptr; }
+    return PDX_OK;
+}
+
+// F = F_ext - K u (masked to free rows) for a full DOF vector u (host)
+static int force_from(pdx_ctx *pc, const std::vector<double> &u_all, bool with_ext, double *F, std::string &err) {
+    Ctx &c = *pc->c;
+    const Model &m = pc->model->m;
+    const size_t n = (size_t)c.nn * c.dim;
+    double *du, *dy;
+    PDX_HIP(hipMalloc((void **)&du, n * 8));
+    PDX_HIP(hipMalloc((void **)&dy, n * 8));
+    PDX_HIP(hipMemcpyAsync(du, u_all.data(), n * 8, hipMemcpyHostToDevice, c.stream));
+    if (dev_spmv(c, du, dy, PDX_FP64, true)) { err = "spmv failed"; return PDX_ERR_HIP; }
+    std::vector<double> y(n), fu;
+    if (with_ext && c.n_nbc) {
+        fu.resize((size_t)c.n_nbc * n);
+        PDX_HIP(hipMemcpyAsync(fu.data(), c.f_unit, fu.size() * 8, hipMemcpyDeviceToHost, c.stream));
+    }
+    PDX_HIP(hipMemcpyAsync(y.data(), dy, n * 8, hipMemcpyDeviceToHost, c.stream));
+    PDX_HIP(hipStreamSynchronize(c.stream));
+    hipFree(du);
+    hipFree(dy);
+    for (int k = 0; k < c.nn; k++)
+        for (int i = 0; i < c.dim; i++) {
+            int r = m.eq[(size_t)k * 3 + i];
+            if (r < 0) continue;
+            size_t o = (size_t)k * c.dim + i;
+            double fe = 0.0;
+            if (with_ext)
+                for (int s = 0; s < c.n_nbc; s++) fe = fe + c.nbc_val[s] * fu[(size_t)s * n + o];
+            F[r] = fe - y[o];
+        }
+    return PDX_OK;
+}
+
+int pdx_assemble(pdx_ctx *pc, int mode, const double *UN, double *Ku, double *F, int with_ext) {
+    if (!pc || !F) return fail(PDX_ERR_ARG, "null argument");
+    Ctx &c = *pc->c;
+    const Model &m = pc->model->m;
+    std::string err;
+    if (mode == PDX_MODE_NEWMARK && !UN) return fail(PDX_ERR_ARG, "Newmark assembly needs U_N");
+    int rc = dev_assemble_K(c, err);
+    if (rc) return fail(rc, err);
+    const size_t n = (size_t)c.nn * c.dim;
+    std::vector<double> u(n, 0.0);
+    if (mode == PDX_MODE_STATIC) {
+        for (int k = 0; k < c.nn; k++)
+            for (int i = 0; i < c.dim; i++) {
+                int ps = m.presc_set[(size_t)k * 3 + i];
+                if (ps >= 0) u[(size_t)k * c.dim + i] = pc->dofval[(size_t)k * c.dim + i];
+            }
+    } else if (mode == PDX_MODE_NEWMARK) {
+        std::copy(UN, UN + n, u.begin());
+    } else {
+        u = pc->dofval;
+    }
+    if ((rc = force_from(pc, u, with_ext != 0, F, err))) return fail(rc, err);
+    if (Ku && mode != PDX_MODE_EXPLICIT) {
+        double *dk;
+        if (hipMalloc((void **)&dk, std::max<size_t>(c.nnz, 1) * 8) != hipSuccess) return fail(PDX_ERR_HIP, "alloc");
+        dev_extract_csr(c, c.K, dk);
+        hipMemcpyAsync(Ku, dk, c.nnz * 8, hipMemcpyDeviceToHost, c.stream);
+        hipStreamSynchronize(c.stream);
+        hipFree(dk);
+    }
+    return hipGetLastError() == hipSuccess ? PDX_OK : fail(PDX_ERR_HIP, "HIP error after assembly");
+}
+
+int pdx_assemble_timed(pdx_ctx *pc, double *ms) {
+    if (!pc) return fail(PDX_ERR_ARG, "null argument");
+    Ctx &c = *pc->c;
+    std::string err;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail(PDX_ERR_HIP, "event");
+    hipEventRecord(e0, c.stream);
+    int rc = dev_assemble_K(c, err);
+    hipEventRecord(e1, c.stream);
+    hipEventSynchronize(e1);
+    float t = 0;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (ms) *ms = t;
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
+int pdx_lumped_mass(pdx_ctx *pc, double *mout) {
+    if (!pc || !mout) return fail(PDX_ERR_ARG, "null argument");
+    Ctx &c = *pc->c;
+    const Model &m = pc->model->m;
+    std::string err;
+    int rc = dev_lumped_mass(c, err);
+    if (rc) return fail(rc, err);
+    std::vector<double> h((size_t)c.nn * c.dim);
+    hipMemcpyAsync(h.data(), c.mlump, h.size() * 8, hipMemcpyDeviceToHost, c.stream);
+    hipStreamSynchronize(c.stream);
+    for (int k = 0; k < c.nn; k++)
+        for (int i = 0; i < c.dim; i++) {
+            int r = m.eq[(size_t)k * 3 + i];
+            if (r >= 0) mout[r] = h[(size_t)k * c.dim + i];
+        }
+    return PDX_OK;
+}
+
+int pdx_explicit_run(pdx_ctx *pc, int nsteps, int precision, double *u_out, double *u_prev) {
+    if (!pc || nsteps < 0) return fail(PDX_ERR_ARG, "bad argument");
+    std::string err;
+    int rc = dev_explicit_steps(*pc->c, nsteps, precision, 1, err);
+    if (rc) return fail(rc, err);
+    rc = dev_explicit_state(*pc->c, u_out, u_prev, err);
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
+int pdx_explicit_steps(pdx_ctx *pc, int nsteps, int precision, int use_graph) {
+    if (!pc || nsteps < 0) return fail(PDX_ERR_ARG, "bad argument");
+    std::string err;
+    int rc = dev_explicit_steps(*pc->c, nsteps, precision, use_graph, err);
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
+int pdx_last_timing(pdx_ctx *pc, double *total_ms, double *per_step_ms) {
+    if (!pc) return fail(PDX_ERR_ARG, "null argument");
+    if (total_ms) *total_ms = pc->c->last_total_ms;
+    if (per_step_ms) *per_step_ms = pc->c->last_step_ms;
+    return PDX_OK;
+}
+
+int pdx_max_eig(pdx_ctx *pc, int iters, double *lmax) {
+    if (!pc || !lmax || iters <= 0) return fail(PDX_ERR_ARG, "bad argument");
+    std::string err;
+    int rc = dev_max_eig(*pc->c, iters, lmax, err);
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
+static pdx_solve_opts default_opts() {
+    pdx_solve_opts o;
+    o.krylov = PDX_KRYLOV_BICGSTAB;
+    o.precond = PDX_PREC_JACOBI;
+    o.precision = PDX_FP64;
+    o.tol = 1e-8;
+    o.max_iter = 20000;
+    o.inner_tol = 1e-4;
+    o.warm_start = 0;
+    return o;
+}
+
+int pdx_static_solve(pdx_ctx *pc, const pdx_solve_opts *op, double *u, pdx_solve_stats *st) {
+    if (!pc || !u) return fail(PDX_ERR_ARG, "null argument");
+    Ctx &c = *pc->c;
+    const Model &m = pc->model->m;
+    pdx_solve_opts o = op ? *op : default_opts();
+    std::string err;
+    int rc;
+    if (!c.K_valid && (rc = dev_assemble_K(c, err))) return fail(rc, err);
+    const size_t n = (size_t)c.nn * c.dim;
+    // b = F_ext - K_fp u_p, as a full DOF vector with prescribed entries zero
+    std::vector<double> up(n, 0.0), Fq(std::max(c.neq, 1));
+    for (int k = 0; k < c.nn; k++)
+        for (int i = 0; i < c.dim; i++)
+            if (m.presc_set[(size_t)k * 3 + i] >= 0) up[(size_t)k * c.dim + i] = pc->dofval[(size_t)k * c.dim + i];
+    if ((rc = force_from(pc, up, true, Fq.data(), err))) return fail(rc, err);
+    std::vector<double> b(n, 0.0), x(n, 0.0);
+    for (int k = 0; k < c.nn; k++)
+        for (int i = 0; i < c.dim; i++) {
+            int r = m.eq[(size_t)k * 3 + i];
+            if (r >= 0) {
+                b[(size_t)k * c.dim + i] = Fq[r];
+                if (o.warm_start) x[(size_t)k * c.dim + i] = u[r];
+            }
+        }
+    double *db, *dx;
+    if (hipMalloc((void **)&db, n * 8) != hipSuccess || hipMalloc((void **)&dx, n * 8) != hipSuccess)
+        return fail(PDX_ERR_HIP, "alloc");
+    hipMemcpyAsync(db, b.data(), n * 8, hipMemcpyHostToDevice, c.stream);
+    hipMemcpyAsync(dx, x.data(), n * 8, hipMemcpyHostToDevice, c.stream);
+    pdx_solve_stats s{};
+    rc = dev_solve(c, o, db, dx, s, err);
+    hipMemcpyAsync(x.data(), dx, n * 8, hipMemcpyDeviceToHost, c.stream);
+    hipStreamSynchronize(c.stream);
+    hipFree(db);
+    hipFree(dx);
+    for (int k = 0; k < c.nn; k++)
+        for (int i = 0; i < c.dim; i++) {
+            int r = m.eq[(size_t)k * 3 + i];
+            if (r >= 0) u[r] = x[(size_t)k * c.dim + i];
+        }
+    if (st) *st = s;
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
+}  // extern "C"

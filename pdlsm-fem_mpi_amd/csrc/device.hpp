@@ -1,0 +1,132 @@
+// Device context of the PDLSM-FEM hot path on one MI355X (gfx950).
+//
+// Layout in HBM
+//   * stiffness: node-block CSR ("BSR") over the node graph of the reference's
+//     pattern (every neighbour node of node k, prescribed DOFs included), with
+//     d x d blocks stored component-major: value (i,j) of block b lives at
+//     K[(i*d + j) * nblk + b].  Lanes that walk consecutive blocks of a row
+//     read consecutive addresses for every component.
+//   * vectors: full node-major DOF vectors of length nn*d; prescribed DOFs
+//     are masked where a free-DOF quantity is meant.
+//   * families: CSR of member node ids (self first) with the block index of
+//     every member in the family node's row (the scatter map: one store per
+//     block, no atomics).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "model.hpp"
+
+namespace pdx {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    const Model *host = nullptr;  // not owned; copied fields below
+    int dim = 3, dd = 9, nn = 0, ne = 0, neq = 0, nfam = 0, max_family = 0, npdbe = 0;
+    long long nblk = 0, nnz = 0;
+    double lambda = 0, mu = 0, nlf = 0.33, rho = 0, dt = 1, beta = 0.25, gamma = 0.5;
+
+    // geometry / model
+    double *x = nullptr, *nv = nullptr, *ev = nullptr;
+    double *Dm = nullptr;  // [36]
+    int *econ = nullptr, *nen = nullptr;
+    int *eq = nullptr, *presc_set = nullptr;  // [nn*3]
+    int *fam_node = nullptr, *fam_off = nullptr, *fam_mem = nullptr, *bond = nullptr, *fam_blk = nullptr;
+    double *fam_v = nullptr, *fam_delta = nullptr;
+    int *fp_off = nullptr, *fp_list = nullptr, *pdbe = nullptr;
+    int nfe = 0;
+    int *fe_elem = nullptr, *fe_blk = nullptr, *nfe_off = nullptr, *nfe_item = nullptr;
+    int *nall_off = nullptr, *nall_ele = nullptr;
+    int *adj_off = nullptr, *adj = nullptr;
+    int64_t *ia = nullptr;
+    int *row_node = nullptr;  // [neq] node*3+dof of each free row
+    // rows of the node graph binned by length for the SpMV (short: one group of 8 lanes, long: a wave)
+    int n_short = 0, n_long = 0;
+    int *rows_short = nullptr, *rows_long = nullptr;
+
+    // natural BCs
+    int n_nbc = 0, n_nbc_ele = 0;
+    int *nbc_con = nullptr, *nbc_set = nullptr;
+    double *nbc_fe = nullptr;     // unit-traction element forces [n_nbc_ele * 12]
+    int *nbcn_off = nullptr, *nbcn_item = nullptr;  // node -> (element*4+local)
+    double *f_unit = nullptr;     // [n_nbc * nn*d] unit-traction nodal forces
+    std::vector<double> nbc_val, nbc_vel;
+    std::vector<int> nbc_vary;
+    std::vector<double> ebc_val, ebc_vel;
+    std::vector<int> ebc_vary;
+    double *d_ebc_vel = nullptr;   // [n_ebc] per-step increment dt*vel (0 if constant)
+    double *d_nbc_vel = nullptr;   // [n_nbc]
+
+    // matrices
+    double *K = nullptr;   // [dd*nblk]
+    float *K32 = nullptr;  // [dd*nblk]
+    double *Mk = nullptr;  // consistent mass on K blocks (Newmark)
+    bool K_valid = false, K32_valid = false, Mk_valid = false;
+    double *mlump = nullptr;  // [nn*d] lumped mass (1 for prescribed)
+    bool mlump_valid = false;
+
+    // explicit state: three rotating full DOF vectors, two NBC value buffers
+    double *U[3] = {nullptr, nullptr, nullptr};
+    double *nbcv = nullptr;  // [2 * n_nbc]
+    int step_phase = 0;      // buffer rotation counter
+    long long steps_done = -1;  // -1: start-up step not yet taken
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
+    int graph_phase[2] = {-1, -1};
+
+    // scratch for Krylov
+    std::vector<DevBuf> scratch;
+    double *red = nullptr;      // partial sums [RED_SLOTS * RED_PARTS]
+    double *red_host = nullptr; // pinned mirror
+    double last_total_ms = 0, last_step_ms = 0;
+
+    // helpers
+    std::vector<void *> owned;
+};
+
+constexpr int RED_PARTS = 512;  // blocks of every reducing kernel
+constexpr int RED_SLOTS = 16;
+
+int ctx_build(const Model &m, int device, Ctx *&out, std::string &err);
+void ctx_free(Ctx *c);
+
+// assembly (assemble.hip)
+int dev_assemble_K(Ctx &c, std::string &err);
+int dev_lumped_mass(Ctx &c, std::string &err);
+int dev_nbc_unit(Ctx &c, std::string &err);
+int dev_mass_on_K(Ctx &c, std::string &err);
+int dev_extract_csr(Ctx &c, const double *Kblk, double *Ku_dev);
+int dev_spmv(Ctx &c, const double *x, double *y, int prec, bool mask_free);
+int dev_fill(Ctx &c, double *p, double v, size_t n);
+
+int dev_spmv_K(Ctx &c, const double *K64, const double *x, double *y);
+int set_error(int rc, const std::string &msg);
+
+}  // namespace pdx
+
+// handle types behind the opaque C-ABI pointers of include/pdx.h
+struct pdx_model {
+    pdx::Model m;
+};
+struct pdx_ctx {
+    pdx::Ctx *c = nullptr;
+    const pdx_model *model = nullptr;
+    std::vector<double> dofval;  // DOF values used by the assembly modes, [nn*dim]
+};
+
+#define PDX_HIP(call)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+            return PDX_ERR_HIP;                                                       \
+        }                                                                             \
+    } while (0)

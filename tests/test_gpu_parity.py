@@ -1,0 +1,173 @@
+"""GPU parity of the HIP hot path against the CPU oracle, through the C-ABI.
+
+Tolerances (FP64 unless stated):
+  assembled K        max over CSR rows of ||row_gpu - row_oracle|| / ||row_oracle|| <= 1e-12
+  assembled F        ||F_gpu - F_oracle|| / ||F_oracle|| <= 1e-12
+  masses             <= 1e-14 relative
+  explicit FP64      trajectory ||u_gpu - u_oracle|| / ||u_oracle|| <= 1e-10 after 60 steps
+  explicit FP32 force (FP64 state): <= 1e-5 after 60 steps
+  solves             true relative residual ||b - K x|| / ||b|| <= 1e-8, checked on the CPU
+                     with the oracle's own K and b
+  box example        ||U_gpu - U_vtk|| / ||U_vtk|| <= 1e-6 (the VTK stores float32)
+"""
+import numpy as np
+import pytest
+
+from helpers import box_fixture, load_oracle, load_pdx, rel, row_block_rel, workloads
+
+pytestmark = pytest.mark.gpu
+
+O = load_oracle()
+W = workloads()
+pdx = load_pdx()
+
+
+def _pair(mesh):
+    mo = O.Model(mesh)
+    mp = pdx.Model(mesh)
+    cx = pdx.Context(mp)
+    return mo, mp, cx
+
+
+CASES = {
+    "plate2d": lambda: W.plate2d(),
+    "crack2d": lambda: W.crack2d(seed=5, nx=121, ny=61, dx=1e-3),
+    "box3d": lambda: W.box3d(n=16, pd_block=8),
+    "penny3d": lambda: W.penny3d(n=20, pd_block=10),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_static_assembly_matches_oracle(name):
+    mesh = CASES[name]()
+    mo, mp, cx = _pair(mesh)
+    K0, F0 = mo.assemble(1)
+    K1, F1 = cx.assemble(pdx.PDX_MODE_STATIC)
+    ia, _ = mo.csr()
+    assert row_block_rel(K1, K0, ia) <= 1e-12
+    assert rel(F1, F0) <= 1e-12
+
+
+def test_assembly_modes_match_oracle():
+    """explicit (F = F_ext - K u), quasi-static and Newmark (U_N) branches."""
+    mesh = W.box3d(n=14, pd_block=6)
+    mesh["ebc"][1]["vary"], mesh["ebc"][1]["vel"] = True, 0.3
+    mo, mp, cx = _pair(mesh)
+    rng = np.random.default_rng(0)
+    mo.store_u(rng.normal(size=mo.neq) * 1e-6)
+    vals = mo.values()
+    cx.set_dof_values(vals)
+    _, F0 = mo.assemble(0)
+    _, F1 = cx.assemble(pdx.PDX_MODE_EXPLICIT)
+    assert rel(F1, F0) <= 1e-12
+    K0, F0 = mo.assemble(2)
+    K1, F1 = cx.assemble(pdx.PDX_MODE_QUASI)
+    ia, _ = mo.csr()
+    assert row_block_rel(K1, K0, ia) <= 1e-12 and rel(F1, F0) <= 1e-12
+    UN = rng.normal(size=mp.nn * 3) * 1e-6
+    K0, F0 = mo.assemble(0, newmark=True, UN=UN)
+    K1, F1 = cx.assemble(pdx.PDX_MODE_NEWMARK, UN=UN)
+    assert row_block_rel(K1, K0, ia) <= 1e-12 and rel(F1, F0) <= 1e-12
+
+
+def test_masses_match_oracle():
+    mesh = W.box3d(n=12, pd_block=6)
+    mo, mp, cx = _pair(mesh)
+    assert rel(cx.lumped_mass(), mo.lumped_mass()) <= 1e-14
+    assert rel(cx.mass_on_K(), mo.mass_on_K()) <= 1e-14
+
+
+def test_assembly_is_deterministic():
+    mesh = W.box3d(n=14, pd_block=8)
+    mp = pdx.Model(mesh)
+    cx = pdx.Context(mp)
+    K1, F1 = cx.assemble(pdx.PDX_MODE_STATIC)
+    K2, F2 = cx.assemble(pdx.PDX_MODE_STATIC)
+    assert np.array_equal(K1, K2) and np.array_equal(F1, F2)
+
+
+def _explicit_case():
+    mesh = W.box3d(n=16, pd_block=8, traction=1e6)
+    mesh["ebc"].append(dict(nid=np.array([16 * 16 * 15 + 1], np.int32), dof=0, val=0.0, vel=0.05, vary=True))
+    mesh["nbc"][0].update(vel=1e9, vary=True)
+    return mesh
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-5)])
+def test_explicit_trajectory_matches_oracle(prec, tol):
+    mesh = _explicit_case()
+    mp = pdx.Model(mesh)
+    cx = pdx.Context(mp)
+    lmax = cx.max_eig(200)
+    dt = 0.8 * 2.0 / np.sqrt(lmax)
+    mesh["dt"] = dt
+    mo = O.Model(mesh)
+    u0, up0 = mo.explicit_run(60)
+    mp = pdx.Model(mesh)
+    cx = pdx.Context(mp)
+    u1, up1 = cx.explicit_run(60, precision=prec)
+    assert np.abs(u0).max() > 0
+    assert rel(u1, u0) <= tol and rel(up1, up0) <= tol
+
+
+def test_explicit_graph_replay_matches_eager():
+    mesh = _explicit_case()
+    mesh["dt"] = 2e-8
+    a = pdx.Context(pdx.Model(mesh))
+    b = pdx.Context(pdx.Model(mesh))
+    a.explicit_steps(1)
+    b.explicit_steps(1)
+    for _ in range(3):
+        a.explicit_steps(12, use_graph=True)
+        b.explicit_steps(12, use_graph=False)
+    ua, _ = a.explicit_run(0)
+    ub, _ = b.explicit_run(0)
+    assert np.array_equal(ua, ub)
+
+
+def _true_relres(mo, K, F, u):
+    A = mo.sparse(K)
+    return np.linalg.norm(F - A @ u) / np.linalg.norm(F)
+
+
+@pytest.mark.parametrize("precision,precond", [(0, 1), (0, 2), (1, 1), (1, 2)])
+def test_static_solve_plate2d(precision, precond):
+    mesh = W.plate2d()
+    mo, mp, cx = _pair(mesh)
+    K, F = mo.assemble(1)
+    u, st = cx.static_solve(pdx.solve_opts(precision=precision, precond=precond, tol=1e-8))
+    assert st["converged"] and st["true_relres"] <= 1e-8
+    assert _true_relres(mo, K, F, u) <= 1.2e-8
+
+
+def test_box_example_matches_reference_vtk():
+    mesh, U = box_fixture()
+    mp = pdx.Model(mesh)
+    cx = pdx.Context(mp)
+    u, st = cx.static_solve(pdx.solve_opts(precision=1, precond=2, tol=1e-10, max_iter=50000))
+    eq = mp.nodes()["eq"]
+    Ug = np.zeros(mp.nn * 3)
+    free = eq.ravel() >= 0
+    Ug[free] = u[eq.ravel()[free]]
+    assert rel(Ug, U.astype(np.float64).ravel()) <= 1e-6, rel(Ug, U.ravel())
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_csr_seam_solves_oracle_system(precision):
+    mo = O.Model(W.plate2d())
+    K, F = mo.assemble(1)
+    ia, ja = mo.csr()
+    x, st = pdx.solve_csr(ia, ja, K, F, pdx.solve_opts(precision=precision, tol=1e-8))
+    assert st["converged"]
+    assert _true_relres(mo, K, F, x) <= 1.2e-8
+
+
+def test_newmark_matches_oracle():
+    mesh = W.plate2d(nx=41, ny=21, dx=1e-3, pd_x=(0.015, 0.025))
+    mesh["ebc"][2].update(val=0.0, vel=1e-2, vary=True)
+    mesh["dt"] = 1e-6
+    mo = O.Model(mesh)
+    u0, v0, a0 = O.newmark_run(mo, 4)
+    cx = pdx.Context(pdx.Model(mesh))
+    u1, v1, a1, st = cx.newmark_run(4, pdx.solve_opts(precision=0, precond=2, tol=1e-12))
+    assert rel(u1, u0) <= 1e-8 and rel(v1, v0) <= 1e-7 and rel(a1, a0) <= 1e-6

@@ -1,0 +1,389 @@
+"""Pins of the CPU oracle (oracle/pdlsm_oracle.c) -- what fixes each function.
+
+  a1-a5 inflFunc / shapTens / vec_gd / matG / matH  -> Navier operator of a
+        quadratic field on jittered 2D and 3D families (closed form), and the
+        reference's golden static result examples/box (end to end)
+  a6    matC                       -> exact strain of a quadratic field
+  a7    PD assembly (-H cd_v, free/fixed split)  -> box golden; pattern counts
+  a8    PD-boundary terms 2D/3D    -> traction sigma.n closed form on the PD
+        region boundary; box golden (3D); a planted drop is detected
+  a9    CSR pattern                -> plate2d N_eq / nnz measured with the
+        reference binary (40,299 / 1,043,589); box N_eq 102,500
+  a12   element stiffness          -> textbook quad4 K11, strain energy of a
+        constant-strain field (quad4, tri3, hex8, tet4), rigid-body null space
+  a14   natural-BC force           -> total force = t * area
+  a15   lumped / consistent mass   -> totals = rho * V
+  a17   explicit driver            -> closed form of the central-difference
+        recurrence of a one-DOF model, with and without a ramped EBC (pins the
+        start-up step and the advance-before-force order)
+  a19   Newmark driver             -> energy conservation of the average
+        acceleration rule on a one-DOF model from rest (pins beta, gamma, the
+        predictor and the update)
+"""
+import numpy as np
+import pytest
+
+from helpers import box_fixture, load_oracle, one_quad, workloads
+
+O = load_oracle()
+W = workloads()
+
+
+# ---------------------------------------------------------------------------
+# golden: the reference's examples/box static result
+# ---------------------------------------------------------------------------
+
+def _box_correction(skip=0):
+    import scipy.sparse.linalg as spla
+    mesh, U = box_fixture()
+    M = O.Model(mesh)
+    O.set_skip(skip)
+    try:
+        K, F = M.assemble(1)
+    finally:
+        O.set_skip(0)
+    eq = M.nodes()["eq"]
+    u = np.zeros(M.neq)
+    u[eq[eq >= 0]] = U.astype(np.float64)[eq >= 0]
+    A = M.sparse(K)
+    r = F - A @ u
+    d = A.diagonal()
+    P = spla.LinearOperator(A.shape, matvec=lambda v: v / d)
+    delta, _ = spla.bicgstab(A, r, M=P, rtol=1e-3, maxiter=400)
+    return M, np.linalg.norm(delta) / np.linalg.norm(u), np.linalg.norm(r) / np.linalg.norm(F)
+
+
+@pytest.mark.slow
+def test_box_golden_static_solution():
+    """The oracle's exact static solution equals the reference's committed
+    boxVpd4_static.vtk U to float32 rounding (correction ||delta||/||u||)."""
+    M, corr, res = _box_correction()
+    assert M.neq == 102500
+    assert corr < 2e-7, corr
+    assert res < 1e-4, res
+
+
+@pytest.mark.slow
+def test_box_golden_detects_dropped_boundary_terms():
+    """Planted mistake in the oracle: without the PD-boundary terms the VTK
+    solution is far from the oracle's."""
+    _, corr, _ = _box_correction(skip=1)
+    assert corr > 1e-3, corr
+
+
+def test_plate2d_pattern_counts_match_reference_binary():
+    M = O.Model(W.plate2d())
+    assert (M.neq, M.nnz, M.nfam) == (40299, 1043589, 4141)
+    ia, ja = M.csr()
+    for r in range(0, M.neq, 997):
+        row = ja[ia[r]: ia[r + 1]]
+        assert np.all(np.diff(row) > 0)
+
+
+# ---------------------------------------------------------------------------
+# PDLSM family: closed forms
+# ---------------------------------------------------------------------------
+
+def _quad_field(dim, seed):
+    rng = np.random.default_rng(seed)
+    c0 = rng.normal(size=dim)
+    B = rng.normal(size=(dim, dim))
+    Q = rng.normal(size=(dim, dim, dim))
+    Q = 0.5 * (Q + Q.transpose(0, 2, 1))  # u_i = c_i + B_ij x_j + Q_ijk x_j x_k
+
+    def u(x):
+        x = np.asarray(x)[:, :dim]
+        return c0 + x @ B.T + np.einsum("ijk,nj,nk->ni", Q, x, x)
+
+    def grad(x):
+        x = np.asarray(x)[:dim]
+        return B + 2 * np.einsum("ijk,k->ij", Q, x)
+
+    def navier(lam, mu):
+        # L(u)_i = (lam+mu) d_i(div u) + mu lap u_i ; second derivatives d_jk u_i = 2 Q_ijk
+        H = 2 * Q
+        div_grad = np.einsum("jji->i", H)
+        lap = np.einsum("ijj->i", H)
+        return (lam + mu) * div_grad + mu * lap
+
+    return u, grad, navier
+
+
+def _interior_families(M, dim, n):
+    fam = M.families()
+    x = M.mesh["x"]
+    lo, hi = x.min(0), x.max(0)
+    out = []
+    for f in range(M.nfam):
+        k = fam["node"][f]
+        d = fam["delta"][f]
+        if np.all(x[k, :dim] - 1.2 * d > lo[:dim]) and np.all(x[k, :dim] + 1.2 * d < hi[:dim]):
+            out.append(f)
+        if len(out) >= n:
+            break
+    return fam, out
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_family_H_reproduces_navier_operator(dim):
+    """sum_m G_m V_m (u_m - u_k) = (lam+mu) grad div u + mu lap u for any
+    quadratic u: pins inflFunc, the shape tensor with its 1/2 factors, the per-bond
+    solve and matG/matH."""
+    if dim == 2:
+        mesh = W.plate2d(nx=41, ny=41, pd_x=(0.0, 1.0))
+        mesh["x"][:, :2] += np.random.default_rng(1).uniform(-0.15, 0.15, (len(mesh["x"]), 2)) * 5e-4
+    else:
+        mesh = W.box3d(n=14, pd_block=14, jitter=0.2)
+    M = O.Model(mesh)
+    g = M.geom()
+    fam, fams = _interior_families(M, dim, 6)
+    assert fams
+    u, _, navier = _quad_field(dim, 7)
+    L = navier(g["lam"], g["mu"])
+    x = mesh["x"]
+    for f in fams:
+        mem = fam["mem"][fam["off"][f]: fam["off"][f + 1]] - 1
+        H = M.family_H(f)
+        val = H @ u(x[mem]).ravel()
+        assert np.allclose(val, L, rtol=1e-6, atol=1e-6 * np.abs(L).max()), (val, L)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_family_C_gives_exact_strain(dim):
+    if dim == 2:
+        mesh = W.plate2d(nx=31, ny=31, pd_x=(0.0, 1.0))
+        mesh["x"][:, :2] += np.random.default_rng(2).uniform(-0.15, 0.15, (len(mesh["x"]), 2)) * 5e-4
+    else:
+        mesh = W.box3d(n=12, pd_block=12, jitter=0.2, seed=3)
+    M = O.Model(mesh)
+    fam, fams = _interior_families(M, dim, 4)
+    u, grad, _ = _quad_field(dim, 11)
+    x = mesh["x"]
+    for f in fams:
+        mem = fam["mem"][fam["off"][f]: fam["off"][f + 1]] - 1
+        k = mem[0]
+        eps = M.family_C(f) @ u(x[mem]).ravel()
+        G = grad(x[k])
+        if dim == 2:
+            ex = np.array([G[0, 0], G[1, 1], G[0, 1] + G[1, 0]])
+        else:
+            ex = np.array([G[0, 0], G[1, 1], G[2, 2], G[0, 1] + G[1, 0], G[1, 2] + G[2, 1], G[0, 2] + G[2, 0]])
+        assert np.allclose(eps, ex, rtol=1e-7, atol=1e-7 * np.abs(ex).max())
+
+
+def _linear_u(M, dim, B):
+    x = M.mesh["x"]
+    return (x[:, :dim] @ B.T).ravel()
+
+
+def test_pdbe_2d_traction_closed_form():
+    """PD-boundary terms alone, linear u (constant stress sigma): at a node on
+    the straight left edge of the PD strip the rows receive -dx * sigma.n with
+    the outward normal -x, i.e. F = dx * (sxx, sxy)."""
+    dx = 5e-4
+    mesh = W.plate2d(nx=41, ny=21, dx=dx, pd_x=(0.005, 0.015))
+    mesh["ebc"] = []
+    M = O.Model(mesh)
+    B = np.array([[1e-3, 2e-4], [-5e-4, 3e-4]])
+    M.store_u(_linear_u(M, 2, B))
+    O.set_skip(2 | 4)
+    try:
+        _, F = M.assemble(2, with_ext=False)
+    finally:
+        O.set_skip(0)
+    g = M.geom()
+    lam, mu = g["lam"], g["mu"]
+    eps = np.array([B[0, 0], B[1, 1], B[0, 1] + B[1, 0]])
+    E, nu = mesh["E"], mesh["nu"]
+    D = E / (1 - nu ** 2) * np.array([[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]])
+    s = D @ eps
+    eq = M.nodes()["eq"]
+    i0 = 10  # node column x = 10 dx = 0.005: left edge of the PD strip
+    for j in range(3, 18, 4):
+        k = j * 41 + i0
+        got = F[eq[k, :2]]
+        assert np.allclose(got, dx * np.array([s[0], s[2]]), rtol=1e-9), (got, dx * s)
+
+
+def test_pdbe_3d_traction_closed_form():
+    dx = 1e-3
+    mesh = W.box3d(n=14, dx=dx, jitter=0.0, pd_block=8)
+    mesh["ebc"] = []
+    M = O.Model(mesh)
+    B = np.array([[1e-3, 2e-4, -1e-4], [-5e-4, 3e-4, 2e-4], [1e-4, -2e-4, 4e-4]])
+    M.store_u(_linear_u(M, 3, B))
+    O.set_skip(2 | 4)
+    try:
+        _, F = M.assemble(2, with_ext=False)
+    finally:
+        O.set_skip(0)
+    E, nu = mesh["E"], mesh["nu"]
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    eps = 0.5 * (B + B.T)
+    sig = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+    eq = M.nodes()["eq"]
+    lo = (14 - 8) // 2  # PD node block [3, 10]; face x = 3 dx has outward normal -x
+    k = 6 * 14 * 14 + 6 * 14 + lo
+    got = F[eq[k]]
+    # F = -sum_e w A_e sigma.n over the four faces around the node, n = -x, sum w A = dx^2
+    assert np.allclose(got, dx * dx * sig[:, 0], rtol=1e-9), (got, dx * dx * sig[:, 0])
+
+
+# ---------------------------------------------------------------------------
+# elements
+# ---------------------------------------------------------------------------
+
+def _single(dim, nen, x, E=2.0e3, nu=0.3):
+    econ = np.zeros((1, 8), np.int32)
+    econ[0, :nen] = np.arange(1, nen + 1)
+    return O.Model(dict(dim=dim, proType=1 if dim == 2 else -1, E=E, nu=nu, rho=3.0, x=x, econ=econ,
+                        nen=np.array([nen], np.int32), algo=np.array([2], np.int32), ebc=[], nbc=[],
+                        cracks=np.zeros((0, 3, 3)), fac=3.0, nlf=0.33))
+
+
+def _D(dim, E, nu):
+    if dim == 2:
+        return E / (1 - nu ** 2) * np.array([[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]])
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    D[np.arange(3), np.arange(3)] += 2 * mu
+    D[3:, 3:] = mu * np.eye(3)
+    return D
+
+
+def test_quad4_textbook_diagonal():
+    x = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], float)
+    E, nu = 2.0e3, 0.3
+    K = _single(2, 4, x, E, nu).element_K(0)
+    assert np.isclose(K[0, 0], E / (1 - nu ** 2) * (0.5 - nu / 6), rtol=1e-13)
+    assert np.allclose(K, K.T, rtol=1e-13, atol=1e-12 * abs(K).max())
+
+
+@pytest.mark.parametrize("kind", ["quad4", "tri3", "hex8", "tet4"])
+def test_element_constant_strain_energy_and_rigid_modes(kind):
+    rng = np.random.default_rng(5)
+    if kind == "quad4":
+        dim, x = 2, np.array([[0, 0, 0], [1.1, 0.1, 0], [1.0, 0.9, 0], [-0.1, 1.2, 0]], float)
+    elif kind == "tri3":
+        dim, x = 2, np.array([[0, 0, 0], [1.2, 0.1, 0], [0.3, 0.9, 0]], float)
+    elif kind == "hex8":
+        dim = 3
+        x = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+        x += rng.uniform(-0.1, 0.1, x.shape)
+    else:
+        dim, x = 3, np.array([[0, 0, 0], [1, 0.1, 0], [0.2, 1, 0.1], [0.1, 0.2, 1.1]], float)
+    nen = len(x)
+    E, nu = 2.0e3, 0.3
+    M = _single(dim, nen, x, E, nu)
+    K = M.element_K(0)
+    V = M.nodes()["ev"][0]
+    B = rng.normal(size=(dim, dim))
+    u = (x[:, :dim] @ B.T).ravel()
+    eps = 0.5 * (B + B.T)
+    ev = np.array([eps[0, 0], eps[1, 1], 2 * eps[0, 1]]) if dim == 2 else \
+        np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[0, 1], 2 * eps[1, 2], 2 * eps[0, 2]])
+    assert np.isclose(u @ K @ u, ev @ _D(dim, E, nu) @ ev * V, rtol=1e-11)
+    W_ = np.zeros((dim, dim))
+    W_[0, 1], W_[1, 0] = 1.0, -1.0  # rigid rotation + translation
+    r = (x[:, :dim] @ W_.T + 0.7).ravel()
+    assert np.abs(K @ r).max() < 1e-10 * np.abs(K).max()
+    Me = M.element_M(0)
+    assert np.isclose(Me.sum(), dim * 3.0 * V, rtol=1e-12)
+
+
+def test_natural_bc_total_force_and_lumped_mass():
+    mesh = W.box3d(n=8, dx=1e-3, jitter=0.2, pd_block=4, traction=2.5e6)
+    mesh["ebc"] = []
+    M = O.Model(mesh)
+    M.store_u(np.zeros(M.nn * 3))
+    O.set_skip(1 | 2 | 4)
+    try:
+        _, F = M.assemble(2, with_ext=True)
+    finally:
+        O.set_skip(0)
+    L = 7e-3
+    Fz = F.reshape(-1, 3)
+    assert np.isclose(Fz[:, 2].sum(), 2.5e6 * L * L, rtol=1e-12)
+    assert abs(Fz[:, 0].sum()) < 1e-9 * 2.5e6 * L * L
+    m = M.lumped_mass()
+    assert np.isclose(m.sum(), 3 * mesh["rho"] * M.nodes()["ev"].sum(), rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# drivers
+# ---------------------------------------------------------------------------
+
+def _one_dof_terms(mesh):
+    M = O.Model(mesh)
+    K, F = M.assemble(1)
+    m = M.lumped_mass()
+    return M, K, F, m
+
+
+def test_explicit_one_dof_closed_form():
+    """u_n = (f/k)(1 - cos n theta), 2 - 2 cos theta = dt^2 k/m, for the
+    reference's start-up step u1 = u0 + dt^2 a0 / 2 and the recurrence."""
+    mesh = one_quad()
+    M, K, F, m = _one_dof_terms(mesh)
+    assert M.neq == 1
+    k, f, mm, dt = K[0], F[0], m[0], mesh["dt"]
+    th = np.arccos(1 - dt * dt * k / (2 * mm))
+    for n in (0, 1, 5, 40):
+        u, _ = O.Model(mesh).explicit_run(n)
+        want = f / k * (1 - np.cos((n + 1) * th))
+        assert np.isclose(u[0], want, rtol=1e-9, atol=1e-15), (n, u[0], want)
+
+
+def test_explicit_ramped_bc_order():
+    """Ramped EBC on a coupled DOF: the force of loop step n uses the BC value
+    advanced n+1 times (advance before force).  Exact discrete solution of
+    u_{n+1} - 2u_n + u_{n-1} = dt^2/m (f - k u_n - c v_{n+1})."""
+    mesh = one_quad(vebc=True)
+    M0 = O.Model(one_quad(vebc=False))
+    K, F = M0.assemble(1)
+    k, f = K[0], F[0]
+    mm = M0.lumped_mass()[0]
+    # coupling c = K(u3x, u2x): force per unit prescribed value
+    Mq = O.Model(dict(one_quad(), ebc=one_quad()["ebc"][:2]))
+    Kq, _ = Mq.assemble(1)
+    Sq = Mq.sparse(Kq).toarray()
+    eqq = Mq.nodes()["eq"]
+    c = Sq[eqq[2, 0], eqq[1, 0]]
+    dt, vel = mesh["dt"], 1.0
+    v = lambda j: j * dt * vel
+    # reference recurrence, computed independently of the oracle driver
+    u_prev, u = 0.0, 0.0
+    a0 = (f - c * v(1)) / mm
+    u_next = u + 0.5 * dt * dt * a0
+    hist = [u_next]
+    for n in range(1, 30):
+        u_prev, u = u, u_next
+        u_next = 2 * u + dt * dt * (f - k * u - c * v(n + 1)) / mm - u_prev
+        hist.append(u_next)
+    got, _ = O.Model(mesh).explicit_run(29)
+    assert np.isclose(got[0], hist[-1], rtol=1e-9)
+    # the off-by-one order (force with v_n) gives a measurably different answer
+    u_prev, u = 0.0, 0.0
+    u_next = 0.5 * dt * dt * (f - c * v(0)) / mm
+    for n in range(1, 30):
+        u_prev, u = u, u_next
+        u_next = 2 * u + dt * dt * (f - k * u - c * v(n)) / mm - u_prev
+    assert abs(got[0] - u_next) > 1e-3 * abs(got[0])
+
+
+def test_newmark_average_acceleration_conserves_energy():
+    """From rest (a0 = 0 as in the reference), after the first step
+    E = m v^2/2 + k (u - f/k)^2/2 is constant for beta = 1/4, gamma = 1/2."""
+    mesh = one_quad(dt=2e-4)
+    M = O.Model(mesh)
+    K, F = M.assemble(1)
+    k, f, mm = K[0], F[0], M.mass_on_K()[0]
+    Es = []
+    for n in (1, 2, 7, 20):
+        u, v, a = O.newmark_run(O.Model(mesh), n)
+        Es.append(0.5 * mm * v[0] ** 2 + 0.5 * k * (u[0] - f / k) ** 2)
+        assert np.isclose(mm * a[0] + k * u[0], f, rtol=1e-9)
+    assert np.allclose(Es, Es[0], rtol=1e-9)

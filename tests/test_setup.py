@@ -1,0 +1,155 @@
+"""Host side of the library (no GPU): the set-up that feeds the kernels
+(DOF numbering, volumes, PD/FE split, boundary entities, families, bond
+states, CSR pattern) against the oracle, the .lis/.in reader, error codes, and
+the exported C-ABI."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import ROOT, box_fixture, load_oracle, load_pdx, workloads
+
+O = load_oracle()
+W = workloads()
+pdx = load_pdx()
+
+
+def _compare(mesh):
+    mo, mp = O.Model(mesh), pdx.Model(mesh)
+    d = mo.dim
+    ia0, ja0 = mo.csr()
+    ia1, ja1 = mp.csr()
+    assert (mo.neq, mo.nnz, mo.nfam) == (mp.neq, mp.nnz, mp.nfam)
+    assert np.array_equal(ia0, ia1) and np.array_equal(ja0, ja1)
+    fo, fp = mo.families(), mp.families()
+    for k in ("node", "off", "mem", "bond"):
+        assert np.array_equal(fo[k], fp[k]), k
+    for k in ("cd_v", "delta"):
+        assert np.array_equal(fo[k], fp[k]), k  # bit-exact
+    no, npd = mo.nodes(), mp.nodes()
+    assert np.array_equal(no["eq"][:, :d], npd["eq"][:, :d])
+    for k in ("ntype", "nv", "ev", "algo"):
+        assert np.array_equal(no[k], npd[k]), k
+    w = 2 if d == 2 else 4
+    assert np.array_equal(no["pdbe"][:, :w], npd["pdbe"][:, :w])
+    return mo, mp
+
+
+@pytest.mark.parametrize("name", ["plate2d", "crack2d", "box3d", "penny3d", "tri_tet"])
+def test_setup_matches_oracle(name):
+    if name == "plate2d":
+        mesh = W.plate2d()
+    elif name == "crack2d":
+        mesh = W.crack2d(seed=5, nx=121, ny=61, dx=1e-3)
+    elif name == "box3d":
+        mesh = W.box3d(n=18, pd_block=8)
+    elif name == "penny3d":
+        mesh = W.penny3d(n=22, pd_block=10)
+    else:
+        mesh = tri_tet_mesh()
+    mo, mp = _compare(mesh)
+    if name == "crack2d":
+        assert (mo.families()["bond"] == 0).sum() > 0
+    if name == "penny3d":
+        assert (mp.families()["bond"] == 0).sum() > 0
+
+
+def tri_tet_mesh():
+    """Mixed 2D mesh: quads in the left half, each right-half quad split into two triangles."""
+    m = W.plate2d(nx=31, ny=17, dx=1e-3, pd_x=(0.008, 0.02))
+    econ, nen, algo = [], [], []
+    for e in range(len(m["econ"])):
+        c = m["econ"][e]
+        cx = m["x"][c[:4] - 1, 0].mean()
+        if cx > 0.015:
+            for tri in ([c[0], c[1], c[2]], [c[0], c[2], c[3]]):
+                row = np.zeros(8, np.int32)
+                row[:3] = tri
+                econ.append(row); nen.append(3); algo.append(m["algo"][e])
+        else:
+            econ.append(c); nen.append(4); algo.append(m["algo"][e])
+    m.update(econ=np.array(econ), nen=np.array(nen, np.int32), algo=np.array(algo, np.int32))
+    return m
+
+
+def test_box_example_setup_matches_oracle():
+    mesh, _ = box_fixture()
+    mo, mp = _compare(mesh)
+    assert mp.neq == 102500
+
+
+def write_lis(path, mesh, title="t"):
+    """Write a mesh dict in the reference .lis layout (datModel.cpp:58-326)."""
+    dim = mesh["dim"]
+    with open(path, "w") as f:
+        f.write(f"{title}\n***generated***\n{dim}D\t{mesh['proType']}\n")
+        f.write(f"{mesh['E']:.17g}\t{mesh['nu']:.17g}\t{mesh['rho']:.17g}\t1e6\t1e6\n")
+        f.write(f"{len(mesh['x'])}\t{len(mesh['econ'])}\n")
+        for i, p in enumerate(mesh["x"]):
+            f.write(f"{i + 1}\t{p[0]:.17g}\t{p[1]:.17g}\t{p[2]:.17g}\n")
+        for e, c in enumerate(mesh["econ"]):
+            if dim == 2:
+                cc = list(c[:4]) if mesh["nen"][e] == 4 else [c[0], c[1], c[2], c[2]]
+            else:
+                cc = list(c[:8]) if mesh["nen"][e] == 8 else [c[0], c[1], c[2], c[2], c[3], c[3], c[3], c[3]]
+            f.write(f"{e + 1}\t{mesh['algo'][e]}\t" + " ".join(str(int(v)) for v in cc) + "\n")
+        f.write("=====Essential BCs=====\n")
+        f.write(f"{len(mesh['ebc'])}\n")
+        for s in mesh["ebc"]:
+            f.write(f"{len(s['nid'])}\t{['UX', 'UY', 'UZ'][s['dof']]}\t{s['val']:.17g}\n")
+        for s in mesh["ebc"]:
+            f.write(" ".join(str(int(v)) for v in s["nid"]) + "\n")
+        f.write("=====Natural BCs=====\n")
+        f.write(f"{len(mesh['nbc'])}\n")
+        for s in mesh["nbc"]:
+            f.write(f"{len(s['con'])}\t{s['val']:.17g}\n")
+        for s in mesh["nbc"]:
+            for c in s["con"]:
+                f.write(" ".join(str(int(v)) for v in c) + "\n")
+        f.write("=====NO FAIL region=====\n0\n=====pre-exist crack=====\n")
+        cr = mesh.get("cracks", np.zeros((0, 3, 3)))
+        f.write(f"{len(cr)}\n")
+        for c in cr:
+            for j in range(3 if dim == 3 else 2):
+                f.write(" ".join(f"{v:.17g}" for v in c[j]) + "\n")
+
+
+@pytest.mark.parametrize("name", ["box3d", "crack2d"])
+def test_lis_reader_matches_arrays(tmp_path, name):
+    mesh = W.box3d(n=10, pd_block=4) if name == "box3d" else W.crack2d(seed=2, nx=61, ny=31, dx=1e-3)
+    write_lis(tmp_path / "m.lis", mesh)
+    (tmp_path / "c.in").write_text(f"MSHFILE m.lis\nSOLVER STATIC\nSETSOLVING 1 3 1 {mesh['fac']!r} {mesh['nlf']!r}\n")
+    a, b = pdx.Model(mesh), pdx.Model(cmd_file=str(tmp_path / "c.in"))
+    assert b.solver_flag == 1 and b.num_steps == 3
+    assert np.array_equal(a.csr()[1], b.csr()[1])
+    fa, fb = a.families(), b.families()
+    assert np.array_equal(fa["mem"], fb["mem"]) and np.array_equal(fa["bond"], fb["bond"])
+    # the oracle's Python reader agrees too
+    mo = O.Model(O.read_case(str(tmp_path / "c.in")))
+    assert np.array_equal(mo.csr()[1], b.csr()[1])
+
+
+def test_error_codes():
+    mesh = W.box3d(n=6, pd_block=2)
+    mesh["ebc"] = mesh["ebc"] + [dict(nid=mesh["ebc"][0]["nid"][:1], dof=2, val=0.0)]
+    with pytest.raises(pdx.PdxError, match="repeatly defined"):
+        pdx.Model(mesh)
+    mesh = W.box3d(n=6, pd_block=2)
+    mesh["econ"] = mesh["econ"].copy()
+    mesh["econ"][0, 0] = 10 ** 6
+    with pytest.raises(pdx.PdxError, match="out of range"):
+        pdx.Model(mesh)
+    with pytest.raises(pdx.PdxError, match="is not exist"):
+        pdx.Model(cmd_file="/nonexistent/x.in")
+
+
+def test_c_abi_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "pdx.h")).read()
+    names = sorted(set(re.findall(r"\b(pdx_[a-z0-9_]+)\s*\(", hdr)))
+    assert len(names) > 20
+    L = ctypes.CDLL(pdx.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert pdx.lib().pdx_version().decode().startswith("pdx")
