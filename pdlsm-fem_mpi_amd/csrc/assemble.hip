@@ -7,8 +7,9 @@
 //             PD-region boundary traction terms N.D.C of the same family, all
 //             written to the family node's block row through the scatter map
 //             (plain stores: each family owns its row).
-//  k_fe_rows  one thread per (node, dof) row: element stiffness rows of every
-//             FE element around the node, added in element order.
+//  k_ke_*     element stiffness matrices (B per element and Gauss point in LDS)
+//             into a staging array; k_fe_gather adds them node-centrically in
+//             element order.
 //  k_lumped   lumped mass rho * sum V_e / n_e, node-centric.
 //  k_nbc_*    equivalent nodal forces of natural BCs for a unit traction.
 //  k_spmv_*   node-block SpMV, short rows on 8-lane groups, long rows on a wave.
@@ -19,6 +20,7 @@
 #include <cmath>
 
 #include "device.hpp"
+#include "rowblock.hpp"
 
 namespace pdx {
 
@@ -314,7 +316,7 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
         }
         const size_t blk = (size_t)A.fam_blk[off + m];
 #pragma unroll
-        for (int e = 0; e < DD; e++) A.K[e * nb + blk] = val[e];
+        for (int e = 0; e < DD; e++) A.K[blk * DD + e] = val[e];
     }
     __syncthreads();
     // self block: sequential sums over members in reference order
@@ -336,7 +338,7 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
             c_block<D>(cs, Cb);
             for (int q = p0; q < p1; q++) add_pdbe_term<D>(A, A.fp_list[q], Cb, val);
         }
-        A.K[e * nb + (size_t)A.fam_blk[off]] = val[e];
+        A.K[(size_t)A.fam_blk[off] * DD + e] = val[e];
     }
 }
 
@@ -345,14 +347,6 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
 // pdfemEleBrick8N.cpp:324-552, pdfemEleTetra4N.cpp:38-170), node-centric
 // ------------------------------------------------------------------------
 
-struct FeArgs {
-    const double *x, *Dm;
-    const int *econ, *nen, *fe_elem, *fe_blk, *nfe_off, *nfe_item;
-    double *K;
-    long long nblk;
-    int nn;
-    int *status;
-};
 
 __device__ inline double det3(const double J[9]) {
     // cofactor form of pdfemEles::detMat33 (pdfemEles.cpp:155-172)
@@ -459,121 +453,188 @@ __device__ inline void btdb_row2(const double (*g)[2], int nnod, int la, int ii,
     }
 }
 
-template <int D>
-__global__ void __launch_bounds__(128) k_fe_rows(FeArgs A) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.nn * D) return;
-    const int a = t / D, ii = t % D;
+// Element stiffness of every FE element into a staging array S (one Ke per
+// element, row stride ND = 24 in 3D / 8 in 2D).  Hex8 / quad4: a workgroup
+// takes EPB elements; first one thread per (element, Gauss point) builds the
+// Jacobian, its partial-pivot LU and dN/dX of all nodes into LDS, then one
+// thread per (element, Ke row) accumulates B^T D B w detJ over the Gauss
+// points in the reference loop order.
+struct KeArgs {
+    const double *x, *Dm;
+    const int *econ, *nen, *fe_elem;
+    const int *list;  // FE-list positions of the elements of this type
+    int nlist;
+    double *S;
+    int *status;
+};
+
+constexpr int HEX_EPB = 8;  // 8 elements x 24 rows = 192 threads
+__global__ void __launch_bounds__(HEX_EPB * 24) k_ke_hex8(KeArgs A) {
+    __shared__ double sg[HEX_EPB][8][8][3];
+    __shared__ double sw[HEX_EPB][8];
+    const int t = threadIdx.x;
+    const int e0 = blockIdx.x * HEX_EPB;
     const double gp = 0.57735026918962573;
     const double gpt[2] = {-gp, gp};
-    const size_t nb = (size_t)A.nblk;
-    for (int q = A.nfe_off[a]; q < A.nfe_off[a + 1]; q++) {
-        const int item = A.nfe_item[q];
-        const int fp = item >> 3, la = item & 7;
-        const int e = A.fe_elem[fp];
-        const int n = A.nen[e];
-        double xN[8][3];
-        for (int b = 0; b < n; b++)
-            for (int c = 0; c < 3; c++) xN[b][c] = A.x[(size_t)A.econ[(size_t)e * 8 + b] * 3 + c];
-        double Krow[24];
-        for (int c = 0; c < 24; c++) Krow[c] = 0.0;
-        if constexpr (D == 3) {
-            if (n == 8) {
-                for (int mr = 0; mr < 2; mr++)
-                    for (int mq = 0; mq < 2; mq++)
-                        for (int mp = 0; mp < 2; mp++) {
-                            const double p = gpt[mp], qq = gpt[mq], r = gpt[mr];
-                            double dN[8][3], J[9];
-                            for (int i = 0; i < 8; i++) {
-                                dN[i][0] = 1.0 / 8 * c_hp[i] * (1 + qq * c_hq[i]) * (1 + r * c_hr[i]);
-                                dN[i][1] = 1.0 / 8 * c_hq[i] * (1 + p * c_hp[i]) * (1 + r * c_hr[i]);
-                                dN[i][2] = 1.0 / 8 * c_hr[i] * (1 + p * c_hp[i]) * (1 + qq * c_hq[i]);
-                            }
-                            for (int i = 0; i < 9; i++) J[i] = 0.0;
-                            for (int cc = 0; cc < 3; cc++)
-                                for (int i = 0; i < 8; i++) {
-                                    J[0 * 3 + cc] = dN[i][0] * xN[i][cc] + J[0 * 3 + cc];
-                                    J[1 * 3 + cc] = dN[i][1] * xN[i][cc] + J[1 * 3 + cc];
-                                    J[2 * 3 + cc] = dN[i][2] * xN[i][cc] + J[2 * 3 + cc];
-                                }
-                            const double detJ = det3(J);
-                            double g[8][3];
-                            if (!lu3_solve8(J, dN, g)) { atomicOr(A.status, 2); return; }
-                            btdb_row3(g, 8, la, ii, A.Dm, 1.0 * 1.0 * 1.0 * detJ, Krow);
-                        }
-            } else {
-                // tet4 closed form (pdfemEleTetra4N.cpp:84-170): B = [b c d]/6V, Ke = Bt D B V
-                double m3[9];
-                auto det_rows = [&](int r0, int r1, int r2, int c0, int c1) {
-                    m3[0] = 1; m3[1] = xN[r0][c0]; m3[2] = xN[r0][c1];
-                    m3[3] = 1; m3[4] = xN[r1][c0]; m3[5] = xN[r1][c1];
-                    m3[6] = 1; m3[7] = xN[r2][c0]; m3[8] = xN[r2][c1];
-                    return det3(m3);
-                };
-                double bb[4], cc[4], dd[4];
-                bb[0] = -det_rows(1, 2, 3, 1, 2); bb[1] = det_rows(0, 2, 3, 1, 2);
-                bb[2] = -det_rows(0, 1, 3, 1, 2); bb[3] = det_rows(0, 1, 2, 1, 2);
-                cc[0] = det_rows(1, 2, 3, 0, 2); cc[1] = -det_rows(0, 2, 3, 0, 2);
-                cc[2] = det_rows(0, 1, 3, 0, 2); cc[3] = -det_rows(0, 1, 2, 0, 2);
-                dd[0] = -det_rows(1, 2, 3, 0, 1); dd[1] = det_rows(0, 2, 3, 0, 1);
-                dd[2] = -det_rows(0, 1, 3, 0, 1); dd[3] = det_rows(0, 1, 2, 0, 1);
-                double Jv[9];
-                for (int r = 0; r < 3; r++)
-                    for (int c = 0; c < 3; c++) Jv[r * 3 + c] = xN[r + 1][c] - xN[0][c];
-                const double V = fabs(det3(Jv)) / 6.0, V6 = 6.0 * (fabs(det3(Jv)) / 6.0);
-                double g[8][3];
-                for (int i = 0; i < 4; i++) { g[i][0] = bb[i] / V6; g[i][1] = cc[i] / V6; g[i][2] = dd[i] / V6; }
-                // Ke = (BtD B) * V: accumulate with unit weight then scale (product order kept)
-                double tmp[24];
-                for (int c = 0; c < 24; c++) tmp[c] = 0.0;
-                btdb_row3(g, 4, la, ii, A.Dm, 1.0, tmp);
-                for (int c = 0; c < 12; c++) Krow[c] = tmp[c] * V;
+    if (t < HEX_EPB * 8) {
+        const int el = t / 8, g = t % 8, idx = e0 + el;
+        if (idx < A.nlist) {
+            const int e = A.fe_elem[A.list[idx]];
+            double xN[8][3];
+            for (int b = 0; b < 8; b++)
+                for (int c = 0; c < 3; c++) xN[b][c] = A.x[(size_t)A.econ[(size_t)e * 8 + b] * 3 + c];
+            // Gauss point order of pdfemEleBrick8N::eleStiffMatFEM: r outer, q, p inner
+            const double p = gpt[g & 1], qq = gpt[(g >> 1) & 1], r = gpt[g >> 2];
+            double dN[8][3], J[9];
+            for (int i = 0; i < 8; i++) {
+                dN[i][0] = 1.0 / 8 * c_hp[i] * (1 + qq * c_hq[i]) * (1 + r * c_hr[i]);
+                dN[i][1] = 1.0 / 8 * c_hq[i] * (1 + p * c_hp[i]) * (1 + r * c_hr[i]);
+                dN[i][2] = 1.0 / 8 * c_hr[i] * (1 + p * c_hp[i]) * (1 + qq * c_hq[i]);
             }
-        } else {
-            if (n == 4) {
-                for (int mm = 0; mm < 2; mm++)
-                    for (int nn2 = 0; nn2 < 2; nn2++) {
-                        const double p = gpt[mm], qq = gpt[nn2];
-                        double dNdp[4] = {0.25 * (-1 + qq), 0.25 * (1 - qq), 0.25 * (1 + qq), 0.25 * (-1 - qq)};
-                        double dNdq[4] = {0.25 * (-1 + p), 0.25 * (-1 - p), 0.25 * (1 + p), 0.25 * (1 - p)};
-                        double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
-                        for (int i = 0; i < 4; i++) {
-                            J00 = dNdp[i] * xN[i][0] + J00;
-                            J01 = dNdp[i] * xN[i][1] + J01;
-                            J10 = dNdq[i] * xN[i][0] + J10;
-                            J11 = dNdq[i] * xN[i][1] + J11;
-                        }
-                        const double detJ = J00 * J11 - J01 * J10;
-                        const double i00 = J11 / detJ, i01 = -J01 / detJ, i10 = -J10 / detJ, i11 = J00 / detJ;
-                        double g[4][2];
-                        for (int i = 0; i < 4; i++) {
-                            g[i][0] = i00 * dNdp[i] + i01 * dNdq[i];
-                            g[i][1] = i10 * dNdp[i] + i11 * dNdq[i];
-                        }
-                        btdb_row2(g, 4, la, ii, A.Dm, 1.0 * 1.0 * detJ, Krow);
-                    }
-            } else {
-                // tri3 closed form (pdfemEleTri3N.cpp:33-47, :185-210)
-                double m3[9] = {1, xN[0][0], xN[0][1], 1, xN[1][0], xN[1][1], 1, xN[2][0], xN[2][1]};
-                const double V = 0.5 * fabs(det3(m3)), V2 = 2.0 * (0.5 * fabs(det3(m3)));
-                double bb[3] = {xN[1][1] - xN[2][1], xN[2][1] - xN[0][1], xN[0][1] - xN[1][1]};
-                double cc[3] = {-xN[1][0] + xN[2][0], -xN[2][0] + xN[0][0], -xN[0][0] + xN[1][0]};
-                double g[4][2];
-                for (int i = 0; i < 3; i++) { g[i][0] = bb[i] / V2; g[i][1] = cc[i] / V2; }
-                double tmp[24];
-                for (int c = 0; c < 24; c++) tmp[c] = 0.0;
-                btdb_row2(g, 3, la, ii, A.Dm, 1.0, tmp);
-                for (int c = 0; c < 6; c++) Krow[c] = tmp[c] * V;
-            }
+            for (int i = 0; i < 9; i++) J[i] = 0.0;
+            for (int cc = 0; cc < 3; cc++)
+                for (int i = 0; i < 8; i++) {
+                    J[0 * 3 + cc] = dN[i][0] * xN[i][cc] + J[0 * 3 + cc];
+                    J[1 * 3 + cc] = dN[i][1] * xN[i][cc] + J[1 * 3 + cc];
+                    J[2 * 3 + cc] = dN[i][2] * xN[i][cc] + J[2 * 3 + cc];
+                }
+            const double detJ = det3(J);
+            double g8[8][3];
+            if (!lu3_solve8(J, dN, g8)) atomicOr(A.status, 2);
+            for (int b = 0; b < 8; b++)
+                for (int c = 0; c < 3; c++) sg[el][g][b][c] = g8[b][c];
+            sw[el][g] = 1.0 * 1.0 * 1.0 * detJ;
         }
-        // add the element row into the node's block row (element order = reference order)
-        const int *blk = &A.fe_blk[(size_t)fp * 64 + la * 8];
-        for (int b = 0; b < n; b++) {
-            const size_t bi = (size_t)blk[b];
-            for (int jj = 0; jj < D; jj++) {
-                const size_t idx = (size_t)(ii * D + jj) * nb + bi;
-                A.K[idx] = A.K[idx] + Krow[D * b + jj];
+    }
+    __syncthreads();
+    const int el = t / 24, row = t % 24, idx = e0 + el;
+    if (idx >= A.nlist) return;
+    const int la = row / 3, ii = row % 3;
+    double Krow[24];
+    for (int c = 0; c < 24; c++) Krow[c] = 0.0;
+    for (int g = 0; g < 8; g++) btdb_row3(sg[el][g], 8, la, ii, A.Dm, sw[el][g], Krow);
+    double *out = A.S + (size_t)A.list[idx] * 576 + row * 24;
+    for (int c = 0; c < 24; c++) out[c] = Krow[c];
+}
+
+constexpr int QUAD_EPB = 32;  // 32 elements x 8 rows = 256 threads
+__global__ void __launch_bounds__(QUAD_EPB * 8) k_ke_quad4(KeArgs A) {
+    __shared__ double sg[QUAD_EPB][4][4][2];
+    __shared__ double sw[QUAD_EPB][4];
+    const int t = threadIdx.x;
+    const int e0 = blockIdx.x * QUAD_EPB;
+    const double gp = 0.57735026918962573;
+    const double gpt[2] = {-gp, gp};
+    if (t < QUAD_EPB * 4) {
+        const int el = t / 4, g = t % 4, idx = e0 + el;
+        if (idx < A.nlist) {
+            const int e = A.fe_elem[A.list[idx]];
+            double xN[4][2];
+            for (int b = 0; b < 4; b++)
+                for (int c = 0; c < 2; c++) xN[b][c] = A.x[(size_t)A.econ[(size_t)e * 8 + b] * 3 + c];
+            // pdfemEleQuad4N::eleStiffMatFEM: p outer, q inner
+            const double p = gpt[g >> 1], qq = gpt[g & 1];
+            const double dNdp[4] = {0.25 * (-1 + qq), 0.25 * (1 - qq), 0.25 * (1 + qq), 0.25 * (-1 - qq)};
+            const double dNdq[4] = {0.25 * (-1 + p), 0.25 * (-1 - p), 0.25 * (1 + p), 0.25 * (1 - p)};
+            double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+            for (int i = 0; i < 4; i++) {
+                J00 = dNdp[i] * xN[i][0] + J00;
+                J01 = dNdp[i] * xN[i][1] + J01;
+                J10 = dNdq[i] * xN[i][0] + J10;
+                J11 = dNdq[i] * xN[i][1] + J11;
             }
+            const double detJ = J00 * J11 - J01 * J10;
+            const double i00 = J11 / detJ, i01 = -J01 / detJ, i10 = -J10 / detJ, i11 = J00 / detJ;
+            for (int i = 0; i < 4; i++) {
+                sg[el][g][i][0] = i00 * dNdp[i] + i01 * dNdq[i];
+                sg[el][g][i][1] = i10 * dNdp[i] + i11 * dNdq[i];
+            }
+            sw[el][g] = 1.0 * 1.0 * detJ;
+        }
+    }
+    __syncthreads();
+    const int el = t / 8, row = t % 8, idx = e0 + el;
+    if (idx >= A.nlist) return;
+    const int la = row / 2, ii = row % 2;
+    double Krow[8];
+    for (int c = 0; c < 8; c++) Krow[c] = 0.0;
+    for (int g = 0; g < 4; g++) btdb_row2(sg[el][g], 4, la, ii, A.Dm, sw[el][g], Krow);
+    double *out = A.S + (size_t)A.list[idx] * 64 + row * 8;
+    for (int c = 0; c < 8; c++) out[c] = Krow[c];
+}
+
+// tri3 / tet4: constant B, Ke = (Bt D B) V (pdfemEleTri3N.cpp:33-47, :185-210;
+// pdfemEleTetra4N.cpp:38-170); one thread per (element, row)
+template <int D>
+__global__ void k_ke_simplex(KeArgs A) {
+    constexpr int NEN = D + 1, ND = D == 3 ? 24 : 8;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nlist * NEN * D) return;
+    const int idx = t / (NEN * D), row = t % (NEN * D);
+    const int e = A.fe_elem[A.list[idx]];
+    const int la = row / D, ii = row % D;
+    double xN[4][3];
+    for (int b = 0; b < NEN; b++)
+        for (int c = 0; c < 3; c++) xN[b][c] = A.x[(size_t)A.econ[(size_t)e * 8 + b] * 3 + c];
+    double Krow[24];
+    for (int c = 0; c < 24; c++) Krow[c] = 0.0;
+    double V;
+    if constexpr (D == 3) {
+        double m3[9];
+        auto det_rows = [&](int r0, int r1, int r2, int c0, int c1) {
+            m3[0] = 1; m3[1] = xN[r0][c0]; m3[2] = xN[r0][c1];
+            m3[3] = 1; m3[4] = xN[r1][c0]; m3[5] = xN[r1][c1];
+            m3[6] = 1; m3[7] = xN[r2][c0]; m3[8] = xN[r2][c1];
+            return det3(m3);
+        };
+        double bb[4], cc[4], dd[4];
+        bb[0] = -det_rows(1, 2, 3, 1, 2); bb[1] = det_rows(0, 2, 3, 1, 2);
+        bb[2] = -det_rows(0, 1, 3, 1, 2); bb[3] = det_rows(0, 1, 2, 1, 2);
+        cc[0] = det_rows(1, 2, 3, 0, 2); cc[1] = -det_rows(0, 2, 3, 0, 2);
+        cc[2] = det_rows(0, 1, 3, 0, 2); cc[3] = -det_rows(0, 1, 2, 0, 2);
+        dd[0] = -det_rows(1, 2, 3, 0, 1); dd[1] = det_rows(0, 2, 3, 0, 1);
+        dd[2] = -det_rows(0, 1, 3, 0, 1); dd[3] = det_rows(0, 1, 2, 0, 1);
+        double Jv[9];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) Jv[r * 3 + c] = xN[r + 1][c] - xN[0][c];
+        V = fabs(det3(Jv)) / 6.0;
+        const double V6 = 6.0 * (fabs(det3(Jv)) / 6.0);
+        double g[4][3];
+        for (int i = 0; i < 4; i++) { g[i][0] = bb[i] / V6; g[i][1] = cc[i] / V6; g[i][2] = dd[i] / V6; }
+        btdb_row3(g, 4, la, ii, A.Dm, 1.0, Krow);
+    } else {
+        double m3[9] = {1, xN[0][0], xN[0][1], 1, xN[1][0], xN[1][1], 1, xN[2][0], xN[2][1]};
+        V = 0.5 * fabs(det3(m3));
+        const double V2 = 2.0 * (0.5 * fabs(det3(m3)));
+        const double bb[3] = {xN[1][1] - xN[2][1], xN[2][1] - xN[0][1], xN[0][1] - xN[1][1]};
+        const double cc[3] = {-xN[1][0] + xN[2][0], -xN[2][0] + xN[0][0], -xN[0][0] + xN[1][0]};
+        double g[3][2];
+        for (int i = 0; i < 3; i++) { g[i][0] = bb[i] / V2; g[i][1] = cc[i] / V2; }
+        btdb_row2(g, 3, la, ii, A.Dm, 1.0, Krow);
+    }
+    double *out = A.S + (size_t)A.list[idx] * ND * ND + row * ND;
+    for (int c = 0; c < NEN * D; c++) out[c] = Krow[c] * V;
+}
+
+// node-centric add of the staged element rows into the node's block row, in
+// element order (the reference's summation order of FE terms into K)
+template <int D>
+__global__ void k_fe_gather(int nn, const int *nfe_off, const int *nfe_item, const int *fe_elem, const int *nen,
+                            const int *fe_blk, const double *S, double *K) {
+    constexpr int ND = D == 3 ? 24 : 8, DD = D * D;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn * D) return;
+    const int a = t / D, ii = t % D;
+    for (int q = nfe_off[a]; q < nfe_off[a + 1]; q++) {
+        const int item = nfe_item[q];
+        const int fp = item >> 3, la = item & 7;
+        const int n = nen[fe_elem[fp]];
+        const double *srow = S + (size_t)fp * ND * ND + (size_t)(D * la + ii) * ND;
+        const int *blk = &fe_blk[(size_t)fp * 64 + la * 8];
+        for (int b = 0; b < n; b++) {
+            double *kb = K + (size_t)blk[b] * DD + ii * D;
+#pragma unroll
+            for (int jj = 0; jj < D; jj++) kb[jj] = kb[jj] + srow[D * b + jj];
         }
     }
 }
@@ -663,37 +724,29 @@ __global__ void k_nbc_gather(const int *off, const int *item, const int *nbc_set
 // ------------------------------------------------------------------------
 // node-block SpMV: y = K x (full DOF vectors), optional mask of prescribed rows
 // ------------------------------------------------------------------------
-template <int D, typename T, int G>
-__global__ void __launch_bounds__(256) k_spmv(const int *rows, int nrows, const int *adj_off, const int *adj,
-                                              const T *K, long long nblk, const double *x, double *y,
-                                              const int *eq, int mask) {
-    const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const int lane = threadIdx.x % G;
-    if (gid >= nrows) return;
-    const int r = rows[gid];
-    const size_t nb = (size_t)nblk;
+// one launch for both row classes: blocks [0, nbs) take short rows on groups of
+// 8 lanes, the remaining blocks take long (PD) rows on whole wavefronts
+template <int D, typename T>
+__global__ void __launch_bounds__(256) k_spmv(const int *rows_s, int ns, int nbs, const int *rows_l, int nl,
+                                              const int *adj_off, const int *adj, const T *K, const double *x,
+                                              double *y, const int *eq, int mask) {
     T acc[D];
-#pragma unroll
-    for (int i = 0; i < D; i++) acc[i] = T(0);
-    for (int b = adj_off[r] + lane; b < adj_off[r + 1]; b += G) {
-        const int c = adj[b];
-        T xv[D];
-#pragma unroll
-        for (int j = 0; j < D; j++) xv[j] = T(x[(size_t)c * D + j]);
-#pragma unroll
-        for (int i = 0; i < D; i++)
-#pragma unroll
-            for (int j = 0; j < D; j++) acc[i] += K[(size_t)(i * D + j) * nb + b] * xv[j];
+    int r, lane;
+    if ((int)blockIdx.x < nbs) {
+        const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / 8;
+        lane = threadIdx.x % 8;
+        if (gid >= ns) return;
+        r = rows_s[gid];
+        row_matvec<D, T, double, 8, false>(r, lane, adj_off, adj, K, x, acc);
+    } else {
+        const int gid = ((blockIdx.x - nbs) * blockDim.x + threadIdx.x) / 64;
+        lane = threadIdx.x % 64;
+        if (gid >= nl) return;
+        r = rows_l[gid];
+        row_matvec<D, T, double, 64, false>(r, lane, adj_off, adj, K, x, acc);
     }
-#pragma unroll
-    for (int s = G / 2; s > 0; s >>= 1)
-#pragma unroll
-        for (int i = 0; i < D; i++) acc[i] += __shfl_xor(acc[i], s, G);
     if (lane < D) {
-        T v = acc[0];
-#pragma unroll
-        for (int i = 1; i < D; i++)
-            if (lane == i) v = acc[i];
+        const T v = pick<D, T>(acc, lane);
         const size_t o = (size_t)r * D + lane;
         y[o] = (mask && eq[(size_t)r * 3 + lane] < 0) ? 0.0 : double(v);
     }
@@ -713,7 +766,7 @@ __global__ void k_csr_extract(const int *row_node, int neq, const int64_t *ia, c
     for (int b = adj_off[k]; b < adj_off[k + 1]; b++) {
         const int c = adj[b];
         for (int jj = 0; jj < D; jj++)
-            if (eq[(size_t)c * 3 + jj] >= 0) Ku[p++] = K[(size_t)(ii * D + jj) * nb + b];
+            if (eq[(size_t)c * 3 + jj] >= 0) Ku[p++] = K[(size_t)b * D * D + ii * D + jj];
     }
 }
 
@@ -765,12 +818,34 @@ int dev_assemble_K(Ctx &c, std::string &err) {
         PDX_HIP(hipGetLastError());
     }
     if (c.nfe > 0) {
-        FeArgs F;
-        F.x = c.x; F.Dm = c.Dm; F.econ = c.econ; F.nen = c.nen; F.fe_elem = c.fe_elem; F.fe_blk = c.fe_blk;
-        F.nfe_off = c.nfe_off; F.nfe_item = c.nfe_item; F.K = c.K; F.nblk = c.nblk; F.nn = c.nn; F.status = status;
-        int th = c.nn * c.dim;
-        if (c.dim == 3) k_fe_rows<3><<<(th + 127) / 128, 128, 0, c.stream>>>(F);
-        else k_fe_rows<2><<<(th + 127) / 128, 128, 0, c.stream>>>(F);
+        const int ND = c.dim == 3 ? 24 : 8;
+        if (!c.Ke_stage) PDX_HIP(hipMalloc((void **)&c.Ke_stage, (size_t)c.nfe * ND * ND * sizeof(double)));
+        KeArgs A;
+        A.x = c.x; A.Dm = c.Dm; A.econ = c.econ; A.nen = c.nen; A.fe_elem = c.fe_elem; A.S = c.Ke_stage;
+        A.status = status;
+        if (c.dim == 3) {
+            if (c.n_fe_a) {
+                A.list = c.fe_list_a; A.nlist = c.n_fe_a;
+                k_ke_hex8<<<(c.n_fe_a + HEX_EPB - 1) / HEX_EPB, HEX_EPB * 24, 0, c.stream>>>(A);
+            }
+            if (c.n_fe_b) {
+                A.list = c.fe_list_b; A.nlist = c.n_fe_b;
+                k_ke_simplex<3><<<(c.n_fe_b * 12 + 127) / 128, 128, 0, c.stream>>>(A);
+            }
+            k_fe_gather<3><<<(c.nn * 3 + 255) / 256, 256, 0, c.stream>>>(c.nn, c.nfe_off, c.nfe_item, c.fe_elem, c.nen,
+                                                                         c.fe_blk, c.Ke_stage, c.K);
+        } else {
+            if (c.n_fe_a) {
+                A.list = c.fe_list_a; A.nlist = c.n_fe_a;
+                k_ke_quad4<<<(c.n_fe_a + QUAD_EPB - 1) / QUAD_EPB, QUAD_EPB * 8, 0, c.stream>>>(A);
+            }
+            if (c.n_fe_b) {
+                A.list = c.fe_list_b; A.nlist = c.n_fe_b;
+                k_ke_simplex<2><<<(c.n_fe_b * 6 + 127) / 128, 128, 0, c.stream>>>(A);
+            }
+            k_fe_gather<2><<<(c.nn * 2 + 255) / 256, 256, 0, c.stream>>>(c.nn, c.nfe_off, c.nfe_item, c.fe_elem, c.nen,
+                                                                         c.fe_blk, c.Ke_stage, c.K);
+        }
         PDX_HIP(hipGetLastError());
     }
     int hs = 0;
@@ -817,16 +892,10 @@ int dev_extract_csr(Ctx &c, const double *Kblk, double *Ku_dev) {
 
 template <int D, typename T>
 static void launch_spmv(Ctx &c, const T *K, const double *x, double *y, bool mask) {
-    if (c.n_short) {
-        const int th = c.n_short * 8;
-        k_spmv<D, T, 8><<<(th + 255) / 256, 256, 0, c.stream>>>(c.rows_short, c.n_short, c.adj_off, c.adj, K, c.nblk,
-                                                                 x, y, c.eq, mask);
-    }
-    if (c.n_long) {
-        const int th = c.n_long * 64;
-        k_spmv<D, T, 64><<<(th + 255) / 256, 256, 0, c.stream>>>(c.rows_long, c.n_long, c.adj_off, c.adj, K, c.nblk,
-                                                                 x, y, c.eq, mask);
-    }
+    const int nbs = (c.n_short * 8 + 255) / 256, nbl = (c.n_long * 64 + 255) / 256;
+    if (nbs + nbl == 0) return;
+    k_spmv<D, T><<<nbs + nbl, 256, 0, c.stream>>>(c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, c.adj_off,
+                                                   c.adj, K, x, y, c.eq, mask);
 }
 
 int dev_spmv_K(Ctx &c, const double *K64, const double *x, double *y) {

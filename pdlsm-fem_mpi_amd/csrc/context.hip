@@ -150,6 +150,12 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
         }
         UP(c->fe_elem, m.fe);
         UP(c->fe_blk, fe_blk);
+        std::vector<int> la, lb;
+        for (int t = 0; t < c->nfe; t++) (m.nen[m.fe[t]] == (D == 3 ? 8 : 4) ? la : lb).push_back(t);
+        c->n_fe_a = int(la.size());
+        c->n_fe_b = int(lb.size());
+        UP(c->fe_list_a, la);
+        UP(c->fe_list_b, lb);
         UP(c->nfe_off, cnt);
         UP(c->nfe_item, item);
         // all elements (lumped mass)
@@ -215,6 +221,8 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
     AL(c->K, (size_t)c->dd * c->nblk);
     AL(c->mlump, (size_t)m.nn * D);
     for (int i = 0; i < 3; i++) AL(c->U[i], (size_t)m.nn * D);
+    // the zero fills above run on the null stream: finish them before any work on c->stream
+    PDX_HIP(hipDeviceSynchronize());
     if ((rc = dev_nbc_unit(*c, err))) return rc;
     PDX_HIP(hipStreamSynchronize(c->stream));
     return PDX_OK;
@@ -229,6 +237,7 @@ void ctx_free(Ctx *c) {
     for (void *p : c->owned) hipFree(p);
     if (c->K32) hipFree(c->K32);
     if (c->Mk) hipFree(c->Mk);
+    if (c->Ke_stage) hipFree(c->Ke_stage);
     if (c->red) hipFree(c->red);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

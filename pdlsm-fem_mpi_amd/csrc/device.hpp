@@ -3,9 +3,9 @@
 // Layout in HBM
 //   * stiffness: node-block CSR ("BSR") over the node graph of the reference's
 //     pattern (every neighbour node of node k, prescribed DOFs included), with
-//     d x d blocks stored component-major: value (i,j) of block b lives at
-//     K[(i*d + j) * nblk + b].  Lanes that walk consecutive blocks of a row
-//     read consecutive addresses for every component.
+//     d x d blocks stored contiguously: value (i,j) of block b lives at
+//     K[b*d*d + i*d + j], so one lane loads a whole block with 2-5 wide loads
+//     (dwordx4) and the lanes of a row segment read consecutive blocks.
 //   * vectors: full node-major DOF vectors of length nn*d; prescribed DOFs
 //     are masked where a free-DOF quantity is meant.
 //   * families: CSR of member node ids (self first) with the block index of
@@ -45,6 +45,9 @@ struct Ctx {
     int *fp_off = nullptr, *fp_list = nullptr, *pdbe = nullptr;
     int nfe = 0;
     int *fe_elem = nullptr, *fe_blk = nullptr, *nfe_off = nullptr, *nfe_item = nullptr;
+    int n_fe_a = 0, n_fe_b = 0;  // FE-list positions of hex8/quad4 (a) and tet4/tri3 (b) elements
+    int *fe_list_a = nullptr, *fe_list_b = nullptr;
+    double *Ke_stage = nullptr;  // element stiffness staging [nfe][ND*ND]
     int *nall_off = nullptr, *nall_ele = nullptr;
     int *adj_off = nullptr, *adj = nullptr;
     int64_t *ia = nullptr;
@@ -80,6 +83,8 @@ struct Ctx {
     int step_phase = 0;      // buffer rotation counter
     long long steps_done = -1;  // -1: start-up step not yet taken
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
+    int tune_explicit[2] = {-1, -1};               // chosen step-kernel variant per precision
+    float tune_ms[2][8] = {{0}};                   // measured ms per variant
     int graph_phase[2] = {-1, -1};
 
     // scratch for Krylov

@@ -121,7 +121,7 @@ __global__ void k_mass_rows(MassArgs A) {
         }
         const int *blk = &A.fe_blk[(size_t)fp * 64 + la * 8];
         for (int b = 0; b < n; b++) {
-            const size_t idx = (size_t)(ii * D + ii) * nb + (size_t)blk[b];
+            const size_t idx = (size_t)blk[b] * D * D + ii * D + ii;
             A.M[idx] = A.M[idx] + row[b];
         }
     }
@@ -134,7 +134,7 @@ __global__ void k_mass_pd(MassArgs A, int D) {
     const size_t b = (size_t)A.fam_blk[A.fam_off[f]];
     const size_t nb = (size_t)A.nblk;
     for (int i = 0; i < D; i++) {
-        const size_t idx = (size_t)(i * D + i) * nb + b;
+        const size_t idx = b * D * D + i * D + i;
         A.M[idx] = A.M[idx] + A.fam_v[f] * A.rho;
     }
 }

@@ -1,0 +1,56 @@
+// Row segment of the node-block matrix: G lanes walk the blocks of one node
+// row, each lane loading whole d x d blocks (contiguous, wide loads) and the
+// d-vector of the column node, then the G partial sums are combined with
+// cross-lane shuffles.  Shared by the explicit step, the SpMV and the Krylov
+// SpMV kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pdx {
+
+// acc = sum_b K_b * x_c  (plain form), or sum_b K_b * (x_c - x_r) (difference
+// form: every full block row of K sums to zero because each family, element
+// and boundary operator annihilates rigid translations, so both forms are
+// equal in exact arithmetic; the difference form keeps FP32 products away
+// from the cancellation of large displacement terms).
+template <int D, typename T, typename X, int G, bool DIFF, bool NT = false>
+__device__ inline void row_matvec(int r, int lane, const int *__restrict__ adj_off, const int *__restrict__ adj,
+                                  const T *__restrict__ K, const X *__restrict__ x, T acc[D]) {
+    constexpr int DD = D * D;
+#pragma unroll
+    for (int i = 0; i < D; i++) acc[i] = T(0);
+    X xr[D];
+    if (DIFF) {
+#pragma unroll
+        for (int j = 0; j < D; j++) xr[j] = x[(size_t)r * D + j];
+    }
+    const int b1 = adj_off[r + 1];
+    for (int b = adj_off[r] + lane; b < b1; b += G) {
+        const int c = adj[b];
+        T kb[DD];
+#pragma unroll
+        for (int e = 0; e < DD; e++) kb[e] = NT ? __builtin_nontemporal_load(&K[(size_t)b * DD + e]) : K[(size_t)b * DD + e];
+        T xv[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) xv[j] = DIFF ? T(x[(size_t)c * D + j] - xr[j]) : T(x[(size_t)c * D + j]);
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) acc[i] += kb[i * D + j] * xv[j];
+    }
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1)
+#pragma unroll
+        for (int i = 0; i < D; i++) acc[i] += __shfl_xor(acc[i], s, G);
+}
+
+template <int D, typename T>
+__device__ inline T pick(const T acc[D], int lane) {
+    T v = acc[0];
+#pragma unroll
+    for (int i = 1; i < D; i++)
+        if (lane == i) v = acc[i];
+    return v;
+}
+
+}  // namespace pdx

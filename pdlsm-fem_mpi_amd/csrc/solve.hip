@@ -18,47 +18,42 @@
 #include <cmath>
 
 #include "device.hpp"
+#include "rowblock.hpp"
 
 namespace pdx {
 
 // ------------------------------------------------------------------------
 // explicit step
 // ------------------------------------------------------------------------
-template <int D, typename T, int G>
-__global__ void __launch_bounds__(256) k_explicit(const int *rows, int nrows, const int *adj_off, const int *adj,
-                                                  const T *K, long long nblk, const double *Un, const double *Unm1,
-                                                  double *Unp1, const double *m, const double *f_unit, int n_nbc,
-                                                  int nn, const double *nbc_cur, double *nbc_next,
-                                                  const double *nbc_inc, const int *eq, const int *presc_set,
-                                                  const double *ebc_inc, double dt, int startup, int write_nbc) {
-    const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const int lane = threadIdx.x % G;
-    if (write_nbc && blockIdx.x == 0 && threadIdx.x < n_nbc) nbc_next[threadIdx.x] = nbc_cur[threadIdx.x] + nbc_inc[threadIdx.x];
-    if (gid >= nrows) return;
-    const int r = rows[gid];
-    const size_t nb = (size_t)nblk;
+// FP32 values use the difference form (see rowblock.hpp), FP64 the plain form.
+// GS = lanes per short row, NT = non-temporal loads of K (chosen by the tuner).
+template <int D, typename T, int GS, bool NT>
+__global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int nbs, const int *rows_l, int nl,
+                                                  const int *adj_off, const int *adj, const T *K, const double *Un,
+                                                  const double *Unm1, double *Unp1, const double *m,
+                                                  const double *f_unit, int n_nbc, int nn, const double *nbc_cur,
+                                                  double *nbc_next, const double *nbc_inc, const int *eq,
+                                                  const int *presc_set, const double *ebc_inc, double dt,
+                                                  int startup) {
+    constexpr bool DIFF = sizeof(T) == 4;
+    if (blockIdx.x == 0 && threadIdx.x < n_nbc) nbc_next[threadIdx.x] = nbc_cur[threadIdx.x] + nbc_inc[threadIdx.x];
     T acc[D];
-#pragma unroll
-    for (int i = 0; i < D; i++) acc[i] = T(0);
-    for (int b = adj_off[r] + lane; b < adj_off[r + 1]; b += G) {
-        const int c = adj[b];
-        T xv[D];
-#pragma unroll
-        for (int j = 0; j < D; j++) xv[j] = T(Un[(size_t)c * D + j]);
-#pragma unroll
-        for (int i = 0; i < D; i++)
-#pragma unroll
-            for (int j = 0; j < D; j++) acc[i] += K[(size_t)(i * D + j) * nb + b] * xv[j];
+    int r, lane;
+    if ((int)blockIdx.x < nbs) {
+        const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / GS;
+        lane = threadIdx.x % GS;
+        if (gid >= ns) return;
+        r = rows_s[gid];
+        row_matvec<D, T, double, GS, DIFF, NT>(r, lane, adj_off, adj, K, Un, acc);
+    } else {
+        const int gid = ((blockIdx.x - nbs) * blockDim.x + threadIdx.x) / 64;
+        lane = threadIdx.x % 64;
+        if (gid >= nl) return;
+        r = rows_l[gid];
+        row_matvec<D, T, double, 64, DIFF, NT>(r, lane, adj_off, adj, K, Un, acc);
     }
-#pragma unroll
-    for (int s = G / 2; s > 0; s >>= 1)
-#pragma unroll
-        for (int i = 0; i < D; i++) acc[i] += __shfl_xor(acc[i], s, G);
     if (lane < D) {
-        T v = acc[0];
-#pragma unroll
-        for (int i = 1; i < D; i++)
-            if (lane == i) v = acc[i];
+        const T v = pick<D, T>(acc, lane);
         const size_t o = (size_t)r * D + lane;
         const int code = r * 3 + lane;
         if (eq[code] >= 0) {
@@ -122,39 +117,16 @@ __global__ void __launch_bounds__(256) k_dot(const V *a, const V *b, size_t n, d
 
 // SpMV with fused dot partials: y = mask(K x); part0 += (w0, y); part1 += (y, y) when w1 set
 template <int D, typename T, typename V, int G>
-__global__ void __launch_bounds__(256) k_spmv_dot(const int *rows, int nrows, const int *adj_off, const int *adj,
-                                                  const T *K, long long nblk, const V *x, V *y, const int *eq,
-                                                  const V *w0, double *part0, const V *w1, double *part1,
-                                                  int part_base) {
-    __shared__ double sm[16];
+__device__ inline void spmv_dot_rows(const int *rows, int nrows, int first_group, int groups, const int *adj_off,
+                                     const int *adj, const T *K, const V *x, V *y, const int *eq, const V *w0,
+                                     const V *w1, double &d0, double &d1) {
     const int lane = threadIdx.x % G;
-    const int groups = (gridDim.x * blockDim.x) / G;
-    const size_t nb = (size_t)nblk;
-    double d0 = 0.0, d1 = 0.0;
-    for (int gid = (blockIdx.x * blockDim.x + threadIdx.x) / G; gid < nrows; gid += groups) {
+    for (int gid = first_group; gid < nrows; gid += groups) {
         const int r = rows[gid];
         T acc[D];
-#pragma unroll
-        for (int i = 0; i < D; i++) acc[i] = T(0);
-        for (int b = adj_off[r] + lane; b < adj_off[r + 1]; b += G) {
-            const int c = adj[b];
-            T xv[D];
-#pragma unroll
-            for (int j = 0; j < D; j++) xv[j] = T(x[(size_t)c * D + j]);
-#pragma unroll
-            for (int i = 0; i < D; i++)
-#pragma unroll
-                for (int j = 0; j < D; j++) acc[i] += K[(size_t)(i * D + j) * nb + b] * xv[j];
-        }
-#pragma unroll
-        for (int s = G / 2; s > 0; s >>= 1)
-#pragma unroll
-            for (int i = 0; i < D; i++) acc[i] += __shfl_xor(acc[i], s, G);
+        row_matvec<D, T, V, G, false>(r, lane, adj_off, adj, K, x, acc);
         if (lane < D) {
-            T v = acc[0];
-#pragma unroll
-            for (int i = 1; i < D; i++)
-                if (lane == i) v = acc[i];
+            const T v = pick<D, T>(acc, lane);
             const size_t o = (size_t)r * D + lane;
             V yv = eq[r * 3 + lane] >= 0 ? V(v) : V(0);
             y[o] = yv;
@@ -162,14 +134,33 @@ __global__ void __launch_bounds__(256) k_spmv_dot(const int *rows, int nrows, co
             if (w1) d1 += double(w1[o]) * double(yv);
         }
     }
+}
+
+// SpMV with fused dot partials over both row classes (grid-stride, RED_PARTS
+// blocks): y = mask(K x); part0 = (w0, y); part1 = (w1, y)
+template <int D, typename T, typename V>
+__global__ void __launch_bounds__(256) k_spmv_dot(const int *rows_s, int ns, const int *rows_l, int nl,
+                                                  const int *adj_off, const int *adj, const T *K, const V *x, V *y,
+                                                  const int *eq, const V *w0, double *part0, const V *w1,
+                                                  double *part1) {
+    __shared__ double sm[16];
+    double d0 = 0.0, d1 = 0.0;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    spmv_dot_rows<D, T, V, 8>(rows_s, ns, t / 8, nt / 8, adj_off, adj, K, x, y, eq, w0, w1, d0, d1);
+    spmv_dot_rows<D, T, V, 64>(rows_l, nl, t / 64, nt / 64, adj_off, adj, K, x, y, eq, w0, w1, d0, d1);
     if (part0) {
         d0 = block_sum(d0, sm);
-        if (threadIdx.x == 0) part0[part_base + blockIdx.x] = d0;
+        if (threadIdx.x == 0) part0[blockIdx.x] = d0;
     }
     if (part1) {
         d1 = block_sum(d1, sm);
-        if (threadIdx.x == 0) part1[part_base + blockIdx.x] = d1;
+        if (threadIdx.x == 0) part1[blockIdx.x] = d1;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int i = gridDim.x; i < NPART; i++) {
+            if (part0) part0[i] = 0.0;
+            if (part1) part1[i] = 0.0;
+        }
 }
 
 // apply the node-diagonal block preconditioner z = Minv r (Minv: [nn][D*D])
@@ -197,9 +188,15 @@ __global__ void __launch_bounds__(256) k_bicg_p(int nn, const V *r, V *p, const 
                                                 const double *rho_part, double *sc, int first) {
     __shared__ double sm[16];
     const double rho = sum_partials(rho_part, NPART, sm);
-    double beta = 0.0;
-    if (!first) beta = (rho / sc[SC_RHO_OLD]) * (sc[SC_ALPHA] / sc[SC_OMEGA]);
-    const V bv = V(beta), om = V(first ? 0.0 : sc[SC_OMEGA]);
+    // a vanishing rho_old or omega (breakdown, or exact convergence in low
+    // precision) restarts the search direction with p = r
+    double beta = 0.0, omg = 0.0;
+    if (!first && sc[SC_RHO_OLD] != 0.0 && sc[SC_OMEGA] != 0.0) {
+        beta = (rho / sc[SC_RHO_OLD]) * (sc[SC_ALPHA] / sc[SC_OMEGA]);
+        omg = sc[SC_OMEGA];
+    }
+    if (!isfinite(beta)) beta = 0.0;
+    const V bv = V(beta), om = V(omg);
     for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < nn; a += gridDim.x * blockDim.x) {
 #pragma unroll
         for (int i = 0; i < D; i++) {
@@ -216,7 +213,8 @@ __global__ void __launch_bounds__(256) k_bicg_s(int nn, const V *r, const V *v, 
     __shared__ double sm[16];
     const double rho = sum_partials(rho_part, NPART, sm);
     const double rv = sum_partials(rv_part, NPART, sm);
-    const double alpha = rho / rv;
+    double alpha = rv != 0.0 ? rho / rv : 0.0;
+    if (!isfinite(alpha)) alpha = 0.0;
     if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_ALPHA] = alpha;
     const V al = V(alpha);
     for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < nn; a += gridDim.x * blockDim.x) {
@@ -238,7 +236,8 @@ __global__ void __launch_bounds__(256) k_bicg_x(int nn, V *x, V *r, const V *rh,
     const double rho = sum_partials(rho_part, NPART, sm);
     const double ts = sum_partials(ts_part, NPART, sm);
     const double tt = sum_partials(tt_part, NPART, sm);
-    const double omega = tt > 0 ? ts / tt : 0.0;
+    double omega = tt > 0 ? ts / tt : 0.0;
+    if (!isfinite(omega)) omega = 0.0;
     const double alpha = sc[SC_ALPHA];
     const V al = V(alpha), om = V(omega);
     double d0 = 0.0, d1 = 0.0;
@@ -316,7 +315,7 @@ __global__ void k_blockjacobi(int nn, const int *adj_off, const int *adj, const 
     for (int i = 0; i < D; i++) fr[i] = eq[a * 3 + i] >= 0;
     for (int i = 0; i < D; i++)
         for (int j = 0; j < D; j++) {
-            double v = (b >= 0) ? K[(size_t)(i * D + j) * nblk + b] : 0.0;
+            double v = (b >= 0) ? K[(size_t)b * D * D + i * D + j] : 0.0;
             if (diag_only && i != j) v = 0.0;
             if (!fr[i] || !fr[j]) v = (i == j) ? 1.0 : 0.0;
             M[i * D + j] = v;
@@ -389,21 +388,75 @@ static int ensure_K32(Ctx &c, std::string &err) {
     return PDX_OK;
 }
 
+// explicit-step variants searched by the tuner: (lanes per short row, non-temporal K)
+constexpr int N_EXPLICIT_VARIANTS = 6;
+static const int kVarGS[N_EXPLICIT_VARIANTS] = {8, 4, 16, 8, 4, 16};
+static const int kVarNT[N_EXPLICIT_VARIANTS] = {0, 0, 0, 1, 1, 1};
+
+template <int D, typename T, int GS, bool NT>
+static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double *Unm1, double *Unp1,
+                              const double *nbc_cur, double *nbc_next, int startup) {
+    const int nbs = (c.n_short * GS + 255) / 256, nbl = (c.n_long * 64 + 255) / 256;
+    k_explicit<D, T, GS, NT><<<std::max(nbs + nbl, 1), 256, 0, c.stream>>>(
+        c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, c.adj_off, c.adj, K, Un, Unm1, Unp1, c.mlump, c.f_unit,
+        c.n_nbc, c.nn, nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup);
+}
+
 template <int D, typename T>
-static void launch_explicit(Ctx &c, const T *K, const double *Un, const double *Unm1, double *Unp1,
+static void launch_explicit(Ctx &c, int variant, const T *K, const double *Un, const double *Unm1, double *Unp1,
                             const double *nbc_cur, double *nbc_next, int startup) {
-    if (c.n_short) {
-        const int th = c.n_short * 8;
-        k_explicit<D, T, 8><<<(th + 255) / 256, 256, 0, c.stream>>>(
-            c.rows_short, c.n_short, c.adj_off, c.adj, K, c.nblk, Un, Unm1, Unp1, c.mlump, c.f_unit, c.n_nbc, c.nn,
-            nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup, 1);
+    switch (variant) {
+    case 1: launch_explicit_v<D, T, 4, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 2: launch_explicit_v<D, T, 16, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 3: launch_explicit_v<D, T, 8, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 4: launch_explicit_v<D, T, 4, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 5: launch_explicit_v<D, T, 16, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    default: launch_explicit_v<D, T, 8, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
     }
-    if (c.n_long) {
-        const int th = c.n_long * 64;
-        k_explicit<D, T, 64><<<(th + 255) / 256, 256, 0, c.stream>>>(
-            c.rows_long, c.n_long, c.adj_off, c.adj, K, c.nblk, Un, Unm1, Unp1, c.mlump, c.f_unit, c.n_nbc, c.nn,
-            nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup, c.n_short ? 0 : 1);
+}
+
+static void launch_explicit_any(Ctx &c, int prec, int variant, const double *Un, const double *Unm1, double *Unp1,
+                                const double *nbc_cur, double *nbc_next, int startup) {
+    if (prec == PDX_FP32) {
+        if (c.dim == 3) launch_explicit<3, float>(c, variant, c.K32, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
+        else launch_explicit<2, float>(c, variant, c.K32, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
+    } else {
+        if (c.dim == 3) launch_explicit<3, double>(c, variant, c.K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
+        else launch_explicit<2, double>(c, variant, c.K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
     }
+}
+
+// Autotuner: time every variant on the current state with the result written to
+// scratch buffers (the simulation state is not advanced), keep the fastest.
+static int tune_explicit(Ctx &c, int prec, std::string &err) {
+    const size_t n = (size_t)c.nn * c.dim;
+    double *scratch, *nscratch;
+    PDX_HIP(hipMalloc((void **)&scratch, n * 8));
+    PDX_HIP(hipMalloc((void **)&nscratch, 2 * (size_t)std::max(c.n_nbc, 1) * 8));
+    hipEvent_t e0, e1;
+    PDX_HIP(hipEventCreate(&e0));
+    PDX_HIP(hipEventCreate(&e1));
+    const int p = c.step_phase % 3;
+    float best = 1e30f;
+    int bestv = 0;
+    for (int v = 0; v < N_EXPLICIT_VARIANTS; v++) {
+        launch_explicit_any(c, prec, v, c.U[p], c.U[(p + 2) % 3], scratch, c.nbcv, nscratch, 0);  // warm
+        PDX_HIP(hipEventRecord(e0, c.stream));
+        for (int r = 0; r < 5; r++)
+            launch_explicit_any(c, prec, v, c.U[p], c.U[(p + 2) % 3], scratch, c.nbcv, nscratch, 0);
+        PDX_HIP(hipEventRecord(e1, c.stream));
+        PDX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        PDX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        c.tune_ms[prec][v] = ms / 5;
+        if (ms < best) { best = ms; bestv = v; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(scratch);
+    hipFree(nscratch);
+    c.tune_explicit[prec] = bestv;
+    return PDX_OK;
 }
 
 // one step on the device state: phase p = index of u_n buffer, s = step index
@@ -413,13 +466,8 @@ static void explicit_step(Ctx &c, int prec, int startup) {
     const long long s = startup ? 0 : c.steps_done + 1;
     double *nbc_cur = c.nbcv + (s % 2) * std::max(c.n_nbc, 1);
     double *nbc_next = c.nbcv + ((s + 1) % 2) * std::max(c.n_nbc, 1);
-    if (prec == PDX_FP32) {
-        if (c.dim == 3) launch_explicit<3, float>(c, c.K32, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
-        else launch_explicit<2, float>(c, c.K32, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
-    } else {
-        if (c.dim == 3) launch_explicit<3, double>(c, c.K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
-        else launch_explicit<2, double>(c, c.K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
-    }
+    launch_explicit_any(c, prec, std::max(c.tune_explicit[prec == PDX_FP32 ? 1 : 0], 0), Un, Unm1, Unp1, nbc_cur,
+                        nbc_next, startup);
     c.step_phase = (p + 1) % 3;
     c.steps_done = s;
 }
@@ -474,9 +522,19 @@ int explicit_prepare(Ctx &c, int prec, std::string &err) {
     return PDX_OK;
 }
 
+int explicit_tune_if_needed(Ctx &c, int prec, std::string &err) {
+    const int pi = prec == PDX_FP32 ? 1 : 0;
+    if (c.tune_explicit[pi] >= 0) return PDX_OK;
+    int rc = tune_explicit(c, pi, err);
+    for (auto &g : c.graph)
+        if (g) { hipGraphExecDestroy(g); g = nullptr; }
+    return rc;
+}
+
 int dev_explicit_steps(Ctx &c, int nsteps, int prec, int use_graph, std::string &err) {
     int e = explicit_prepare(c, prec, err);
     if (e) return e;
+    if ((e = explicit_tune_if_needed(c, prec, err))) return e;
     hipEvent_t e0, e1;
     PDX_HIP(hipEventCreate(&e0));
     PDX_HIP(hipEventCreate(&e1));
@@ -621,20 +679,8 @@ static void ws_free(KrylovWs<V> &w) {
 
 template <int D, typename T, typename V>
 static void spmv_dot(Ctx &c, const T *K, const V *x, V *y, const V *w0, double *p0, const V *w1, double *p1) {
-    if (c.n_short)
-        k_spmv_dot<D, T, V, 8><<<RED_PARTS, 256, 0, c.stream>>>(c.rows_short, c.n_short, c.adj_off, c.adj, K, c.nblk,
-                                                                 x, y, c.eq, w0, p0, w1, p1, 0);
-    else {
-        if (p0) k_fill_s<<<2, 256, 0, c.stream>>>(p0, 0.0, RED_PARTS);
-        if (p1) k_fill_s<<<2, 256, 0, c.stream>>>(p1, 0.0, RED_PARTS);
-    }
-    if (c.n_long)
-        k_spmv_dot<D, T, V, 64><<<RED_PARTS, 256, 0, c.stream>>>(c.rows_long, c.n_long, c.adj_off, c.adj, K, c.nblk,
-                                                                  x, y, c.eq, w0, p0, w1, p1, RED_PARTS);
-    else {
-        if (p0) k_fill_s<<<2, 256, 0, c.stream>>>(p0 + RED_PARTS, 0.0, RED_PARTS);
-        if (p1) k_fill_s<<<2, 256, 0, c.stream>>>(p1 + RED_PARTS, 0.0, RED_PARTS);
-    }
+    k_spmv_dot<D, T, V><<<RED_PARTS, 256, 0, c.stream>>>(c.rows_short, c.n_short, c.rows_long, c.n_long, c.adj_off,
+                                                          c.adj, K, x, y, c.eq, w0, p0, w1, p1);
 }
 
 // one BiCGStab iteration (it = iteration index, parity selects the rho slot)
@@ -675,14 +721,27 @@ static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm
     hipFree(y);
     double rr0 = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
     double rel = std::sqrt(rr0) / bnorm;
-    int it = 0;
+    int it = 0, since_best = 0;
     const int chunk = 10;
+    double best = rel;
     while (rel > tol && it < max_iter) {
         for (int k = 0; k < chunk && it < max_iter; k++, it++) bicg_iter<D, T, V>(c, K, w, red, sc, it);
         PDX_HIP(hipGetLastError());
         double rr = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
-        if (!std::isfinite(rr)) { err = "Krylov breakdown (non-finite residual)"; *iters = it; return PDX_ERR_NOCONV; }
+        if (!std::isfinite(rr)) {
+            std::vector<double> hs(8);
+            hipMemcpy(hs.data(), sc, 8 * sizeof(double), hipMemcpyDeviceToHost);
+            err = "Krylov breakdown (non-finite residual) at iteration " + std::to_string(it) + " after relres " +
+                  std::to_string(rel) + " (best " + std::to_string(best) + "), rho_old " + std::to_string(hs[0]) +
+                  " alpha " + std::to_string(hs[1]) + " omega " + std::to_string(hs[2]) + ", value bytes " +
+                  std::to_string(sizeof(V));
+            *iters = it;
+            return PDX_ERR_NOCONV;
+        }
         rel = std::sqrt(rr) / bnorm;
+        // stagnation (rounding floor of the working precision): hand back to the caller
+        if (rel < 0.95 * best) { best = rel; since_best = 0; }
+        else if (++since_best >= 40) break;
     }
     *iters = it;
     *relres = rel;
@@ -724,7 +783,7 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
     const size_t n = (size_t)c.nn * c.dim;
     if (!c.red) {
         PDX_HIP(hipMalloc((void **)&c.red, sizeof(double) * ((size_t)RED_SLOTS * NPART + 16)));
-        PDX_HIP(hipMemset(c.red, 0, sizeof(double) * ((size_t)RED_SLOTS * NPART + 16)));
+        PDX_HIP(hipMemsetAsync(c.red, 0, sizeof(double) * ((size_t)RED_SLOTS * NPART + 16), c.stream));
     }
     double *part = c.red + (size_t)P_BB * NPART;
     k_dot<double><<<RED_PARTS, 256, 0, c.stream>>>(b_dev, b_dev, n, part);

@@ -171,3 +171,22 @@ def test_newmark_matches_oracle():
     cx = pdx.Context(pdx.Model(mesh))
     u1, v1, a1, st = cx.newmark_run(4, pdx.solve_opts(precision=0, precond=2, tol=1e-12))
     assert rel(u1, u0) <= 1e-8 and rel(v1, v0) <= 1e-7 and rel(a1, a0) <= 1e-6
+
+
+@pytest.fixture(scope="module")
+def crack_case():
+    mesh = W.crack2d(seed=0)
+    mo = O.Model(mesh)
+    K, F = mo.assemble(1)
+    return mesh, mo, K, F
+
+
+@pytest.mark.parametrize("precision,precond", [(0, 1), (0, 2), (1, 1), (1, 2)])
+def test_static_solve_cracked_plate(crack_case, precision, precond):
+    """Edge-cracked plate 501 x 251 (broken bonds across the crack): every
+    precision / preconditioner combination reaches the 1e-8 true residual."""
+    mesh, mo, K, F = crack_case
+    cx = pdx.Context(pdx.Model(mesh))
+    u, st = cx.static_solve(pdx.solve_opts(precision=precision, precond=precond, tol=1e-8, max_iter=100000))
+    assert st["converged"]
+    assert _true_relres(mo, K, F, u) <= 1.2e-8
