@@ -157,12 +157,19 @@ def main():
         smod = pdx.Model(sm)
         scx = pdx.Context(smod)
         solves = {}
-        for name, o in (("plain_fp64_jacobi_bicgstab", pdx.solve_opts(precision=0, precond=1, tol=1e-8, max_iter=100000)),
-                        ("fp32_ir_blockjacobi_bicgstab", pdx.solve_opts(precision=1, precond=2, tol=1e-8, max_iter=100000))):
+        cfgs = (("plain_fp64_jacobi_bicgstab", pdx.solve_opts(precision=0, precond=1, tol=1e-8, max_iter=100000)),
+                ("fp32_ir_blockjacobi_bicgstab", pdx.solve_opts(precision=1, precond=2, tol=1e-8, max_iter=100000)),
+                ("fp64_two_level_bicgstab", pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
+                ("fp32_ir_two_level_bicgstab", pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)))
+        for name, o in cfgs:
             try:
+                # first call builds the coarse space of the two-level kinds; the
+                # second reuses it (repeated solves with one operator)
+                u, st0 = scx.static_solve(o)
                 u, st = scx.static_solve(o)
-                solves[name] = {"seconds": st["seconds"], "iterations": st["iterations"],
-                                "refinements": st["refinements"], "true_relres": st["true_relres"]}
+                solves[name] = {"seconds": st["seconds"], "seconds_first_call": st0["seconds"],
+                                "iterations": st["iterations"], "refinements": st["refinements"],
+                                "true_relres": st["true_relres"]}
             except Exception as e:  # reported, not hidden
                 solves[name] = {"error": str(e)}
             log(name, solves[name])

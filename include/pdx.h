@@ -66,7 +66,8 @@ extern "C" {
 #define PDX_PREC_NONE 0
 #define PDX_PREC_JACOBI 1
 #define PDX_PREC_BLOCK_JACOBI 2 /* node d x d blocks */
-#define PDX_PREC_LEARNED 3      /* graph-network sparse approximate inverse */
+#define PDX_PREC_LEARNED 3      /* two-level, node-block smoother predicted by the graph network */
+#define PDX_PREC_TWO_LEVEL 4    /* two-level hybrid: rigid-body aggregate coarse space + node-block Jacobi */
 
 typedef struct pdx_model pdx_model; /* host-side data model (families, pattern) */
 typedef struct pdx_ctx pdx_ctx;     /* device-resident state on one GPU         */
@@ -111,6 +112,7 @@ typedef struct pdx_solve_opts {
     int max_iter;      /* total inner iterations */
     double inner_tol;  /* FP32 inner relative tolerance (refinement), e.g. 1e-4 */
     int warm_start;    /* use x as the initial guess */
+    int coarse_agg;    /* two-level: aggregate edge in node spacings (0: 16 in 2D, 6 in 3D) */
 } pdx_solve_opts;
 
 typedef struct pdx_solve_stats {
@@ -132,6 +134,10 @@ void pdx_model_destroy(pdx_model *m);
  *             n_adj (node-graph entries), n_presc, nb0, nb1, nb2, max_family */
 int pdx_model_sizes(const pdx_model *m, long long *out);
 int pdx_model_csr(const pdx_model *m, long long *ia, long long *ja);
+/* node graph of the pattern (setCSRIndexes_gloStiffMat pdsolve.cpp:992-1051 before
+ * the DOF expansion): sorted neighbour nodes per node, prescribed DOFs included;
+ * adj_off [n_node+1], adj [sizes[10]]; the order of the node blocks on the device */
+int pdx_model_adjacency(const pdx_model *m, int *adj_off, int *adj);
 /* families: node (0-based), offsets, members (1-based, self first), bond state, cd_v, horizon */
 int pdx_model_families(const pdx_model *m, int *fam_node, int *fam_off, int *fam_mem, int *bond,
                        double *cd_v, double *delta);
@@ -177,6 +183,31 @@ int pdx_set_dt(pdx_ctx *c, double dt);
 int pdx_static_solve(pdx_ctx *c, const pdx_solve_opts *o, double *u, pdx_solve_stats *st);
 int pdx_newmark_run(pdx_ctx *c, int nsteps, const pdx_solve_opts *o, double *u, double *v, double *a,
                     pdx_solve_stats *st);
+
+/* ---- learned smoother of the two-level preconditioner ------------------
+ * No counterpart in the reference (which factorises with PARDISO,
+ * calMatrixOperations.cpp:378-498): a graph network over the node graph of K
+ * predicts a bounded correction of the node-block Jacobi smoother,
+ * M_a = (I + 0.5 tanh(o_a)) (K_aa)^-1, used by PDX_PREC_LEARNED.  Training
+ * minimises ||f - M K f||^2 / ||f||^2 over coarse-corrected Gaussian probes
+ * f = (I - Q K) z, on the GPU. */
+typedef struct pdx_gnn pdx_gnn;
+int pdx_gnn_create(int dim, int device, unsigned long long seed, pdx_gnn **out);
+void pdx_gnn_destroy(pdx_gnn *g);
+long long pdx_gnn_num_params(const pdx_gnn *g);
+int pdx_gnn_get_params(const pdx_gnn *g, float *w);
+int pdx_gnn_set_params(pdx_gnn *g, const float *w); /* also resets the Adam state */
+/* steps Adam steps on n_probes probes of the context's current K; loss: [steps+1] or NULL */
+int pdx_gnn_train(pdx_ctx *c, pdx_gnn *g, int n_probes, unsigned long long probe_seed, int coarse_agg, int steps,
+                  float lr, double *loss);
+/* loss and gradient (flat, pdx_gnn_num_params floats) of the current weights */
+int pdx_gnn_loss_grad(pdx_ctx *c, pdx_gnn *g, int n_probes, unsigned long long probe_seed, int coarse_agg,
+                      double *loss, float *grad);
+/* the network inputs: X [nn*8], c [nblk], A_aa [nn*d*d], B [nn*d*d], f, y [n_probes*nn*d]; any may be NULL */
+int pdx_gnn_inputs(pdx_ctx *c, int n_probes, unsigned long long probe_seed, int coarse_agg, float *X, float *cw,
+                   float *Aaa, float *B, float *f, float *y);
+/* predict the smoother for the context's current K (used by PDX_PREC_LEARNED) */
+int pdx_gnn_apply(pdx_ctx *c, const pdx_gnn *g);
 
 /* PARDISO seam: solve A x = b for a zero-based int64 CSR (n, ia, ja, a). */
 int pdx_solve_csr(int device, long long n, const long long *ia, const long long *ja, const double *a,

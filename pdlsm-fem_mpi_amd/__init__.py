@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(HERE, "libpdx.so")
 PDX_MODE_EXPLICIT, PDX_MODE_STATIC, PDX_MODE_QUASI, PDX_MODE_NEWMARK = 0, 1, 2, 3
 PDX_FP64, PDX_FP32 = 0, 1
 PDX_KRYLOV_BICGSTAB, PDX_KRYLOV_CG = 0, 1
-PDX_PREC_NONE, PDX_PREC_JACOBI, PDX_PREC_BLOCK_JACOBI, PDX_PREC_LEARNED = 0, 1, 2, 3
+PDX_PREC_NONE, PDX_PREC_JACOBI, PDX_PREC_BLOCK_JACOBI, PDX_PREC_LEARNED, PDX_PREC_TWO_LEVEL = 0, 1, 2, 3, 4
 
 _P, _I, _D, _LL = C.c_void_p, C.c_int, C.c_double, C.c_longlong
 
@@ -42,7 +42,7 @@ class MeshDesc(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("krylov", _I), ("precond", _I), ("precision", _I), ("tol", _D), ("max_iter", _I),
-                ("inner_tol", _D), ("warm_start", _I)]
+                ("inner_tol", _D), ("warm_start", _I), ("coarse_agg", _I)]
 
 
 class SolveStats(C.Structure):
@@ -63,6 +63,7 @@ EXPORTS = {
     "pdx_model_destroy": (None, [_P]),
     "pdx_model_sizes": (_I, [_P, _P]),
     "pdx_model_csr": (_I, [_P, _P, _P]),
+    "pdx_model_adjacency": (_I, [_P, _P, _P]),
     "pdx_model_families": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "pdx_model_nodes": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "pdx_model_values": (_I, [_P, _P]),
@@ -82,6 +83,15 @@ EXPORTS = {
     "pdx_newmark_run": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "pdx_solve_csr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
     "pdx_last_timing": (_I, [_P, _P, _P]),
+    "pdx_gnn_create": (_I, [_I, _I, C.c_ulonglong, _P]),
+    "pdx_gnn_destroy": (None, [_P]),
+    "pdx_gnn_num_params": (_LL, [_P]),
+    "pdx_gnn_get_params": (_I, [_P, _P]),
+    "pdx_gnn_set_params": (_I, [_P, _P]),
+    "pdx_gnn_train": (_I, [_P, _P, _I, C.c_ulonglong, _I, _I, C.c_float, _P]),
+    "pdx_gnn_loss_grad": (_I, [_P, _P, _I, C.c_ulonglong, _I, _P, _P]),
+    "pdx_gnn_inputs": (_I, [_P, _I, C.c_ulonglong, _I, _P, _P, _P, _P, _P, _P]),
+    "pdx_gnn_apply": (_I, [_P, _P]),
 }
 
 
@@ -182,6 +192,12 @@ class Model:
             lib().pdx_model_destroy(self.h)
             self.h = None
 
+    def adjacency(self):
+        off = np.zeros(self.nn + 1, np.int32)
+        adj = np.zeros(max(self.nadj, 1), np.int32)
+        _chk(lib().pdx_model_adjacency(self.h, _p(off), _p(adj)))
+        return off, adj[: self.nadj]
+
     def csr(self):
         ia = np.zeros(self.neq + 1, np.int64)
         ja = np.zeros(max(self.nnz, 1), np.int64)
@@ -216,10 +232,10 @@ class Model:
 
 
 def solve_opts(krylov=PDX_KRYLOV_BICGSTAB, precond=PDX_PREC_JACOBI, precision=PDX_FP64, tol=1e-8, max_iter=20000,
-               inner_tol=1e-4, warm_start=False):
+               inner_tol=1e-4, warm_start=False, coarse_agg=0):
     o = SolveOpts()
     o.krylov, o.precond, o.precision, o.tol, o.max_iter = krylov, precond, precision, tol, max_iter
-    o.inner_tol, o.warm_start = inner_tol, int(warm_start)
+    o.inner_tol, o.warm_start, o.coarse_agg = inner_tol, int(warm_start), int(coarse_agg)
     return o
 
 
@@ -318,3 +334,61 @@ def solve_csr(ia, ja, a, b, opts=None, device=0, x0=None):
     st = SolveStats()
     _chk(lib().pdx_solve_csr(device, len(b), _p(ia), _p(ja), _p(a), _p(b), _p(x), C.byref(opts), C.byref(st)))
     return x, st.as_dict()
+
+
+class GNN:
+    """Graph network predicting the learned smoother of PDX_PREC_LEARNED."""
+
+    def __init__(self, dim: int, device: int = 0, seed: int = 0):
+        h = C.c_void_p()
+        _chk(lib().pdx_gnn_create(dim, device, seed, C.byref(h)))
+        self.h, self.dim = h, dim
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pdx_gnn_destroy(self.h)
+            self.h = None
+
+    def num_params(self) -> int:
+        return int(lib().pdx_gnn_num_params(self.h))
+
+    def params(self) -> np.ndarray:
+        w = np.zeros(self.num_params(), np.float32)
+        _chk(lib().pdx_gnn_get_params(self.h, _p(w)))
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.num_params()
+        _chk(lib().pdx_gnn_set_params(self.h, _p(w)))
+
+    def train(self, cx: "Context", steps: int, n_probes: int = 8, probe_seed: int = 0, lr: float = 1e-3,
+              coarse_agg: int = 0) -> np.ndarray:
+        loss = np.zeros(steps + 1)
+        _chk(lib().pdx_gnn_train(cx.h, self.h, n_probes, probe_seed, coarse_agg, steps, lr, _p(loss)))
+        return loss
+
+    def loss_grad(self, cx: "Context", n_probes: int = 4, probe_seed: int = 0, coarse_agg: int = 0):
+        g = np.zeros(self.num_params(), np.float32)
+        loss = C.c_double()
+        _chk(lib().pdx_gnn_loss_grad(cx.h, self.h, n_probes, probe_seed, coarse_agg, C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def apply(self, cx: "Context"):
+        _chk(lib().pdx_gnn_apply(cx.h, self.h))
+
+
+def gnn_inputs(cx: "Context", n_probes: int = 4, probe_seed: int = 0, coarse_agg: int = 0) -> dict:
+    """The exact network inputs the library uses for the context's current K."""
+    m = cx.model
+    nn, dim = m.nn, m.dim
+    adj_off, adj = m.adjacency()
+    nblk = int(adj_off[-1])
+    X = np.zeros((nn, 8), np.float32)
+    cw = np.zeros(nblk, np.float32)
+    Aaa = np.zeros((nn, dim * dim), np.float32)
+    B = np.zeros((nn, dim * dim), np.float32)
+    f = np.zeros((n_probes, nn * dim), np.float32)
+    y = np.zeros((n_probes, nn * dim), np.float32)
+    _chk(lib().pdx_gnn_inputs(cx.h, n_probes, probe_seed, coarse_agg, _p(X), _p(cw), _p(Aaa), _p(B), _p(f), _p(y)))
+    return dict(X=X, cw=cw, Aaa=Aaa, B=B, f=f, y=y, adj_off=adj_off, adj=adj, eq=m.nodes()["eq"][:, :dim])

@@ -857,6 +857,7 @@ int dev_assemble_K(Ctx &c, std::string &err) {
     if (hs & 4) { err = "Area is zero (PD boundary face)"; return PDX_ERR_MESH; }
     if (hs & 8) { err = "family exceeds the LDS capacity"; return PDX_ERR_LIMIT; }
     c.K_valid = true;
+    c.K_gen++;
     c.K32_valid = false;
     return PDX_OK;
 }

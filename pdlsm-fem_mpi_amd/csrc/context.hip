@@ -239,6 +239,9 @@ void ctx_free(Ctx *c) {
     if (c->Mk) hipFree(c->Mk);
     if (c->Ke_stage) hipFree(c->Ke_stage);
     if (c->red) hipFree(c->red);
+    if (c->learned_Minv) hipFree(c->learned_Minv);
+    free_two_level(*c);
+    free_gnn_work(*c);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -301,6 +304,14 @@ int pdx_model_sizes(const pdx_model *pm, long long *out) {
                        (long long)m.pde.size(), (long long)m.fe.size(), (long long)m.adj.size(), npresc,
                        m.nb[0], m.nb[1], m.nb[2], m.max_family};
     std::memcpy(out, v, sizeof(v));
+    return PDX_OK;
+}
+
+int pdx_model_adjacency(const pdx_model *pm, int *adj_off, int *adj) {
+    if (!pm) return fail(PDX_ERR_ARG, "null argument");
+    const Model &m = pm->m;
+    if (adj_off) std::copy(m.adj_off.begin(), m.adj_off.end(), adj_off);
+    if (adj) std::copy(m.adj.begin(), m.adj.end(), adj);
     return PDX_OK;
 }
 

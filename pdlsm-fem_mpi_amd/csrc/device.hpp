@@ -74,6 +74,7 @@ struct Ctx {
     float *K32 = nullptr;  // [dd*nblk]
     double *Mk = nullptr;  // consistent mass on K blocks (Newmark)
     bool K_valid = false, K32_valid = false, Mk_valid = false;
+    long long K_gen = 0;  // incremented by every assembly of K
     double *mlump = nullptr;  // [nn*d] lumped mass (1 for prescribed)
     bool mlump_valid = false;
 
@@ -92,6 +93,10 @@ struct Ctx {
     double *red = nullptr;      // partial sums [RED_SLOTS * RED_PARTS]
     double *red_host = nullptr; // pinned mirror
     double last_total_ms = 0, last_step_ms = 0;
+    void *two_level = nullptr;       // coarse space of the last two-level solve (krylov.hip)
+    double *learned_Minv = nullptr;  // [nn*dd] node-block smoother predicted by the graph network
+    long long learned_gen = -1;      // K_gen the smoother was predicted for
+    void *gnn_work = nullptr;        // graph-network inputs and activations (learned.hip)
 
     // helpers
     std::vector<void *> owned;
@@ -113,6 +118,9 @@ int dev_spmv(Ctx &c, const double *x, double *y, int prec, bool mask_free);
 int dev_fill(Ctx &c, double *p, double v, size_t n);
 
 int dev_spmv_K(Ctx &c, const double *K64, const double *x, double *y);
+void free_two_level(Ctx &c);
+void free_gnn_work(Ctx &c);
+double two_level_setup_ms(Ctx &c);
 int set_error(int rc, const std::string &msg);
 
 }  // namespace pdx

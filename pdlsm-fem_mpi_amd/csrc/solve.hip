@@ -1,5 +1,5 @@
-// Explicit central-difference steps, power iteration and Krylov solvers on
-// the node-block matrix, for CDNA4 (gfx950).
+// Explicit central-difference steps and the power iteration for the
+// critical time step, on the node-block matrix, for CDNA4 (gfx950).
 //
 // Explicit step (pdsolve.cpp:5842-6090, timeIntegration_LM :5113-5132): one
 // fused kernel per row class computes y = K u_n for a node's rows and updates
@@ -7,10 +7,6 @@
 //   v_{n+2} = v_{n+1} + dt * velocity                  (prescribed DOFs)
 // with three rotating state vectors, so a step is a pure function of device
 // state and a run of steps can be replayed from a hipGraph.
-//
-// BiCGStab keeps every scalar on the device: reducing kernels write per-block
-// partial sums, consumer kernels re-reduce them in a fixed order (bitwise
-// deterministic), so no host synchronisation is needed inside the iteration.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,6 +14,7 @@
 #include <cmath>
 
 #include "device.hpp"
+#include "reduce.hpp"
 #include "rowblock.hpp"
 
 namespace pdx {
@@ -73,279 +70,9 @@ __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int
     }
 }
 
-// ------------------------------------------------------------------------
-// reductions and vector kernels (V = double or float)
-// ------------------------------------------------------------------------
-
-// deterministic block reduction of one value per thread (256 threads)
-__device__ inline double block_sum(double v, double *sm) {
-    const int t = threadIdx.x;
-    for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
-    if ((t & 63) == 0) sm[t >> 6] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (t == 0) {
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += sm[w];
-        sm[8] = r;
-    }
-    __syncthreads();
-    r = sm[8];
-    __syncthreads();
-    return r;
-}
-
-// every block re-reduces n partial sums in the same fixed order
-__device__ inline double sum_partials(const double *p, int n, double *sm) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
-    return block_sum(v, sm);
-}
-
-constexpr int NPART = 2 * RED_PARTS;  // short-row + long-row SpMV partials
-
-template <typename V>
-__global__ void __launch_bounds__(256) k_dot(const V *a, const V *b, size_t n, double *part) {
-    __shared__ double sm[16];
-    double v = 0.0;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        v += double(a[i]) * double(b[i]);
-    v = block_sum(v, sm);
-    if (threadIdx.x == 0) part[blockIdx.x] = v;
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        for (int i = gridDim.x; i < NPART; i++) part[i] = 0.0;
-}
-
-// SpMV with fused dot partials: y = mask(K x); part0 += (w0, y); part1 += (y, y) when w1 set
-template <int D, typename T, typename V, int G>
-__device__ inline void spmv_dot_rows(const int *rows, int nrows, int first_group, int groups, const int *adj_off,
-                                     const int *adj, const T *K, const V *x, V *y, const int *eq, const V *w0,
-                                     const V *w1, double &d0, double &d1) {
-    const int lane = threadIdx.x % G;
-    for (int gid = first_group; gid < nrows; gid += groups) {
-        const int r = rows[gid];
-        T acc[D];
-        row_matvec<D, T, V, G, false>(r, lane, adj_off, adj, K, x, acc);
-        if (lane < D) {
-            const T v = pick<D, T>(acc, lane);
-            const size_t o = (size_t)r * D + lane;
-            V yv = eq[r * 3 + lane] >= 0 ? V(v) : V(0);
-            y[o] = yv;
-            if (w0) d0 += double(w0[o]) * double(yv);
-            if (w1) d1 += double(w1[o]) * double(yv);
-        }
-    }
-}
-
-// SpMV with fused dot partials over both row classes (grid-stride, RED_PARTS
-// blocks): y = mask(K x); part0 = (w0, y); part1 = (w1, y)
-template <int D, typename T, typename V>
-__global__ void __launch_bounds__(256) k_spmv_dot(const int *rows_s, int ns, const int *rows_l, int nl,
-                                                  const int *adj_off, const int *adj, const T *K, const V *x, V *y,
-                                                  const int *eq, const V *w0, double *part0, const V *w1,
-                                                  double *part1) {
-    __shared__ double sm[16];
-    double d0 = 0.0, d1 = 0.0;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-    spmv_dot_rows<D, T, V, 8>(rows_s, ns, t / 8, nt / 8, adj_off, adj, K, x, y, eq, w0, w1, d0, d1);
-    spmv_dot_rows<D, T, V, 64>(rows_l, nl, t / 64, nt / 64, adj_off, adj, K, x, y, eq, w0, w1, d0, d1);
-    if (part0) {
-        d0 = block_sum(d0, sm);
-        if (threadIdx.x == 0) part0[blockIdx.x] = d0;
-    }
-    if (part1) {
-        d1 = block_sum(d1, sm);
-        if (threadIdx.x == 0) part1[blockIdx.x] = d1;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        for (int i = gridDim.x; i < NPART; i++) {
-            if (part0) part0[i] = 0.0;
-            if (part1) part1[i] = 0.0;
-        }
-}
-
-// apply the node-diagonal block preconditioner z = Minv r (Minv: [nn][D*D])
-template <int D, typename V>
-__device__ inline void apply_blk(const V *Minv, const V *r, V *z, int a) {
-    V rv[D];
-#pragma unroll
-    for (int j = 0; j < D; j++) rv[j] = r[(size_t)a * D + j];
-#pragma unroll
-    for (int i = 0; i < D; i++) {
-        V s = V(0);
-#pragma unroll
-        for (int j = 0; j < D; j++) s += Minv[(size_t)a * D * D + i * D + j] * rv[j];
-        z[(size_t)a * D + i] = s;
-    }
-}
-
-// scalar slots in the device array sc[]
-enum { SC_RHO_OLD = 0, SC_ALPHA = 1, SC_OMEGA = 2 };
-// partial-sum slots (each NPART doubles)
-enum { P_RHO0 = 0, P_RHO1 = 1, P_RV = 2, P_TS = 3, P_TT = 4, P_RR = 5, P_BB = 6, P_TMP = 7 };
-
-template <int D, typename V>
-__global__ void __launch_bounds__(256) k_bicg_p(int nn, const V *r, V *p, const V *v, V *ph, const V *Minv,
-                                                const double *rho_part, double *sc, int first) {
-    __shared__ double sm[16];
-    const double rho = sum_partials(rho_part, NPART, sm);
-    // a vanishing rho_old or omega (breakdown, or exact convergence in low
-    // precision) restarts the search direction with p = r
-    double beta = 0.0, omg = 0.0;
-    if (!first && sc[SC_RHO_OLD] != 0.0 && sc[SC_OMEGA] != 0.0) {
-        beta = (rho / sc[SC_RHO_OLD]) * (sc[SC_ALPHA] / sc[SC_OMEGA]);
-        omg = sc[SC_OMEGA];
-    }
-    if (!isfinite(beta)) beta = 0.0;
-    const V bv = V(beta), om = V(omg);
-    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < nn; a += gridDim.x * blockDim.x) {
-#pragma unroll
-        for (int i = 0; i < D; i++) {
-            const size_t o = (size_t)a * D + i;
-            p[o] = first ? r[o] : V(r[o] + bv * (p[o] - om * v[o]));
-        }
-        apply_blk<D, V>(Minv, p, ph, a);
-    }
-}
-
-template <int D, typename V>
-__global__ void __launch_bounds__(256) k_bicg_s(int nn, const V *r, const V *v, V *s, V *sh, const V *Minv,
-                                                const double *rho_part, const double *rv_part, double *sc) {
-    __shared__ double sm[16];
-    const double rho = sum_partials(rho_part, NPART, sm);
-    const double rv = sum_partials(rv_part, NPART, sm);
-    double alpha = rv != 0.0 ? rho / rv : 0.0;
-    if (!isfinite(alpha)) alpha = 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_ALPHA] = alpha;
-    const V al = V(alpha);
-    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < nn; a += gridDim.x * blockDim.x) {
-#pragma unroll
-        for (int i = 0; i < D; i++) {
-            const size_t o = (size_t)a * D + i;
-            s[o] = r[o] - al * v[o];
-        }
-        apply_blk<D, V>(Minv, s, sh, a);
-    }
-}
-
-template <int D, typename V>
-__global__ void __launch_bounds__(256) k_bicg_x(int nn, V *x, V *r, const V *rh, const V *ph, const V *sh,
-                                                const V *s, const V *t, const double *rho_part,
-                                                const double *ts_part, const double *tt_part, double *sc,
-                                                double *rho_next, double *rr_part) {
-    __shared__ double sm[16];
-    const double rho = sum_partials(rho_part, NPART, sm);
-    const double ts = sum_partials(ts_part, NPART, sm);
-    const double tt = sum_partials(tt_part, NPART, sm);
-    double omega = tt > 0 ? ts / tt : 0.0;
-    if (!isfinite(omega)) omega = 0.0;
-    const double alpha = sc[SC_ALPHA];
-    const V al = V(alpha), om = V(omega);
-    double d0 = 0.0, d1 = 0.0;
-    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < nn; a += gridDim.x * blockDim.x) {
-#pragma unroll
-        for (int i = 0; i < D; i++) {
-            const size_t o = (size_t)a * D + i;
-            x[o] = x[o] + al * ph[o] + om * sh[o];
-            const V rn = s[o] - om * t[o];
-            r[o] = rn;
-            d0 += double(rh[o]) * double(rn);
-            d1 += double(rn) * double(rn);
-        }
-    }
-    d0 = block_sum(d0, sm);
-    d1 = block_sum(d1, sm);
-    if (threadIdx.x == 0) {
-        rho_next[blockIdx.x] = d0;
-        rr_part[blockIdx.x] = d1;
-        if (blockIdx.x == 0) {
-            for (int i = gridDim.x; i < NPART; i++) { rho_next[i] = 0.0; rr_part[i] = 0.0; }
-            sc[SC_OMEGA] = omega;
-            sc[SC_RHO_OLD] = rho;
-        }
-    }
-}
-
-// r = b - y (masked by y already), plus partial (r, r); optional copy to rh
-template <typename V, typename B>
-__global__ void __launch_bounds__(256) k_resid(size_t n, const B *b, const double *y, V *r, V *rh, double *part) {
-    __shared__ double sm[16];
-    double v = 0.0;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double ri = double(b[i]) - y[i];
-        r[i] = V(ri);
-        if (rh) rh[i] = V(ri);
-        v += ri * ri;
-    }
-    v = block_sum(v, sm);
-    if (threadIdx.x == 0) part[blockIdx.x] = v;
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        for (int i = gridDim.x; i < NPART; i++) part[i] = 0.0;
-}
-
-template <typename A, typename B>
-__global__ void k_copy(size_t n, const A *a, B *b) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        b[i] = B(a[i]);
-}
-
 __global__ void k_axpy_d(size_t n, double a, const double *x, double *y) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         y[i] = y[i] + a * x[i];
-}
-
-__global__ void k_add_f2d(size_t n, const float *d, double *x) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        x[i] = x[i] + double(d[i]);
-}
-
-// block-Jacobi / Jacobi inverse from the diagonal blocks; prescribed rows/cols removed
-template <int D, typename V>
-__global__ void k_blockjacobi(int nn, const int *adj_off, const int *adj, const double *K, long long nblk,
-                              const int *eq, int diag_only, V *Minv, int *status) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nn) return;
-    int lo = adj_off[a], hi = adj_off[a + 1] - 1, b = -1;
-    while (lo <= hi) {
-        int mid = (lo + hi) >> 1;
-        if (adj[mid] == a) { b = mid; break; }
-        if (adj[mid] < a) lo = mid + 1; else hi = mid - 1;
-    }
-    double M[D * D], I[D * D];
-    int fr[D];
-    for (int i = 0; i < D; i++) fr[i] = eq[a * 3 + i] >= 0;
-    for (int i = 0; i < D; i++)
-        for (int j = 0; j < D; j++) {
-            double v = (b >= 0) ? K[(size_t)b * D * D + i * D + j] : 0.0;
-            if (diag_only && i != j) v = 0.0;
-            if (!fr[i] || !fr[j]) v = (i == j) ? 1.0 : 0.0;
-            M[i * D + j] = v;
-            I[i * D + j] = i == j ? 1.0 : 0.0;
-        }
-    // Gauss-Jordan with partial pivoting on a D x D block
-    for (int k = 0; k < D; k++) {
-        int p = k;
-        for (int i = k + 1; i < D; i++)
-            if (fabs(M[i * D + k]) > fabs(M[p * D + k])) p = i;
-        if (M[p * D + k] == 0.0) { atomicOr(status, 1); return; }
-        for (int j = 0; j < D; j++) {
-            double t = M[k * D + j]; M[k * D + j] = M[p * D + j]; M[p * D + j] = t;
-            t = I[k * D + j]; I[k * D + j] = I[p * D + j]; I[p * D + j] = t;
-        }
-        const double piv = M[k * D + k];
-        for (int j = 0; j < D; j++) { M[k * D + j] /= piv; I[k * D + j] /= piv; }
-        for (int i = 0; i < D; i++) {
-            if (i == k) continue;
-            const double f = M[i * D + k];
-            for (int j = 0; j < D; j++) { M[i * D + j] -= f * M[k * D + j]; I[i * D + j] -= f * I[k * D + j]; }
-        }
-    }
-    for (int i = 0; i < D; i++)
-        for (int j = 0; j < D; j++) Minv[(size_t)a * D * D + i * D + j] = (fr[i] && fr[j]) ? V(I[i * D + j]) : V(0);
-}
-
-__global__ void k_fill_s(double *p, double v, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        p[i] = v;
 }
 
 __global__ void k_to_f32(size_t n, const double *a, float *b) {
@@ -362,22 +89,7 @@ __global__ void k_scale_mass(size_t n, const double *y, const double *m, const i
     }
 }
 
-// ------------------------------------------------------------------------
-// host side
-// ------------------------------------------------------------------------
-static double host_sum(Ctx &c, const double *part, int n, std::string &err) {
-    std::vector<double> h(n);
-    if (hipMemcpyAsync(h.data(), part, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
-        hipStreamSynchronize(c.stream) != hipSuccess) {
-        err = "reduction copy failed";
-        return NAN;
-    }
-    double s = 0.0;
-    for (int i = 0; i < n; i++) s += h[i];
-    return s;
-}
-
-static int ensure_K32(Ctx &c, std::string &err) {
+int ensure_K32(Ctx &c, std::string &err) {
     if (!c.K_valid) { err = "K not assembled"; return PDX_ERR_ARG; }
     if (c.K32_valid) return PDX_OK;
     if (!c.K32) PDX_HIP(hipMalloc((void **)&c.K32, sizeof(float) * c.dd * c.nblk));
@@ -649,208 +361,6 @@ int dev_max_eig(Ctx &c, int iters, double *lmax, std::string &err) {
     return PDX_OK;
 }
 
-// ------------------------------------------------------------------------
-// BiCGStab on the masked node-block operator (right block-Jacobi preconditioning)
-// ------------------------------------------------------------------------
-template <typename V>
-struct KrylovWs {
-    V *x = nullptr, *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *s = nullptr, *t = nullptr,
-      *ph = nullptr, *sh = nullptr, *Minv = nullptr;
-};
-
-template <typename V>
-static int ws_alloc(Ctx &c, KrylovWs<V> &w, std::string &err) {
-    const size_t n = (size_t)c.nn * c.dim;
-    V **ptrs[] = {&w.x, &w.r, &w.rh, &w.p, &w.v, &w.s, &w.t, &w.ph, &w.sh};
-    for (auto pp : ptrs) {
-        PDX_HIP(hipMalloc((void **)pp, n * sizeof(V)));
-        PDX_HIP(hipMemsetAsync(*pp, 0, n * sizeof(V), c.stream));
-    }
-    PDX_HIP(hipMalloc((void **)&w.Minv, (size_t)c.nn * c.dd * sizeof(V)));
-    return PDX_OK;
-}
-
-template <typename V>
-static void ws_free(KrylovWs<V> &w) {
-    V *ptrs[] = {w.x, w.r, w.rh, w.p, w.v, w.s, w.t, w.ph, w.sh, w.Minv};
-    for (auto p : ptrs)
-        if (p) hipFree(p);
-}
-
-template <int D, typename T, typename V>
-static void spmv_dot(Ctx &c, const T *K, const V *x, V *y, const V *w0, double *p0, const V *w1, double *p1) {
-    k_spmv_dot<D, T, V><<<RED_PARTS, 256, 0, c.stream>>>(c.rows_short, c.n_short, c.rows_long, c.n_long, c.adj_off,
-                                                          c.adj, K, x, y, c.eq, w0, p0, w1, p1);
-}
-
-// one BiCGStab iteration (it = iteration index, parity selects the rho slot)
-template <int D, typename T, typename V>
-static void bicg_iter(Ctx &c, const T *K, KrylovWs<V> &w, double *red, double *sc, int it) {
-    double *rho = red + (size_t)((it & 1) ? P_RHO1 : P_RHO0) * NPART;
-    double *rho_next = red + (size_t)((it & 1) ? P_RHO0 : P_RHO1) * NPART;
-    double *rv = red + (size_t)P_RV * NPART, *ts = red + (size_t)P_TS * NPART, *tt = red + (size_t)P_TT * NPART,
-           *rr = red + (size_t)P_RR * NPART;
-    k_bicg_p<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.r, w.p, w.v, w.ph, w.Minv, rho, sc, it == 0);
-    spmv_dot<D, T, V>(c, K, w.ph, w.v, w.rh, rv, (const V *)nullptr, nullptr);
-    k_bicg_s<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.r, w.v, w.s, w.sh, w.Minv, rho, rv, sc);
-    spmv_dot<D, T, V>(c, K, w.sh, w.t, w.s, ts, w.t, tt);
-    k_bicg_x<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.x, w.r, w.rh, w.ph, w.sh, w.s, w.t, rho, ts, tt, sc,
-                                                    rho_next, rr);
-}
-
-// Solve K_ff x = b (b, x: full DOF vectors, prescribed entries zero) with
-// preconditioned BiCGStab in precision V/T to relative residual tol (recurrence
-// residual), starting from x (device, type V).  Returns iterations.
-template <int D, typename T, typename V>
-static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, double tol, int max_iter,
-                    int *iters, double *relres, std::string &err) {
-    const size_t n = (size_t)c.nn * c.dim;
-    double *red = c.red, *sc = c.red + (size_t)RED_SLOTS * NPART;
-    double *y;
-    PDX_HIP(hipMalloc((void **)&y, n * sizeof(double)));
-    // r = b - K x ; rh = r
-    {
-        V *tmp = w.t;
-        spmv_dot<D, T, V>(c, K, w.x, tmp, (const V *)nullptr, nullptr, (const V *)nullptr, nullptr);
-        k_copy<V, double><<<2048, 256, 0, c.stream>>>(n, tmp, y);
-        k_resid<V, V><<<RED_PARTS, 256, 0, c.stream>>>(n, b, y, w.r, w.rh, red + (size_t)P_RR * NPART);
-        k_dot<V><<<RED_PARTS, 256, 0, c.stream>>>(w.rh, w.r, n, red + (size_t)P_RHO0 * NPART);
-        PDX_HIP(hipMemsetAsync(w.p, 0, n * sizeof(V), c.stream));
-        PDX_HIP(hipMemsetAsync(w.v, 0, n * sizeof(V), c.stream));
-    }
-    hipFree(y);
-    double rr0 = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
-    double rel = std::sqrt(rr0) / bnorm;
-    int it = 0, since_best = 0;
-    const int chunk = 10;
-    double best = rel;
-    while (rel > tol && it < max_iter) {
-        for (int k = 0; k < chunk && it < max_iter; k++, it++) bicg_iter<D, T, V>(c, K, w, red, sc, it);
-        PDX_HIP(hipGetLastError());
-        double rr = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
-        if (!std::isfinite(rr)) {
-            std::vector<double> hs(8);
-            hipMemcpy(hs.data(), sc, 8 * sizeof(double), hipMemcpyDeviceToHost);
-            err = "Krylov breakdown (non-finite residual) at iteration " + std::to_string(it) + " after relres " +
-                  std::to_string(rel) + " (best " + std::to_string(best) + "), rho_old " + std::to_string(hs[0]) +
-                  " alpha " + std::to_string(hs[1]) + " omega " + std::to_string(hs[2]) + ", value bytes " +
-                  std::to_string(sizeof(V));
-            *iters = it;
-            return PDX_ERR_NOCONV;
-        }
-        rel = std::sqrt(rr) / bnorm;
-        // stagnation (rounding floor of the working precision): hand back to the caller
-        if (rel < 0.95 * best) { best = rel; since_best = 0; }
-        else if (++since_best >= 40) break;
-    }
-    *iters = it;
-    *relres = rel;
-    return PDX_OK;
-}
-
-template <int D, typename V>
-static int build_precond(Ctx &c, const double *K64, int kind, V *Minv, std::string &err) {
-    int *status;
-    PDX_HIP(hipMalloc((void **)&status, sizeof(int)));
-    PDX_HIP(hipMemsetAsync(status, 0, sizeof(int), c.stream));
-    if (kind == PDX_PREC_NONE) {
-        std::vector<V> h((size_t)c.nn * c.dd, V(0));
-        std::vector<int> eqh((size_t)c.nn * 3);
-        PDX_HIP(hipMemcpy(eqh.data(), c.eq, eqh.size() * sizeof(int), hipMemcpyDeviceToHost));
-        for (int a = 0; a < c.nn; a++)
-            for (int i = 0; i < D; i++)
-                if (eqh[(size_t)a * 3 + i] >= 0) h[(size_t)a * c.dd + i * D + i] = V(1);
-        PDX_HIP(hipMemcpyAsync(Minv, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice, c.stream));
-        PDX_HIP(hipStreamSynchronize(c.stream));
-    } else {
-        k_blockjacobi<D, V><<<(c.nn + 255) / 256, 256, 0, c.stream>>>(c.nn, c.adj_off, c.adj, K64, c.nblk, c.eq,
-                                                                      kind == PDX_PREC_JACOBI, Minv, status);
-    }
-    int hs = 0;
-    PDX_HIP(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    PDX_HIP(hipStreamSynchronize(c.stream));
-    hipFree(status);
-    if (hs) { err = "singular diagonal block in the preconditioner"; return PDX_ERR_SINGULAR; }
-    return PDX_OK;
-}
-
-// Solve K_ff x = b in FP64 (plain) or FP32 inner + FP64 refinement.
-// b_dev: full DOF vector (double, prescribed zero).  x_dev: in/out (double).
-template <int D>
-static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
-                        double *x_dev, pdx_solve_stats &st, std::string &err) {
-    auto t0 = std::chrono::steady_clock::now();
-    const size_t n = (size_t)c.nn * c.dim;
-    if (!c.red) {
-        PDX_HIP(hipMalloc((void **)&c.red, sizeof(double) * ((size_t)RED_SLOTS * NPART + 16)));
-        PDX_HIP(hipMemsetAsync(c.red, 0, sizeof(double) * ((size_t)RED_SLOTS * NPART + 16), c.stream));
-    }
-    double *part = c.red + (size_t)P_BB * NPART;
-    k_dot<double><<<RED_PARTS, 256, 0, c.stream>>>(b_dev, b_dev, n, part);
-    double bnorm = std::sqrt(host_sum(c, part, NPART, err));
-    if (bnorm == 0.0) {
-        PDX_HIP(hipMemsetAsync(x_dev, 0, n * 8, c.stream));
-        st.true_relres = 0; st.converged = 1; st.iterations = 0; st.refinements = 0;
-        return PDX_OK;
-    }
-    if (!o.warm_start) PDX_HIP(hipMemsetAsync(x_dev, 0, n * 8, c.stream));
-    int total = 0, refine = 0, rc = PDX_OK;
-    double rel = 1.0;
-    double *y;
-    PDX_HIP(hipMalloc((void **)&y, n * 8));
-    if (o.precision == PDX_FP64) {
-        KrylovWs<double> w;
-        if ((rc = ws_alloc(c, w, err)) || (rc = build_precond<D, double>(c, K64, o.precond, w.Minv, err))) { ws_free(w); hipFree(y); return rc; }
-        PDX_HIP(hipMemcpyAsync(w.x, x_dev, n * 8, hipMemcpyDeviceToDevice, c.stream));
-        for (int restart = 0; restart < 5; restart++) {
-            int it = 0;
-            double rr;
-            rc = bicgstab<D, double, double>(c, K64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total, &it, &rr, err);
-            total += it;
-            if (rc) break;
-            // true residual
-            spmv_dot<D, double, double>(c, K64, w.x, y, (const double *)nullptr, nullptr, (const double *)nullptr, nullptr);
-            k_resid<double, double><<<RED_PARTS, 256, 0, c.stream>>>(n, b_dev, y, w.t, nullptr, part);
-            rel = std::sqrt(host_sum(c, part, NPART, err)) / bnorm;
-            if (rel <= o.tol || total >= o.max_iter) break;
-        }
-        PDX_HIP(hipMemcpyAsync(x_dev, w.x, n * 8, hipMemcpyDeviceToDevice, c.stream));
-        ws_free(w);
-    } else {
-        if (!K32) { err = "FP32 matrix missing"; hipFree(y); return PDX_ERR_ARG; }
-        KrylovWs<float> w;
-        float *b32;
-        PDX_HIP(hipMalloc((void **)&b32, n * 4));
-        if ((rc = ws_alloc(c, w, err)) || (rc = build_precond<D, float>(c, K64, o.precond, w.Minv, err))) { ws_free(w); hipFree(y); hipFree(b32); return rc; }
-        const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-4;
-        for (refine = 0; refine < 40; refine++) {
-            // FP64 residual r = b - K x
-            spmv_dot<D, double, double>(c, K64, x_dev, y, (const double *)nullptr, nullptr, (const double *)nullptr, nullptr);
-            k_resid<float, double><<<RED_PARTS, 256, 0, c.stream>>>(n, b_dev, y, b32, nullptr, part);
-            rel = std::sqrt(host_sum(c, part, NPART, err)) / bnorm;
-            if (rel <= o.tol || total >= o.max_iter) break;
-            PDX_HIP(hipMemsetAsync(w.x, 0, n * 4, c.stream));
-            int it = 0;
-            double rr;
-            rc = bicgstab<D, float, float>(c, K32, w, b32, rel * bnorm, inner, o.max_iter - total, &it, &rr, err);
-            total += it;
-            if (rc) break;
-            k_add_f2d<<<2048, 256, 0, c.stream>>>(n, w.x, x_dev);
-        }
-        ws_free(w);
-        hipFree(b32);
-    }
-    hipFree(y);
-    PDX_HIP(hipStreamSynchronize(c.stream));
-    st.iterations = total;
-    st.refinements = refine;
-    st.true_relres = rel;
-    st.converged = rel <= o.tol;
-    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (rc) return rc;
-    if (!st.converged) { err = "Krylov solver did not reach the tolerance"; return PDX_ERR_NOCONV; }
-    return PDX_OK;
-}
 
 int to_f32(Ctx &c, const double *K64, float *&K32, std::string &err) {
     size_t n = (size_t)c.dd * c.nblk;
@@ -860,21 +370,5 @@ int to_f32(Ctx &c, const double *K64, float *&K32, std::string &err) {
     return PDX_OK;
 }
 
-// solve with an explicit node-block matrix (K, K_eff or a 1x1-block CSR)
-int dev_solve_mat(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
-                  double *x_dev, pdx_solve_stats &st, std::string &err) {
-    if (o.krylov != PDX_KRYLOV_BICGSTAB) { err = "only BiCGStab is available on the node-block operator"; return PDX_ERR_ARG; }
-    if (c.dim == 3) return solve_masked<3>(c, K64, K32, o, b_dev, x_dev, st, err);
-    if (c.dim == 2) return solve_masked<2>(c, K64, K32, o, b_dev, x_dev, st, err);
-    return solve_masked<1>(c, K64, K32, o, b_dev, x_dev, st, err);
-}
-
-int dev_solve(Ctx &c, const pdx_solve_opts &o, const double *b_dev, double *x_dev, pdx_solve_stats &st,
-              std::string &err) {
-    if (!c.K_valid) { err = "K not assembled"; return PDX_ERR_ARG; }
-    int rc;
-    if (o.precision == PDX_FP32 && (rc = ensure_K32(c, err))) return rc;
-    return dev_solve_mat(c, c.K, o.precision == PDX_FP32 ? c.K32 : nullptr, o, b_dev, x_dev, st, err);
-}
 
 }  // namespace pdx

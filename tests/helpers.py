@@ -28,6 +28,12 @@ def load_oracle():
     return oracle
 
 
+def load_gnn_oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import gnn_oracle
+    return gnn_oracle
+
+
 def workloads():
     sys.path.insert(0, PKG_DIR)
     import workloads as W

@@ -130,7 +130,7 @@ def _true_relres(mo, K, F, u):
     return np.linalg.norm(F - A @ u) / np.linalg.norm(F)
 
 
-@pytest.mark.parametrize("precision,precond", [(0, 1), (0, 2), (1, 1), (1, 2)])
+@pytest.mark.parametrize("precision,precond", [(0, 1), (0, 2), (1, 1), (1, 2), (0, 4), (1, 4)])
 def test_static_solve_plate2d(precision, precond):
     mesh = W.plate2d()
     mo, mp, cx = _pair(mesh)
@@ -162,14 +162,15 @@ def test_csr_seam_solves_oracle_system(precision):
     assert _true_relres(mo, K, F, x) <= 1.2e-8
 
 
-def test_newmark_matches_oracle():
+@pytest.mark.parametrize("precond", [2, 4])
+def test_newmark_matches_oracle(precond):
     mesh = W.plate2d(nx=41, ny=21, dx=1e-3, pd_x=(0.015, 0.025))
     mesh["ebc"][2].update(val=0.0, vel=1e-2, vary=True)
     mesh["dt"] = 1e-6
     mo = O.Model(mesh)
     u0, v0, a0 = O.newmark_run(mo, 4)
     cx = pdx.Context(pdx.Model(mesh))
-    u1, v1, a1, st = cx.newmark_run(4, pdx.solve_opts(precision=0, precond=2, tol=1e-12))
+    u1, v1, a1, st = cx.newmark_run(4, pdx.solve_opts(precision=0, precond=precond, tol=1e-12, coarse_agg=6))
     assert rel(u1, u0) <= 1e-8 and rel(v1, v0) <= 1e-7 and rel(a1, a0) <= 1e-6
 
 
@@ -181,12 +182,55 @@ def crack_case():
     return mesh, mo, K, F
 
 
-@pytest.mark.parametrize("precision,precond", [(0, 1), (0, 2), (1, 1), (1, 2)])
+@pytest.mark.parametrize("precision,precond", [(0, 1), (0, 2), (0, 4), (1, 4)])
 def test_static_solve_cracked_plate(crack_case, precision, precond):
-    """Edge-cracked plate 501 x 251 (broken bonds across the crack): every
-    precision / preconditioner combination reaches the 1e-8 true residual."""
+    """Edge-cracked plate 501 x 251 (broken bonds across the crack): FP64 with
+    (block) Jacobi, and FP64 / FP32-refined with the two-level preconditioner,
+    reach the 1e-8 true residual."""
     mesh, mo, K, F = crack_case
     cx = pdx.Context(pdx.Model(mesh))
     u, st = cx.static_solve(pdx.solve_opts(precision=precision, precond=precond, tol=1e-8, max_iter=100000))
     assert st["converged"]
     assert _true_relres(mo, K, F, u) <= 1.2e-8
+
+
+@pytest.mark.parametrize("precond", [1, 2])
+def test_fp32_weak_preconditioner_ends_cleanly(crack_case, precond):
+    """FP32 inner iterations with (block) Jacobi cannot resolve the cracked plate
+    (condition number ~1e7, near 1/eps of FP32): the refinement must either
+    converge or stop with PDX_ERR_NOCONV and a finite residual, never diverge."""
+    mesh, mo, K, F = crack_case
+    cx = pdx.Context(pdx.Model(mesh))
+    try:
+        u, st = cx.static_solve(pdx.solve_opts(precision=1, precond=precond, tol=1e-8, max_iter=20000))
+    except pdx.PdxError as e:
+        assert "did not reach the tolerance" in str(e), str(e)
+        return
+    assert _true_relres(mo, K, F, u) <= 1.2e-8
+
+
+@pytest.mark.parametrize("name", ["crack2d", "box3d", "penny3d"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_two_level_solves(name, precision):
+    """Two-level hybrid preconditioner (rigid-body aggregates + block Jacobi) on
+    2D and 3D meshes with PD regions and cracks: 1e-8 true residual."""
+    mesh = CASES[name]()
+    for s in mesh["nbc"]:  # the penny case ramps its traction from zero: give it a static load
+        s["val"] = 5e6
+    mo, mp, cx = _pair(mesh)
+    K, F = mo.assemble(1)
+    assert np.linalg.norm(F) > 0
+    u, st = cx.static_solve(pdx.solve_opts(precision=precision, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8,
+                                           coarse_agg=4 if mesh["dim"] == 3 else 8))
+    assert st["converged"]
+    assert _true_relres(mo, K, F, u) <= 1.2e-8
+
+
+def test_two_level_cuts_iterations(crack_case):
+    """On the full cracked plate the coarse correction cuts BiCGStab iterations
+    at least tenfold against block Jacobi (FP64 both)."""
+    mesh, mo, K, F = crack_case
+    cx = pdx.Context(pdx.Model(mesh))
+    _, s_bj = cx.static_solve(pdx.solve_opts(precision=0, precond=2, tol=1e-8, max_iter=100000))
+    _, s_tl = cx.static_solve(pdx.solve_opts(precision=0, precond=4, tol=1e-8))
+    assert s_tl["iterations"] * 10 <= s_bj["iterations"], (s_tl, s_bj)
