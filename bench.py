@@ -89,6 +89,66 @@ def cpu_baseline(budget_s=15.0):
                       f"({M.neq} DOF), start-up + {steps} steps, {dt:.1f} s"}
 
 
+def copy_probe(mib):
+    """Five device-to-device copies of a known size: under rocprofv3 --pmc the
+    FETCH_SIZE / WRITE_SIZE of this copy kernel against mib MiB read and written
+    gives the scale of those counters on this GPU."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "copies": 5}), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of the explicit kernel on the 60^3 box, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+    profiles/k_explicit_box3d_60_traffic.md (copy-calibrated); None if that
+    kernel form was not profiled."""
+    path = os.path.join(ROOT, "profiles", "k_explicit_box3d_60_traffic.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return k["traffic_bytes"] if k else None
+
+
+def family_batch_bench(pdx, W):
+    """The 2,000,376-node PD cloud (126^3 jittered grid, per-node horizons
+    U(2.5, 3.5) dx, members by the OR-of-horizons rule): shape tensor + LU (FP64)
+    and bond blocks H (FP32) of every family, one family per block vs the
+    autotuned persistent launch.  Algorithmic bytes count every array element
+    once (coordinates, volumes, horizons, offsets, member ids in; A, LU,
+    pivots, H out); FP64 flops: 3 per member per symmetric shape-tensor entry
+    (45 in 3D), 2/3 p^3 per LU, 2 p^2 + 20 per bond solve and G."""
+    E, nu = 70e9, 0.33
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    t0 = time.time()
+    c = W.family_batch()
+    fb = pdx.FamilyBatch(c["x"], c["vol"], c["delta"], nlf=0.33, lam=lam, mu=mu)
+    setup_s = time.time() - t0
+    N, nb, P = fb.n_fam, fb.n_bond, 9
+    alg_bytes = N * (24 + 8 + 8 + 8) + nb * 4 + N * (2 * P * P * 8 + P * 4) + nb * 9 * 4
+    flops = 45 * 3 * (nb - N) + (2 * P ** 3 // 3) * N + (2 * P * P + 20) * (nb - N)
+    fb.run(0)
+    block_ms = min(fb.run(0) for _ in range(3))
+    fb.run(-1)  # times every persistent form once, keeps the fastest
+    best, tune = fb.tuning()
+    pers_ms = min(fb.run(-1) for _ in range(3))
+    return {"workload": "126^3 jittered node cloud (2,000,376 nodes), horizons U(2.5,3.5) dx, OR-rule families",
+            "n_family": N, "n_bond": nb, "mean_family": nb / N, "max_family": fb.max_family,
+            "host_setup_s": setup_s, "one_family_per_block_ms": block_ms, "persistent_autotuned_ms": pers_ms,
+            "persistent_variant": best, "tuning_ms": [float(t) for t in tune],
+            "speedup_persistent": block_ms / pers_ms, "families_per_s": N / (pers_ms * 1e-3),
+            "algorithmic_GBps": alg_bytes / (pers_ms * 1e-3) / 1e9, "fp64_TFLOPs": flops / (pers_ms * 1e-3) / 1e12,
+            "alg_bytes": alg_bytes, "flops": flops}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,7 +157,13 @@ def main():
     ap.add_argument("--n", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-fb", action="store_true", help="skip the 2M-node family batch")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the whole mesh lives in one HBM and this build has no "
               "multi-GPU path; run with --gpus 1", file=sys.stderr)
@@ -135,9 +201,12 @@ def main():
     prec = 1
     bytes_step = step_bytes(model, nblk, prec)
     achieved = bytes_step / (res[prec]["step_ms"] * 1e-3)
+    lanes, nt, tune_ms = ctxs[prec].explicit_tuning(prec)
+    kname = f"pdx::k_explicit<{model.dim}, float, {lanes}, {'true' if nt else 'false'}>"
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None,
-                "kernel": "k_explicit (node-block SpMV + central-difference update, FP32 values)",
+                "frac": achieved / HBM_PEAK, "traffic": measured_traffic(kname) if args.n == 60 else None,
+                "kernel": kname + " (node-block SpMV + central-difference update, FP32 values)",
+                "tuner_ms_per_form": [float(t) for t in tune_ms],
                 "bytes_per_launch": bytes_step,
                 "scalar_csr_bytes_per_step": survey_step_bytes(model, prec)}
     out = {"metric": METRIC, "value": res[1]["dofs"], "unit": "DOF-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -174,6 +243,9 @@ def main():
                 solves[name] = {"error": str(e)}
             log(name, solves[name])
         out["solve_to_1e-8"] = {"workload": f"crack2d 501x251 seed 0, neq {smod.neq}", **solves}
+    if not args.no_fb:
+        out["family_batch"] = family_batch_bench(pdx, W)
+        log("family batch", out["family_batch"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         log("cpu baseline", out["cpu_baseline"])

@@ -209,6 +209,27 @@ int pdx_gnn_inputs(pdx_ctx *c, int n_probes, unsigned long long probe_seed, int 
 /* predict the smoother for the context's current K (used by PDX_PREC_LEARNED) */
 int pdx_gnn_apply(pdx_ctx *c, const pdx_gnn *g);
 
+/* ---- batch of PD families (a node cloud, every node a family) ------------
+ * The family kernel of the reference -- inflFunc, shapTens2D/3D, vec_gd2D/3D
+ * with PLUSolve, matG2D/3D, matH2D/3D (pdsolve.cpp:1929-2416) -- for a whole
+ * cloud with per-node horizons.  Members by the rule of initialPDN_Fami
+ * (pdsolve.cpp:1442-1568): j joins k iff |x_j - x_k| < delta_k or < delta_j
+ * (self excluded), self first, then in cell-scan order.  Results stay on the
+ * device: A and its partial-pivot LU (FP64, p x p per family, p = 5 / 9),
+ * pivots, and H (FP32, d x d per member; member 0 holds -sum of the others). */
+typedef struct pdx_fbatch pdx_fbatch;
+int pdx_fb_create(int device, int dim, long long n, const double *xyz /*[n*3]*/, const double *vol /*[n]*/,
+                  const double *delta /*[n]*/, double nlf, double lambda, double mu, pdx_fbatch **out);
+void pdx_fb_destroy(pdx_fbatch *b);
+int pdx_fb_sizes(const pdx_fbatch *b, long long *n_fam, long long *n_bond, int *max_family);
+int pdx_fb_families(const pdx_fbatch *b, long long *off /*[n+1]*/, int *mem /*[n_bond]*/);
+/* variant -1: the fastest persistent launch (timed on the first call); 0: one
+ * family per 64-thread block; 1..4: persistent launch forms. ms: kernel time */
+int pdx_fb_run(pdx_fbatch *b, int variant, float *ms);
+int pdx_fb_tuning(const pdx_fbatch *b, int *best_variant, float *ms /*[5]*/);
+int pdx_fb_results(const pdx_fbatch *b, long long first, long long count, double *A, double *LU, int *piv,
+                   float *H);
+
 /* PARDISO seam: solve A x = b for a zero-based int64 CSR (n, ia, ja, a). */
 int pdx_solve_csr(int device, long long n, const long long *ia, const long long *ja, const double *a,
                   const double *b, double *x, const pdx_solve_opts *o, pdx_solve_stats *st);
@@ -216,6 +237,9 @@ int pdx_solve_csr(int device, long long n, const long long *ia, const long long 
 /* timing of the last pdx_explicit_steps call measured with HIP events on the
  * context stream: total ms and per-step ms */
 int pdx_last_timing(pdx_ctx *c, double *total_ms, double *per_step_ms);
+/* the explicit-step kernel form the tuner chose (lanes per short row, non-temporal K loads;
+ * 0/0 before the first step) and its timings of the 6 forms (ms per step) */
+int pdx_explicit_tuning(pdx_ctx *c, int precision, int *lanes, int *nontemporal, float *ms);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,10 @@ def lib():
         L.om_newmark_UN.argtypes = [P, P, P, P]
         L.om_set_skip.restype = None
         L.om_set_skip.argtypes = [I]
+        L.om_fb_members.restype = I
+        L.om_fb_members.argtypes = [I, I, P, P, P, P, I]
+        L.om_fb_family.restype = I
+        L.om_fb_family.argtypes = [I, P, P, P, D, D, D, I, P, P, P]
         _lib = L
     return _lib
 
@@ -447,3 +451,31 @@ def read_case(cmd_path: str) -> dict:
              gamma=opts["gamma"])
     m["opts"] = opts
     return m
+
+
+# ---- PD family batch ------------------------------------------------------
+def fb_members(x, delta, cap=None):
+    """Brute-force members of every node's family (initialPDN_Fami rule)."""
+    x = np.ascontiguousarray(x, np.float64)
+    delta = np.ascontiguousarray(delta, np.float64)
+    n = len(delta)
+    cap = cap or n * 200
+    off = np.zeros(n + 1, np.int32)
+    mem = np.zeros(cap, np.int32)
+    c = lib().om_fb_members(3, n, _p(x), _p(delta), _p(off), _p(mem), cap)
+    if c < 0:
+        raise RuntimeError("member capacity exceeded")
+    return off, mem[:c]
+
+
+def fb_family(dim, x, vol, delta, nlf, lam, mu, mem):
+    """Shape tensor A and H blocks [n, d*d] (self block first) of one family."""
+    x = np.ascontiguousarray(x, np.float64)
+    vol = np.ascontiguousarray(vol, np.float64)
+    delta = np.ascontiguousarray(delta, np.float64)
+    mem = np.ascontiguousarray(mem, np.int32)
+    p = 5 if dim == 2 else 9
+    A = np.zeros(p * p)
+    H = np.zeros(len(mem) * dim * dim)
+    _chk(lib().om_fb_family(dim, _p(x), _p(vol), _p(delta), nlf, lam, mu, len(mem), _p(mem), _p(A), _p(H)))
+    return A.reshape(p, p), H.reshape(len(mem), dim * dim)

@@ -1210,8 +1210,34 @@ static void fam_xi(const om_model *M, int f, int m, double xi[3]) {
     for (int i = 0; i < 3; i++) xi[i] = M->x[j * 3 + i] - M->x[k * 3 + i];
 }
 
-/* pdsolve::shapTens2D / shapTens3D (pdsolve.cpp:1938-2064) with the reference's
- * expression order for every entry */
+/* one member's term of pdsolve::shapTens2D / shapTens3D (pdsolve.cpp:1938-2064),
+ * in the reference's expression order for every entry */
+static void shape_term(int dim, const double xi[3], double om, double dv, double *A) {
+    double x = xi[0], y = xi[1], z = xi[2];
+    if (dim == 2) {
+        double t[25] = {
+            om * x * x * dv, om * dv * x * y, om * dv * x * x * x * 0.5, om * dv * x * y * y * 0.5, om * dv * x * x * y,
+            om * dv * x * y, om * dv * y * y, om * dv * x * x * y * 0.5, om * dv * y * y * y * 0.5, om * dv * x * y * y,
+            om * dv * x * x * x, om * dv * x * x * y, om * dv * x * x * x * x * 0.5, om * dv * x * x * y * y * 0.5, om * dv * x * x * x * y,
+            om * dv * x * y * y, om * dv * y * y * y, om * dv * x * x * y * y * 0.5, om * dv * y * y * y * y * 0.5, om * dv * x * y * y * y,
+            om * dv * x * x * y, om * dv * x * y * y, om * dv * x * x * x * y * 0.5, om * dv * x * y * y * y * 0.5, om * dv * x * x * y * y};
+        for (int e = 0; e < 25; e++) A[e] = t[e] + A[e];
+    } else {
+        double t[81] = {
+            x * x, x * y, x * z, (x * x * x) * 0.5, (x * y * y) * 0.5, (x * z * z) * 0.5, x * x * y, x * y * z, x * x * z,
+            x * y, y * y, y * z, (x * x * y) * 0.5, (y * y * y) * 0.5, (y * z * z) * 0.5, x * y * y, y * y * z, x * y * z,
+            x * z, y * z, z * z, (x * x * z) * 0.5, (y * y * z) * 0.5, (z * z * z) * 0.5, x * y * z, y * z * z, x * z * z,
+            x * x * x, x * x * y, x * x * z, (x * x * x * x) * 0.5, (x * x * y * y) * 0.5, (x * x * z * z) * 0.5, x * x * x * y, x * x * y * z, x * x * x * z,
+            x * y * y, y * y * y, y * y * z, (x * x * y * y) * 0.5, (y * y * y * y) * 0.5, (y * y * z * z) * 0.5, x * y * y * y, y * y * y * z, x * y * y * z,
+            x * z * z, y * z * z, z * z * z, (x * x * z * z) * 0.5, (y * y * z * z) * 0.5, (z * z * z * z) * 0.5, x * y * z * z, y * z * z * z, x * z * z * z,
+            x * x * y, x * y * y, x * y * z, (x * x * x * y) * 0.5, (x * y * y * y) * 0.5, (x * y * z * z) * 0.5, x * x * y * y, x * y * y * z, x * x * y * z,
+            x * y * z, y * y * z, y * z * z, (x * x * y * z) * 0.5, (y * y * y * z) * 0.5, (y * z * z * z) * 0.5, x * y * y * z, y * y * z * z, x * y * z * z,
+            x * x * z, x * y * z, x * z * z, (x * x * x * z) * 0.5, (x * y * y * z) * 0.5, (x * z * z * z) * 0.5, x * x * y * z, x * y * z * z, x * x * z * z};
+        for (int e = 0; e < 81; e++) A[e] = om * dv * t[e] + A[e];
+    }
+}
+
+/* pdsolve::shapTens2D / shapTens3D (pdsolve.cpp:1938-2064) */
 static void shape_tensor(const om_model *M, int f, double *A) {
     int n = M->fam_off[f + 1] - M->fam_off[f];
     double delt = M->fam_delta[f];
@@ -1221,29 +1247,7 @@ static void shape_tensor(const om_model *M, int f, double *A) {
         double xi[3];
         fam_xi(M, f, j, xi);
         double dv = M->nv[M->fam_mem[M->fam_off[f] + j] - 1];
-        double om = infl(M, xi, delt);
-        double x = xi[0], y = xi[1], z = xi[2];
-        if (M->dim == 2) {
-            double t[25] = {
-                om * x * x * dv, om * dv * x * y, om * dv * x * x * x * 0.5, om * dv * x * y * y * 0.5, om * dv * x * x * y,
-                om * dv * x * y, om * dv * y * y, om * dv * x * x * y * 0.5, om * dv * y * y * y * 0.5, om * dv * x * y * y,
-                om * dv * x * x * x, om * dv * x * x * y, om * dv * x * x * x * x * 0.5, om * dv * x * x * y * y * 0.5, om * dv * x * x * x * y,
-                om * dv * x * y * y, om * dv * y * y * y, om * dv * x * x * y * y * 0.5, om * dv * y * y * y * y * 0.5, om * dv * x * y * y * y,
-                om * dv * x * x * y, om * dv * x * y * y, om * dv * x * x * x * y * 0.5, om * dv * x * y * y * y * 0.5, om * dv * x * x * y * y};
-            for (int e = 0; e < 25; e++) A[e] = t[e] + A[e];
-        } else {
-            double t[81] = {
-                x * x, x * y, x * z, (x * x * x) * 0.5, (x * y * y) * 0.5, (x * z * z) * 0.5, x * x * y, x * y * z, x * x * z,
-                x * y, y * y, y * z, (x * x * y) * 0.5, (y * y * y) * 0.5, (y * z * z) * 0.5, x * y * y, y * y * z, x * y * z,
-                x * z, y * z, z * z, (x * x * z) * 0.5, (y * y * z) * 0.5, (z * z * z) * 0.5, x * y * z, y * z * z, x * z * z,
-                x * x * x, x * x * y, x * x * z, (x * x * x * x) * 0.5, (x * x * y * y) * 0.5, (x * x * z * z) * 0.5, x * x * x * y, x * x * y * z, x * x * x * z,
-                x * y * y, y * y * y, y * y * z, (x * x * y * y) * 0.5, (y * y * y * y) * 0.5, (y * y * z * z) * 0.5, x * y * y * y, y * y * y * z, x * y * y * z,
-                x * z * z, y * z * z, z * z * z, (x * x * z * z) * 0.5, (y * y * z * z) * 0.5, (z * z * z * z) * 0.5, x * y * z * z, y * z * z * z, x * z * z * z,
-                x * x * y, x * y * y, x * y * z, (x * x * x * y) * 0.5, (x * y * y * y) * 0.5, (x * y * z * z) * 0.5, x * x * y * y, x * y * y * z, x * x * y * z,
-                x * y * z, y * y * z, y * z * z, (x * x * y * z) * 0.5, (y * y * y * z) * 0.5, (y * z * z * z) * 0.5, x * y * y * z, y * y * z * z, x * y * z * z,
-                x * x * z, x * y * z, x * z * z, (x * x * x * z) * 0.5, (x * y * y * z) * 0.5, (x * z * z * z) * 0.5, x * x * y * z, x * y * z * z, x * x * z * z};
-            for (int e = 0; e < 81; e++) A[e] = om * dv * t[e] + A[e];
-        }
+        shape_term(M->dim, xi, infl(M, xi, delt), dv, A);
     }
 }
 
@@ -1265,11 +1269,30 @@ static int vec_gd(const om_model *M, const double *A, const double xi[3], double
     return OM_OK;
 }
 
-/* pdsolve::matH2D / matH3D (pdsolve.cpp:2349-2416) with matG2D/3D (:2277-2347);
+/* pdsolve::matG2D / matG3D (pdsolve.cpp:2277-2347): bond stiffness from the d-weights */
+static void bond_G(int d, double lam, double mu, double om, double mukm, const double *dd, double *G) {
+    if (d == 2) {
+        double t = mukm * om * ((2 * mu + lam) * dd[0] + mu * dd[1]);
+        G[0] = t;
+        t = mukm * om * (lam + mu) * dd[2];
+        G[1] = t; G[2] = t;
+        G[3] = mukm * om * ((2 * mu + lam) * dd[1] + mu * dd[0]);
+    } else {
+        double tr = dd[0] + dd[1] + dd[2], t;
+        G[0] = mukm * om * ((mu + lam) * dd[0] + mu * tr);
+        t = mukm * om * (lam + mu) * dd[3]; G[1] = t; G[3] = t;
+        t = mukm * om * (lam + mu) * dd[5]; G[2] = t; G[6] = t;
+        G[4] = mukm * om * ((mu + lam) * dd[1] + mu * tr);
+        t = mukm * om * (lam + mu) * dd[4]; G[5] = t; G[7] = t;
+        G[8] = mukm * om * ((mu + lam) * dd[2] + mu * tr);
+    }
+}
+
+/* pdsolve::matH2D / matH3D (pdsolve.cpp:2349-2416) with matG2D/3D;
  * H is d x (d*n) row-major */
 static int mat_H(const om_model *M, int f, double *H) {
     int n = M->fam_off[f + 1] - M->fam_off[f], d = M->dim, w = d * n;
-    double A[81], lam = M->lambda, mu = M->mu;
+    double A[81];
     memset(H, 0, sizeof(double) * d * w);
     shape_tensor(M, f, A);
     for (int m = 1; m < n; m++) {
@@ -1279,22 +1302,7 @@ static int mat_H(const om_model *M, int f, double *H) {
         fam_xi(M, f, m, xi);
         double mukm = M->bond[idx];
         if (vec_gd(M, A, xi, g, dd)) return OM_ERR_SINGULAR;
-        double om = infl(M, xi, M->fam_delta[f]);
-        if (d == 2) {
-            double t = mukm * om * ((2 * mu + lam) * dd[0] + mu * dd[1]);
-            G[0] = t;
-            t = mukm * om * (lam + mu) * dd[2];
-            G[1] = t; G[2] = t;
-            G[3] = mukm * om * ((2 * mu + lam) * dd[1] + mu * dd[0]);
-        } else {
-            double tr = dd[0] + dd[1] + dd[2], t;
-            G[0] = mukm * om * ((mu + lam) * dd[0] + mu * tr);
-            t = mukm * om * (lam + mu) * dd[3]; G[1] = t; G[3] = t;
-            t = mukm * om * (lam + mu) * dd[5]; G[2] = t; G[6] = t;
-            G[4] = mukm * om * ((mu + lam) * dd[1] + mu * tr);
-            t = mukm * om * (lam + mu) * dd[4]; G[5] = t; G[7] = t;
-            G[8] = mukm * om * ((mu + lam) * dd[2] + mu * tr);
-        }
+        bond_G(d, M->lambda, M->mu, infl(M, xi, M->fam_delta[f]), mukm, dd, G);
         for (int i = 0; i < d; i++)
             for (int j = 0; j < d; j++) {
                 double t = G[i * d + j] * dvm;
@@ -1768,4 +1776,73 @@ void om_newmark_UN(const om_model *M, const double *v, const double *a, double *
             UN[k * d + M->ebc_dof[s]] = UN[k * d + M->ebc_dof[s]] + dU;
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* PD family batch: a node cloud in which every node is a family            */
+/* ------------------------------------------------------------------------ */
+
+/* Members by the rule of pdsolve::initialPDN_Fami (pdsolve.cpp:1442-1568):
+ * j belongs to the family of k iff |x_j - x_k|^2 > md2 and
+ * (|x_j - x_k|^2 < delta_k^2 + md2 or |x_j - x_k|^2 < delta_j^2 + md2),
+ * md2 = (min delta)^2 * 1e-16; self first, then the others in node order
+ * (brute force, for small clouds).  off [n+1]; returns -1 if cap is exceeded. */
+int om_fb_members(int dim, int n, const double *x, const double *delta, int *off, int *mem, int cap) {
+    double mind = delta[0];
+    for (int k = 1; k < n; k++) mind = delta[k] < mind ? delta[k] : mind;
+    double md2 = mind * mind * 1.0e-16;
+    int c = 0;
+    off[0] = 0;
+    for (int k = 0; k < n; k++) {
+        if (c >= cap) return -1;
+        mem[c++] = k;
+        for (int j = 0; j < n; j++) {
+            double dist = 0.0;
+            for (int i = 0; i < 3; i++) dist += (x[j * 3 + i] - x[k * 3 + i]) * (x[j * 3 + i] - x[k * 3 + i]);
+            if (dist > md2 && (dist < delta[k] * delta[k] + md2 || dist < delta[j] * delta[j] + md2)) {
+                if (c >= cap) return -1;
+                mem[c++] = j;
+            }
+        }
+        off[k + 1] = c;
+    }
+    (void)dim;
+    return c;
+}
+
+/* Shape tensor A (p x p) and the H blocks of one family: d x d per member,
+ * H_m = G_m V_m (matG/matH, pdsolve.cpp:2277-2416, vec_gd :2174-2232 with one
+ * dgesv per bond), self block H_0 = -sum_m H_m; all bonds intact. */
+int om_fb_family(int dim, const double *x, const double *vol, const double *delta, double nlf, double lam, double mu,
+                 int n, const int *mem, double *A, double *H) {
+    int p = dim == 2 ? 5 : 9, dd2 = dim * dim, k = mem[0];
+    double delt = delta[k], Aa = nlf * nlf;
+    memset(A, 0, sizeof(double) * p * p);
+    memset(H, 0, sizeof(double) * dd2 * n);
+    for (int m = 1; m < n; m++) {
+        double xi[3];
+        for (int i = 0; i < 3; i++) xi[i] = x[mem[m] * 3 + i] - x[k * 3 + i];
+        double om = exp(-(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]) / Aa / delt / delt);
+        shape_term(dim, xi, om, vol[mem[m]], A);
+    }
+    for (int m = 1; m < n; m++) {
+        double xi[3], b[9], sl[9], G[9];
+        for (int i = 0; i < 3; i++) xi[i] = x[mem[m] * 3 + i] - x[k * 3 + i];
+        if (dim == 2) {
+            b[0] = xi[0]; b[1] = xi[1]; b[2] = xi[0] * xi[0]; b[3] = xi[1] * xi[1]; b[4] = xi[0] * xi[1];
+        } else {
+            b[0] = xi[0]; b[1] = xi[1]; b[2] = xi[2];
+            b[3] = xi[0] * xi[0]; b[4] = xi[1] * xi[1]; b[5] = xi[2] * xi[2];
+            b[6] = xi[0] * xi[1]; b[7] = xi[1] * xi[2]; b[8] = xi[2] * xi[0];
+        }
+        if (lu_solve(p, A, b, sl)) return OM_ERR_SINGULAR;
+        double om = exp(-(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]) / Aa / delt / delt);
+        bond_G(dim, lam, mu, om, 1.0, sl + dim, G);
+        for (int e = 0; e < dd2; e++) {
+            double t = G[e] * vol[mem[m]];
+            H[m * dd2 + e] = t;
+            H[e] = -t + H[e];
+        }
+    }
+    return OM_OK;
 }

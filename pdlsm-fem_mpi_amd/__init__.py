@@ -83,6 +83,7 @@ EXPORTS = {
     "pdx_newmark_run": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "pdx_solve_csr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
     "pdx_last_timing": (_I, [_P, _P, _P]),
+    "pdx_explicit_tuning": (_I, [_P, _I, _P, _P, _P]),
     "pdx_gnn_create": (_I, [_I, _I, C.c_ulonglong, _P]),
     "pdx_gnn_destroy": (None, [_P]),
     "pdx_gnn_num_params": (_LL, [_P]),
@@ -92,6 +93,13 @@ EXPORTS = {
     "pdx_gnn_loss_grad": (_I, [_P, _P, _I, C.c_ulonglong, _I, _P, _P]),
     "pdx_gnn_inputs": (_I, [_P, _I, C.c_ulonglong, _I, _P, _P, _P, _P, _P, _P]),
     "pdx_gnn_apply": (_I, [_P, _P]),
+    "pdx_fb_create": (_I, [_I, _I, _LL, _P, _P, _P, _D, _D, _D, _P]),
+    "pdx_fb_destroy": (None, [_P]),
+    "pdx_fb_sizes": (_I, [_P, _P, _P, _P]),
+    "pdx_fb_families": (_I, [_P, _P, _P]),
+    "pdx_fb_run": (_I, [_P, _I, _P]),
+    "pdx_fb_tuning": (_I, [_P, _P, _P]),
+    "pdx_fb_results": (_I, [_P, _LL, _LL, _P, _P, _P, _P]),
 }
 
 
@@ -297,6 +305,12 @@ class Context:
         _chk(lib().pdx_last_timing(self.h, C.byref(t), C.byref(s)))
         return t.value, s.value
 
+    def explicit_tuning(self, precision):
+        lanes, nt = C.c_int(0), C.c_int(0)
+        ms = np.zeros(6, np.float32)
+        _chk(lib().pdx_explicit_tuning(self.h, precision, C.byref(lanes), C.byref(nt), _p(ms)))
+        return lanes.value, bool(nt.value), ms
+
     def max_eig(self, iters=100):
         v = C.c_double(0)
         _chk(lib().pdx_max_eig(self.h, iters, C.byref(v)))
@@ -392,3 +406,53 @@ def gnn_inputs(cx: "Context", n_probes: int = 4, probe_seed: int = 0, coarse_agg
     y = np.zeros((n_probes, nn * dim), np.float32)
     _chk(lib().pdx_gnn_inputs(cx.h, n_probes, probe_seed, coarse_agg, _p(X), _p(cw), _p(Aaa), _p(B), _p(f), _p(y)))
     return dict(X=X, cw=cw, Aaa=Aaa, B=B, f=f, y=y, adj_off=adj_off, adj=adj, eq=m.nodes()["eq"][:, :dim])
+
+
+class FamilyBatch:
+    """Device-resident batch of PD families over a node cloud (every node a family)."""
+
+    def __init__(self, x, vol, delta, nlf=0.33, lam=0.0, mu=0.0, dim=3, device=0):
+        x = np.ascontiguousarray(x, np.float64)
+        vol = np.ascontiguousarray(vol, np.float64)
+        delta = np.ascontiguousarray(delta, np.float64)
+        h = C.c_void_p()
+        _chk(lib().pdx_fb_create(device, dim, len(vol), _p(x), _p(vol), _p(delta), nlf, lam, mu, C.byref(h)))
+        self.h, self.dim = h, dim
+        nf, nb, mf = C.c_longlong(), C.c_longlong(), C.c_int()
+        _chk(lib().pdx_fb_sizes(h, C.byref(nf), C.byref(nb), C.byref(mf)))
+        self.n_fam, self.n_bond, self.max_family = nf.value, nb.value, mf.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pdx_fb_destroy(self.h)
+            self.h = None
+
+    def families(self):
+        off = np.zeros(self.n_fam + 1, np.int64)
+        mem = np.zeros(max(self.n_bond, 1), np.int32)
+        _chk(lib().pdx_fb_families(self.h, _p(off), _p(mem)))
+        return off, mem[: self.n_bond]
+
+    def run(self, variant=-1) -> float:
+        ms = C.c_float()
+        _chk(lib().pdx_fb_run(self.h, variant, C.byref(ms)))
+        return ms.value
+
+    def tuning(self):
+        best = C.c_int()
+        ms = np.zeros(5, np.float32)
+        _chk(lib().pdx_fb_tuning(self.h, C.byref(best), _p(ms)))
+        return best.value, ms
+
+    def results(self, first, count, off=None):
+        p = 9 if self.dim == 3 else 5
+        dd = self.dim * self.dim
+        if off is None:
+            off, _ = self.families()
+        nb = int(off[first + count] - off[first])
+        A = np.zeros((count, p, p))
+        LU = np.zeros((count, p, p))
+        piv = np.zeros((count, p), np.int32)
+        H = np.zeros((max(nb, 1), dd), np.float32)
+        _chk(lib().pdx_fb_results(self.h, first, count, _p(A), _p(LU), _p(piv), _p(H)))
+        return A, LU, piv, H[:nb]

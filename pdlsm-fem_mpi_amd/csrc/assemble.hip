@@ -260,15 +260,18 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
             b[3] = x0 * x0; b[4] = x1 * x1; b[5] = x2 * x2;
             b[6] = x0 * x1; b[7] = x1 * x2; b[8] = x2 * x0;
         }
+        // the LU swapped whole rows (multipliers included): permute b completely, then eliminate
 #pragma unroll
         for (int kk = 0; kk < P; kk++) {
             const int p = spiv[kk];
 #pragma unroll
             for (int i = kk + 1; i < P; i++)
                 if (p == i) { double t = b[kk]; b[kk] = b[i]; b[i] = t; }
+        }
+#pragma unroll
+        for (int kk = 0; kk < P; kk++)
 #pragma unroll
             for (int i = kk + 1; i < P; i++) b[i] = b[i] - sA[i * P + kk] * b[kk];
-        }
         double s[P];
 #pragma unroll
         for (int i = P - 1; i >= 0; i--) {
@@ -384,11 +387,13 @@ __device__ inline bool lu3_solve8(double J[9], double dN[8][3], double out[8][3]
     for (int a = 0; a < 8; a++) {
         double b0 = dN[a][0], b1 = dN[a][1], b2 = dN[a][2];
         double bb[3] = {b0, b1, b2};
+        // rows were swapped whole (multipliers included): permute first, then eliminate
         for (int k = 0; k < 3; k++) {
             int p = piv[k];
             if (p != k) { double t = bb[k]; bb[k] = bb[p]; bb[p] = t; }
-            for (int i = k + 1; i < 3; i++) bb[i] = bb[i] - J[i * 3 + k] * bb[k];
         }
+        for (int k = 0; k < 3; k++)
+            for (int i = k + 1; i < 3; i++) bb[i] = bb[i] - J[i * 3 + k] * bb[k];
         for (int i = 2; i >= 0; i--) {
             double t = bb[i];
             for (int c = i + 1; c < 3; c++) t = t - J[i * 3 + c] * out[a][c];

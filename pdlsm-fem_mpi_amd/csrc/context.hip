@@ -522,6 +522,17 @@ int pdx_last_timing(pdx_ctx *pc, double *total_ms, double *per_step_ms) {
     return PDX_OK;
 }
 
+int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal, float *ms) {
+    if (!pc || (precision != PDX_FP64 && precision != PDX_FP32)) return fail(PDX_ERR_ARG, "bad argument");
+    const Ctx &c = *pc->c;
+    const int v = c.tune_explicit[precision];
+    static const int gs[6] = {8, 4, 16, 8, 4, 16}, nt[6] = {0, 0, 0, 1, 1, 1};
+    if (lanes) *lanes = v >= 0 ? gs[v] : 0;
+    if (nontemporal) *nontemporal = v >= 0 ? nt[v] : 0;
+    if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + 6, ms);
+    return PDX_OK;
+}
+
 int pdx_max_eig(pdx_ctx *pc, int iters, double *lmax) {
     if (!pc || !lmax || iters <= 0) return fail(PDX_ERR_ARG, "bad argument");
     std::string err;

@@ -147,7 +147,8 @@ def penny3d(n=100, dx=1e-3, pd_block=30, radius_frac=0.2, nfan=32, m=3.015, nlf=
     top = np.stack([nid(a, b, n - 1), nid(a + 1, b, n - 1), nid(a + 1, b + 1, n - 1), nid(a, b + 1, n - 1)],
                    axis=1).astype(np.int32)
     nbc = [dict(con=top, val=0.0, vel=t_final / (ramp_steps * dt), vary=True)]
-    c = np.array([0.5 * L + 0.5 * dx, 0.5 * L + 0.5 * dx, 0.5 * L + 0.5 * dx])
+    # crack plane halfway between two node layers (a plane through nodes is degenerate)
+    c = np.array([0.5 * L, 0.5 * L, (n // 2 - 0.5) * dx])
     R = radius_frac * L
     cracks = np.zeros((nfan, 3, 3))
     for q in range(nfan):

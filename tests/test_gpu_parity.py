@@ -34,6 +34,8 @@ CASES = {
     "crack2d": lambda: W.crack2d(seed=5, nx=121, ny=61, dx=1e-3),
     "box3d": lambda: W.box3d(n=16, pd_block=8),
     "penny3d": lambda: W.penny3d(n=20, pd_block=10),
+    # PD up to the free surfaces: one-sided families whose LU pivots after the first column
+    "box3d_allpd": lambda: W.box3d(n=10, pd_block=10, jitter=0.3),
 }
 
 
