@@ -181,6 +181,15 @@ int pdx_max_eig(pdx_ctx *c, int iters, double *lmax);
 int pdx_set_dt(pdx_ctx *c, double dt);
 
 int pdx_static_solve(pdx_ctx *c, const pdx_solve_opts *o, double *u, pdx_solve_stats *st);
+
+/* Nodal stresses, pdsolve::calGlobalNodeStresses (pdsolve.cpp:9540-9604) with
+ * calPDNodeStresses (:9081-9166) and calFEMNodeStresses_EXP (:9168-9335), as the
+ * drivers call it before every result output (e.g. :6062).  u_all: DOF values
+ * [n_node*dim] (the reference's pdDof values), or NULL for the current explicit
+ * state u_{n+1}.  sigma [n_node*6]: xx, yy, zz, xy, yz, zx (2D: xx, yy, zz, xy). */
+int pdx_node_stresses(pdx_ctx *c, const double *u_all, double *sigma);
+/* the same on the device only, timed (ms of one recovery, inputs resident) */
+int pdx_node_stresses_timed(pdx_ctx *c, const double *u_all, double *ms);
 int pdx_newmark_run(pdx_ctx *c, int nsteps, const pdx_solve_opts *o, double *u, double *v, double *a,
                     pdx_solve_stats *st);
 
@@ -238,7 +247,7 @@ int pdx_solve_csr(int device, long long n, const long long *ia, const long long 
  * context stream: total ms and per-step ms */
 int pdx_last_timing(pdx_ctx *c, double *total_ms, double *per_step_ms);
 /* the explicit-step kernel form the tuner chose (lanes per short row, non-temporal K loads;
- * 0/0 before the first step) and its timings of the 6 forms (ms per step) */
+ * 0/0 before the first step) and its timings of the 8 forms (ms per step) */
 int pdx_explicit_tuning(pdx_ctx *c, int precision, int *lanes, int *nontemporal, float *ms);
 
 #ifdef __cplusplus

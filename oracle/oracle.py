@@ -74,6 +74,8 @@ def lib():
         L.om_newmark_UN.argtypes = [P, P, P, P]
         L.om_set_skip.restype = None
         L.om_set_skip.argtypes = [I]
+        L.om_node_stresses.restype = I
+        L.om_node_stresses.argtypes = [P, P, P]
         L.om_fb_members.restype = I
         L.om_fb_members.argtypes = [I, I, P, P, P, P, I]
         L.om_fb_family.restype = I
@@ -201,6 +203,13 @@ class Model:
         A = np.zeros((p, p))
         _chk(lib().om_family_A(self.h, f, _p(A)))
         return A
+
+    def node_stresses(self, u_all):
+        """calGlobalNodeStresses for DOF values u_all [nn*dim] -> sigma [nn, 6]."""
+        u = np.ascontiguousarray(u_all, np.float64)
+        sig = np.zeros(self.nn * 6)
+        _chk(lib().om_node_stresses(self.h, _p(u), _p(sig)))
+        return sig.reshape(self.nn, 6)
 
     def element_K(self, e):
         out = np.zeros(576)

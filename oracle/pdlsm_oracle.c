@@ -318,6 +318,55 @@ static void add_btdb(int nv, int nd, const double *B, const double *D, double w,
 }
 
 /* element stiffness of every element type; nd = dofs per element */
+/* pdfemEleTri3N::BmatFEM (pdfemEleTri3N.cpp:185-210): B (3 x 6) */
+static void tri3_B(double xN[][3], double *B) {
+    double V2 = 2.0 * tri3_volume(xN);
+    double b[3] = {xN[1][1] - xN[2][1], xN[2][1] - xN[0][1], xN[0][1] - xN[1][1]};
+    double c[3] = {-xN[1][0] + xN[2][0], -xN[2][0] + xN[0][0], -xN[0][0] + xN[1][0]};
+    memset(B, 0, sizeof(double) * 18);
+    for (int i = 0; i < 3; i++) {
+        B[0 * 6 + 2 * i] = b[i] / V2;
+        B[1 * 6 + 2 * i + 1] = c[i] / V2;
+        B[2 * 6 + 2 * i] = c[i] / V2;
+        B[2 * 6 + 2 * i + 1] = b[i] / V2;
+    }
+}
+
+/* pdfemEleTetra4N::BmatFEM (pdfemEleTetra4N.cpp:84-170): B (6 x 12) */
+static void tet4_B(double xN[][3], double *B) {
+    double m[3][3], b[4], c[4], d[4];
+#define SETM(r0, r1, r2, c0, c1) \
+    m[0][0] = 1, m[0][1] = xN[r0][c0], m[0][2] = xN[r0][c1]; \
+    m[1][0] = 1, m[1][1] = xN[r1][c0], m[1][2] = xN[r1][c1]; \
+    m[2][0] = 1, m[2][1] = xN[r2][c0], m[2][2] = xN[r2][c1];
+    SETM(1, 2, 3, 1, 2) b[0] = -det33(m);
+    SETM(0, 2, 3, 1, 2) b[1] = det33(m);
+    SETM(0, 1, 3, 1, 2) b[2] = -det33(m);
+    SETM(0, 1, 2, 1, 2) b[3] = det33(m);
+    SETM(1, 2, 3, 0, 2) c[0] = det33(m);
+    SETM(0, 2, 3, 0, 2) c[1] = -det33(m);
+    SETM(0, 1, 3, 0, 2) c[2] = det33(m);
+    SETM(0, 1, 2, 0, 2) c[3] = -det33(m);
+    SETM(1, 2, 3, 0, 1) d[0] = -det33(m);
+    SETM(0, 2, 3, 0, 1) d[1] = det33(m);
+    SETM(0, 1, 3, 0, 1) d[2] = -det33(m);
+    SETM(0, 1, 2, 0, 1) d[3] = det33(m);
+#undef SETM
+    double V6 = 6.0 * tet4_volume(xN);
+    memset(B, 0, sizeof(double) * 72);
+    for (int i = 0; i < 4; i++) {
+        B[0 * 12 + 3 * i] = b[i] / V6;
+        B[1 * 12 + 3 * i + 1] = c[i] / V6;
+        B[2 * 12 + 3 * i + 2] = d[i] / V6;
+        B[3 * 12 + 3 * i] = c[i] / V6;
+        B[3 * 12 + 3 * i + 1] = b[i] / V6;
+        B[4 * 12 + 3 * i + 1] = d[i] / V6;
+        B[4 * 12 + 3 * i + 2] = c[i] / V6;
+        B[5 * 12 + 3 * i] = d[i] / V6;
+        B[5 * 12 + 3 * i + 2] = b[i] / V6;
+    }
+}
+
 static int ele_stiff(const om_model *M, int e, double *Ke, int *nd_out) {
     double xN[8][3];
     ele_xyz(M, e, xN);
@@ -333,17 +382,8 @@ static int ele_stiff(const om_model *M, int e, double *Ke, int *nd_out) {
         *nd_out = 8;
     } else if (M->dim == 2) {
         /* pdfemEleTri3N::eleStiffMatFEM / BmatFEM (pdfemEleTri3N.cpp:33-47, :185-210) */
-        double V = tri3_volume(xN), V2 = 2.0 * tri3_volume(xN);
-        double b[3] = {xN[1][1] - xN[2][1], xN[2][1] - xN[0][1], xN[0][1] - xN[1][1]};
-        double c[3] = {-xN[1][0] + xN[2][0], -xN[2][0] + xN[0][0], -xN[0][0] + xN[1][0]};
-        double B[18];
-        memset(B, 0, sizeof(B));
-        for (int i = 0; i < 3; i++) {
-            B[0 * 6 + 2 * i] = b[i] / V2;
-            B[1 * 6 + 2 * i + 1] = c[i] / V2;
-            B[2 * 6 + 2 * i] = c[i] / V2;
-            B[2 * 6 + 2 * i + 1] = b[i] / V2;
-        }
+        double V = tri3_volume(xN), B[18];
+        tri3_B(xN, B);
         double Bt[18], BtD[18];
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 6; j++) Bt[j * 3 + i] = B[i * 6 + j];
@@ -364,37 +404,8 @@ static int ele_stiff(const om_model *M, int e, double *Ke, int *nd_out) {
         *nd_out = 24;
     } else {
         /* pdfemEleTetra4N::BmatFEM (pdfemEleTetra4N.cpp:84-170) */
-        double m[3][3], b[4], c[4], d[4];
-#define SETM(r0, r1, r2, c0, c1) \
-    m[0][0] = 1, m[0][1] = xN[r0][c0], m[0][2] = xN[r0][c1]; \
-    m[1][0] = 1, m[1][1] = xN[r1][c0], m[1][2] = xN[r1][c1]; \
-    m[2][0] = 1, m[2][1] = xN[r2][c0], m[2][2] = xN[r2][c1];
-        SETM(1, 2, 3, 1, 2) b[0] = -det33(m);
-        SETM(0, 2, 3, 1, 2) b[1] = det33(m);
-        SETM(0, 1, 3, 1, 2) b[2] = -det33(m);
-        SETM(0, 1, 2, 1, 2) b[3] = det33(m);
-        SETM(1, 2, 3, 0, 2) c[0] = det33(m);
-        SETM(0, 2, 3, 0, 2) c[1] = -det33(m);
-        SETM(0, 1, 3, 0, 2) c[2] = det33(m);
-        SETM(0, 1, 2, 0, 2) c[3] = -det33(m);
-        SETM(1, 2, 3, 0, 1) d[0] = -det33(m);
-        SETM(0, 2, 3, 0, 1) d[1] = det33(m);
-        SETM(0, 1, 3, 0, 1) d[2] = -det33(m);
-        SETM(0, 1, 2, 0, 1) d[3] = det33(m);
-#undef SETM
-        double V = tet4_volume(xN), V6 = 6.0 * tet4_volume(xN), B[72];
-        memset(B, 0, sizeof(B));
-        for (int i = 0; i < 4; i++) {
-            B[0 * 12 + 3 * i] = b[i] / V6;
-            B[1 * 12 + 3 * i + 1] = c[i] / V6;
-            B[2 * 12 + 3 * i + 2] = d[i] / V6;
-            B[3 * 12 + 3 * i] = c[i] / V6;
-            B[3 * 12 + 3 * i + 1] = b[i] / V6;
-            B[4 * 12 + 3 * i + 1] = d[i] / V6;
-            B[4 * 12 + 3 * i + 2] = c[i] / V6;
-            B[5 * 12 + 3 * i] = d[i] / V6;
-            B[5 * 12 + 3 * i + 2] = b[i] / V6;
-        }
+        double V = tet4_volume(xN), B[72];
+        tet4_B(xN, B);
         double Bt[72], BtD[72];
         for (int i = 0; i < 6; i++)
             for (int j = 0; j < 12; j++) Bt[j * 6 + i] = B[i * 12 + j];
@@ -1844,5 +1855,133 @@ int om_fb_family(int dim, const double *x, const double *vol, const double *delt
             H[e] = -t + H[e];
         }
     }
+    return OM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* nodal stresses                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* pdsolve::calGlobalNodeStresses (pdsolve.cpp:9540-9604) for the DOF values
+ * u_all [nn*dim]: every PD family node first (calPDNodeStresses :9081-9166,
+ * sigma = D (C u_family) stored, count 1), then every FE element in FE order
+ * (calFEMNodeStresses_EXP :9168-9335 with eleFitStresses flag 1: Gauss-point
+ * stresses D B u_e extrapolated to the corner nodes with L, added, count + 1
+ * per element node), then sigma / count and, for 2D plane strain,
+ * sigma_zz = nu (sigma_xx + sigma_yy) (pdNode::calSigzz).  sigma [nn*6]; the
+ * 2D components sit at 0 (xx), 1 (yy), 3 (xy).  A node with count 0 keeps 0
+ * (the reference would divide by zero). */
+int om_node_stresses(const om_model *M, const double *u_all, double *sigma) {
+    int d = M->dim, nv = d == 2 ? 3 : 6, nn = M->nn;
+    int *count = xcalloc(nn, sizeof(int));
+    memset(sigma, 0, sizeof(double) * 6 * nn);
+    for (int f = 0; f < M->nfam; f++) {
+        int n = M->fam_off[f + 1] - M->fam_off[f], w = d * n, k = M->fam_mem[M->fam_off[f]] - 1;
+        double *C = xcalloc((size_t)nv * w, sizeof(double)), *uk = xcalloc(w, sizeof(double));
+        double eps[6], sig[6];
+        count[k]++;
+        if (mat_C(M, f, C)) { free(C); free(uk); free(count); return OM_ERR_SINGULAR; }
+        for (int m = 0; m < n; m++) {
+            int j = M->fam_mem[M->fam_off[f] + m] - 1;
+            for (int i = 0; i < d; i++) uk[d * m + i] = u_all[(size_t)j * d + i];
+        }
+        matmul(nv, w, 1, C, uk, eps);
+        matmul(nv, nv, 1, M->D, eps, sig);
+        if (d == 2) {
+            sigma[6 * k + 0] = sig[0]; sigma[6 * k + 1] = sig[1]; sigma[6 * k + 3] = sig[2];
+        } else {
+            for (int i = 0; i < 6; i++) sigma[6 * k + i] = sig[i];
+        }
+        free(C); free(uk);
+    }
+    /* extrapolation matrices L (pdsolve.cpp:9176-9196 for 3D, :9254-9277 for 2D) */
+    double L8[64], L4[16];
+    {
+        double pI[8] = {-1.0, 1., 1., -1., -1., 1., 1., -1.}, qI[8] = {-1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0},
+               rI[8] = {-1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0}, p[8], q[8], r[8];
+        for (int i = 0; i < 8; i++) { p[i] = sqrt(3.0) * pI[i]; q[i] = sqrt(3.0) * qI[i]; r[i] = sqrt(3.0) * rI[i]; }
+        for (int i = 0; i < 8; i++)
+            for (int j = 0; j < 8; j++)
+                L8[i * 8 + j] = 1.0 / 8 * (1 + pI[j] * p[i]) * (1 + qI[j] * q[i]) * (1 + rI[j] * r[i]);
+        double a = 1 + 0.5 * sqrt(3.0), b = -0.5, c = 1 - 0.5 * sqrt(3.0);
+        double row0[4] = {a, b, c, b};
+        memset(L4, 0, sizeof(L4));
+        for (int i = 0; i < 4; i++) L4[i] = row0[i];
+        L4[1 * 4 + 1] = a; L4[1 * 4 + 2] = b; L4[1 * 4 + 3] = c;
+        L4[2 * 4 + 2] = a; L4[2 * 4 + 3] = b;
+        L4[3 * 4 + 3] = a;
+        for (int i = 1; i < 4; i++)
+            for (int j = 0; j < i; j++) L4[i * 4 + j] = L4[j * 4 + i];
+    }
+    for (int t = 0; t < M->nfe; t++) {
+        int e = M->fe[t], ne = M->nen[e];
+        double xN[8][3], Ue[24], Ns[6][8];
+        ele_xyz(M, e, xN);
+        for (int a = 0; a < ne; a++) {
+            int j = M->econ[e * 8 + a] - 1;
+            for (int i = 0; i < d; i++) Ue[d * a + i] = u_all[(size_t)j * d + i];
+            count[j]++;
+        }
+        if (d == 2 && ne == 4) {
+            /* pdfemEleQuad4N::eleFitStresses (pdfemEleQuad4N.cpp:73-124): q outer, p inner */
+            double gp[3][4];
+            int cnt = 0;
+            for (int nq = 0; nq < 2; nq++)
+                for (int np = 0; np < 2; np++) {
+                    double B[24], detJ, eps[3], sig[3];
+                    quad4_B(xN, GP2[np], GP2[nq], B, &detJ);
+                    matmul(3, 8, 1, B, Ue, eps);
+                    matmul(3, 3, 1, M->D, eps, sig);
+                    for (int i = 0; i < 3; i++) gp[i][cnt] = sig[i];
+                    cnt++;
+                }
+            for (int i = 0; i < 3; i++) matmul(4, 4, 1, L4, gp[i], Ns[i]);
+        } else if (d == 2) {
+            /* pdfemEleTri3N::eleFitStresses (pdfemEleTri3N.cpp:82-103): constant */
+            double B[18], eps[3], sig[3];
+            tri3_B(xN, B);
+            matmul(3, 6, 1, B, Ue, eps);
+            matmul(3, 3, 1, M->D, eps, sig);
+            for (int a = 0; a < 3; a++)
+                for (int i = 0; i < 3; i++) Ns[i][a] = sig[i];
+        } else if (ne == 8) {
+            /* pdfemEleBrick8N::eleFitStresses (pdfemEleBrick8N.cpp:119-166) */
+            double g = 1.0 / sqrt(3), p[8] = {-g, g, g, -g, -g, g, g, -g}, q[8] = {-g, -g, g, g, -g, -g, g, g},
+                   r[8] = {-g, -g, -g, -g, g, g, g, g}, gp[6][8];
+            for (int i = 0; i < 8; i++) {
+                double B[144], eps[6], sig[6];
+                if (hex8_B(xN, p[i], q[i], r[i], B)) { free(count); return OM_ERR_SINGULAR; }
+                matmul(6, 24, 1, B, Ue, eps);
+                matmul(6, 6, 1, M->D, eps, sig);
+                for (int j = 0; j < 6; j++) gp[j][i] = sig[j];
+            }
+            for (int j = 0; j < 6; j++) matmul(8, 8, 1, L8, gp[j], Ns[j]);
+        } else {
+            /* pdfemEleTetra4N::eleFitStresses: constant */
+            double B[72], eps[6], sig[6];
+            tet4_B(xN, B);
+            matmul(6, 12, 1, B, Ue, eps);
+            matmul(6, 6, 1, M->D, eps, sig);
+            for (int a = 0; a < 4; a++)
+                for (int i = 0; i < 6; i++) Ns[i][a] = sig[i];
+        }
+        for (int a = 0; a < ne; a++) {
+            int j = M->econ[e * 8 + a] - 1;
+            if (d == 2) {
+                sigma[6 * j + 0] = sigma[6 * j + 0] + Ns[0][a];
+                sigma[6 * j + 1] = sigma[6 * j + 1] + Ns[1][a];
+                sigma[6 * j + 3] = sigma[6 * j + 3] + Ns[2][a];
+            } else {
+                for (int i = 0; i < 6; i++) sigma[6 * j + i] = sigma[6 * j + i] + Ns[i][a];
+            }
+        }
+    }
+    for (int j = 0; j < nn; j++) {
+        if (count[j] == 0) continue;
+        for (int i = 0; i < 6; i++) sigma[6 * j + i] = sigma[6 * j + i] / count[j];
+    }
+    if (d == 2 && M->proType == 2)
+        for (int j = 0; j < nn; j++) sigma[6 * j + 2] = M->nu * (sigma[6 * j + 0] + sigma[6 * j + 1]);
+    free(count);
     return OM_OK;
 }

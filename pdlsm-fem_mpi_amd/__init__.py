@@ -84,6 +84,8 @@ EXPORTS = {
     "pdx_solve_csr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
     "pdx_last_timing": (_I, [_P, _P, _P]),
     "pdx_explicit_tuning": (_I, [_P, _I, _P, _P, _P]),
+    "pdx_node_stresses": (_I, [_P, _P, _P]),
+    "pdx_node_stresses_timed": (_I, [_P, _P, _P]),
     "pdx_gnn_create": (_I, [_I, _I, C.c_ulonglong, _P]),
     "pdx_gnn_destroy": (None, [_P]),
     "pdx_gnn_num_params": (_LL, [_P]),
@@ -305,9 +307,25 @@ class Context:
         _chk(lib().pdx_last_timing(self.h, C.byref(t), C.byref(s)))
         return t.value, s.value
 
+    def node_stresses(self, u_all=None):
+        """Nodal stresses [nn, 6] of the DOF values u_all [nn*dim] (None: explicit state)."""
+        sig = np.zeros(self.model.nn * 6)
+        if u_all is None:
+            _chk(lib().pdx_node_stresses(self.h, None, _p(sig)))
+        else:
+            u = np.ascontiguousarray(u_all, np.float64)
+            _chk(lib().pdx_node_stresses(self.h, _p(u), _p(sig)))
+        return sig.reshape(-1, 6)
+
+    def node_stresses_timed(self, u_all) -> float:
+        u = np.ascontiguousarray(u_all, np.float64)
+        ms = C.c_double()
+        _chk(lib().pdx_node_stresses_timed(self.h, _p(u), C.byref(ms)))
+        return ms.value
+
     def explicit_tuning(self, precision):
         lanes, nt = C.c_int(0), C.c_int(0)
-        ms = np.zeros(6, np.float32)
+        ms = np.zeros(8, np.float32)
         _chk(lib().pdx_explicit_tuning(self.h, precision, C.byref(lanes), C.byref(nt), _p(ms)))
         return lanes.value, bool(nt.value), ms
 

@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result", "-Wno-unused-value"]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["model.cpp", "assemble.hip", "solve.hip", "krylov.hip", "context.hip", "newmark.hip", "csr_solve.hip",
-           "learned.hip", "family_batch.hip"]
+           "learned.hip", "family_batch.hip", "stress.hip"]
 
 
 def _compile(src: str) -> str:

@@ -20,6 +20,7 @@
 #include <cmath>
 
 #include "device.hpp"
+#include "element.hpp"
 #include "rowblock.hpp"
 
 namespace pdx {
@@ -47,6 +48,7 @@ __device__ inline double a_term(int e, double x, double y, double z, double om, 
 
 template <>
 __device__ inline double a_term<3>(int e, double x, double y, double z, double om, double vm) {
+#pragma clang fp contract(off)
     int r = e / 9, c = e % 9;
     int ex = c_pe3[r][0] + c_pe3[c][0], ey = c_pe3[r][1] + c_pe3[c][1], ez = c_pe3[r][2] + c_pe3[c][2];
     double t = 1.0;
@@ -59,6 +61,7 @@ __device__ inline double a_term<3>(int e, double x, double y, double z, double o
 
 template <>
 __device__ inline double a_term<2>(int e, double x, double y, double, double om, double vm) {
+#pragma clang fp contract(off)
     if (e == 0) return ((om * x) * x) * vm;
     int r = e / 5, c = e % 5;
     int ex = c_pe2[r][0] + c_pe2[c][0], ey = c_pe2[r][1] + c_pe2[c][1];
@@ -77,6 +80,7 @@ struct FamArgs {
     const double *Dm;
     double lam, mu, nlf;
     double *K;
+    double *fam_c;  // [members * d] strain weights mu_km omega g V of every bond (kept for stresses)
     long long nblk;
     int *status;
     int mf;  // LDS capacity in members
@@ -84,6 +88,7 @@ struct FamArgs {
 
 // normMatArea (pdsolve.cpp:2917-2967): area-weighted face normal matrix 3x6
 __device__ inline bool norm_mat_area(const double *x, const int *con, double Nm[18]) {
+#pragma clang fp contract(off)
     double xN[4][3];
     for (int a = 0; a < 4; a++)
         for (int c = 0; c < 3; c++) xN[a][c] = x[(size_t)con[a] * 3 + c];
@@ -117,6 +122,7 @@ __device__ inline bool norm_mat_area(const double *x, const int *con, double Nm[
 // pdsolve.cpp:3212-3252, :3324-3398): NV x D
 template <int D>
 __device__ inline void c_block(const double c[D], double Cb[Dm<D>::NV * D]) {
+#pragma clang fp contract(off)
     if constexpr (D == 2) {
         Cb[0] = c[0]; Cb[1] = 0;
         Cb[2] = 0; Cb[3] = c[1];
@@ -134,6 +140,7 @@ __device__ inline void c_block(const double c[D], double Cb[Dm<D>::NV * D]) {
 // 2D: 1/2 * N . (D . C)  (pdsolve.cpp:4049-4253); 3D: (Nm . D) . C  (pdsolve.cpp:2785-2915)
 template <int D>
 __device__ inline void add_pdbe_term(const FamArgs &A, int p, const double *Cb, double *val) {
+#pragma clang fp contract(off)
     constexpr int NV = Dm<D>::NV;
     const double *Dg = A.Dm;
     if constexpr (D == 2) {
@@ -177,6 +184,9 @@ __device__ inline void add_pdbe_term(const FamArgs &A, int p, const double *Cb, 
 
 template <int D>
 __global__ void __launch_bounds__(64) k_family(FamArgs A) {
+    // no FMA contraction: the shape tensor of a one-sided family is ill-conditioned, and the
+    // reference's arithmetic (separate multiply and add) must be followed to keep K to rounding
+#pragma clang fp contract(off)
     constexpr int P = Dm<D>::P, DD = Dm<D>::DD;
     extern __shared__ double lds[];
     const int f = blockIdx.x, lane = threadIdx.x;
@@ -309,8 +319,9 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
         double c[D];
 #pragma unroll
         for (int i = 0; i < D; i++) {
-            c[i] = mukm * om * s[i] * vm;
+            c[i] = mukm * om * s[i] * vm;  // matC2D / matC3D (pdsolve.cpp:3212-3252, :3324-3398)
             sc[(size_t)m * D + i] = c[i];
+            A.fam_c[(size_t)(off + m) * D + i] = c[i];
         }
         if (p1 > p0) {
             double Cb[Dm<D>::NV * D];
@@ -351,57 +362,6 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
 // ------------------------------------------------------------------------
 
 
-__device__ inline double det3(const double J[9]) {
-    // cofactor form of pdfemEles::detMat33 (pdfemEles.cpp:155-172)
-    double det = 0.0;
-    for (int i = 0; i < 3; i++) {
-        double a1 = 1., a2 = 1.;
-        for (int j = 0; j < 3; j++) {
-            a1 *= J[j * 3 + (i + j) % 3];
-            a2 *= J[j * 3 + (i - j + 3) % 3];
-        }
-        det = det + a1 - a2;
-    }
-    return det;
-}
-
-// 3x3 partial-pivot LU solve for 8 right-hand sides (PLUSolve per shape function,
-// pdfemEleBrick8N.cpp:380)
-__device__ inline bool lu3_solve8(double J[9], double dN[8][3], double out[8][3]) {
-    int piv[3];
-    for (int k = 0; k < 3; k++) {
-        int p = k;
-        double mx = fabs(J[k * 3 + k]);
-        for (int i = k + 1; i < 3; i++)
-            if (fabs(J[i * 3 + k]) > mx) { mx = fabs(J[i * 3 + k]); p = i; }
-        piv[k] = p;
-        if (J[p * 3 + k] == 0.0) return false;
-        if (p != k)
-            for (int c = 0; c < 3; c++) { double t = J[k * 3 + c]; J[k * 3 + c] = J[p * 3 + c]; J[p * 3 + c] = t; }
-        for (int i = k + 1; i < 3; i++) {
-            double l = J[i * 3 + k] / J[k * 3 + k];
-            J[i * 3 + k] = l;
-            for (int c = k + 1; c < 3; c++) J[i * 3 + c] = J[i * 3 + c] - l * J[k * 3 + c];
-        }
-    }
-    for (int a = 0; a < 8; a++) {
-        double b0 = dN[a][0], b1 = dN[a][1], b2 = dN[a][2];
-        double bb[3] = {b0, b1, b2};
-        // rows were swapped whole (multipliers included): permute first, then eliminate
-        for (int k = 0; k < 3; k++) {
-            int p = piv[k];
-            if (p != k) { double t = bb[k]; bb[k] = bb[p]; bb[p] = t; }
-        }
-        for (int k = 0; k < 3; k++)
-            for (int i = k + 1; i < 3; i++) bb[i] = bb[i] - J[i * 3 + k] * bb[k];
-        for (int i = 2; i >= 0; i--) {
-            double t = bb[i];
-            for (int c = i + 1; c < 3; c++) t = t - J[i * 3 + c] * out[a][c];
-            out[a][i] = t / J[i * 3 + i];
-        }
-    }
-    return true;
-}
 
 __constant__ double c_hp[8] = {-1., 1., 1., -1., -1., 1., 1., -1.};
 __constant__ double c_hq[8] = {-1., -1., 1., 1., -1., -1., 1., 1.};
@@ -807,6 +767,13 @@ int dev_assemble_K(Ctx &c, std::string &err) {
         A.fp_off = c.fp_off; A.fp_list = c.fp_list; A.pdbe = c.pdbe; A.Dm = c.Dm;
         A.lam = c.lambda; A.mu = c.mu; A.nlf = c.nlf; A.K = c.K; A.nblk = c.nblk; A.status = status;
         A.mf = c.max_family;
+        if (!c.fam_c) {
+            int nmem = 0;
+            PDX_HIP(hipMemcpy(&nmem, c.fam_off + c.nfam, sizeof(int), hipMemcpyDeviceToHost));
+            PDX_HIP(hipMalloc((void **)&c.fam_c, std::max<size_t>((size_t)nmem * c.dim, 1) * sizeof(double)));
+            PDX_HIP(hipMemsetAsync(c.fam_c, 0, std::max<size_t>((size_t)nmem * c.dim, 1) * sizeof(double), c.stream));
+        }
+        A.fam_c = c.fam_c;
         size_t lds = family_lds_bytes(c.dim, A.mf);
         if (lds > 160 * 1024) {
             err = "family too large for LDS (" + std::to_string(c.max_family) + " members)";

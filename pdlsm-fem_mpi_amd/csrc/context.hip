@@ -240,6 +240,8 @@ void ctx_free(Ctx *c) {
     if (c->Ke_stage) hipFree(c->Ke_stage);
     if (c->red) hipFree(c->red);
     if (c->learned_Minv) hipFree(c->learned_Minv);
+    if (c->fam_c) hipFree(c->fam_c);
+    if (c->stress_stage) hipFree(c->stress_stage);
     free_two_level(*c);
     free_gnn_work(*c);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -522,14 +524,75 @@ int pdx_last_timing(pdx_ctx *pc, double *total_ms, double *per_step_ms) {
     return PDX_OK;
 }
 
+int pdx_node_stresses(pdx_ctx *pc, const double *u_all, double *sigma) {
+    if (!pc || !sigma) return fail(PDX_ERR_ARG, "null argument");
+    Ctx &c = *pc->c;
+    std::string err;
+    const size_t n = (size_t)c.nn * c.dim;
+    double *du = nullptr, *ds = nullptr;
+    auto run = [&]() -> int {
+        PDX_HIP(hipMalloc((void **)&ds, (size_t)c.nn * 6 * sizeof(double)));
+        const double *src;
+        if (u_all) {
+            PDX_HIP(hipMalloc((void **)&du, n * sizeof(double)));
+            PDX_HIP(hipMemcpyAsync(du, u_all, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+            src = du;
+        } else {
+            if (c.steps_done < 0) { err = "no explicit state yet: pass the DOF values"; return PDX_ERR_ARG; }
+            src = c.U[c.step_phase % 3];
+        }
+        int rc = dev_node_stresses(c, src, ds, err);
+        if (rc) return rc;
+        PDX_HIP(hipMemcpyAsync(sigma, ds, (size_t)c.nn * 6 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        PDX_HIP(hipStreamSynchronize(c.stream));
+        return PDX_OK;
+    };
+    int rc = run();
+    if (du) hipFree(du);
+    if (ds) hipFree(ds);
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
+int pdx_node_stresses_timed(pdx_ctx *pc, const double *u_all, double *ms) {
+    if (!pc || !u_all || !ms) return fail(PDX_ERR_ARG, "null argument");
+    Ctx &c = *pc->c;
+    std::string err;
+    const size_t n = (size_t)c.nn * c.dim;
+    double *du = nullptr, *ds = nullptr;
+    auto run = [&]() -> int {
+        PDX_HIP(hipMalloc((void **)&ds, (size_t)c.nn * 6 * sizeof(double)));
+        PDX_HIP(hipMalloc((void **)&du, n * sizeof(double)));
+        PDX_HIP(hipMemcpyAsync(du, u_all, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        int rc = dev_node_stresses(c, du, ds, err);  // warm (assembles K if needed)
+        if (rc) return rc;
+        hipEvent_t e0, e1;
+        PDX_HIP(hipEventCreate(&e0));
+        PDX_HIP(hipEventCreate(&e1));
+        PDX_HIP(hipEventRecord(e0, c.stream));
+        rc = dev_node_stresses(c, du, ds, err);
+        PDX_HIP(hipEventRecord(e1, c.stream));
+        PDX_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        PDX_HIP(hipEventElapsedTime(&t, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        *ms = t;
+        return rc;
+    };
+    int rc = run();
+    if (du) hipFree(du);
+    if (ds) hipFree(ds);
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
 int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal, float *ms) {
     if (!pc || (precision != PDX_FP64 && precision != PDX_FP32)) return fail(PDX_ERR_ARG, "bad argument");
     const Ctx &c = *pc->c;
     const int v = c.tune_explicit[precision];
-    static const int gs[6] = {8, 4, 16, 8, 4, 16}, nt[6] = {0, 0, 0, 1, 1, 1};
+    static const int gs[8] = {8, 4, 16, 8, 4, 16, 32, 32}, nt[8] = {0, 0, 0, 1, 1, 1, 0, 1};
     if (lanes) *lanes = v >= 0 ? gs[v] : 0;
     if (nontemporal) *nontemporal = v >= 0 ? nt[v] : 0;
-    if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + 6, ms);
+    if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + 8, ms);
     return PDX_OK;
 }
 

@@ -42,6 +42,7 @@ struct Ctx {
     int *eq = nullptr, *presc_set = nullptr;  // [nn*3]
     int *fam_node = nullptr, *fam_off = nullptr, *fam_mem = nullptr, *bond = nullptr, *fam_blk = nullptr;
     double *fam_v = nullptr, *fam_delta = nullptr;
+    double *fam_c = nullptr;  // [members*d] strain weights of the bonds (written by the family kernel)
     int *fp_off = nullptr, *fp_list = nullptr, *pdbe = nullptr;
     int nfe = 0;
     int *fe_elem = nullptr, *fe_blk = nullptr, *nfe_off = nullptr, *nfe_item = nullptr;
@@ -97,6 +98,7 @@ struct Ctx {
     double *learned_Minv = nullptr;  // [nn*dd] node-block smoother predicted by the graph network
     long long learned_gen = -1;      // K_gen the smoother was predicted for
     void *gnn_work = nullptr;        // graph-network inputs and activations (learned.hip)
+    double *stress_stage = nullptr;  // [nfe*8*6 + nfam*6] element corner stresses, family stresses
 
     // helpers
     std::vector<void *> owned;
@@ -120,6 +122,7 @@ int dev_fill(Ctx &c, double *p, double v, size_t n);
 int dev_spmv_K(Ctx &c, const double *K64, const double *x, double *y);
 void free_two_level(Ctx &c);
 void free_gnn_work(Ctx &c);
+int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err);
 double two_level_setup_ms(Ctx &c);
 int set_error(int rc, const std::string &msg);
 

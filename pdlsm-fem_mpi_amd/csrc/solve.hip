@@ -101,9 +101,9 @@ int ensure_K32(Ctx &c, std::string &err) {
 }
 
 // explicit-step variants searched by the tuner: (lanes per short row, non-temporal K)
-constexpr int N_EXPLICIT_VARIANTS = 6;
-static const int kVarGS[N_EXPLICIT_VARIANTS] = {8, 4, 16, 8, 4, 16};
-static const int kVarNT[N_EXPLICIT_VARIANTS] = {0, 0, 0, 1, 1, 1};
+constexpr int N_EXPLICIT_VARIANTS = 8;
+static const int kVarGS[N_EXPLICIT_VARIANTS] = {8, 4, 16, 8, 4, 16, 32, 32};
+static const int kVarNT[N_EXPLICIT_VARIANTS] = {0, 0, 0, 1, 1, 1, 0, 1};
 
 template <int D, typename T, int GS, bool NT>
 static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double *Unm1, double *Unp1,
@@ -123,6 +123,8 @@ static void launch_explicit(Ctx &c, int variant, const T *K, const double *Un, c
     case 3: launch_explicit_v<D, T, 8, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
     case 4: launch_explicit_v<D, T, 4, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
     case 5: launch_explicit_v<D, T, 16, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 6: launch_explicit_v<D, T, 32, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 7: launch_explicit_v<D, T, 32, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
     default: launch_explicit_v<D, T, 8, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
     }
 }

@@ -19,11 +19,14 @@
   a19   Newmark driver             -> energy conservation of the average
         acceleration rule on a one-DOF model from rest (pins beta, gamma, the
         predictor and the update)
+  S9    nodal stresses             -> sigma = D eps of a uniform strain at every
+        node (PD, FE, averaging, plane-strain sigma_zz); corner values of a
+        linear stress field built from the reference's Gauss-point order and L
 """
 import numpy as np
 import pytest
 
-from helpers import box_fixture, load_oracle, one_quad, workloads
+from helpers import box_fixture, load_oracle, one_quad, tet_box_mesh, tri_tet_mesh, workloads
 
 O = load_oracle()
 W = workloads()
@@ -387,3 +390,96 @@ def test_newmark_average_acceleration_conserves_energy():
         Es.append(0.5 * mm * v[0] ** 2 + 0.5 * k * (u[0] - f / k) ** 2)
         assert np.isclose(mm * a[0] + k * u[0], f, rtol=1e-9)
     assert np.allclose(Es, Es[0], rtol=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# nodal stresses (calGlobalNodeStresses)
+# ---------------------------------------------------------------------------
+
+def _voigt_sigma(dim, E, nu, G, plane_strain=False):
+    """sigma of the linear field u = G x in the oracle's 6-slot layout."""
+    eps = 0.5 * (G + G.T)
+    if dim == 2:
+        if plane_strain:
+            lam = E * nu / ((1 + nu) * (1 - 2 * nu)); mu = E / (2 * (1 + nu))
+            D = np.array([[lam + 2 * mu, lam, 0], [lam, lam + 2 * mu, 0], [0, 0, mu]])
+        else:
+            D = _D(2, E, nu)
+        s = D @ np.array([eps[0, 0], eps[1, 1], 2 * eps[0, 1]])
+        out = np.array([s[0], s[1], 0.0, s[2], 0.0, 0.0])
+        if plane_strain:
+            out[2] = nu * (s[0] + s[1])
+        return out
+    return _D(3, E, nu) @ np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[0, 1], 2 * eps[1, 2], 2 * eps[0, 2]])
+
+
+@pytest.mark.parametrize("case", ["plate2d_stress", "plate2d_strain", "box3d", "tri_tet", "tet_box"])
+def test_node_stresses_of_uniform_strain(case):
+    """A linear displacement gives sigma = D eps at every node: PD nodes through
+    matC (polynomial reproduction), FE nodes through B at the Gauss points and
+    the extrapolation L (unit row sums), all through the averaging."""
+    rng = np.random.default_rng(3)
+    if case.startswith("plate2d"):
+        m = W.plate2d(nx=41, ny=21, dx=1e-3, pd_x=(0.015, 0.025))
+        if case == "plate2d_strain":
+            m["proType"] = 2
+        dim = 2
+    elif case == "tri_tet":
+        m, dim = tri_tet_mesh(), 2
+    elif case == "tet_box":
+        m, dim = tet_box_mesh(), 3
+    else:
+        m = W.box3d(n=12, pd_block=6)
+        dim = 3
+    M = O.Model(m)
+    G = rng.normal(size=(dim, dim)) * 1e-4
+    x = np.asarray(m["x"])[:, :dim]
+    u = (x @ G.T).ravel()
+    sig = M.node_stresses(u)
+    want = _voigt_sigma(dim, m["E"], m["nu"], G, plane_strain=(case == "plate2d_strain"))
+    assert np.abs(sig - want[None, :]).max() <= 1e-9 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("kind", ["quad4", "hex8"])
+def test_node_stresses_extrapolate_linear_stress(kind):
+    """On one square/cube element, u_x = x*y gives stresses linear in x and y.
+    Expected corner values are built here from the reference's written
+    algorithm, independently of the oracle's code: Gauss-point stresses in the
+    reference's point order, then its matrix L (pdsolve.cpp:9176-9196, :9254-9277).
+    hex8: points and L follow the node order, so the corners are exact.
+    quad4: the points are stored p-fastest, (-,-), (+,-), (-,+), (+,+), while L is
+    written for the node order (-,-), (+,-), (+,+), (-,+); the reference therefore
+    extrapolates fields varying along x with points 2 and 3 exchanged, and the
+    oracle must do the same."""
+    E, nu = 2.0e3, 0.3
+    g = 1 / np.sqrt(3)
+    if kind == "quad4":
+        dim, x = 2, np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], float)
+        gp = [(-g, -g), (g, -g), (-g, g), (g, g)]  # q outer, p inner (pdfemEleQuad4N.cpp:93-110)
+        a, b, c = 1 + 0.5 * np.sqrt(3), -0.5, 1 - 0.5 * np.sqrt(3)
+        L = np.array([[a, b, c, b], [b, a, b, c], [c, b, a, b], [b, c, b, a]])
+    else:
+        dim = 3
+        x = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+        pI, qI, rI = [-1, 1, 1, -1, -1, 1, 1, -1], [-1, -1, 1, 1, -1, -1, 1, 1], [-1, -1, -1, -1, 1, 1, 1, 1]
+        gp = [(g * pI[i], g * qI[i], g * rI[i]) for i in range(8)]
+        L = np.array([[(1 + pI[j] * np.sqrt(3) * pI[i]) * (1 + qI[j] * np.sqrt(3) * qI[i]) *
+                       (1 + rI[j] * np.sqrt(3) * rI[i]) / 8 for j in range(8)] for i in range(8)])
+    M = _single(dim, len(x), x, E, nu)
+    u = np.zeros((len(x), dim))
+    u[:, 0] = x[:, 0] * x[:, 1]
+    sig = M.node_stresses(u.ravel())
+    D = _D(dim, E, nu)
+    sgp = []
+    for pt in gp:
+        X, Y = 0.5 * (pt[0] + 1), 0.5 * (pt[1] + 1)  # unit element: natural -> physical
+        eps = np.array([Y, 0.0, X]) if dim == 2 else np.array([Y, 0.0, 0.0, X, 0.0, 0.0])
+        sgp.append(D @ eps)
+    nodal = L @ np.array(sgp)
+    for i in range(len(x)):
+        want = np.array([nodal[i, 0], nodal[i, 1], 0, nodal[i, 2], 0, 0]) if dim == 2 else nodal[i]
+        assert np.abs(sig[i] - want).max() <= 1e-10 * np.abs(nodal).max(), (i, sig[i], want)
+    if kind == "hex8":  # node-ordered points: exact corner values
+        for i in range(len(x)):
+            X, Y = x[i, 0], x[i, 1]
+            assert np.allclose(sig[i], D @ np.array([Y, 0, 0, X, 0, 0]), rtol=0, atol=1e-9 * np.abs(D).max())

@@ -9,7 +9,7 @@ import re
 import numpy as np
 import pytest
 
-from helpers import ROOT, box_fixture, load_oracle, load_pdx, workloads
+from helpers import tri_tet_mesh, ROOT, box_fixture, load_oracle, load_pdx, workloads
 
 O = load_oracle()
 W = workloads()
@@ -54,24 +54,6 @@ def test_setup_matches_oracle(name):
         assert (mo.families()["bond"] == 0).sum() > 0
     if name == "penny3d":
         assert (mp.families()["bond"] == 0).sum() > 0
-
-
-def tri_tet_mesh():
-    """Mixed 2D mesh: quads in the left half, each right-half quad split into two triangles."""
-    m = W.plate2d(nx=31, ny=17, dx=1e-3, pd_x=(0.008, 0.02))
-    econ, nen, algo = [], [], []
-    for e in range(len(m["econ"])):
-        c = m["econ"][e]
-        cx = m["x"][c[:4] - 1, 0].mean()
-        if cx > 0.015:
-            for tri in ([c[0], c[1], c[2]], [c[0], c[2], c[3]]):
-                row = np.zeros(8, np.int32)
-                row[:3] = tri
-                econ.append(row); nen.append(3); algo.append(m["algo"][e])
-        else:
-            econ.append(c); nen.append(4); algo.append(m["algo"][e])
-    m.update(econ=np.array(econ), nen=np.array(nen, np.int32), algo=np.array(algo, np.int32))
-    return m
 
 
 def test_box_example_setup_matches_oracle():
