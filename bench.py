@@ -118,6 +118,33 @@ def measured_traffic(kernel):
     return k["traffic_bytes"] if k else None
 
 
+def newmark_bench(pdx, W, n=100, steps=50):
+    """The penny-crack Newmark configuration: 100^3 nodes (dx = 1 mm), penny
+    crack of radius 0.2 L between two node layers, PD region the central 30^3
+    nodes, bottom fixed, top traction ramped linearly to 5 MPa over the 50 steps,
+    beta = 1/4, gamma = 1/2, dt = 10x the explicit critical step, a_{n+1}
+    warm-started; each step solves (M + beta dt^2 K) a = F - K U_N with
+    node-block-Jacobi BiCGStab to a 1e-8 true residual."""
+    t0 = time.time()
+    m0 = W.penny3d(n=n)
+    cx0 = pdx.Context(pdx.Model(m0))
+    lmax = cx0.max_eig(200)
+    dt = 10.0 * 2.0 / np.sqrt(lmax)
+    del cx0
+    m = W.penny3d(n=n, dt=dt, ramp_steps=steps)
+    mp = pdx.Model(m)
+    cx = pdx.Context(mp)
+    setup_s = time.time() - t0
+    opts = pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8, max_iter=100000)
+    t1 = time.time()
+    u, v, a, st = cx.newmark_run(steps, opts)
+    wall = time.time() - t1
+    return {"workload": f"penny3d {n}^3 nodes, PD 30^3, dt = 10 dt_crit = {dt:.3e} s, {steps} Newmark steps",
+            "neq": mp.neq, "host_setup_s": setup_s, "seconds": wall, "ms_per_step": wall / steps * 1e3,
+            "krylov_iterations": st["iterations"], "last_true_relres": st["true_relres"],
+            "dof_steps_per_s": mp.neq * steps / wall, "u_max": float(np.abs(u).max())}
+
+
 def family_batch_bench(pdx, W):
     """The 2,000,376-node PD cloud (126^3 jittered grid, per-node horizons
     U(2.5, 3.5) dx, members by the OR-of-horizons rule): shape tensor + LU (FP64)
@@ -158,6 +185,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-fb", action="store_true", help="skip the 2M-node family batch")
+    ap.add_argument("--no-newmark", action="store_true", help="skip the 100^3 penny-crack Newmark run")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -201,8 +229,8 @@ def main():
     prec = 1
     bytes_step = step_bytes(model, nblk, prec)
     achieved = bytes_step / (res[prec]["step_ms"] * 1e-3)
-    lanes, nt, tune_ms = ctxs[prec].explicit_tuning(prec)
-    kname = f"pdx::k_explicit<{model.dim}, float, {lanes}, {'true' if nt else 'false'}>"
+    lanes, nt, xm, tune_ms = ctxs[prec].explicit_tuning(prec)
+    kname = f"pdx::k_explicit<{model.dim}, float, {lanes}, {'true' if nt else 'false'}, {'true' if xm else 'false'}>"
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": measured_traffic(kname) if args.n == 60 else None,
                 "kernel": kname + " (node-block SpMV + central-difference update, FP32 values)",
@@ -243,6 +271,9 @@ def main():
                 solves[name] = {"error": str(e)}
             log(name, solves[name])
         out["solve_to_1e-8"] = {"workload": f"crack2d 501x251 seed 0, neq {smod.neq}", **solves}
+    if not args.no_newmark:
+        out["newmark_penny"] = newmark_bench(pdx, W)
+        log("newmark", out["newmark_penny"])
     if not args.no_fb:
         out["family_batch"] = family_batch_bench(pdx, W)
         log("family batch", out["family_batch"])

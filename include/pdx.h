@@ -112,7 +112,7 @@ typedef struct pdx_solve_opts {
     int max_iter;      /* total inner iterations */
     double inner_tol;  /* FP32 inner relative tolerance (refinement), e.g. 1e-4 */
     int warm_start;    /* use x as the initial guess */
-    int coarse_agg;    /* two-level: aggregate edge in node spacings (0: 16 in 2D, 6 in 3D) */
+    int coarse_agg;    /* two-level: aggregate edge in node spacings (0: 10 in 2D, 6 in 3D) */
 } pdx_solve_opts;
 
 typedef struct pdx_solve_stats {
@@ -224,8 +224,9 @@ int pdx_gnn_apply(pdx_ctx *c, const pdx_gnn *g);
  * cloud with per-node horizons.  Members by the rule of initialPDN_Fami
  * (pdsolve.cpp:1442-1568): j joins k iff |x_j - x_k| < delta_k or < delta_j
  * (self excluded), self first, then in cell-scan order.  Results stay on the
- * device: A and its partial-pivot LU (FP64, p x p per family, p = 5 / 9),
- * pivots, and H (FP32, d x d per member; member 0 holds -sum of the others). */
+ * device: A (FP64, p x p per family, p = 5 / 9), the lower Cholesky factor L of
+ * its symmetric part S (A = S diag(s), s = 1/2 on the squared terms, S = L L^T),
+ * and H (FP32, d x d per member; member 0 holds minus the sum of the others). */
 typedef struct pdx_fbatch pdx_fbatch;
 int pdx_fb_create(int device, int dim, long long n, const double *xyz /*[n*3]*/, const double *vol /*[n]*/,
                   const double *delta /*[n]*/, double nlf, double lambda, double mu, pdx_fbatch **out);
@@ -233,11 +234,11 @@ void pdx_fb_destroy(pdx_fbatch *b);
 int pdx_fb_sizes(const pdx_fbatch *b, long long *n_fam, long long *n_bond, int *max_family);
 int pdx_fb_families(const pdx_fbatch *b, long long *off /*[n+1]*/, int *mem /*[n_bond]*/);
 /* variant -1: the fastest persistent launch (timed on the first call); 0: one
- * family per 64-thread block; 1..4: persistent launch forms. ms: kernel time */
+ * family per 64-thread block; 1..3: persistent, 4 / 2 / 1 waves per block.
+ * ms: kernel time */
 int pdx_fb_run(pdx_fbatch *b, int variant, float *ms);
-int pdx_fb_tuning(const pdx_fbatch *b, int *best_variant, float *ms /*[5]*/);
-int pdx_fb_results(const pdx_fbatch *b, long long first, long long count, double *A, double *LU, int *piv,
-                   float *H);
+int pdx_fb_tuning(const pdx_fbatch *b, int *best_variant, float *ms /*[4]*/);
+int pdx_fb_results(const pdx_fbatch *b, long long first, long long count, double *A, double *L, float *H);
 
 /* PARDISO seam: solve A x = b for a zero-based int64 CSR (n, ia, ja, a). */
 int pdx_solve_csr(int device, long long n, const long long *ia, const long long *ja, const double *a,
@@ -246,9 +247,10 @@ int pdx_solve_csr(int device, long long n, const long long *ia, const long long 
 /* timing of the last pdx_explicit_steps call measured with HIP events on the
  * context stream: total ms and per-step ms */
 int pdx_last_timing(pdx_ctx *c, double *total_ms, double *per_step_ms);
-/* the explicit-step kernel form the tuner chose (lanes per short row, non-temporal K loads;
- * 0/0 before the first step) and its timings of the 8 forms (ms per step) */
-int pdx_explicit_tuning(pdx_ctx *c, int precision, int *lanes, int *nontemporal, float *ms);
+/* the explicit-step kernel form the tuner chose (lanes per short row, non-temporal K loads,
+ * XCD-contiguous block order; 0/0/0 before the first step) and its timings of the 10 forms
+ * (ms per step) */
+int pdx_explicit_tuning(pdx_ctx *c, int precision, int *lanes, int *nontemporal, int *xcd_order, float *ms);
 
 #ifdef __cplusplus
 }

@@ -83,7 +83,7 @@ EXPORTS = {
     "pdx_newmark_run": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "pdx_solve_csr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
     "pdx_last_timing": (_I, [_P, _P, _P]),
-    "pdx_explicit_tuning": (_I, [_P, _I, _P, _P, _P]),
+    "pdx_explicit_tuning": (_I, [_P, _I, _P, _P, _P, _P]),
     "pdx_node_stresses": (_I, [_P, _P, _P]),
     "pdx_node_stresses_timed": (_I, [_P, _P, _P]),
     "pdx_gnn_create": (_I, [_I, _I, C.c_ulonglong, _P]),
@@ -101,7 +101,7 @@ EXPORTS = {
     "pdx_fb_families": (_I, [_P, _P, _P]),
     "pdx_fb_run": (_I, [_P, _I, _P]),
     "pdx_fb_tuning": (_I, [_P, _P, _P]),
-    "pdx_fb_results": (_I, [_P, _LL, _LL, _P, _P, _P, _P]),
+    "pdx_fb_results": (_I, [_P, _LL, _LL, _P, _P, _P]),
 }
 
 
@@ -324,10 +324,10 @@ class Context:
         return ms.value
 
     def explicit_tuning(self, precision):
-        lanes, nt = C.c_int(0), C.c_int(0)
-        ms = np.zeros(8, np.float32)
-        _chk(lib().pdx_explicit_tuning(self.h, precision, C.byref(lanes), C.byref(nt), _p(ms)))
-        return lanes.value, bool(nt.value), ms
+        lanes, nt, xm = C.c_int(0), C.c_int(0), C.c_int(0)
+        ms = np.zeros(10, np.float32)
+        _chk(lib().pdx_explicit_tuning(self.h, precision, C.byref(lanes), C.byref(nt), C.byref(xm), _p(ms)))
+        return lanes.value, bool(nt.value), bool(xm.value), ms
 
     def max_eig(self, iters=100):
         v = C.c_double(0)
@@ -458,7 +458,7 @@ class FamilyBatch:
 
     def tuning(self):
         best = C.c_int()
-        ms = np.zeros(5, np.float32)
+        ms = np.zeros(4, np.float32)
         _chk(lib().pdx_fb_tuning(self.h, C.byref(best), _p(ms)))
         return best.value, ms
 
@@ -469,8 +469,7 @@ class FamilyBatch:
             off, _ = self.families()
         nb = int(off[first + count] - off[first])
         A = np.zeros((count, p, p))
-        LU = np.zeros((count, p, p))
-        piv = np.zeros((count, p), np.int32)
+        L = np.zeros((count, p, p))
         H = np.zeros((max(nb, 1), dd), np.float32)
-        _chk(lib().pdx_fb_results(self.h, first, count, _p(A), _p(LU), _p(piv), _p(H)))
-        return A, LU, piv, H[:nb]
+        _chk(lib().pdx_fb_results(self.h, first, count, _p(A), _p(L), _p(H)))
+        return A, L, H[:nb]

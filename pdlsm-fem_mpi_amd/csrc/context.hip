@@ -585,14 +585,16 @@ int pdx_node_stresses_timed(pdx_ctx *pc, const double *u_all, double *ms) {
     return rc ? fail(rc, err) : PDX_OK;
 }
 
-int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal, float *ms) {
+int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal, int *xcd_order, float *ms) {
     if (!pc || (precision != PDX_FP64 && precision != PDX_FP32)) return fail(PDX_ERR_ARG, "bad argument");
     const Ctx &c = *pc->c;
     const int v = c.tune_explicit[precision];
-    static const int gs[8] = {8, 4, 16, 8, 4, 16, 32, 32}, nt[8] = {0, 0, 0, 1, 1, 1, 0, 1};
+    static const int gs[10] = {8, 4, 16, 8, 4, 16, 32, 32, 16, 8}, nt[10] = {0, 0, 0, 1, 1, 1, 0, 1, 0, 0},
+                     xm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1};
     if (lanes) *lanes = v >= 0 ? gs[v] : 0;
     if (nontemporal) *nontemporal = v >= 0 ? nt[v] : 0;
-    if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + 8, ms);
+    if (xcd_order) *xcd_order = v >= 0 ? xm[v] : 0;
+    if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + 10, ms);
     return PDX_OK;
 }
 
@@ -604,7 +606,7 @@ int pdx_max_eig(pdx_ctx *pc, int iters, double *lmax) {
 }
 
 static pdx_solve_opts default_opts() {
-    pdx_solve_opts o;
+    pdx_solve_opts o{};
     o.krylov = PDX_KRYLOV_BICGSTAB;
     o.precond = PDX_PREC_JACOBI;
     o.precision = PDX_FP64;

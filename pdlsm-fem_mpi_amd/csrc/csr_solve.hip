@@ -94,7 +94,7 @@ static int csr_solve(int device, long long n, const long long *ia, const long lo
 
 extern "C" int pdx_solve_csr(int device, long long n, const long long *ia, const long long *ja, const double *a,
                              const double *b, double *x, const pdx_solve_opts *op, pdx_solve_stats *st) {
-    pdx_solve_opts o;
+    pdx_solve_opts o{};
     if (op) o = *op;
     else {
         o.krylov = PDX_KRYLOV_BICGSTAB; o.precond = PDX_PREC_JACOBI; o.precision = PDX_FP64;

@@ -86,7 +86,7 @@ struct Ctx {
     long long steps_done = -1;  // -1: start-up step not yet taken
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
     int tune_explicit[2] = {-1, -1};               // chosen step-kernel variant per precision
-    float tune_ms[2][8] = {{0}};                   // measured ms per variant
+    float tune_ms[2][12] = {{0}};                  // measured ms per variant
     int graph_phase[2] = {-1, -1};
 
     // scratch for Krylov

@@ -1,23 +1,23 @@
 // Batched PDLSM family kernel: every node of a PD node cloud is a family with
-// its own horizon; per family the shape tensor A (FP64, p x p), its
-// partial-pivot LU and the bond blocks H_m = G_m V_m (FP32, d x d per member,
-// self block H_0 = -sum_m H_m) -- shapTens3D, vec_gd3D + PLUSolve, matG3D and
+// its own horizon; per family the shape tensor A (FP64, p x p), a Cholesky
+// factor of its symmetric part, and the bond blocks H_m = G_m V_m (FP32, d x d
+// per member, self block H_0 = -sum_m H_m) -- shapTens3D, vec_gd3D, matG3D and
 // matH3D of pdsolve.cpp:1938-2416 for a whole batch.
 //
-// Two launch forms of the same per-family code:
-//   * one family per 64-thread block (grid = number of families);
-//   * persistent: a grid sized to the device's resident waves; each wave takes
-//     the next families from a global counter (one atomic per wave), lanes
-//     per family LPF in {64, 32} (1 or 2 families per wave) and 1 to 4 waves
-//     per block; the fastest form is picked by timing them on the batch.
-// Lanes of a family cooperate through a per-family LDS slot; a family never
-// spans waves, so only wave-level ordering is needed (no block barriers in the
-// persistent loop).
-//
-// The shape tensor is accumulated as A_rc = sum_m (w_m p_r p_c) s_c with
-// p = [x,y,z,x^2,y^2,z^2,xy,yz,zx] (2D: [x,y,x^2,y^2,xy]), s_c = 1/2 on the
-// squared terms and w_m = omega_m V_m, i.e. the reference's terms in a
-// different (symmetric) evaluation order; member order is the batch's.
+// A = S diag(s) with S = sum_m w_m p_m p_m^T symmetric positive definite
+// (w_m = omega_m V_m > 0, p = [x,y,z,x^2,y^2,z^2,xy,yz,zx], 2D [x,y,x^2,y^2,xy])
+// and s = 1/2 on the squared terms.  So per family:
+//   * S on the matrix cores: one wave, v_mfma_f64_16x16x4f64 over the members
+//     four at a time (a lane holds w p_i and p_i of one member for the A and B
+//     operands; the 9x9 / 5x5 result sits in the 16x16 tile);
+//   * S = L L^T by one lane in registers (no pivoting is needed for an SPD
+//     matrix; the reference's per-bond dgesv factors A with partial pivoting,
+//     which gives the same solution to rounding);
+//   * every bond on its own lane: two triangular solves with the shared L
+//     (reciprocal diagonal), x = y / s, then G and H.
+// Launch forms of the same per-family code: one family per 64-thread block, or
+// persistent waves pulling families from an atomic counter with 1, 2 or 4
+// waves per block; the fastest is timed once per batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,8 +49,7 @@ struct FbArgs {
     const int *mem;                 // members, self first
     double nlf, lam, mu;
     int mf;                         // largest family (LDS slot size)
-    double *A, *LU;                 // [nf*P*P]
-    int *piv;                       // [nf*P]
+    double *A, *LU;                 // [nf*P*P]: A; lower Cholesky factor of its symmetric part
     float *H;                       // [off[nf]*D*D]
     int *status;                    // bit 1: singular shape tensor, bit 8: family larger than mf
     unsigned long long *counter;    // persistent work counter
@@ -79,30 +78,32 @@ __device__ inline bool is_square(int c) {
 }
 
 // slot layout (doubles): member basis values [mf*P], weights w = omega V [mf],
-// LU [P*P], pivots [P] (+ singular flag)
+// S / L [16*16], reciprocal diagonal of L [P] (+ singular flag)
 template <int D>
 __host__ __device__ constexpr int fb_slot_doubles(int mf) {
-    return mf * (FbDim<D>::P + 1) + FbDim<D>::P * FbDim<D>::P + FbDim<D>::P + 1;
+    return mf * (FbDim<D>::P + 1) + 256 + FbDim<D>::P + 1;
 }
 
-// one family on LPF lanes (sl = lane within the family); slot = its LDS region
-template <int D, int LPF>
-__device__ void fb_family(const FbArgs &a, long long f, int sl, double *slot) {
+typedef double fb_d4 __attribute__((ext_vector_type(4)));
+
+// one family on one wave; slot = its LDS region
+template <int D>
+__device__ void fb_family(const FbArgs &a, long long f, int lane, double *slot) {
     constexpr int P = FbDim<D>::P, PP = P * P, DD = FbDim<D>::DD;
     const long long o0 = a.off[f];
     const int n = int(a.off[f + 1] - o0);
     if (n > a.mf) {
-        if (sl == 0) atomicOr(a.status, 8);
+        if (lane == 0) atomicOr(a.status, 8);
         return;
     }
-    double *sb = slot;                  // [mf*P] basis values p(xi_m)
-    double *sw = slot + a.mf * P;       // [mf] omega_m V_m
-    double *sA = sw + a.mf;             // [PP]
-    double *spv = sA + PP;              // [P] pivots (as doubles) + singular flag
+    double *sb = slot;             // [mf*P] basis values p(xi_m)
+    double *sw = slot + a.mf * P;  // [mf] omega_m V_m (0 for the self member)
+    double *sS = sw + a.mf;        // [16*16] S, then L (row-major, ld 16)
+    double *srd = sS + 256;        // [P] 1 / L_ii, [P]: singular flag
     const int k = a.mem[o0];
     const double xk = a.x[(size_t)k * 3], yk = a.x[(size_t)k * 3 + 1], zk = a.x[(size_t)k * 3 + 2];
     const double delt = a.delta[k], Aa = a.nlf * a.nlf;
-    for (int m = sl; m < n; m += LPF) {
+    for (int m = lane; m < n; m += 64) {
         const int j = a.mem[o0 + m];
         const double x0 = a.x[(size_t)j * 3] - xk, x1 = a.x[(size_t)j * 3 + 1] - yk, x2 = a.x[(size_t)j * 3 + 2] - zk;
         double p[P];
@@ -113,94 +114,96 @@ __device__ void fb_family(const FbArgs &a, long long f, int sl, double *slot) {
         sw[m] = m == 0 ? 0.0 : exp(-(x0 * x0 + x1 * x1 + x2 * x2) / Aa / delt / delt) * a.vol[j];
     }
     wave_sync();
-    // symmetric part S_rc = sum_m w p_r p_c, entries r <= c over the lanes
-    constexpr int NS = P * (P + 1) / 2;
-    for (int e = sl; e < NS; e += LPF) {
-        int r = 0, c = e;
-        while (c >= P - r) { c -= P - r; r++; }
-        c += r;
-        double acc = 0.0;
-        for (int m = 1; m < n; m++) acc += sw[m] * sb[m * P + r] * sb[m * P + c];
-        const double arc = is_square<D>(c) ? 0.5 * acc : acc, acr = is_square<D>(r) ? 0.5 * acc : acc;
-        sA[r * P + c] = arc;
-        sA[c * P + r] = acr;
-        a.A[f * PP + r * P + c] = arc;
-        a.A[f * PP + c * P + r] = acr;
+    // S = sum_m w_m p_m p_m^T on the matrix cores: A-operand lane (i = lane%16, member k0 + lane/16)
+    // holds w p_i, the B-operand lane (member k0 + lane/16, column j = lane%16 = i) holds p_i
+    const int i16 = lane & 15, kq = lane >> 4;
+    fb_d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < n; k0 += 4) {
+        const int m = k0 + kq;
+        const bool ok = m < n && i16 < P;
+        const double bi = ok ? sb[m * P + i16] : 0.0;
+        const double wm = m < n ? sw[m] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wm * bi, bi, acc, 0, 0, 0);
+    }
+    // result layout (checked on gfx950): lane l item r = S[l/16 + 4 r][l%16]
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int row = kq + 4 * r, col = i16;
+        sS[row * 16 + col] = acc[r];
+        if (row < P && col < P) a.A[f * PP + row * P + col] = is_square<D>(col) ? 0.5 * acc[r] : acc[r];
     }
     wave_sync();
-    // partial-pivot LU (first maximum), one lane
-    if (sl == 0) {
+    // Cholesky S = L L^T in registers on one lane (fixed indices, fully unrolled)
+    if (lane == 0) {
+        double L[PP];
+#pragma unroll
+        for (int r = 0; r < P; r++)
+#pragma unroll
+            for (int c = 0; c < P; c++) L[r * P + c] = c <= r ? sS[r * 16 + c] : 0.0;
         int sing = 0;
-        for (int kk = 0; kk < P; kk++) {
-            int p = kk;
-            double mx = fabs(sA[kk * P + kk]);
-            for (int i = kk + 1; i < P; i++) {
-                const double v = fabs(sA[i * P + kk]);
-                if (v > mx) { mx = v; p = i; }
-            }
-            spv[kk] = p;
-            if (sA[p * P + kk] == 0.0) { sing = 1; break; }
-            if (p != kk)
-                for (int c = 0; c < P; c++) {
-                    const double t = sA[kk * P + c];
-                    sA[kk * P + c] = sA[p * P + c];
-                    sA[p * P + c] = t;
-                }
-            for (int i = kk + 1; i < P; i++) {
-                const double l = sA[i * P + kk] / sA[kk * P + kk];
-                sA[i * P + kk] = l;
-                for (int c = kk + 1; c < P; c++) sA[i * P + c] = sA[i * P + c] - l * sA[kk * P + c];
+#pragma unroll
+        for (int c = 0; c < P; c++) {
+            double d = L[c * P + c];
+#pragma unroll
+            for (int q = 0; q < c; q++) d -= L[c * P + q] * L[c * P + q];
+            sing |= !(d > 0.0);
+            const double lc = sqrt(d > 0.0 ? d : 1.0);
+            const double il = 1.0 / lc;
+            L[c * P + c] = lc;
+            srd[c] = il;
+#pragma unroll
+            for (int r = c + 1; r < P; r++) {
+                double t = L[r * P + c];
+#pragma unroll
+                for (int q = 0; q < c; q++) t -= L[r * P + q] * L[c * P + q];
+                L[r * P + c] = t * il;
             }
         }
-        spv[P] = sing;
+#pragma unroll
+        for (int e = 0; e < PP; e++) sS[(e / P) * 16 + (e % P)] = L[e];
+        srd[P] = sing;
     }
     wave_sync();
-    for (int e = sl; e < PP; e += LPF) a.LU[f * PP + e] = sA[e];
-    for (int e = sl; e < P; e += LPF) a.piv[f * P + e] = int(spv[e]);
-    if (spv[P] != 0.0) {
-        if (sl == 0) atomicOr(a.status, 1);
+    for (int e = lane; e < PP; e += 64) a.LU[f * PP + e] = sS[(e / P) * 16 + (e % P)];
+    if (srd[P] != 0.0) {
+        if (lane == 0) atomicOr(a.status, 1);
         return;
     }
-    int pv[P];
-#pragma unroll
-    for (int kk = 0; kk < P; kk++) pv[kk] = int(spv[kk]);
     const double lam = a.lam, mu = a.mu;
     double hs[DD];
 #pragma unroll
     for (int e = 0; e < DD; e++) hs[e] = 0.0;
-    for (int m = 1 + sl; m < n; m += LPF) {
-        double b[P];
+    for (int m = 1 + lane; m < n; m += 64) {
+        // S y = p_m: L z = p_m, L^T y = z; then x = y / s (A = S diag(s))
+        double z[P];
 #pragma unroll
-        for (int q = 0; q < P; q++) b[q] = sb[m * P + q];
-        // the LU swapped whole rows (multipliers included): permute b completely, then eliminate
+        for (int r = 0; r < P; r++) {
+            double t = sb[m * P + r];
 #pragma unroll
-        for (int kk = 0; kk < P; kk++) {
-#pragma unroll
-            for (int i = kk + 1; i < P; i++)
-                if (pv[kk] == i) { const double t = b[kk]; b[kk] = b[i]; b[i] = t; }
+            for (int q = 0; q < r; q++) t -= sS[r * 16 + q] * z[q];
+            z[r] = t * srd[r];
         }
+        double y[P];
 #pragma unroll
-        for (int kk = 0; kk < P; kk++)
+        for (int r = P - 1; r >= 0; r--) {
+            double t = z[r];
 #pragma unroll
-            for (int i = kk + 1; i < P; i++) b[i] = b[i] - sA[i * P + kk] * b[kk];
-        double s[P];
-#pragma unroll
-        for (int i = P - 1; i >= 0; i--) {
-            double t = b[i];
-#pragma unroll
-            for (int c = i + 1; c < P; c++) t = t - sA[i * P + c] * s[c];
-            s[i] = t / sA[i * P + i];
+            for (int q = r + 1; q < P; q++) t -= sS[q * 16 + r] * y[q];
+            y[r] = t * srd[r];
         }
+        double sx[P];
+#pragma unroll
+        for (int r = 0; r < P; r++) sx[r] = is_square<D>(r) ? 2.0 * y[r] : y[r];
         // matG (pdsolve.cpp:2277-2347) times omega V: H_m = G_m V_m (all bonds intact)
         const double w = sw[m];
         double G[DD];
         if constexpr (D == 2) {
-            const double *dd = s + 2;
+            const double *dd = sx + 2;
             G[0] = w * ((2 * mu + lam) * dd[0] + mu * dd[1]);
             G[1] = G[2] = w * (lam + mu) * dd[2];
             G[3] = w * ((2 * mu + lam) * dd[1] + mu * dd[0]);
         } else {
-            const double *dd = s + 3;
+            const double *dd = sx + 3;
             const double tr = dd[0] + dd[1] + dd[2];
             G[0] = w * ((mu + lam) * dd[0] + mu * tr);
             G[1] = G[3] = w * (lam + mu) * dd[3];
@@ -216,14 +219,14 @@ __device__ void fb_family(const FbArgs &a, long long f, int sl, double *slot) {
             hs[e] += G[e];
         }
     }
-    // self block: reduction over the family's lanes in a fixed tree
+    // self block: reduction over the wave in a fixed tree
 #pragma unroll
     for (int e = 0; e < DD; e++) {
         double v = hs[e];
-        for (int s = LPF / 2; s > 0; s >>= 1) v += __shfl_xor(v, s, LPF);
+        for (int sft = 32; sft > 0; sft >>= 1) v += __shfl_xor(v, sft, 64);
         hs[e] = v;
     }
-    if (sl == 0) {
+    if (lane == 0) {
         float *h = a.H + (size_t)o0 * DD;
 #pragma unroll
         for (int e = 0; e < DD; e++) h[e] = float(-hs[e]);
@@ -234,24 +237,21 @@ template <int D>
 __global__ void __launch_bounds__(64) k_fb_block(FbArgs a) {
     extern __shared__ double lds[];
     const long long f = blockIdx.x;
-    if (f < a.nf) fb_family<D, 64>(a, f, threadIdx.x, lds);
+    if (f < a.nf) fb_family<D>(a, f, threadIdx.x, lds);
 }
 
-// persistent: each wave pulls FPW = 64/LPF families per atomic fetch until the batch is drained
-template <int D, int LPF>
+// persistent: every wave pulls the next family per atomic fetch until the batch is drained
+template <int D>
 __global__ void __launch_bounds__(256) k_fb_persistent(FbArgs a) {
-    constexpr int FPW = 64 / LPF;
     extern __shared__ double lds[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int sub = lane / LPF, sl = lane % LPF;
-    double *slot = lds + (size_t)(wave * FPW + sub) * fb_slot_doubles<D>(a.mf);
+    double *slot = lds + (size_t)wave * fb_slot_doubles<D>(a.mf);
     for (;;) {
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)FPW);
-        base = __shfl(base, 0, 64);
-        if (base >= (unsigned long long)a.nf) break;  // uniform across the wave: every wave exits
-        const long long f = (long long)base + sub;
-        if (f < a.nf) fb_family<D, LPF>(a, f, sl, slot);
+        unsigned long long f = 0;
+        if (lane == 0) f = atomicAdd(a.counter, 1ull);
+        f = __shfl(f, 0, 64);
+        if (f >= (unsigned long long)a.nf) break;  // uniform across the wave: every wave exits
+        fb_family<D>(a, (long long)f, lane, slot);
         wave_sync();
     }
 }
@@ -271,7 +271,7 @@ struct pdx_fbatch {  // handle behind the opaque C-ABI pointer
     double nlf = 0.33, lam = 0, mu = 0;
     double *x = nullptr, *vol = nullptr, *delta = nullptr, *A = nullptr, *LU = nullptr;
     long long *doff = nullptr;
-    int *dmem = nullptr, *piv = nullptr, *status = nullptr;
+    int *dmem = nullptr, *status = nullptr;
     float *H = nullptr;
     unsigned long long *counter = nullptr;
     int tuned = -1;
@@ -282,9 +282,8 @@ struct pdx_fbatch {  // handle behind the opaque C-ABI pointer
 
 namespace pdx {
 
-// variants: 0 one family per block; 1 persistent LPF 64, 4 waves/block; 2 LPF 64, 2 waves;
-// 3 LPF 32, 2 waves; 4 LPF 32, 1 wave
-constexpr int FB_VARIANTS = 5;
+// launch forms: 0 one family per block; 1..3 persistent with 4, 2, 1 waves per block
+constexpr int FB_VARIANTS = 4;
 
 static FbArgs fb_args(pdx_fbatch &b) {
     FbArgs a;
@@ -300,28 +299,26 @@ static FbArgs fb_args(pdx_fbatch &b) {
     a.mf = b.mf;
     a.A = b.A;
     a.LU = b.LU;
-    a.piv = b.piv;
     a.H = b.H;
     a.status = b.status;
     a.counter = b.counter;
     return a;
 }
 
-template <int D, int LPF, int WPB>
+template <int D, int WPB>
 static hipError_t launch_persistent(pdx_fbatch &b, FbArgs a) {
-    constexpr int FPW = 64 / LPF;
-    const size_t lds = (size_t)WPB * FPW * fb_slot_doubles<D>(b.mf) * sizeof(double);
+    const size_t lds = (size_t)WPB * fb_slot_doubles<D>(b.mf) * sizeof(double);
     if (lds > b.lds_max) return hipErrorInvalidValue;  // this form does not fit the families
-    hipError_t e = hipFuncSetAttribute((const void *)k_fb_persistent<D, LPF>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute((const void *)k_fb_persistent<D>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
     if (e != hipSuccess) return e;
     int per_sm = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_sm, (const void *)k_fb_persistent<D, LPF>, WPB * 64, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_sm, (const void *)k_fb_persistent<D>, WPB * 64, lds);
     if (e != hipSuccess) return e;
     const int grid = std::max(1, per_sm) * b.n_sm;
     e = hipMemsetAsync(b.counter, 0, sizeof(unsigned long long), b.stream);
     if (e != hipSuccess) return e;
-    k_fb_persistent<D, LPF><<<grid, WPB * 64, lds, b.stream>>>(a);
+    k_fb_persistent<D><<<grid, WPB * 64, lds, b.stream>>>(a);
     return hipGetLastError();
 }
 
@@ -329,10 +326,9 @@ template <int D>
 static hipError_t fb_launch(pdx_fbatch &b, int variant) {
     FbArgs a = fb_args(b);
     switch (variant) {
-    case 1: return launch_persistent<D, 64, 4>(b, a);
-    case 2: return launch_persistent<D, 64, 2>(b, a);
-    case 3: return launch_persistent<D, 32, 2>(b, a);
-    case 4: return launch_persistent<D, 32, 1>(b, a);
+    case 1: return launch_persistent<D, 4>(b, a);
+    case 2: return launch_persistent<D, 2>(b, a);
+    case 3: return launch_persistent<D, 1>(b, a);
     default: {
         const size_t lds = (size_t)fb_slot_doubles<D>(b.mf) * sizeof(double);
         if (lds > b.lds_max) return hipErrorInvalidValue;
@@ -442,7 +438,7 @@ void pdx_fb_destroy(pdx_fbatch *b) {
     if (!b) return;
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
-    void *ptrs[] = {b->x, b->vol, b->delta, b->A, b->LU, b->doff, b->dmem, b->piv, b->status, b->H, b->counter};
+    void *ptrs[] = {b->x, b->vol, b->delta, b->A, b->LU, b->doff, b->dmem, b->status, b->H, b->counter};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);
@@ -480,7 +476,6 @@ int pdx_fb_create(int device, int dim, long long n, const double *xyz, const dou
         PDX_HIP(hipMalloc((void **)&b->dmem, std::max<long long>(b->nbond, 1) * 4));
         PDX_HIP(hipMalloc((void **)&b->A, n * P * P * 8));
         PDX_HIP(hipMalloc((void **)&b->LU, n * P * P * 8));
-        PDX_HIP(hipMalloc((void **)&b->piv, n * P * 4));
         PDX_HIP(hipMalloc((void **)&b->H, std::max<long long>(b->nbond, 1) * DD * 4));
         PDX_HIP(hipMalloc((void **)&b->status, 4));
         PDX_HIP(hipMalloc((void **)&b->counter, 8));
@@ -567,7 +562,7 @@ int pdx_fb_run(pdx_fbatch *b, int variant, float *ms) {
         int st = 0;
         PDX_HIP(hipMemcpy(&st, b->status, 4, hipMemcpyDeviceToHost));
         if (st & 8) { err = "pdx_fb_run: family larger than the LDS slot"; return PDX_ERR_ARG; }
-        if (st & 1) { err = "pdx_fb_run: singular shape tensor"; return PDX_ERR_SINGULAR; }
+        if (st & 1) { err = "pdx_fb_run: shape tensor not positive definite (degenerate family)"; return PDX_ERR_SINGULAR; }
         return PDX_OK;
     };
     int rc = run();
@@ -581,7 +576,7 @@ int pdx_fb_tuning(const pdx_fbatch *b, int *best, float *ms) {
     return PDX_OK;
 }
 
-int pdx_fb_results(const pdx_fbatch *b, long long first, long long count, double *A, double *LU, int *piv, float *H) {
+int pdx_fb_results(const pdx_fbatch *b, long long first, long long count, double *A, double *L, float *H) {
     if (!b || first < 0 || count < 0 || first + count > b->nf) return set_error(PDX_ERR_ARG, "pdx_fb_results: range");
     std::string err;
     auto run = [&]() -> int {
@@ -589,8 +584,7 @@ int pdx_fb_results(const pdx_fbatch *b, long long first, long long count, double
         PDX_HIP(hipStreamSynchronize(b->stream));
         const int P = b->dim == 3 ? 9 : 5, DD = b->dim * b->dim;
         if (A) PDX_HIP(hipMemcpy(A, b->A + first * P * P, count * P * P * 8, hipMemcpyDeviceToHost));
-        if (LU) PDX_HIP(hipMemcpy(LU, b->LU + first * P * P, count * P * P * 8, hipMemcpyDeviceToHost));
-        if (piv) PDX_HIP(hipMemcpy(piv, b->piv + first * P, count * P * 4, hipMemcpyDeviceToHost));
+        if (L) PDX_HIP(hipMemcpy(L, b->LU + first * P * P, count * P * P * 8, hipMemcpyDeviceToHost));
         const long long o0 = b->off[first], o1 = b->off[first + count];
         if (H && o1 > o0) PDX_HIP(hipMemcpy(H, b->H + o0 * DD, (o1 - o0) * DD * 4, hipMemcpyDeviceToHost));
         return PDX_OK;

@@ -226,6 +226,56 @@ __global__ void __launch_bounds__(256) k_bicg_x(int nn, V *x, V *r, const V *rh,
     }
 }
 
+// ---- preconditioned conjugate gradients (symmetric operators) ----
+// x += alpha p, r -= alpha q, z = Minv r; partials rho_next = (r, z), rr = (r, r)
+template <int D, typename V>
+__global__ void __launch_bounds__(256) k_cg_xr(int nn, V *x, V *r, V *z, const V *p, const V *q, const V *Minv,
+                                               const double *rho_part, const double *pq_part, double *rho_next,
+                                               double *rr_part) {
+    __shared__ double sm[17];
+    const double rho = sum_partials(rho_part, NPART, sm);
+    const double pq = sum_partials(pq_part, NPART, sm);
+    double alpha = pq != 0.0 ? rho / pq : 0.0;
+    if (!isfinite(alpha)) alpha = 0.0;
+    const V al = V(alpha);
+    double d0 = 0.0, d1 = 0.0;
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < nn; a += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            const size_t o = (size_t)a * D + i;
+            x[o] = x[o] + al * p[o];
+            r[o] = r[o] - al * q[o];
+        }
+        apply_blk<D, V>(Minv, r, z, a);
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            const size_t o = (size_t)a * D + i;
+            d0 += double(r[o]) * double(z[o]);
+            d1 += double(r[o]) * double(r[o]);
+        }
+    }
+    d0 = block_sum(d0, sm);
+    d1 = block_sum(d1, sm);
+    if (threadIdx.x == 0) {
+        rho_next[blockIdx.x] = d0;
+        rr_part[blockIdx.x] = d1;
+    }
+}
+
+// p = z + beta p, beta = rho_new / rho_old (p = z on the first iteration)
+template <typename V>
+__global__ void __launch_bounds__(256) k_cg_p(size_t n, V *p, const V *z, const double *rho_new_part,
+                                              const double *rho_old_part, int first) {
+    __shared__ double sm[17];
+    const double rn = sum_partials(rho_new_part, NPART, sm);
+    const double ro = sum_partials(rho_old_part, NPART, sm);
+    double beta = (!first && ro != 0.0) ? rn / ro : 0.0;
+    if (!isfinite(beta)) beta = 0.0;
+    const V bv = V(beta);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = first ? z[i] : V(z[i] + bv * p[i]);
+}
+
 // ---- two-level variants: vector update in aggregate order + restriction ----
 // One wave per aggregate: updates the vector on the aggregate's nodes and
 // writes rc = P_I^T vec (NM modes) with a fixed-order wave reduction.
@@ -495,11 +545,11 @@ __global__ void k_diag_copy(int n, int ld, const double *A, double *d) {
 // row and column are removed (zero in the inverse).
 constexpr int GJB = 64;
 
-__global__ void __launch_bounds__(256) k_gj_diag(int ld, int k, const double *A, const double *diag0, double *Dk) {
+__global__ void __launch_bounds__(1024) k_gj_diag(int ld, int k, const double *A, const double *diag0, double *Dk) {
     __shared__ double S[GJB][GJB + 1];
     __shared__ double colq[GJB];
-    const int t = threadIdx.x, base = k * GJB;
-    for (int e = t; e < GJB * GJB; e += 256) S[e / GJB][e % GJB] = A[(size_t)(base + e / GJB) * ld + base + e % GJB];
+    const int t = threadIdx.x, base = k * GJB, nt = blockDim.x;
+    for (int e = t; e < GJB * GJB; e += nt) S[e / GJB][e % GJB] = A[(size_t)(base + e / GJB) * ld + base + e % GJB];
     __syncthreads();
     for (int q = 0; q < GJB; q++) {
         const double piv = S[q][q];
@@ -515,7 +565,7 @@ __global__ void __launch_bounds__(256) k_gj_diag(int ld, int k, const double *A,
         const double ip = 1.0 / piv;
         if (t < GJB && t != q) S[q][t] *= ip;
         __syncthreads();
-        for (int e = t; e < GJB * GJB; e += 256) {
+        for (int e = t; e < GJB * GJB; e += nt) {
             const int i = e / GJB, j = e % GJB;
             if (i == q) continue;
             if (j == q) S[i][j] = -colq[i] * ip;
@@ -525,10 +575,14 @@ __global__ void __launch_bounds__(256) k_gj_diag(int ld, int k, const double *A,
         if (t == 0) S[q][q] = ip;
         __syncthreads();
     }
-    for (int e = t; e < GJB * GJB; e += 256) Dk[e] = S[e / GJB][e % GJB];
+    for (int e = t; e < GJB * GJB; e += nt) Dk[e] = S[e / GJB][e % GJB];
 }
 
-// C = alpha * A_blk * B_blk (+ C when acc): 64 x 64 tiles through LDS, 4 x 4 per thread
+typedef double gj_d4 __attribute__((ext_vector_type(4)));
+
+// C = alpha * A_blk * B_blk (+ C when acc): 64 x 64 tiles through LDS; each of the 4
+// waves computes a 32 x 32 quadrant as 2 x 2 tiles of v_mfma_f64_16x16x4f64 (operand lane l:
+// A[i = l%16][k = l/16], B[k = l/16][j = l%16]; result lane l item r: row l/16 + 4 r, col l%16)
 __device__ inline void tile_mm(const double *A, int lda, const double *B, int ldb, double *C, int ldc, double alpha,
                                bool acc, double (*As)[GJB + 1], double (*Bs)[GJB + 1]) {
     const int t = threadIdx.x;
@@ -537,25 +591,29 @@ __device__ inline void tile_mm(const double *A, int lda, const double *B, int ld
         Bs[e / GJB][e % GJB] = B[(size_t)(e / GJB) * ldb + e % GJB];
     }
     __syncthreads();
-    const int r0 = (t / 16) * 4, c0 = (t % 16) * 4;
-    double s[4][4] = {};
-    for (int k = 0; k < GJB; k++) {
-        double av[4], bv[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) { av[i] = As[r0 + i][k]; bv[i] = Bs[k][c0 + i]; }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) s[i][j] += av[i] * bv[j];
+    const int w = t >> 6, l = t & 63;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32, li = l & 15, lk = l >> 4;
+    gj_d4 c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < GJB; k0 += 4) {
+        const int kk = k0 + lk;
+        const double a0 = As[wr + li][kk], a1 = As[wr + 16 + li][kk];
+        const double b0 = Bs[kk][wc + li], b1 = Bs[kk][wc + 16 + li];
+        c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c00, 0, 0, 0);
+        c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c01, 0, 0, 0);
+        c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c10, 0, 0, 0);
+        c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c11, 0, 0, 0);
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            double *o = C + (size_t)(r0 + i) * ldc + c0 + j;
-            *o = acc ? *o + alpha * s[i][j] : alpha * s[i][j];
-        }
+    for (int r = 0; r < 4; r++) {
+        const int row0 = wr + lk + 4 * r, row1 = row0 + 16, col0 = wc + li, col1 = col0 + 16;
+        double *o00 = C + (size_t)row0 * ldc + col0, *o01 = C + (size_t)row0 * ldc + col1;
+        double *o10 = C + (size_t)row1 * ldc + col0, *o11 = C + (size_t)row1 * ldc + col1;
+        *o00 = acc ? *o00 + alpha * c00[r] : alpha * c00[r];
+        *o01 = acc ? *o01 + alpha * c01[r] : alpha * c01[r];
+        *o10 = acc ? *o10 + alpha * c10[r] : alpha * c10[r];
+        *o11 = acc ? *o11 + alpha * c11[r] : alpha * c11[r];
+    }
 }
 
 // block row k: A_kj <- Dk A_kj (j != k)
@@ -783,7 +841,7 @@ static int build_two_level(Ctx &c, const double *K64, int agg, TwoLevel &t, std:
     k_diag_copy<<<(t.ld + 255) / 256, 256, 0, c.stream>>>(t.ld, t.ld, A, diag0);
     const int nb = t.ld / GJB;
     for (int k = 0; k < nb; k++) {
-        k_gj_diag<<<1, 256, 0, c.stream>>>(t.ld, k, A, diag0, Dk);
+        k_gj_diag<<<1, 1024, 0, c.stream>>>(t.ld, k, A, diag0, Dk);
         k_gj_row<<<nb, 256, 0, c.stream>>>(t.ld, nb, k, A, Dk, tmpb);
         k_gj_update<<<nb * nb, 256, 0, c.stream>>>(t.ld, nb, k, A);
         k_gj_col<<<nb, 256, 0, c.stream>>>(t.ld, nb, k, A, Dk, tmpb);
@@ -810,7 +868,7 @@ static int ensure_two_level_f32(Ctx &c, TwoLevel &t, std::string &err) {
     return PDX_OK;
 }
 
-static int default_agg(int dim) { return dim == 3 ? 6 : 16; }
+static int default_agg(int dim) { return dim == 3 ? 6 : 10; }
 
 // the two-level data of operator K64 (cached on the context)
 static int get_two_level(Ctx &c, const double *K64, int agg, TwoLevel *&out, std::string &err) {
@@ -971,6 +1029,68 @@ static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm
     return PDX_OK;
 }
 
+// Preconditioned CG on a symmetric operator (block-diagonal preconditioners):
+// same stopping rules and best-iterate restore as bicgstab().  Workspace: p = w.p,
+// q = w.v, z = w.s.
+template <int D, typename T, typename V>
+static int pcg(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, double tol, int max_iter, int *iters,
+               double *relres, std::string &err) {
+    const size_t n = (size_t)c.nn * c.dim;
+    double *red = c.red;
+    double *y;
+    PDX_HIP(hipMalloc((void **)&y, n * sizeof(double)));
+    spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.x, w.t, (const V *)nullptr, nullptr, (const V *)nullptr, nullptr);
+    k_copy<V, double><<<2048, 256, 0, c.stream>>>(n, w.t, y);
+    k_resid<V, V><<<RED_PARTS, 256, 0, c.stream>>>(n, b, y, w.r, nullptr, red + (size_t)P_RR * NPART);
+    hipFree(y);
+    // z = Minv r and rho0 = (r, z) through a zero step
+    PDX_HIP(hipMemsetAsync(w.p, 0, n * sizeof(V), c.stream));
+    PDX_HIP(hipMemsetAsync(w.v, 0, n * sizeof(V), c.stream));
+    k_fill_d<<<1, 256, 0, c.stream>>>(NPART, red + (size_t)P_RV * NPART, 0.0);
+    k_fill_d<<<1, 256, 0, c.stream>>>(NPART, red + (size_t)P_TMP * NPART, 1.0);  // pq != 0, alpha = 0
+    k_cg_xr<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.x, w.r, w.s, w.p, w.v, w.Minv, red + (size_t)P_RV * NPART,
+                                                  red + (size_t)P_TMP * NPART, red + (size_t)P_RHO0 * NPART,
+                                                  red + (size_t)P_RR * NPART);
+    double rel = std::sqrt(host_sum(c, red + (size_t)P_RR * NPART, NPART, err)) / bnorm;
+    int it = 0, since_best = 0;
+    double best = rel;
+    PDX_HIP(hipMemcpyAsync(w.xb, w.x, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
+    while (rel > tol && it < max_iter) {
+        for (int k = 0; k < 10 && it < max_iter; k++, it++) {
+            double *rho = red + (size_t)((it & 1) ? P_RHO1 : P_RHO0) * NPART;
+            // the other slot holds the previous rho until this step's k_cg_xr overwrites it
+            double *rho_new = red + (size_t)((it & 1) ? P_RHO0 : P_RHO1) * NPART;
+            double *rho_prev = rho_new;
+            k_cg_p<V><<<RED_PARTS, 256, 0, c.stream>>>(n, w.p, w.s, rho, rho_prev, it == 0);
+            spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.p, w.v, w.p, red + (size_t)P_RV * NPART, (const V *)nullptr,
+                              nullptr);
+            k_cg_xr<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.x, w.r, w.s, w.p, w.v, w.Minv, rho,
+                                                          red + (size_t)P_RV * NPART, rho_new,
+                                                          red + (size_t)P_RR * NPART);
+        }
+        PDX_HIP(hipGetLastError());
+        const double rr = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
+        rel = std::sqrt(rr) / bnorm;
+        if (!std::isfinite(rr) || rel > 1e3 * best) {
+            PDX_HIP(hipMemcpyAsync(w.x, w.xb, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
+            rel = best;
+            break;
+        }
+        if (rel < 0.95 * best) {
+            best = rel;
+            since_best = 0;
+            PDX_HIP(hipMemcpyAsync(w.xb, w.x, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
+        } else if (++since_best >= 40) {
+            PDX_HIP(hipMemcpyAsync(w.x, w.xb, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
+            rel = best;
+            break;
+        }
+    }
+    *iters = it;
+    *relres = rel;
+    return PDX_OK;
+}
+
 template <int D, typename V>
 static int build_precond(Ctx &c, const double *K64, int kind, V *Minv, std::string &err) {
     int *status;
@@ -1065,7 +1185,10 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
         for (int restart = 0; restart < 8; restart++) {
             int it = 0;
             double rr;
-            rc = bicgstab<D, double, double>(c, K64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total, &it, &rr, err);
+            rc = o.krylov == PDX_KRYLOV_CG
+                     ? pcg<D, double, double>(c, K64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total, &it, &rr, err)
+                     : bicgstab<D, double, double>(c, K64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total, &it, &rr,
+                                                   err);
             total += it;
             if (rc) break;
             spmv_dot<D, double, double>(c, K64, nullptr, w.x, y, (const double *)nullptr, nullptr,
@@ -1101,7 +1224,10 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
             PDX_HIP(hipMemsetAsync(w.x, 0, n * 4, c.stream));
             int it = 0;
             double rr;
-            rc = bicgstab<D, float, float>(c, K32, w, b32, rel * bnorm, inner, o.max_iter - total, &it, &rr, err);
+            rc = o.krylov == PDX_KRYLOV_CG
+                     ? pcg<D, float, float>(c, K32, w, b32, rel * bnorm, inner, o.max_iter - total, &it, &rr, err)
+                     : bicgstab<D, float, float>(c, K32, w, b32, rel * bnorm, inner, o.max_iter - total, &it, &rr,
+                                                 err);
             total += it;
             if (rc) break;
             k_add_f2d<<<2048, 256, 0, c.stream>>>(n, w.x, x_dev);
@@ -1128,7 +1254,11 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
 // solve with an explicit node-block matrix (K, K_eff or a 1x1-block CSR)
 int dev_solve_mat(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
                   double *x_dev, pdx_solve_stats &st, std::string &err) {
-    if (o.krylov != PDX_KRYLOV_BICGSTAB) { err = "only BiCGStab is available on the node-block operator"; return PDX_ERR_ARG; }
+    if (o.krylov != PDX_KRYLOV_BICGSTAB && o.krylov != PDX_KRYLOV_CG) { err = "unknown Krylov method"; return PDX_ERR_ARG; }
+    if (o.krylov == PDX_KRYLOV_CG && (o.precond == PDX_PREC_TWO_LEVEL || o.precond == PDX_PREC_LEARNED)) {
+        err = "CG needs a symmetric preconditioner (none, Jacobi, block Jacobi); the hybrid two-level one is not";
+        return PDX_ERR_ARG;
+    }
     if (o.precond < PDX_PREC_NONE || o.precond > PDX_PREC_TWO_LEVEL) { err = "unknown preconditioner"; return PDX_ERR_ARG; }
     if (c.dim == 3) return solve_masked<3>(c, K64, K32, o, b_dev, x_dev, st, err);
     if (c.dim == 2) return solve_masked<2>(c, K64, K32, o, b_dev, x_dev, st, err);

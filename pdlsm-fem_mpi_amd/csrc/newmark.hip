@@ -323,7 +323,7 @@ int pdx_mass_on_K(pdx_ctx *pc, double *Mv) {
 int pdx_newmark_run(pdx_ctx *pc, int nsteps, const pdx_solve_opts *op, double *u, double *v, double *a,
                     pdx_solve_stats *st) {
     if (!pc || nsteps < 0) return set_error(PDX_ERR_ARG, "bad argument");
-    pdx_solve_opts o;
+    pdx_solve_opts o{};
     if (op) o = *op;
     else {
         o.krylov = PDX_KRYLOV_BICGSTAB; o.precond = PDX_PREC_JACOBI; o.precision = PDX_FP64;

@@ -23,49 +23,73 @@ namespace pdx {
 // explicit step
 // ------------------------------------------------------------------------
 // FP32 values use the difference form (see rowblock.hpp), FP64 the plain form.
-// GS = lanes per short row, NT = non-temporal loads of K (chosen by the tuner).
-template <int D, typename T, int GS, bool NT>
+// GS = lanes per short row, NT = non-temporal loads of K, XM = XCD-aware block
+// order (chosen by the tuner).  The hardware deals workgroups to the 8 XCDs
+// round-robin; with XM the launch is padded to a multiple of 8 and block b works
+// on logical block (b % 8) * per + b / 8, so each XCD (and its L2) owns one
+// contiguous eighth of the rows and the gathers of u_n stay local to it.
+template <int D, typename T, int GS, bool NT, bool XM>
 __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int nbs, const int *rows_l, int nl,
-                                                  const int *adj_off, const int *adj, const T *K, const double *Un,
-                                                  const double *Unm1, double *Unp1, const double *m,
-                                                  const double *f_unit, int n_nbc, int nn, const double *nbc_cur,
-                                                  double *nbc_next, const double *nbc_inc, const int *eq,
-                                                  const int *presc_set, const double *ebc_inc, double dt,
-                                                  int startup) {
+                                                  int nbt, const int *adj_off, const int *adj, const T *K,
+                                                  const double *Un, const double *Unm1, double *Unp1,
+                                                  const double *m, const double *f_unit, int n_nbc, int nn,
+                                                  const double *nbc_cur, double *nbc_next, const double *nbc_inc,
+                                                  const int *eq, const int *presc_set, const double *ebc_inc,
+                                                  double dt, int startup) {
     constexpr bool DIFF = sizeof(T) == 4;
-    if (blockIdx.x == 0 && threadIdx.x < n_nbc) nbc_next[threadIdx.x] = nbc_cur[threadIdx.x] + nbc_inc[threadIdx.x];
+    int lb = blockIdx.x;
+    if (XM) {
+        const int per = gridDim.x / 8;
+        lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+        if (lb >= nbt) return;
+    }
+    if (lb == 0 && threadIdx.x < n_nbc) nbc_next[threadIdx.x] = nbc_cur[threadIdx.x] + nbc_inc[threadIdx.x];
     T acc[D];
-    int r, lane;
-    if ((int)blockIdx.x < nbs) {
-        const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / GS;
+    int r, lane, grp;
+    if (lb < nbs) {
+        const int gid = (lb * blockDim.x + threadIdx.x) / GS;
         lane = threadIdx.x % GS;
         if (gid >= ns) return;
         r = rows_s[gid];
-        row_matvec<D, T, double, GS, DIFF, NT>(r, lane, adj_off, adj, K, Un, acc);
+        grp = GS;
     } else {
-        const int gid = ((blockIdx.x - nbs) * blockDim.x + threadIdx.x) / 64;
+        const int gid = ((lb - nbs) * blockDim.x + threadIdx.x) / 64;
         lane = threadIdx.x % 64;
         if (gid >= nl) return;
         r = rows_l[gid];
-        row_matvec<D, T, double, 64, DIFF, NT>(r, lane, adj_off, adj, K, Un, acc);
+        grp = 64;
     }
-    if (lane < D) {
-        const T v = pick<D, T>(acc, lane);
-        const size_t o = (size_t)r * D + lane;
-        const int code = r * 3 + lane;
-        if (eq[code] >= 0) {
-            double fext = 0.0;
+    // the row's DOF state is loaded before the row sum, so its latency overlaps the block gathers
+    const size_t o = (size_t)r * D + (lane < D ? lane : 0);
+    const int code = r * 3 + (lane < D ? lane : 0);
+    double un = 0.0, unm1 = 0.0, mo = 1.0, fext = 0.0, inc = 0.0;
+    const bool act = lane < D;
+    const bool fr = act && eq[code] >= 0;
+    if (act) {
+        un = Un[o];
+        if (fr) {
+            unm1 = Unm1[o];
+            mo = m[o];
             for (int s = 0; s < n_nbc; s++) fext = fext + nbc_cur[s] * f_unit[(size_t)s * nn * D + o];
+        } else {
+            inc = ebc_inc[presc_set[code]];
+        }
+    }
+    if (grp == GS) row_matvec<D, T, double, GS, DIFF, NT>(r, lane, adj_off, adj, K, Un, acc);
+    else row_matvec<D, T, double, 64, DIFF, NT>(r, lane, adj_off, adj, K, Un, acc);
+    if (act) {
+        const T v = pick<D, T>(acc, lane);
+        if (fr) {
             const double F = fext - double(v);
             if (startup) {
-                const double a = F / m[o];
-                Unp1[o] = a * (0.5 * dt * dt) + Un[o];
+                const double a = F / mo;
+                Unp1[o] = a * (0.5 * dt * dt) + un;
             } else {
-                const double tA = dt * dt * F / m[o];
-                Unp1[o] = Un[o] * 2.0 + tA - Unm1[o];
+                const double tA = dt * dt * F / mo;
+                Unp1[o] = un * 2.0 + tA - unm1;
             }
         } else {
-            Unp1[o] = Un[o] + ebc_inc[presc_set[code]];
+            Unp1[o] = un + inc;
         }
     }
 }
@@ -101,31 +125,36 @@ int ensure_K32(Ctx &c, std::string &err) {
 }
 
 // explicit-step variants searched by the tuner: (lanes per short row, non-temporal K)
-constexpr int N_EXPLICIT_VARIANTS = 8;
-static const int kVarGS[N_EXPLICIT_VARIANTS] = {8, 4, 16, 8, 4, 16, 32, 32};
-static const int kVarNT[N_EXPLICIT_VARIANTS] = {0, 0, 0, 1, 1, 1, 0, 1};
+constexpr int N_EXPLICIT_VARIANTS = 10;
+static const int kVarGS[N_EXPLICIT_VARIANTS] = {8, 4, 16, 8, 4, 16, 32, 32, 16, 8};
+static const int kVarNT[N_EXPLICIT_VARIANTS] = {0, 0, 0, 1, 1, 1, 0, 1, 0, 0};
+static const int kVarXM[N_EXPLICIT_VARIANTS] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1};
 
-template <int D, typename T, int GS, bool NT>
+template <int D, typename T, int GS, bool NT, bool XM>
 static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double *Unm1, double *Unp1,
                               const double *nbc_cur, double *nbc_next, int startup) {
     const int nbs = (c.n_short * GS + 255) / 256, nbl = (c.n_long * 64 + 255) / 256;
-    k_explicit<D, T, GS, NT><<<std::max(nbs + nbl, 1), 256, 0, c.stream>>>(
-        c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, c.adj_off, c.adj, K, Un, Unm1, Unp1, c.mlump, c.f_unit,
-        c.n_nbc, c.nn, nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup);
+    const int nbt = std::max(nbs + nbl, 1);
+    const int grid = XM ? (nbt + 7) / 8 * 8 : nbt;
+    k_explicit<D, T, GS, NT, XM><<<grid, 256, 0, c.stream>>>(
+        c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, nbt, c.adj_off, c.adj, K, Un, Unm1, Unp1, c.mlump,
+        c.f_unit, c.n_nbc, c.nn, nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup);
 }
 
 template <int D, typename T>
 static void launch_explicit(Ctx &c, int variant, const T *K, const double *Un, const double *Unm1, double *Unp1,
                             const double *nbc_cur, double *nbc_next, int startup) {
     switch (variant) {
-    case 1: launch_explicit_v<D, T, 4, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 2: launch_explicit_v<D, T, 16, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 3: launch_explicit_v<D, T, 8, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 4: launch_explicit_v<D, T, 4, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 5: launch_explicit_v<D, T, 16, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 6: launch_explicit_v<D, T, 32, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 7: launch_explicit_v<D, T, 32, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    default: launch_explicit_v<D, T, 8, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 1: launch_explicit_v<D, T, 4, false, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 2: launch_explicit_v<D, T, 16, false, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 3: launch_explicit_v<D, T, 8, true, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 4: launch_explicit_v<D, T, 4, true, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 5: launch_explicit_v<D, T, 16, true, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 6: launch_explicit_v<D, T, 32, false, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 7: launch_explicit_v<D, T, 32, true, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 8: launch_explicit_v<D, T, 16, false, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    case 9: launch_explicit_v<D, T, 8, false, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+    default: launch_explicit_v<D, T, 8, false, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
     }
 }
 

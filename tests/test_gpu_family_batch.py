@@ -4,10 +4,11 @@
              rule (initialPDN_Fami), self first
   A          shape tensor within 1e-12 relative of the oracle's (the oracle sums
              in the reference's expression order, the kernel in a symmetric form)
-  LU         P L U reproduces A within 1e-13 relative
+  L          L L^T diag(s) reproduces A within 1e-13 relative (S = L L^T, the
+             symmetric part of A; s = 1/2 on the squared terms)
   H          bond blocks (FP32) within 2e-6 relative of the oracle's FP64 blocks,
              per family; the self block is minus the sum of the others
-  variants   every launch form gives the same A, LU and bond blocks (bitwise)
+  variants   every launch form gives the same A, L and H (bitwise)
 """
 import numpy as np
 import pytest
@@ -34,16 +35,10 @@ def _cloud(n, dim=3, seed=0):
     return c
 
 
-def _lu_rebuild(LU, piv):
-    p = LU.shape[0]
-    L = np.tril(LU, -1) + np.eye(p)
-    U = np.triu(LU)
-    M = L @ U
-    # undo the row interchanges in reverse order
-    for k in reversed(range(p)):
-        if piv[k] != k:
-            M[[k, piv[k]]] = M[[piv[k], k]]
-    return M
+def _rebuild(L, dim):
+    s = np.ones(L.shape[0])
+    s[[3, 4, 5] if dim == 3 else [2, 3]] = 0.5
+    return (L @ L.T) * s[None, :]
 
 
 @pytest.mark.parametrize("dim", [3, 2])
@@ -58,11 +53,11 @@ def test_family_batch_matches_oracle(dim):
         assert a[0] == k and b[0] == k
         assert np.array_equal(np.sort(a[1:]), np.sort(b[1:])), k
     fb.run(0)
-    A, LU, piv, H = fb.results(0, fb.n_fam, off)
+    A, L, H = fb.results(0, fb.n_fam, off)
     for k in range(fb.n_fam):
         A0, H0 = O.fb_family(dim, c["x"], c["vol"], c["delta"], 0.33, LAM, MU, mem[off[k]:off[k + 1]])
         assert np.linalg.norm(A[k] - A0) <= 1e-12 * np.linalg.norm(A0), k
-        assert np.linalg.norm(_lu_rebuild(LU[k], piv[k]) - A[k]) <= 1e-13 * np.linalg.norm(A[k]), k
+        assert np.linalg.norm(_rebuild(L[k], dim) - A[k]) <= 1e-13 * np.linalg.norm(A[k]), k
         Hk = H[off[k]:off[k + 1]].astype(np.float64)
         assert np.linalg.norm(Hk - H0) <= 2e-6 * np.linalg.norm(H0), (k, np.linalg.norm(Hk - H0) / np.linalg.norm(H0))
         assert np.linalg.norm(Hk[0] + Hk[1:].sum(0)) <= 1e-5 * np.linalg.norm(Hk[1:]), k
@@ -74,20 +69,17 @@ def test_family_batch_variants_agree():
     off, _ = fb.families()
     fb.run(0)
     ref = fb.results(0, fb.n_fam, off)
-    for v in (1, 2, 3, 4):
+    for v in (1, 2, 3):
         try:
             fb.run(v)
         except pdx.PdxError:  # a form whose LDS slots exceed the device limit
             continue
         got = fb.results(0, fb.n_fam, off)
-        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2])
-        bond = np.ones(len(got[3]), bool)
-        bond[off[:-1]] = False  # self blocks come from a lane reduction whose tree depends on the form
-        assert np.array_equal(got[3][bond], ref[3][bond]), v
-        np.testing.assert_allclose(got[3][~bond], ref[3][~bond], rtol=1e-6, atol=1e-6 * np.abs(ref[3]).max())
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), v
     ms = fb.run(-1)
     best, times = fb.tuning()
-    assert best in (1, 2, 3, 4) and ms > 0 and times[best] > 0
+    assert best in (1, 2, 3) and ms > 0 and times[best] > 0
 
 
 def test_family_batch_full_cloud_sampled():
@@ -100,7 +92,7 @@ def test_family_batch_full_cloud_sampled():
     fb.run(-1)
     rng = np.random.default_rng(0)
     for k in rng.choice(fb.n_fam, 64, replace=False):
-        A, LU, piv, H = fb.results(int(k), 1, off)
+        A, L, H = fb.results(int(k), 1, off)
         A0, H0 = O.fb_family(3, c["x"], c["vol"], c["delta"], 0.33, LAM, MU, mem[off[k]:off[k + 1]])
         assert np.linalg.norm(A[0] - A0) <= 1e-12 * np.linalg.norm(A0)
         assert np.linalg.norm(H.astype(np.float64) - H0) <= 2e-6 * np.linalg.norm(H0)

@@ -236,3 +236,23 @@ def test_two_level_cuts_iterations(crack_case):
     _, s_bj = cx.static_solve(pdx.solve_opts(precision=0, precond=2, tol=1e-8, max_iter=100000))
     _, s_tl = cx.static_solve(pdx.solve_opts(precision=0, precond=4, tol=1e-8))
     assert s_tl["iterations"] * 10 <= s_bj["iterations"], (s_tl, s_bj)
+
+
+@pytest.mark.parametrize("precision,precond", [(0, 1), (0, 2), (1, 2)])
+def test_pcg_on_symmetric_fe_plate(precision, precond):
+    """PCG on an FE-only plate (symmetric K): 1e-8 true residual; the PD part of
+    PDLSM makes K non-symmetric in general, hence BiCGStab elsewhere."""
+    mesh = W.plate2d(nx=81, ny=41, dx=1e-3, pd_x=(0.0, 0.0))
+    mo, mp, cx = _pair(mesh)
+    K, F = mo.assemble(1)
+    A = mo.sparse(K)
+    assert abs(A - A.T).max() <= 1e-12 * abs(A).max()
+    u, st = cx.static_solve(pdx.solve_opts(krylov=pdx.PDX_KRYLOV_CG, precision=precision, precond=precond, tol=1e-8))
+    assert st["converged"]
+    assert _true_relres(mo, K, F, u) <= 1.2e-8
+
+
+def test_pcg_rejects_the_hybrid_preconditioner():
+    mo, mp, cx = _pair(W.plate2d(nx=21, ny=11, dx=1e-3, pd_x=(0.0, 0.0)))
+    with pytest.raises(pdx.PdxError, match="symmetric preconditioner"):
+        cx.static_solve(pdx.solve_opts(krylov=pdx.PDX_KRYLOV_CG, precond=pdx.PDX_PREC_TWO_LEVEL))
