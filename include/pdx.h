@@ -234,10 +234,11 @@ void pdx_fb_destroy(pdx_fbatch *b);
 int pdx_fb_sizes(const pdx_fbatch *b, long long *n_fam, long long *n_bond, int *max_family);
 int pdx_fb_families(const pdx_fbatch *b, long long *off /*[n+1]*/, int *mem /*[n_bond]*/);
 /* variant -1: the fastest persistent launch (timed on the first call); 0: one
- * family per 64-thread block; 1..3: persistent, 4 / 2 / 1 waves per block.
- * ms: kernel time */
+ * family per 64-thread block; 1..6: persistent, (waves per block, families per
+ * chunk) = (4, 4), (2, 4), (1, 4), (4, 16), (2, 16), (1, 16).  ms: kernel time */
+#define PDX_FB_VARIANTS 7
 int pdx_fb_run(pdx_fbatch *b, int variant, float *ms);
-int pdx_fb_tuning(const pdx_fbatch *b, int *best_variant, float *ms /*[4]*/);
+int pdx_fb_tuning(const pdx_fbatch *b, int *best_variant, float *ms /*[PDX_FB_VARIANTS]*/);
 int pdx_fb_results(const pdx_fbatch *b, long long first, long long count, double *A, double *L, float *H);
 
 /* PARDISO seam: solve A x = b for a zero-based int64 CSR (n, ia, ja, a). */

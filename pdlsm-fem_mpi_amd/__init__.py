@@ -26,6 +26,7 @@ PDX_MODE_EXPLICIT, PDX_MODE_STATIC, PDX_MODE_QUASI, PDX_MODE_NEWMARK = 0, 1, 2, 
 PDX_FP64, PDX_FP32 = 0, 1
 PDX_KRYLOV_BICGSTAB, PDX_KRYLOV_CG = 0, 1
 PDX_PREC_NONE, PDX_PREC_JACOBI, PDX_PREC_BLOCK_JACOBI, PDX_PREC_LEARNED, PDX_PREC_TWO_LEVEL = 0, 1, 2, 3, 4
+FB_VARIANTS = 7  # family-batch launch forms (PDX_FB_VARIANTS)
 
 _P, _I, _D, _LL = C.c_void_p, C.c_int, C.c_double, C.c_longlong
 
@@ -458,7 +459,7 @@ class FamilyBatch:
 
     def tuning(self):
         best = C.c_int()
-        ms = np.zeros(4, np.float32)
+        ms = np.zeros(FB_VARIANTS, np.float32)
         _chk(lib().pdx_fb_tuning(self.h, C.byref(best), _p(ms)))
         return best.value, ms
 
