@@ -104,6 +104,30 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4, "copies": 5}), flush=True)
 
 
+def hbm_probe(mib=2048, reps=10):
+    """Practical HBM bandwidth of plain streaming kernels on this GPU, timed with
+    events: a read-only reduction over mib MiB, and a device copy of mib / 2 MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    half = a[: n // 2]
+    b = torch.empty_like(half)
+    out = {}
+    for name, fn, nbytes in (("read_sum", lambda: a.sum(), n * 4),
+                             ("copy", lambda: b.copy_(half), n * 4)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"bytes": nbytes, "ms": ms, "GBps": nbytes / (ms * 1e-3) / 1e9}
+    print(json.dumps({"hbm_probe": out}), flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of the explicit kernel on the 60^3 box, from the
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
@@ -189,11 +213,16 @@ def main():
     ap.add_argument("--no-newmark", action="store_true", help="skip the 100^3 penny-crack Newmark run")
     ap.add_argument("--fb-only", action="store_true",
                     help="only run the 2M-node family batch and print its figures (for kernel profiling)")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only time a streaming read and a copy (practical HBM bandwidth) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.hbm_probe:
+        hbm_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the whole mesh lives in one HBM and this build has no "
@@ -236,8 +265,9 @@ def main():
     prec = 1
     bytes_step = step_bytes(model, nblk, prec)
     achieved = bytes_step / (res[prec]["step_ms"] * 1e-3)
-    lanes, nt, xm, tune_ms = ctxs[prec].explicit_tuning(prec)
-    kname = f"pdx::k_explicit<{model.dim}, float, {lanes}, {'true' if nt else 'false'}, {'true' if xm else 'false'}>"
+    lanes, nt, xm, soa, tune_ms = ctxs[prec].explicit_tuning(prec)
+    tf = lambda b: "true" if b else "false"
+    kname = f"pdx::k_explicit<{model.dim}, float, {lanes}, {tf(nt)}, {tf(xm)}, {tf(soa)}>"
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": measured_traffic(kname) if args.n == 60 else None,
                 "kernel": kname + " (node-block SpMV + central-difference update, FP32 values)",

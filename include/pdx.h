@@ -249,9 +249,15 @@ int pdx_solve_csr(int device, long long n, const long long *ia, const long long 
  * context stream: total ms and per-step ms */
 int pdx_last_timing(pdx_ctx *c, double *total_ms, double *per_step_ms);
 /* the explicit-step kernel form the tuner chose (lanes per short row, non-temporal K loads,
- * XCD-contiguous block order; 0/0/0 before the first step) and its timings of the 10 forms
- * (ms per step) */
-int pdx_explicit_tuning(pdx_ctx *c, int precision, int *lanes, int *nontemporal, int *xcd_order, float *ms);
+ * XCD-contiguous block order, row-local value order of K; 0/0/0/0 before the first step) and its
+ * timings of the PDX_EXPLICIT_FORMS forms (ms per step) */
+#define PDX_EXPLICIT_FORMS 14
+int pdx_explicit_tuning(pdx_ctx *c, int precision, int *lanes, int *nontemporal, int *xcd_order, int *row_soa,
+                        float *ms /*[PDX_EXPLICIT_FORMS]*/);
+/* use explicit-step form `form` (0 .. PDX_EXPLICIT_FORMS-1) from now on instead of the tuner's
+ * choice, or -1 to tune again at the next step; forms with the same lanes per row compute the
+ * same bits, the lanes per row change the order of the row sum */
+int pdx_explicit_set_form(pdx_ctx *c, int precision, int form);
 
 #ifdef __cplusplus
 }

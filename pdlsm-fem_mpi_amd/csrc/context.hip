@@ -236,6 +236,8 @@ void ctx_free(Ctx *c) {
         if (g) hipGraphExecDestroy(g);
     for (void *p : c->owned) hipFree(p);
     if (c->K32) hipFree(c->K32);
+    if (c->K32s) hipFree(c->K32s);
+    if (c->K64s) hipFree(c->K64s);
     if (c->Mk) hipFree(c->Mk);
     if (c->Ke_stage) hipFree(c->Ke_stage);
     if (c->red) hipFree(c->red);
@@ -585,16 +587,31 @@ int pdx_node_stresses_timed(pdx_ctx *pc, const double *u_all, double *ms) {
     return rc ? fail(rc, err) : PDX_OK;
 }
 
-int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal, int *xcd_order, float *ms) {
+int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal, int *xcd_order, int *row_soa,
+                        float *ms) {
     if (!pc || (precision != PDX_FP64 && precision != PDX_FP32)) return fail(PDX_ERR_ARG, "bad argument");
     const Ctx &c = *pc->c;
     const int v = c.tune_explicit[precision];
-    static const int gs[10] = {8, 4, 16, 8, 4, 16, 32, 32, 16, 8}, nt[10] = {0, 0, 0, 1, 1, 1, 0, 1, 0, 0},
-                     xm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1};
-    if (lanes) *lanes = v >= 0 ? gs[v] : 0;
-    if (nontemporal) *nontemporal = v >= 0 ? nt[v] : 0;
-    if (xcd_order) *xcd_order = v >= 0 ? xm[v] : 0;
-    if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + 10, ms);
+    const ExplicitForm f = v >= 0 ? kExplicitForms[v] : ExplicitForm{0, false, false, false};
+    if (lanes) *lanes = f.lanes;
+    if (nontemporal) *nontemporal = f.nt;
+    if (xcd_order) *xcd_order = f.xm;
+    if (row_soa) *row_soa = f.soa;
+    if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + PDX_EXPLICIT_FORMS, ms);
+    return PDX_OK;
+}
+
+int pdx_explicit_set_form(pdx_ctx *pc, int precision, int form) {
+    if (!pc || (precision != PDX_FP64 && precision != PDX_FP32) || form < -1 || form >= PDX_EXPLICIT_FORMS)
+        return fail(PDX_ERR_ARG, "pdx_explicit_set_form: bad argument");
+    Ctx &c = *pc->c;
+    hipSetDevice(c.device);
+    c.tune_explicit[precision] = form;
+    if (c.graph[precision]) {  // the captured steps launch the previous form
+        hipStreamSynchronize(c.stream);
+        hipGraphExecDestroy(c.graph[precision]);
+        c.graph[precision] = nullptr;
+    }
     return PDX_OK;
 }
 

@@ -27,6 +27,18 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// explicit-step kernel forms searched by the tuner: lanes per short row, non-temporal K loads,
+// XCD-contiguous block order, row-local value order of K
+struct ExplicitForm {
+    int lanes;
+    bool nt, xm, soa;
+};
+constexpr ExplicitForm kExplicitForms[PDX_EXPLICIT_FORMS] = {
+    {8, false, false, false}, {4, false, false, false}, {16, false, false, false}, {8, true, false, false},
+    {4, true, false, false},  {16, true, false, false}, {32, false, false, false}, {32, true, false, false},
+    {16, false, true, false}, {8, false, true, false},  {8, false, true, true},    {16, false, true, true},
+    {4, false, true, true},   {32, false, true, true}};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -73,6 +85,12 @@ struct Ctx {
     // matrices
     double *K = nullptr;   // [dd*nblk]
     float *K32 = nullptr;  // [dd*nblk]
+    // K in row-local value order for the explicit step: value e of block b of the row whose blocks
+    // are [b0, b0 + nb) at b0*dd + e*nb + (b - b0), so that lanes on consecutive blocks read
+    // consecutive words; built on first use for the K of generation ks_gen
+    double *K64s = nullptr;
+    float *K32s = nullptr;
+    long long ks_gen[2] = {-1, -1};
     double *Mk = nullptr;  // consistent mass on K blocks (Newmark)
     bool K_valid = false, K32_valid = false, Mk_valid = false;
     long long K_gen = 0;  // incremented by every assembly of K
@@ -86,7 +104,7 @@ struct Ctx {
     long long steps_done = -1;  // -1: start-up step not yet taken
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
     int tune_explicit[2] = {-1, -1};               // chosen step-kernel variant per precision
-    float tune_ms[2][12] = {{0}};                  // measured ms per variant
+    float tune_ms[2][PDX_EXPLICIT_FORMS] = {{0}};  // measured ms per variant (-1: not run)
     int graph_phase[2] = {-1, -1};
 
     // scratch for Krylov

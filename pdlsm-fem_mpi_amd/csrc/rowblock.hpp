@@ -13,7 +13,9 @@ namespace pdx {
 // and boundary operator annihilates rigid translations, so both forms are
 // equal in exact arithmetic; the difference form keeps FP32 products away
 // from the cancellation of large displacement terms).
-template <int D, typename T, typename X, int G, bool DIFF, bool NT = false>
+// SOA: K is in row-local value order (value e of block b of a row whose blocks are [b0, b0 + nb)
+// at b0*DD + e*nb + (b - b0)): lanes on consecutive blocks then read consecutive words.
+template <int D, typename T, typename X, int G, bool DIFF, bool NT = false, bool SOA = false>
 __device__ inline void row_matvec(int r, int lane, const int *__restrict__ adj_off, const int *__restrict__ adj,
                                   const T *__restrict__ K, const X *__restrict__ x, T acc[D]) {
     constexpr int DD = D * D;
@@ -24,12 +26,15 @@ __device__ inline void row_matvec(int r, int lane, const int *__restrict__ adj_o
 #pragma unroll
         for (int j = 0; j < D; j++) xr[j] = x[(size_t)r * D + j];
     }
-    const int b1 = adj_off[r + 1];
-    for (int b = adj_off[r] + lane; b < b1; b += G) {
+    const int b0 = adj_off[r], b1 = adj_off[r + 1], nb = b1 - b0;
+    for (int b = b0 + lane; b < b1; b += G) {
         const int c = adj[b];
         T kb[DD];
 #pragma unroll
-        for (int e = 0; e < DD; e++) kb[e] = NT ? __builtin_nontemporal_load(&K[(size_t)b * DD + e]) : K[(size_t)b * DD + e];
+        for (int e = 0; e < DD; e++) {
+            const size_t k = SOA ? (size_t)b0 * DD + (size_t)e * nb + (b - b0) : (size_t)b * DD + e;
+            kb[e] = NT ? __builtin_nontemporal_load(&K[k]) : K[k];
+        }
         T xv[D];
 #pragma unroll
         for (int j = 0; j < D; j++) xv[j] = DIFF ? T(x[(size_t)c * D + j] - xr[j]) : T(x[(size_t)c * D + j]);

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 
 #include "device.hpp"
 #include "reduce.hpp"
@@ -28,7 +29,7 @@ namespace pdx {
 // round-robin; with XM the launch is padded to a multiple of 8 and block b works
 // on logical block (b % 8) * per + b / 8, so each XCD (and its L2) owns one
 // contiguous eighth of the rows and the gathers of u_n stay local to it.
-template <int D, typename T, int GS, bool NT, bool XM>
+template <int D, typename T, int GS, bool NT, bool XM, bool SOA>
 __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int nbs, const int *rows_l, int nl,
                                                   int nbt, const int *adj_off, const int *adj, const T *K,
                                                   const double *Un, const double *Unm1, double *Unp1,
@@ -75,8 +76,8 @@ __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int
             inc = ebc_inc[presc_set[code]];
         }
     }
-    if (grp == GS) row_matvec<D, T, double, GS, DIFF, NT>(r, lane, adj_off, adj, K, Un, acc);
-    else row_matvec<D, T, double, 64, DIFF, NT>(r, lane, adj_off, adj, K, Un, acc);
+    if (grp == GS) row_matvec<D, T, double, GS, DIFF, NT, SOA>(r, lane, adj_off, adj, K, Un, acc);
+    else row_matvec<D, T, double, 64, DIFF, NT, SOA>(r, lane, adj_off, adj, K, Un, acc);
     if (act) {
         const T v = pick<D, T>(acc, lane);
         if (fr) {
@@ -124,48 +125,90 @@ int ensure_K32(Ctx &c, std::string &err) {
     return PDX_OK;
 }
 
-// explicit-step variants searched by the tuner: (lanes per short row, non-temporal K)
-constexpr int N_EXPLICIT_VARIANTS = 10;
-static const int kVarGS[N_EXPLICIT_VARIANTS] = {8, 4, 16, 8, 4, 16, 32, 32, 16, 8};
-static const int kVarNT[N_EXPLICIT_VARIANTS] = {0, 0, 0, 1, 1, 1, 0, 1, 0, 0};
-static const int kVarXM[N_EXPLICIT_VARIANTS] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1};
+// K in row-local value order (one wave per node row), from the FP64 K
+template <typename T>
+__global__ void k_row_soa(int nn, int dd, const int *__restrict__ adj_off, const double *__restrict__ K, T *Ks) {
+    const int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x & 63;
+    if (r >= nn) return;
+    const int b0 = adj_off[r], nb = adj_off[r + 1] - b0;
+    for (int v = lane; v < nb * dd; v += 64) {
+        const int b = v / dd, e = v % dd;
+        Ks[(size_t)b0 * dd + (size_t)e * nb + b] = T(K[((size_t)b0 + b) * dd + e]);
+    }
+}
 
-template <int D, typename T, int GS, bool NT, bool XM>
+// the row-local copy of K for the explicit step, rebuilt when K has been assembled again
+static int ensure_Ks(Ctx &c, int prec, std::string &err) {
+    const int pi = prec == PDX_FP32 ? 1 : 0;
+    if (!c.K_valid) { err = "K not assembled"; return PDX_ERR_ARG; }
+    if (c.ks_gen[pi] == c.K_gen) return PDX_OK;
+    const size_t n = (size_t)c.dd * c.nblk;
+    const int grid = (c.nn + 3) / 4;
+    if (pi) {
+        if (!c.K32s) PDX_HIP(hipMalloc((void **)&c.K32s, sizeof(float) * std::max<size_t>(n, 1)));
+        k_row_soa<float><<<grid, 256, 0, c.stream>>>(c.nn, c.dd, c.adj_off, c.K, c.K32s);
+    } else {
+        if (!c.K64s) PDX_HIP(hipMalloc((void **)&c.K64s, sizeof(double) * std::max<size_t>(n, 1)));
+        k_row_soa<double><<<grid, 256, 0, c.stream>>>(c.nn, c.dd, c.adj_off, c.K, c.K64s);
+    }
+    PDX_HIP(hipGetLastError());
+    c.ks_gen[pi] = c.K_gen;
+    return PDX_OK;
+}
+
+template <int D, typename T, int GS, bool NT, bool XM, bool SOA>
 static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double *Unm1, double *Unp1,
                               const double *nbc_cur, double *nbc_next, int startup) {
     const int nbs = (c.n_short * GS + 255) / 256, nbl = (c.n_long * 64 + 255) / 256;
     const int nbt = std::max(nbs + nbl, 1);
     const int grid = XM ? (nbt + 7) / 8 * 8 : nbt;
-    k_explicit<D, T, GS, NT, XM><<<grid, 256, 0, c.stream>>>(
+    k_explicit<D, T, GS, NT, XM, SOA><<<grid, 256, 0, c.stream>>>(
         c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, nbt, c.adj_off, c.adj, K, Un, Unm1, Unp1, c.mlump,
         c.f_unit, c.n_nbc, c.nn, nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup);
 }
 
+// one launch of form `variant` of kExplicitForms (K: the plain or the row-local copy, as the form reads)
 template <int D, typename T>
 static void launch_explicit(Ctx &c, int variant, const T *K, const double *Un, const double *Unm1, double *Unp1,
                             const double *nbc_cur, double *nbc_next, int startup) {
+#define PDX_EXPLICIT_FORM(V, GS, NT, XM, SOA)                                                                   \
+    case V:                                                                                                     \
+        static_assert(kExplicitForms[V].lanes == GS && kExplicitForms[V].nt == NT && kExplicitForms[V].xm == XM && \
+                          kExplicitForms[V].soa == SOA,                                                         \
+                      "form table");                                                                            \
+        launch_explicit_v<D, T, GS, NT, XM, SOA>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);                 \
+        break;
     switch (variant) {
-    case 1: launch_explicit_v<D, T, 4, false, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 2: launch_explicit_v<D, T, 16, false, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 3: launch_explicit_v<D, T, 8, true, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 4: launch_explicit_v<D, T, 4, true, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 5: launch_explicit_v<D, T, 16, true, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 6: launch_explicit_v<D, T, 32, false, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 7: launch_explicit_v<D, T, 32, true, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 8: launch_explicit_v<D, T, 16, false, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    case 9: launch_explicit_v<D, T, 8, false, true>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
-    default: launch_explicit_v<D, T, 8, false, false>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup); break;
+        PDX_EXPLICIT_FORM(1, 4, false, false, false)
+        PDX_EXPLICIT_FORM(2, 16, false, false, false)
+        PDX_EXPLICIT_FORM(3, 8, true, false, false)
+        PDX_EXPLICIT_FORM(4, 4, true, false, false)
+        PDX_EXPLICIT_FORM(5, 16, true, false, false)
+        PDX_EXPLICIT_FORM(6, 32, false, false, false)
+        PDX_EXPLICIT_FORM(7, 32, true, false, false)
+        PDX_EXPLICIT_FORM(8, 16, false, true, false)
+        PDX_EXPLICIT_FORM(9, 8, false, true, false)
+        PDX_EXPLICIT_FORM(10, 8, false, true, true)
+        PDX_EXPLICIT_FORM(11, 16, false, true, true)
+        PDX_EXPLICIT_FORM(12, 4, false, true, true)
+        PDX_EXPLICIT_FORM(13, 32, false, true, true)
+    default:
+        PDX_EXPLICIT_FORM(0, 8, false, false, false)
     }
+#undef PDX_EXPLICIT_FORM
 }
 
 static void launch_explicit_any(Ctx &c, int prec, int variant, const double *Un, const double *Unm1, double *Unp1,
                                 const double *nbc_cur, double *nbc_next, int startup) {
+    const bool soa = kExplicitForms[variant].soa;
     if (prec == PDX_FP32) {
-        if (c.dim == 3) launch_explicit<3, float>(c, variant, c.K32, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
-        else launch_explicit<2, float>(c, variant, c.K32, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
+        const float *K = soa ? c.K32s : c.K32;
+        if (c.dim == 3) launch_explicit<3, float>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
+        else launch_explicit<2, float>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
     } else {
-        if (c.dim == 3) launch_explicit<3, double>(c, variant, c.K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
-        else launch_explicit<2, double>(c, variant, c.K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
+        const double *K = soa ? c.K64s : c.K;
+        if (c.dim == 3) launch_explicit<3, double>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
+        else launch_explicit<2, double>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
     }
 }
 
@@ -182,7 +225,7 @@ static int tune_explicit(Ctx &c, int prec, std::string &err) {
     const int p = c.step_phase % 3;
     float best = 1e30f;
     int bestv = 0;
-    for (int v = 0; v < N_EXPLICIT_VARIANTS; v++) {
+    for (int v = 0; v < PDX_EXPLICIT_FORMS; v++) {
         launch_explicit_any(c, prec, v, c.U[p], c.U[(p + 2) % 3], scratch, c.nbcv, nscratch, 0);  // warm
         PDX_HIP(hipEventRecord(e0, c.stream));
         for (int r = 0; r < 5; r++)
@@ -233,6 +276,7 @@ int explicit_prepare(Ctx &c, int prec, std::string &err) {
         int e = ensure_K32(c, err);
         if (e) return e;
     }
+    if (int e = ensure_Ks(c, prec, err)) return e;
     if (!c.mlump_valid) {
         int e = dev_lumped_mass(c, err);
         if (e) return e;

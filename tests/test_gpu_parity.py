@@ -127,6 +127,33 @@ def test_explicit_graph_replay_matches_eager():
     assert np.array_equal(ua, ub)
 
 
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-5)])
+def test_explicit_forms_agree(prec, tol):
+    """Explicit-step kernel forms with the same lanes per row (non-temporal
+    loads, XCD order, row-local K order) give the same trajectory bit for bit,
+    eager and in graph replay; the lanes per row change only the order of the
+    row sum (rounding), checked to tol.  2D and 3D explicit cases."""
+    for mesh in (_explicit_case(), W.box3d(n=12, pd_block=4)):
+        mesh["dt"] = 2e-8
+        ref, by_lanes = None, {}
+        for form in range(pdx.EXPLICIT_FORMS):
+            cx = pdx.Context(pdx.Model(mesh))
+            cx.explicit_set_form(prec, form)
+            cx.explicit_steps(1, precision=prec)
+            cx.explicit_steps(12, precision=prec, use_graph=(form % 2 == 0))
+            u, up = cx.explicit_run(0, precision=prec)
+            lanes = cx.explicit_tuning(prec)[0]
+            assert np.abs(u).max() > 0
+            if lanes in by_lanes:
+                assert np.array_equal(u, by_lanes[lanes][0]) and np.array_equal(up, by_lanes[lanes][1]), form
+            else:
+                by_lanes[lanes] = (u, up)
+            if ref is None:
+                ref = (u, up)
+            assert rel(u, ref[0]) <= tol and rel(up, ref[1]) <= tol, form
+        assert len(by_lanes) == 4
+
+
 def _true_relres(mo, K, F, u):
     A = mo.sparse(K)
     return np.linalg.norm(F - A @ u) / np.linalg.norm(F)

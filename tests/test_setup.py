@@ -145,3 +145,4 @@ def test_python_constants_match_the_header():
         if name in defs:
             assert getattr(pdx, name) == int(defs[name]), name
     assert pdx.FB_VARIANTS == int(defs["PDX_FB_VARIANTS"])
+    assert pdx.EXPLICIT_FORMS == int(defs["PDX_EXPLICIT_FORMS"])
