@@ -130,6 +130,13 @@ const char *pdx_version(void);
 int pdx_model_create(const pdx_mesh_desc *desc, pdx_model **out);
 int pdx_model_read(const char *cmd_file, pdx_model **out, int *solver_flag, int *num_steps);
 void pdx_model_destroy(pdx_model *m);
+/* Result file of the whole mesh in the reference's legacy VTK layout, byte for byte:
+ * writeReslutsTOTAL_vtk_Binary / _ASCII (fioFile.cpp:562-769), chosen as its VTKFORMAT
+ * command does (binary: big-endian).  u [n_node*3] DOF values (UZ 0 in 2D), sigma [n_node*6]
+ * nodal stresses (xx, yy, zz, xy, yz, zx, as pdx_node_stresses gives them), phi [n_node]
+ * nodal damage; a NULL array writes zeros.  title NULL: the mesh file's title. */
+int pdx_write_vtk(const pdx_model *m, const char *path, const char *title, const double *u, const double *sigma,
+                  const double *phi, int binary);
 /* out[0..15]: n_node, n_ele, dim, neq, nnz, nfam, n_members, npdbe, npde, nfe,
  *             n_adj (node-graph entries), n_presc, nb0, nb1, nb2, max_family */
 int pdx_model_sizes(const pdx_model *m, long long *out);

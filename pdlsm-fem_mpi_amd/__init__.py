@@ -62,6 +62,7 @@ EXPORTS = {
     "pdx_version": (C.c_char_p, []),
     "pdx_model_create": (_I, [_P, _P]),
     "pdx_model_read": (_I, [C.c_char_p, _P, _P, _P]),
+    "pdx_write_vtk": (_I, [_P, C.c_char_p, C.c_char_p, _P, _P, _P, _I]),
     "pdx_model_destroy": (None, [_P]),
     "pdx_model_sizes": (_I, [_P, _P]),
     "pdx_model_csr": (_I, [_P, _P, _P]),
@@ -204,6 +205,24 @@ class Model:
         if getattr(self, "h", None):
             lib().pdx_model_destroy(self.h)
             self.h = None
+
+    def write_vtk(self, path, u=None, sigma=None, phi=None, title=None, binary=True):
+        """Write the whole mesh with nodal displacements u ([nn, dim] or [nn, 3], or
+        flat), stresses sigma [nn, 6] (xx, yy, zz, xy, yz, zx) and damage phi [nn]
+        in the reference's legacy VTK layout, byte for byte
+        (writeReslutsTOTAL_vtk_Binary / _ASCII, fioFile.cpp:562-769); None writes
+        zeros, title None takes the mesh file's title."""
+        nn = self.nn
+        u3 = None
+        if u is not None:
+            u = np.asarray(u, np.float64).reshape(nn, -1)
+            u3 = np.zeros((nn, 3))
+            u3[:, : u.shape[1]] = u
+        sg = None if sigma is None else np.ascontiguousarray(np.asarray(sigma, np.float64).reshape(nn, 6))
+        ph = None if phi is None else np.ascontiguousarray(np.asarray(phi, np.float64).reshape(nn))
+        _chk(lib().pdx_write_vtk(self.h, str(path).encode(), None if title is None else title.encode(),
+                                 None if u3 is None else _p(u3), None if sg is None else _p(sg),
+                                 None if ph is None else _p(ph), int(bool(binary))))
 
     def adjacency(self):
         off = np.zeros(self.nn + 1, np.int32)

@@ -298,6 +298,14 @@ int pdx_model_read(const char *cmd_file, pdx_model **out, int *solver_flag, int 
 
 void pdx_model_destroy(pdx_model *m) { delete m; }
 
+int pdx_write_vtk(const pdx_model *pm, const char *path, const char *title, const double *u, const double *sigma,
+                  const double *phi, int binary) {
+    if (!pm || !path) return fail(PDX_ERR_ARG, "pdx_write_vtk: null argument");
+    std::string err;
+    const int rc = write_vtk(pm->m, path, title ? std::string(title) : pm->m.title, u, sigma, phi, binary != 0, err);
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
 int pdx_model_sizes(const pdx_model *pm, long long *out) {
     if (!pm || !out) return fail(PDX_ERR_ARG, "null argument");
     const Model &m = pm->m;

@@ -657,6 +657,7 @@ int model_read_files(const std::string &cmd, Model &m, int &solver_flag, int &nu
             if (!fin.is_open()) { err = "File \"" + tk[1] + "\" is not exist."; return PDX_ERR_IO; }
             std::string title, label, sdim;
             fin >> title;
+            m.title = title;
             std::getline(fin, label);
             std::getline(fin, label);
             int pro;

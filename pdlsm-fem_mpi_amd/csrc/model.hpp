@@ -39,6 +39,7 @@ struct Model {
     std::vector<NbcSet> nbc;
     std::vector<double> crack;  // [n_crack*9]
     double fac = 3, nlf = 0.33, me = 0, dt = 1, beta = 0.25, gamma = 0.5;
+    std::string title = "pdlsm";  // first token of the mesh file (the VTK title)
 
     // ---- DOFs ----
     std::vector<int> eq;       // [nn*3], -1 prescribed / unused
@@ -88,5 +89,9 @@ int model_read_files(const std::string &cmd, Model &m, int &solver_flag, int &nu
 
 // small numerics shared with the device set-up
 double det33(const double m[3][3]);
+
+// result file in the reference's VTK layout (vtk.cpp); null arrays write zeros
+int write_vtk(const Model &m, const std::string &path, const std::string &title, const double *u,
+              const double *sigma, const double *phi, bool binary, std::string &err);
 
 }  // namespace pdx

@@ -10,7 +10,11 @@ Inputs (data files of the reference, read as text/binary, nothing executed):
 Outputs:
   tests/golden/box/box_mesh.npz  raw mesh arrays + solver settings
   tests/golden/box/box_U.npz     U (float32, big-endian in the VTK) per node
+  tests/golden/box/box_vtk_fields.npz  the VTK's nodal stress tensor (6 components,
+                                 float32) and damage phi (float32), decoded from the file
+  tests/golden/box/box_vtk.sha256      SHA-256 and size of boxVpd4_static.vtk
 """
+import hashlib
 import json
 import os
 import sys
@@ -27,6 +31,12 @@ def read_vtk_vectors(path, name, n):
     key = f"VECTORS {name} float\n".encode()
     i = raw.index(key) + len(key)
     return np.frombuffer(raw[i: i + n * 12], dtype=">f4").reshape(n, 3).astype(np.float32)
+
+
+def read_vtk_section(raw, key, count):
+    """count big-endian float32 values after the line `key`."""
+    i = raw.index(key) + len(key)
+    return np.frombuffer(raw[i: i + count * 4], dtype=">f4").astype(np.float32)
 
 
 def main():
@@ -48,6 +58,15 @@ def main():
                                           fac=m["fac"], nlf=m["nlf"], me=m["me"], dt=m["dt"]))))
     U = read_vtk_vectors(os.path.join(REF, "boxVpd4_static.vtk"), "U", len(m["x"]))
     np.savez_compressed(os.path.join(OUT, "box_U.npz"), U=U)
+    raw = open(os.path.join(REF, "boxVpd4_static.vtk"), "rb").read()
+    n = len(m["x"])
+    t = read_vtk_section(raw, b"TENSORS sigma float\n", 9 * n).reshape(n, 9)
+    sig = t[:, [0, 4, 8, 1, 5, 2]]  # xx, yy, zz, xy, yz, zx from the written rows
+    assert np.array_equal(t, sig[:, [0, 3, 5, 3, 1, 4, 5, 4, 2]])
+    phi = read_vtk_section(raw, b"LOOKUP_TABLE  default  \n", n)
+    np.savez_compressed(os.path.join(OUT, "box_vtk_fields.npz"), sigma=sig, phi=phi)
+    with open(os.path.join(OUT, "box_vtk.sha256"), "w") as f:
+        f.write(f"{hashlib.sha256(raw).hexdigest()} {len(raw)} boxVpd4_static.vtk\n")
     print("wrote", OUT, os.path.getsize(os.path.join(OUT, "box_mesh.npz")), os.path.getsize(os.path.join(OUT, "box_U.npz")))
 
 

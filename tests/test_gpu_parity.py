@@ -181,6 +181,39 @@ def test_box_example_matches_reference_vtk():
     assert rel(Ug, U.astype(np.float64).ravel()) <= 1e-6, rel(Ug, U.ravel())
 
 
+def test_box_example_end_to_end_vtk(tmp_path):
+    """The reference's examples/box end to end on the GPU: static solve, nodal
+    stresses, result file in the reference's binary VTK layout; U and the stress
+    tensor read back from the file match boxVpd4_static.vtk (float32) to 1e-6 and
+    5e-6 (norm-wise), the damage is zero as there."""
+    import os
+    mesh, U = box_fixture()
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "box", "box_vtk_fields.npz"))
+    mp = pdx.Model(mesh)
+    cx = pdx.Context(mp)
+    u, st = cx.static_solve(pdx.solve_opts(precision=0, precond=4, tol=1e-10, max_iter=50000))
+    eq = mp.nodes()["eq"].ravel()
+    u_all = np.zeros(mp.nn * 3)
+    u_all[eq >= 0] = u[eq[eq >= 0]]
+    sig = cx.node_stresses(u_all)
+    path = tmp_path / "box.vtk"
+    mp.write_vtk(path, u=u_all, sigma=sig, title="boxVpd4")
+    raw = path.read_bytes()
+    n = mp.nn
+
+    def section(key, count):
+        i = raw.index(key) + len(key)
+        return np.frombuffer(raw[i: i + count * 4], dtype=">f4").astype(np.float64)
+
+    Uf = section(b"VECTORS U float\n", 3 * n).reshape(n, 3)
+    Tf = section(b"TENSORS sigma float\n", 9 * n).reshape(n, 9)[:, [0, 4, 8, 1, 5, 2]]
+    phi = section(b"LOOKUP_TABLE  default  \n", n)
+    assert rel(Uf, U.astype(np.float64)) <= 1e-6
+    sref = ref["sigma"].astype(np.float64)
+    assert rel(Tf, sref) <= 5e-6, rel(Tf, sref)
+    assert np.array_equal(phi, ref["phi"].astype(np.float64))
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_csr_seam_solves_oracle_system(precision):
     mo = O.Model(W.plate2d())

@@ -23,6 +23,8 @@
         node (PD, FE, averaging, plane-strain sigma_zz); corner values of a
         linear stress field built from the reference's Gauss-point order and L
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -72,6 +74,22 @@ def test_box_golden_detects_dropped_boundary_terms():
     solution is far from the oracle's."""
     _, corr, _ = _box_correction(skip=1)
     assert corr > 1e-3, corr
+
+
+def test_box_golden_node_stresses():
+    """calGlobalNodeStresses against the reference's own output: the oracle's
+    nodal stresses of the boxVpd4_static.vtk displacements equal that file's
+    stress tensor to the float32 rounding of its U (SURVEY a6 / §8f stress row)."""
+    mesh, U = box_fixture()
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "box", "box_vtk_fields.npz"))["sigma"]
+    ref = ref.astype(np.float64)
+    M = O.Model(mesh)
+    S = M.node_stresses(U.astype(np.float64).reshape(-1))
+    assert np.linalg.norm(S - ref) <= 5e-6 * np.linalg.norm(ref)
+    assert np.abs(S - ref).max() <= 1e-5 * np.abs(ref).max()
+    # a planted mistake (strain from u scaled by 1 + 1e-4) is far outside the bound
+    S2 = M.node_stresses(U.astype(np.float64).reshape(-1) * (1 + 1e-4))
+    assert np.linalg.norm(S2 - ref) > 5e-5 * np.linalg.norm(ref)
 
 
 def test_plate2d_pattern_counts_match_reference_binary():

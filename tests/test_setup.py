@@ -146,3 +146,55 @@ def test_python_constants_match_the_header():
             assert getattr(pdx, name) == int(defs[name]), name
     assert pdx.FB_VARIANTS == int(defs["PDX_FB_VARIANTS"])
     assert pdx.EXPLICIT_FORMS == int(defs["PDX_EXPLICIT_FORMS"])
+
+
+def _box_vtk_inputs():
+    from helpers import box_fixture
+    mesh, U = box_fixture()
+    z = np.load(os.path.join(ROOT, "tests", "golden", "box", "box_vtk_fields.npz"))
+    sha, size, _ = open(os.path.join(ROOT, "tests", "golden", "box", "box_vtk.sha256")).read().split()
+    return mesh, U, z["sigma"], z["phi"], sha, int(size)
+
+
+def test_vtk_binary_is_byte_identical_to_the_reference(tmp_path):
+    """The reference's examples/box result file (boxVpd4_static.vtk, 3.5 MB): the
+    writer, given the mesh and the file's own U, stress tensor and damage, reproduces
+    it byte for byte (SHA-256); one changed stress value changes the hash."""
+    import hashlib
+    mesh, U, sig, phi, sha, size = _box_vtk_inputs()
+    m = pdx.Model(mesh)
+    path = tmp_path / "box.vtk"
+    m.write_vtk(path, u=U.astype(np.float64), sigma=sig.astype(np.float64), phi=phi.astype(np.float64),
+                title="boxVpd4", binary=True)
+    raw = path.read_bytes()
+    assert len(raw) == size and hashlib.sha256(raw).hexdigest() == sha
+    sig2 = sig.astype(np.float64).copy()
+    sig2[100, 3] *= 1.5
+    m.write_vtk(path, u=U.astype(np.float64), sigma=sig2, phi=phi.astype(np.float64), title="boxVpd4")
+    assert hashlib.sha256(path.read_bytes()).hexdigest() != sha
+
+
+def test_vtk_ascii_layout(tmp_path):
+    """The ASCII form (VTKFORMAT ASCII): sections, counts and 4-digit scientific
+    values read back to the arrays."""
+    mesh, U, sig, phi, _, _ = _box_vtk_inputs()
+    m = pdx.Model(mesh)
+    path = tmp_path / "box_a.vtk"
+    m.write_vtk(path, u=U.astype(np.float64), sigma=sig.astype(np.float64), phi=phi.astype(np.float64),
+                title="boxVpd4", binary=False)
+    lines = path.read_text().split("\n")
+    assert lines[:5] == ["# vtk DataFile Version 3.0", "boxVpd4", "ASCII", "DATASET UNSTRUCTURED_GRID",
+                         f"POINTS {m.nn} float"]
+    pts = np.array([[float(v) for v in l.split()] for l in lines[5: 5 + m.nn]])
+    assert np.allclose(pts, mesh["x"], rtol=1e-4, atol=0)
+    i = lines.index(f"CELLS {m.ne} {m.ne * 9}")
+    cell = [int(v) for v in lines[i + 1].split()]
+    assert cell == [8] + [int(v) - 1 for v in mesh["econ"][0][:8]]
+    i = lines.index("VECTORS U float")
+    u = np.array([[float(v) for v in l.split()] for l in lines[i + 1: i + 1 + m.nn]])
+    assert np.allclose(u, U, rtol=1e-4, atol=1e-30)
+    i = lines.index("TENSORS sigma float")
+    t = np.array([[float(v) for v in l.split()] for l in lines[i + 1: i + 4]])
+    s0 = sig[0].astype(np.float64)
+    full = np.array([[s0[0], s0[3], s0[5]], [s0[3], s0[1], s0[4]], [s0[5], s0[4], s0[2]]])
+    assert np.allclose(t, full, rtol=1e-4, atol=1e-30) and lines[i + 4] == ""
