@@ -142,13 +142,17 @@ def measured_traffic(kernel):
     return k["traffic_bytes"] if k else None
 
 
-def newmark_bench(pdx, W, n=100, steps=50):
+def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
     """The penny-crack Newmark configuration: 100^3 nodes (dx = 1 mm), penny
     crack of radius 0.2 L between two node layers, PD region the central 30^3
     nodes, bottom fixed, top traction ramped linearly to 5 MPa over the 50 steps,
     beta = 1/4, gamma = 1/2, dt = 10x the explicit critical step, a_{n+1}
-    warm-started; each step solves (M + beta dt^2 K) a = F - K U_N with
-    node-block-Jacobi BiCGStab to a 1e-8 true residual."""
+    warm-started; each step solves (M + beta dt^2 K) a = F - K U_N to a 1e-8 true
+    residual, three ways: plain (FP64 node-block-Jacobi BiCGStab), FP32
+    BiCGStab with FP64 refinement and node-block Jacobi, and the same with the
+    two-level hybrid preconditioner (aggregates of coarse_agg^3 nodes, built
+    once per run).  The mixed runs' displacements are compared with the plain
+    run's."""
     t0 = time.time()
     m0 = W.penny3d(n=n)
     cx0 = pdx.Context(pdx.Model(m0))
@@ -157,16 +161,36 @@ def newmark_bench(pdx, W, n=100, steps=50):
     del cx0
     m = W.penny3d(n=n, dt=dt, ramp_steps=steps)
     mp = pdx.Model(m)
-    cx = pdx.Context(mp)
     setup_s = time.time() - t0
-    opts = pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8, max_iter=100000)
-    t1 = time.time()
-    u, v, a, st = cx.newmark_run(steps, opts)
-    wall = time.time() - t1
-    return {"workload": f"penny3d {n}^3 nodes, PD 30^3, dt = 10 dt_crit = {dt:.3e} s, {steps} Newmark steps",
-            "neq": mp.neq, "host_setup_s": setup_s, "seconds": wall, "ms_per_step": wall / steps * 1e3,
-            "krylov_iterations": st["iterations"], "last_true_relres": st["true_relres"],
-            "dof_steps_per_s": mp.neq * steps / wall, "u_max": float(np.abs(u).max())}
+    out = {"workload": f"penny3d {n}^3 nodes, PD 30^3, dt = 10 dt_crit = {dt:.3e} s, {steps} Newmark steps",
+           "neq": mp.neq, "host_setup_s": setup_s}
+    runs = {"plain_fp64_blockjacobi": pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
+                                                     max_iter=100000),
+            "fp32_ir_blockjacobi": pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
+                                                  max_iter=100000),
+            "fp32_ir_two_level": pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8,
+                                                max_iter=100000, coarse_agg=coarse_agg)}
+    us = {}
+    for name, opts in runs.items():
+        cx = pdx.Context(mp)
+        t1 = time.time()
+        try:
+            u, v, a, st = cx.newmark_run(steps, opts)
+        except Exception as e:  # reported, not hidden
+            out[name] = {"error": str(e)}
+            continue
+        wall = time.time() - t1
+        us[name] = u
+        out[name] = {"seconds": wall, "ms_per_step": wall / steps * 1e3, "krylov_iterations": st["iterations"],
+                     "refinements": st["refinements"], "last_true_relres": st["true_relres"],
+                     "dof_steps_per_s": mp.neq * steps / wall, "u_max": float(np.abs(u).max())}
+        del cx
+    ref = us.get("plain_fp64_blockjacobi")
+    for name in us:
+        if name != "plain_fp64_blockjacobi" and ref is not None:
+            out[name]["u_rel_diff_vs_plain"] = float(np.linalg.norm(us[name] - ref) / max(np.linalg.norm(ref), 1e-300))
+            out[name]["speedup_vs_plain"] = out["plain_fp64_blockjacobi"]["seconds"] / out[name]["seconds"]
+    return out
 
 
 def family_batch_bench(pdx, W):
