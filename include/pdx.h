@@ -102,6 +102,8 @@ typedef struct pdx_mesh_desc {
     double nlf;            /* non-local factor a */
     double me;             /* enrichment radius factor */
     double dt, beta, gamma;
+    double sigma_ult, kic; /* material strength and toughness (mesh file) */
+    int fc, topk;          /* failure criterion and bonds per step (FC / TOPK commands) */
 } pdx_mesh_desc;
 
 typedef struct pdx_solve_opts {
@@ -189,6 +191,25 @@ int pdx_set_dt(pdx_ctx *c, double dt);
 
 int pdx_static_solve(pdx_ctx *c, const pdx_solve_opts *o, double *u, pdx_solve_stats *st);
 
+/* Quasi-static load steps with bond failure: pdfemQuasiStaticSolver_CSRformat
+ * (pdsolve.cpp:6464-6580).  Per step: K_ff du = F_ext - K u at the current DOF values (the
+ * quasi-static assembly, pdsolve.cpp:6434-6462), u += du, the nodal stresses, failureProcess
+ * (:6597-6641; 2D only, as in the reference): with fc == 2 the maximum-principal-stress
+ * criterion breaks bonds (failureCriterion_maxPriSig and Finally_TopK_andUpdate_bondStaus,
+ * :7184-7289, :6981-7182; K is assembled again from the new bond states), then, if no bond
+ * broke, the natural and essential BCs advance (setVaryNaturalBC / setVaryEssentialBC with the
+ * ratio 1.001 sigma_ult / max s1, :5263-5329).  fc, topk < 0 and sigma_ult <= 0 take the
+ * model's FC / TOPK commands and material strength.  u_all [n_node*dim] gets the final DOF
+ * values; step_info [nsteps*4] (or NULL) gets per step: bonds selected, bonds broken, ratio,
+ * largest principal bond stress.  The context keeps its bond states (pdx_ctx_bonds) and DOF
+ * values, so calls continue one another. */
+int pdx_quasi_static_run(pdx_ctx *c, int nsteps, const pdx_solve_opts *o, int fc, int topk, double sigma_ult,
+                         double *u_all, double *step_info, pdx_solve_stats *st);
+/* the context's bond states, laid out like pdx_model_families' bond array */
+int pdx_ctx_bonds(pdx_ctx *c, int *bond);
+/* calLocalDamage (pdsolve.cpp:7338-7373): phi [n_node] = 1 - intact / all bonds of the node's
+ * family (0 for nodes without a family), from the context's bond states */
+int pdx_local_damage(pdx_ctx *c, double *phi);
 /* Nodal stresses, pdsolve::calGlobalNodeStresses (pdsolve.cpp:9540-9604) with
  * calPDNodeStresses (:9081-9166) and calFEMNodeStresses_EXP (:9168-9335), as the
  * drivers call it before every result output (e.g. :6062).  u_all: DOF values

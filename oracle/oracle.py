@@ -80,6 +80,14 @@ def lib():
         L.om_fb_members.argtypes = [I, I, P, P, P, P, I]
         L.om_fb_family.restype = I
         L.om_fb_family.argtypes = [I, P, P, P, D, D, D, I, P, P, P]
+        for name in ("om_set_values", "om_set_bonds", "om_advance_bc_ratio", "om_local_damage"):
+            getattr(L, name).restype = None
+        L.om_set_values.argtypes = [P, P]
+        L.om_set_bonds.argtypes = [P, P]
+        L.om_advance_bc_ratio.argtypes = [P, D]
+        L.om_local_damage.argtypes = [P, P]
+        L.om_fc2_step.restype = I
+        L.om_fc2_step.argtypes = [P, P, D, I, P]
         _lib = L
     return _lib
 
@@ -100,6 +108,7 @@ class Model:
         L = lib()
         self.mesh = mesh
         self.dim = int(mesh["dim"])
+        self.mesh_sigma_ult = float(mesh.get("sigma_ult", 0.0))
         self._keep = []
 
         def arr(x, dt):
@@ -251,6 +260,55 @@ class Model:
         out = np.zeros(self.nn * self.dim)
         lib().om_get_values(self.h, _p(out))
         return out
+
+    def set_values(self, u_all):
+        lib().om_set_values(self.h, _p(np.ascontiguousarray(u_all, np.float64)))
+
+    def set_bonds(self, bond):
+        lib().om_set_bonds(self.h, _p(np.ascontiguousarray(bond, np.int32)))
+
+    def advance_bc_ratio(self, rati):
+        lib().om_advance_bc_ratio(self.h, float(rati))
+
+    def fc2_step(self, sigma, sigma_ult, topk):
+        """(bonds kept, bonds broken, largest principal bond stress) of one FC 2 update."""
+        out = np.zeros(3)
+        _chk(lib().om_fc2_step(self.h, _p(np.ascontiguousarray(sigma, np.float64)), float(sigma_ult), int(topk),
+                               _p(out)))
+        return int(out[0]), int(out[1]), float(out[2])
+
+    def local_damage(self):
+        phi = np.zeros(self.nn)
+        lib().om_local_damage(self.h, _p(phi))
+        return phi
+
+    def quasi_static_run(self, nsteps, fc=2, topk=0, sigma_ult=None):
+        """pdfemQuasiStaticSolver_CSRformat (pdsolve.cpp:6464-6580) with failureProcess
+        (:6597-6641), single rank: per step K_ff du = F_ext - K u (quasi-static assembly,
+        SuperLU in place of PARDISO), u += du, nodal stresses, the FC 2 bond update (2D),
+        then the BCs advance with the load ratio unless a bond was selected.  Returns the DOF
+        values [nn*dim] and per step (bonds kept, bonds broken, ratio, largest s1)."""
+        import scipy.sparse.linalg as spla
+        sigma_ult = self.mesh_sigma_ult if sigma_ult is None else sigma_ult
+        info = np.zeros((nsteps, 4))
+        add_load = True
+        for n in range(nsteps):
+            K, F = self.assemble(2)
+            du = spla.spsolve(self.sparse(K).tocsc(), F)
+            u = self.values()
+            eq = self.nodes()["eq"][:, : self.dim].ravel()
+            u[eq >= 0] += du[eq[eq >= 0]]
+            self.set_values(u)
+            rati, kept, broke, smax = 1.0, 0, 0, 0.0
+            if fc == 2 and self.dim == 2:
+                sig = self.node_stresses(u)
+                kept, broke, smax = self.fc2_step(sig, sigma_ult, topk)
+                add_load = kept == 0
+                rati = 1.0 if abs(smax) < 1e-20 * sigma_ult else 1.001 * sigma_ult / smax
+            info[n] = (kept, broke, rati, smax)
+            if add_load:
+                self.advance_bc_ratio(rati)
+        return self.values(), info
 
     def explicit_run(self, nsteps):
         """u_{nsteps+1} and u_{nsteps} of pdfemDynamicSolver_CSRformat with lumped mass."""

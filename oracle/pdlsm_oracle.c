@@ -1985,3 +1985,127 @@ int om_node_stresses(const om_model *M, const double *u_all, double *sigma) {
     free(count);
     return OM_OK;
 }
+
+/* ---- quasi-static failure (2D, FC 2) -------------------------------------- */
+
+/* DOF values [nn*dim] into the pdDof values (storeDisplacementResult for every DOF) */
+void om_set_values(om_model *M, const double *u_all) {
+    for (int k = 0; k < M->nn; k++)
+        for (int i = 0; i < M->dim; i++) M->val[k * 3 + i] = u_all[k * M->dim + i];
+}
+
+void om_set_bonds(om_model *M, const int *bond) { memcpy(M->bond, bond, sizeof(int) * M->fam_off[M->nfam]); }
+
+/* setVaryEssentialBC with the quasi-static ratio branch (pdsolve.cpp:5263-5312, solvFlag 2)
+ * and setVaryNaturalBC (:5314-5329) */
+void om_advance_bc_ratio(om_model *M, double rati) {
+    for (int s = 0; s < M->n_ebc; s++) {
+        if (!M->ebc_vary[s]) continue;
+        double dv;
+        if (M->ebc_val[s] == 0) {
+            dv = M->ebc_vel[s];
+        } else {
+            double ndv = (rati - 1) * M->ebc_val[s];
+            if (fabs(ndv) > fabs(M->ebc_vel[s])) dv = M->ebc_vel[s];
+            else if (rati < 1) dv = 0;
+            else dv = ndv;
+        }
+        double U = M->ebc_val[s] + M->dt * dv;
+        M->ebc_val[s] = U;
+        for (int t = M->ebc_off[s]; t < M->ebc_off[s + 1]; t++) M->val[(M->ebc_nid[t] - 1) * 3 + M->ebc_dof[s]] = U;
+    }
+    for (int s = 0; s < M->n_nbc; s++)
+        if (M->nbc_vary[s]) M->nbc_val[s] = M->nbc_val[s] + M->dt * M->nbc_vel[s];
+}
+
+typedef struct {
+    double v;
+    long long order;
+    int f, m;
+} om_cand;
+
+static int cand_cmp(const void *a, const void *b) {
+    const om_cand *x = a, *y = b;
+    if (x->v > y->v) return -1;
+    if (x->v < y->v) return 1;
+    return (x->order > y->order) - (x->order < y->order);
+}
+
+/* failureCriterion_maxPriSig + Finally_TopK_andUpdate_bondStaus with flag 2 (pdsolve.cpp:7184-7289,
+ * :6981-7182), single rank, 2D.  sigma [nn*6] (xx, yy, zz, xy, ...).  For every intact bond the
+ * larger eigenvalue s1 of the mean of the two nodes' in-plane stresses, computed in closed form
+ * (the reference calls dsyev; the two agree to rounding); the bonds with s1 > sigult are kept
+ * as the topk largest (0: all; among equal values the earlier in family / member order, as the
+ * reference's min-heap replaces only on a larger value); each kept bond puts a crack segment of
+ * length minDelta through its midpoint, normal to the principal direction, and updateBondstate
+ * breaks the bonds crossing it.  out: [0] bonds kept, [1] bonds broken, [2] largest s1 (>= 0). */
+int om_fc2_step(om_model *M, const double *sigma, double sigult, int topk, double *out) {
+    if (M->dim != 2) return OM_ERR_ARG;
+    int nb = M->fam_off[M->nfam];
+    om_cand *cand = xcalloc(nb + 1, sizeof(om_cand));
+    int nc = 0;
+    long long order = 0;
+    double smax = 0;
+    for (int f = 0; f < M->nfam; f++) {
+        int nk = M->fam_mem[M->fam_off[f]] - 1;
+        const double *sk = &sigma[nk * 6];
+        for (int m = 1; m < M->fam_off[f + 1] - M->fam_off[f]; m++, order++) {
+            int idx = M->fam_off[f] + m;
+            if (M->bond[idx] != 1) continue;
+            const double *sm = &sigma[(M->fam_mem[idx] - 1) * 6];
+            double a = 0.5 * (sk[0] + sm[0]), b = 0.5 * (sk[1] + sm[1]), c = 0.5 * (sk[3] + sm[3]);
+            double h = 0.5 * (a - b);
+            double s1 = 0.5 * (a + b) + sqrt(h * h + c * c);
+            if (s1 > smax) smax = s1;
+            if (s1 > sigult) {
+                cand[nc].v = s1;
+                cand[nc].order = order;
+                cand[nc].f = f;
+                cand[nc].m = m;
+                nc++;
+            }
+        }
+    }
+    qsort(cand, nc, sizeof(om_cand), cand_cmp);
+    if (topk > 0 && nc > topk) nc = topk;
+    int before = 0;
+    for (int i = 0; i < nb; i++) before += M->bond[i];
+    double minD = 0.5 * M->minDelta;
+    for (int i = 0; i < nc; i++) {
+        int f = cand[i].f, idx = M->fam_off[f] + cand[i].m;
+        int nk = M->fam_mem[M->fam_off[f]] - 1, nm = M->fam_mem[idx] - 1;
+        const double *xk = &M->x[nk * 3], *xm = &M->x[nm * 3];
+        double xc[3];
+        for (int q = 0; q < 3; q++) xc[q] = 0.5 * (xk[q] + xm[q]);
+        const double *sk = &sigma[nk * 6], *sm = &sigma[nm * 6];
+        double a = 0.5 * (sk[0] + sm[0]), b = 0.5 * (sk[1] + sm[1]), c = 0.5 * (sk[3] + sm[3]);
+        double h = 0.5 * (a - b), l1 = 0.5 * (a + b) + sqrt(h * h + c * c);
+        double e0, e1;
+        if (c != 0.0) { e0 = l1 - b; e1 = c; }
+        else if (a >= b) { e0 = 1.0; e1 = 0.0; }
+        else { e0 = 0.0; e1 = 1.0; }
+        double nrm = sqrt(e0 * e0 + e1 * e1);
+        e0 /= nrm;
+        e1 /= nrm;
+        double crack[9] = {xc[0] - minD * e1, xc[1] + minD * e0, 0, xc[0] + minD * e1, xc[1] - minD * e0, 0, 0, 0, 0};
+        update_bondstate(M, crack);
+    }
+    int after = 0;
+    for (int i = 0; i < nb; i++) after += M->bond[i];
+    out[0] = nc;
+    out[1] = before - after;
+    out[2] = smax;
+    free(cand);
+    return OM_OK;
+}
+
+/* calLocalDamage (pdsolve.cpp:7338-7373) */
+void om_local_damage(const om_model *M, double *phi) {
+    for (int k = 0; k < M->nn; k++) phi[k] = 0.0;
+    for (int f = 0; f < M->nfam; f++) {
+        double t = 0;
+        int n = M->fam_off[f + 1] - M->fam_off[f];
+        for (int m = 1; m < n; m++) t = t + M->bond[M->fam_off[f] + m];
+        phi[M->fam_mem[M->fam_off[f]] - 1] = 1.0 - t / (n - 1.0);
+    }
+}

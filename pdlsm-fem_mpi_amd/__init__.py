@@ -39,7 +39,7 @@ class MeshDesc(C.Structure):
                 ("ebc_val", _P), ("ebc_vel", _P), ("ebc_vary", _P), ("n_nbc", _I), ("nbc_off", _P),
                 ("nbc_con", _P), ("nbc_val", _P), ("nbc_vel", _P), ("nbc_vary", _P), ("n_crack", _I),
                 ("crack", _P), ("horizon_factor", _D), ("nlf", _D), ("me", _D), ("dt", _D), ("beta", _D),
-                ("gamma", _D)]
+                ("gamma", _D), ("sigma_ult", _D), ("kic", _D), ("fc", _I), ("topk", _I)]
 
 
 class SolveOpts(C.Structure):
@@ -63,6 +63,9 @@ EXPORTS = {
     "pdx_model_create": (_I, [_P, _P]),
     "pdx_model_read": (_I, [C.c_char_p, _P, _P, _P]),
     "pdx_write_vtk": (_I, [_P, C.c_char_p, C.c_char_p, _P, _P, _P, _I]),
+    "pdx_quasi_static_run": (_I, [_P, _I, _P, _I, _I, _D, _P, _P, _P]),
+    "pdx_ctx_bonds": (_I, [_P, _P]),
+    "pdx_local_damage": (_I, [_P, _P]),
     "pdx_model_destroy": (None, [_P]),
     "pdx_model_sizes": (_I, [_P, _P]),
     "pdx_model_csr": (_I, [_P, _P, _P]),
@@ -178,6 +181,8 @@ def mesh_desc(mesh: dict):
     d.n_crack, d.crack = len(cracks), _p(cracks)
     d.horizon_factor, d.nlf, d.me = float(mesh["fac"]), float(mesh["nlf"]), float(mesh.get("me", 0.0))
     d.dt, d.beta, d.gamma = float(mesh.get("dt", 1.0)), float(mesh.get("beta", 0.25)), float(mesh.get("gamma", 0.5))
+    d.sigma_ult, d.kic = float(mesh.get("sigma_ult", 0.0)), float(mesh.get("kic", 0.0))
+    d.fc, d.topk = int(mesh.get("fc", 0)), int(mesh.get("topk", 0))
     return d, keep
 
 
@@ -373,6 +378,29 @@ class Context:
         rc = lib().pdx_static_solve(self.h, C.byref(opts), _p(u), C.byref(st))
         _chk(rc)
         return u[: self.model.neq], st.as_dict()
+
+    def quasi_static_run(self, nsteps, opts=None, fc=-1, topk=-1, sigma_ult=0.0):
+        """Quasi-static load steps with the maximum-principal-stress bond failure (fc=2,
+        2D); returns (u_all [nn*dim], step_info [nsteps, 4]: bonds selected, bonds broken,
+        load ratio, largest principal bond stress, stats)."""
+        opts = opts or solve_opts()
+        u = np.zeros(self.model.nn * self.model.dim)
+        info = np.zeros((max(nsteps, 1), 4))
+        st = SolveStats()
+        _chk(lib().pdx_quasi_static_run(self.h, nsteps, C.byref(opts), fc, topk, sigma_ult, _p(u), _p(info),
+                                        C.byref(st)))
+        return u, info[:nsteps], st.as_dict()
+
+    def bonds(self):
+        """Bond states of this context (1 intact, 0 broken), laid out like Model.families()['bond']."""
+        b = np.zeros(max(self.model.nmem, 1), np.int32)
+        _chk(lib().pdx_ctx_bonds(self.h, _p(b)))
+        return b[: self.model.nmem]
+
+    def local_damage(self):
+        phi = np.zeros(self.model.nn)
+        _chk(lib().pdx_local_damage(self.h, _p(phi)))
+        return phi
 
     def newmark_run(self, nsteps, opts=None):
         opts = opts or solve_opts()

@@ -384,6 +384,7 @@ int pdx_ctx_create(const pdx_model *pm, int device, pdx_ctx **out) {
     const Model &m = pm->m;
     pc->dofval.resize((size_t)m.nn * m.dim);
     pdx_model_values(pm, pc->dofval.data());
+    pc->bond = m.bond;
     *out = pc;
     return PDX_OK;
 }
@@ -684,6 +685,215 @@ int pdx_static_solve(pdx_ctx *pc, const pdx_solve_opts *op, double *u, pdx_solve
         }
     if (st) *st = s;
     return rc ? fail(rc, err) : PDX_OK;
+}
+
+// ---- quasi-static load steps with bond failure ----------------------------
+
+// One step of failureCriterion_maxPriSig + Finally_TopK_andUpdate_bondStaus (flag 2)
+// (pdsolve.cpp:7184-7289, :6981-7182) on the bond states of the context: the largest principal
+// stress s1 of every intact bond (device), the bonds with s1 > sigma_ult kept as the topk largest
+// (0: all; ties keep the bond met first in family / member order, as the reference's min-heap
+// replaces only on a larger value), and for each kept bond a crack segment of length 2 * minD
+// (minD = half the smallest horizon) through the bond's midpoint, normal to the principal
+// direction, which breaks every intact bond that crosses it.  Returns the bonds kept; smax gets
+// the largest s1 over the intact bonds (at least 0), nbroke the bonds broken.
+static int fc2_step(pdx_ctx *pc, const double *sig_dev, const std::vector<double> &sig, double sigma_ult, int topk,
+                    double &smax, int &nkept, int &nbroke, std::string &err) {
+    Ctx &c = *pc->c;
+    const Model &m = pc->model->m;
+    const size_t nb = pc->bond.size();
+    double *s1_dev;
+    int *bond_dev = c.bond;
+    PDX_HIP(hipMalloc((void **)&s1_dev, std::max<size_t>(nb, 1) * 8));
+    int rc = dev_bond_sigma1(c, bond_dev, sig_dev, s1_dev, err);
+    std::vector<double> s1(nb);
+    if (!rc && nb) {
+        PDX_HIP(hipMemcpyAsync(s1.data(), s1_dev, nb * 8, hipMemcpyDeviceToHost, c.stream));
+        PDX_HIP(hipStreamSynchronize(c.stream));
+    }
+    hipFree(s1_dev);
+    if (rc) return rc;
+    smax = 0.0;
+    struct Cand {
+        double v;
+        long long order;  // family / member scan order
+        int f, q;
+    };
+    std::vector<Cand> cand;
+    long long order = 0;
+    for (int f = 0; f < m.nfam; f++)
+        for (int q = m.fam_off[f] + 1; q < m.fam_off[f + 1]; q++, order++) {
+            if (pc->bond[q] != 1) continue;
+            const double v = s1[q];
+            if (v > smax) smax = v;
+            if (v > sigma_ult) cand.push_back({v, order, f, q});
+        }
+    std::stable_sort(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.v > b.v; });
+    if (topk > 0 && (int)cand.size() > topk) cand.resize(topk);
+    nkept = (int)cand.size();
+    nbroke = 0;
+    const double minD = 0.5 * m.minDelta;
+    for (const Cand &k : cand) {
+        const int nk = m.fam_node[k.f], nm = m.fam_mem[k.q];
+        const double *xk = &m.x[(size_t)nk * 3], *xm = &m.x[(size_t)nm * 3];
+        double xc[3];
+        for (int i = 0; i < 3; i++) xc[i] = 0.5 * (xk[i] + xm[i]);
+        const double *sk = &sig[(size_t)nk * 6], *sm = &sig[(size_t)nm * 6];
+        const double a = 0.5 * (sk[0] + sm[0]), b = 0.5 * (sk[1] + sm[1]), cc = 0.5 * (sk[3] + sm[3]);
+        // unit eigenvector of the larger eigenvalue of [[a, cc], [cc, b]] (the first axis for a
+        // multiple of the identity)
+        const double h = 0.5 * (a - b), l1 = 0.5 * (a + b) + std::sqrt(h * h + cc * cc);
+        double e0, e1;
+        if (cc != 0.0) {
+            e0 = l1 - b;
+            e1 = cc;
+        } else if (a >= b) {
+            e0 = 1.0;
+            e1 = 0.0;
+        } else {
+            e0 = 0.0;
+            e1 = 1.0;
+        }
+        const double nrm = std::sqrt(e0 * e0 + e1 * e1);
+        e0 /= nrm;
+        e1 /= nrm;
+        double crack[9] = {xc[0] - minD * e1, xc[1] + minD * e0, 0.0, xc[0] + minD * e1, xc[1] - minD * e0, 0.0,
+                           0.0, 0.0, 0.0};
+        nbroke += break_bonds_across(m, crack, pc->bond);
+    }
+    if (nbroke) {
+        PDX_HIP(hipMemcpy(c.bond, pc->bond.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+        c.K_valid = false;  // the next assembly sees the new bond states
+    }
+    return PDX_OK;
+}
+
+int pdx_quasi_static_run(pdx_ctx *pc, int nsteps, const pdx_solve_opts *op, int fc, int topk, double sigma_ult,
+                         double *u_all, double *step_info, pdx_solve_stats *st) {
+    if (!pc || nsteps < 0) return fail(PDX_ERR_ARG, "pdx_quasi_static_run: bad argument");
+    Ctx &c = *pc->c;
+    const Model &m = pc->model->m;
+    const pdx_solve_opts o = op ? *op : default_opts();
+    if (fc < 0) fc = m.fc;
+    if (topk < 0) topk = m.topk;
+    if (!(sigma_ult > 0)) sigma_ult = m.sig_ult;
+    if (fc != 0 && fc != 2) return fail(PDX_ERR_ARG, "pdx_quasi_static_run: failure criterion 0 or 2");
+    if (fc == 2 && m.dim == 2 && !(sigma_ult > 0)) return fail(PDX_ERR_ARG, "pdx_quasi_static_run: sigma_ult <= 0");
+    std::string err;
+    int rc = 0;
+    const size_t n = (size_t)c.nn * c.dim;
+    const double dt = m.dt;
+    std::vector<double> ebc_cur(m.ebc.size());
+    for (size_t s = 0; s < m.ebc.size(); s++) ebc_cur[s] = m.ebc[s].val;
+    // current prescribed values are the DOF values of the context
+    for (size_t s = 0; s < m.ebc.size(); s++)
+        if (!m.ebc[s].nid.empty()) ebc_cur[s] = pc->dofval[(size_t)m.ebc[s].nid[0] * c.dim + m.ebc[s].dof];
+    double *db = nullptr, *dx = nullptr, *du = nullptr, *dsig = nullptr;
+    auto cleanup = [&]() {
+        for (double *p : {db, dx, du, dsig})
+            if (p) hipFree(p);
+    };
+    if (hipMalloc((void **)&db, n * 8) != hipSuccess || hipMalloc((void **)&dx, n * 8) != hipSuccess ||
+        hipMalloc((void **)&du, n * 8) != hipSuccess || hipMalloc((void **)&dsig, (size_t)c.nn * 6 * 8) != hipSuccess) {
+        cleanup();
+        return fail(PDX_ERR_HIP, "alloc");
+    }
+    std::vector<double> Fq(std::max(c.neq, 1)), b(n), x(n), sig((size_t)c.nn * 6);
+    pdx_solve_stats tot{};
+    bool add_load = true;
+    for (int step = 0; step < nsteps && !rc; step++) {
+        // pdfemQuasiStaticAssembleEquasSys_CSRformat (pdsolve.cpp:6434-6462): K_ff and
+        // F = F_ext - K u over all columns at the current DOF values; K_ff du = F
+        if (!c.K_valid && (rc = dev_assemble_K(c, err))) break;
+        if ((rc = force_from(pc, pc->dofval, true, Fq.data(), err))) break;
+        std::fill(b.begin(), b.end(), 0.0);
+        std::fill(x.begin(), x.end(), 0.0);
+        for (int k = 0; k < c.nn; k++)
+            for (int i = 0; i < c.dim; i++) {
+                const int r = m.eq[(size_t)k * 3 + i];
+                if (r >= 0) b[(size_t)k * c.dim + i] = Fq[r];
+            }
+        bool loaded = false;
+        for (double v : b) loaded = loaded || v != 0.0;
+        if (loaded) {  // an unloaded step (F = 0) has du = 0
+            hipMemcpyAsync(db, b.data(), n * 8, hipMemcpyHostToDevice, c.stream);
+            hipMemcpyAsync(dx, x.data(), n * 8, hipMemcpyHostToDevice, c.stream);
+            pdx_solve_stats s{};
+            if ((rc = dev_solve(c, o, db, dx, s, err))) break;
+            tot.iterations += s.iterations;
+            tot.refinements += s.refinements;
+            tot.true_relres = s.true_relres;
+            tot.seconds += s.seconds;
+            tot.converged = s.converged;
+            hipMemcpyAsync(x.data(), dx, n * 8, hipMemcpyDeviceToHost, c.stream);
+            hipStreamSynchronize(c.stream);
+        }
+        for (int k = 0; k < c.nn; k++)
+            for (int i = 0; i < c.dim; i++)
+                if (m.eq[(size_t)k * 3 + i] >= 0) pc->dofval[(size_t)k * c.dim + i] += x[(size_t)k * c.dim + i];
+        // calGlobalNodeStresses, then failureProcess (pdsolve.cpp:6601-6627; 3D returns at once)
+        double rati = 1.0, smax = 0.0;
+        int nkept = 0, nbroke = 0;
+        if (fc == 2 && c.dim == 2) {
+            hipMemcpyAsync(du, pc->dofval.data(), n * 8, hipMemcpyHostToDevice, c.stream);
+            if ((rc = dev_node_stresses(c, du, dsig, err))) break;
+            hipMemcpyAsync(sig.data(), dsig, sig.size() * 8, hipMemcpyDeviceToHost, c.stream);
+            hipStreamSynchronize(c.stream);
+            if ((rc = fc2_step(pc, dsig, sig, sigma_ult, topk, smax, nkept, nbroke, err))) break;
+            add_load = nkept == 0;
+            rati = std::fabs(smax) < 1e-20 * sigma_ult ? 1.0 : 1.001 * sigma_ult / smax;
+        }
+        if (step_info) {
+            step_info[step * 4] = nkept;
+            step_info[step * 4 + 1] = nbroke;
+            step_info[step * 4 + 2] = rati;
+            step_info[step * 4 + 3] = smax;
+        }
+        if (add_load) {
+            // setVaryNaturalBC, setVaryEssentialBC with the ratio (pdsolve.cpp:5263-5329)
+            for (int s2 = 0; s2 < c.n_nbc; s2++)
+                if (c.nbc_vary[s2]) c.nbc_val[s2] = c.nbc_val[s2] + dt * c.nbc_vel[s2];
+            for (size_t s2 = 0; s2 < m.ebc.size(); s2++) {
+                const EbcSet &E = m.ebc[s2];
+                if (!E.vary) continue;
+                double dv;
+                if (ebc_cur[s2] == 0) {
+                    dv = E.vel;
+                } else {
+                    const double ndv = (rati - 1) * ebc_cur[s2];
+                    if (std::fabs(ndv) > std::fabs(E.vel)) dv = E.vel;
+                    else if (rati < 1) dv = 0;
+                    else dv = ndv;
+                }
+                ebc_cur[s2] = ebc_cur[s2] + dt * dv;
+                for (int k : E.nid) pc->dofval[(size_t)k * c.dim + E.dof] = ebc_cur[s2];
+            }
+        }
+    }
+    cleanup();
+    if (rc) return fail(rc, err);
+    if (u_all) std::copy(pc->dofval.begin(), pc->dofval.end(), u_all);
+    if (st) *st = tot;
+    return PDX_OK;
+}
+
+int pdx_ctx_bonds(pdx_ctx *pc, int *bond) {
+    if (!pc || !bond) return fail(PDX_ERR_ARG, "null argument");
+    std::copy(pc->bond.begin(), pc->bond.end(), bond);
+    return PDX_OK;
+}
+
+int pdx_local_damage(pdx_ctx *pc, double *phi) {
+    // calLocalDamage (pdsolve.cpp:7338-7373): 1 - intact / all bonds of the node's family
+    if (!pc || !phi) return fail(PDX_ERR_ARG, "null argument");
+    const Model &m = pc->model->m;
+    std::fill(phi, phi + m.nn, 0.0);
+    for (int f = 0; f < m.nfam; f++) {
+        double t = 0;
+        for (int q = m.fam_off[f] + 1; q < m.fam_off[f + 1]; q++) t = t + pc->bond[q];
+        phi[m.fam_node[f]] = 1.0 - t / (m.fam_off[f + 1] - m.fam_off[f] - 1.0);
+    }
+    return PDX_OK;
 }
 
 }  // extern "C"

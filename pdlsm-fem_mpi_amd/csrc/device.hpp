@@ -141,6 +141,7 @@ int dev_spmv_K(Ctx &c, const double *K64, const double *x, double *y);
 void free_two_level(Ctx &c);
 void free_gnn_work(Ctx &c);
 int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err);
+int dev_bond_sigma1(Ctx &c, const int *bond_dev, const double *sigma_dev, double *s1_dev, std::string &err);
 double two_level_setup_ms(Ctx &c);
 int set_error(int rc, const std::string &msg);
 
@@ -154,6 +155,7 @@ struct pdx_ctx {
     pdx::Ctx *c = nullptr;
     const pdx_model *model = nullptr;
     std::vector<double> dofval;  // DOF values used by the assembly modes, [nn*dim]
+    std::vector<int> bond;       // bond states of this context (the model's, then broken by failure)
 };
 
 #define PDX_HIP(call)                                                                 \

@@ -323,6 +323,31 @@ void boundary_entities(Model &m) {
 
 }  // namespace
 
+// updateBondstate (pdsolve.cpp:8910-9004): every intact bond of a family whose node lies in the
+// cells of the crack's bounding box (one cell of margin) and whose segment crosses the crack (2D:
+// the segment crack[0..1]; 3D: the triangle crack[0..2]) is broken; returns how many
+int break_bonds_across(const Model &m, const double *crack, std::vector<int> &bond) {
+    int lo[3], hi[3], nb = 0;
+    crack_box(m, crack, lo, hi);
+    for_cells(m, lo, hi, [&](int b) {
+        for (int t = m.cell_node_off[b]; t < m.cell_node_off[b + 1]; t++) {
+            const int f = m.famid[m.cell_node[t]];
+            if (f < 0) continue;
+            const double *xk = &m.x[size_t(m.fam_node[f]) * 3];
+            for (int q = m.fam_off[f] + 1; q < m.fam_off[f + 1]; q++) {
+                if (bond[q] != 1) continue;
+                const double *xm = &m.x[size_t(m.fam_mem[q]) * 3];
+                const bool cut = m.dim == 3 ? seg_tri_plane(xk, xm, crack) : seg_seg(xk, xm, crack, crack + 3);
+                if (cut) {
+                    bond[q] = 0;
+                    nb++;
+                }
+            }
+        }
+    });
+    return nb;
+}
+
 int Model::setup(std::string &err) {
     // ---- D matrix and Lame constants (pdsolve.cpp:37-94) ----
     std::fill(D, D + 36, 0.0);
@@ -506,24 +531,7 @@ int Model::setup(std::string &err) {
     bond.assign(fam_mem.size(), 1);
 
     // ---- bond states across pre-existing cracks (pdsolve.cpp:1823-1915, :8910-9004) ----
-    for (size_t c = 0; c < ncr; c++) {
-        const double *cr = &crack[c * 9];
-        int lo[3], hi[3];
-        crack_box(*this, cr, lo, hi);
-        for_cells(*this, lo, hi, [&](int b) {
-            for (int t = cell_node_off[b]; t < cell_node_off[b + 1]; t++) {
-                int f = famid[cell_node[t]];
-                if (f < 0) continue;
-                const double *xk = &x[size_t(fam_node[f]) * 3];
-                for (int q = fam_off[f] + 1; q < fam_off[f + 1]; q++) {
-                    if (bond[q] != 1) continue;
-                    const double *xm = &x[size_t(fam_mem[q]) * 3];
-                    bool cut = dim == 3 ? seg_tri_plane(xk, xm, cr) : seg_seg(xk, xm, cr, cr + 3);
-                    if (cut) bond[q] = 0;
-                }
-            }
-        });
-    }
+    for (size_t c = 0; c < ncr; c++) break_bonds_across(*this, &crack[c * 9], bond);
 
     // ---- node graph and CSR (pdsolve.cpp:959-1121) ----
     std::vector<std::vector<int>> nbr(nn);
@@ -628,6 +636,10 @@ int model_from_desc(const pdx_mesh_desc &d, Model &m, std::string &err) {
     m.dt = d.dt;
     m.beta = d.beta;
     m.gamma = d.gamma;
+    m.sig_ult = d.sigma_ult;
+    m.kic = d.kic;
+    m.fc = d.fc;
+    m.topk = d.topk;
     return m.setup(err);
 }
 
@@ -667,6 +679,8 @@ int model_read_files(const std::string &cmd, Model &m, int &solver_flag, int &nu
             m.dim = sdim == "3D" ? 3 : (sdim == "2D" ? 2 : 0);
             if (!m.dim) { err = "The problem dimension is not 3D or 2D"; return PDX_ERR_MESH; }
             m.pro_type = pro; m.E = E; m.nu = nu; m.rho = rho;
+            m.kic = k1;
+            m.sig_ult = su;
             fin >> m.nn >> m.ne;
             m.x.assign(size_t(m.nn) * 3, 0.0);
             for (int i = 0; i < m.nn; i++) {
@@ -752,6 +766,12 @@ int model_read_files(const std::string &cmd, Model &m, int &solver_flag, int &nu
             m.fac = std::atof(tk[4].c_str());
             m.nlf = std::atof(tk[5].c_str());
             if (tk.size() > 6) m.me = std::atof(tk[6].c_str());
+        } else if (tk[0] == "FC" && tk.size() > 1) {
+            // fioFiles::excuteCMD (fioFile.cpp:340-386): FC flag [k for flag 2]
+            m.fc = std::atoi(tk[1].c_str());
+            if (m.fc == 2 && tk.size() > 2) m.topk = std::atoi(tk[2].c_str());
+        } else if (tk[0] == "TOPK" && tk.size() > 1) {
+            m.topk = std::atoi(tk[1].c_str());
         } else if (tk[0] == "NEWMARK" && tk.size() > 3) {
             m.gamma = std::atof(tk[2].c_str());
             m.beta = std::atof(tk[3].c_str());

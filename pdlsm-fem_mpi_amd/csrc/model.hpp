@@ -40,6 +40,8 @@ struct Model {
     std::vector<double> crack;  // [n_crack*9]
     double fac = 3, nlf = 0.33, me = 0, dt = 1, beta = 0.25, gamma = 0.5;
     std::string title = "pdlsm";  // first token of the mesh file (the VTK title)
+    double sig_ult = 0, kic = 0;  // material strength and toughness of the mesh file
+    int fc = 0, topk = 0;         // failure criterion (FC) and bonds per step (TOPK) of the command file
 
     // ---- DOFs ----
     std::vector<int> eq;       // [nn*3], -1 prescribed / unused
@@ -89,6 +91,10 @@ int model_read_files(const std::string &cmd, Model &m, int &solver_flag, int &nu
 
 // small numerics shared with the device set-up
 double det33(const double m[3][3]);
+
+// updateBondstate (pdsolve.cpp:8910-9004): breaks the intact bonds (in `bond`, laid out like
+// Model::bond) that cross the crack (2D segment / 3D triangle, 9 doubles); returns how many
+int break_bonds_across(const Model &m, const double *crack, std::vector<int> &bond);
 
 // result file in the reference's VTK layout (vtk.cpp); null arrays write zeros
 int write_vtk(const Model &m, const std::string &path, const std::string &title, const double *u,

@@ -337,6 +337,41 @@ __global__ void k_stress_nodes(int nn, const int *famid, const double *sig_fam, 
 }
 
 // sigma [nn*6] (device) of the DOF values u_dev [nn*d] (device)
+// Largest principal stress of every intact bond of a 2D family: the eigenvalue of the 2x2
+// mean of the two nodes' stresses, (a + b) / 2 + sqrt(((a - b) / 2)^2 + c^2), as
+// failureCriterion_maxPriSig computes it with dsyev (pdsolve.cpp:7184-7247).  One thread per
+// family; broken bonds and the self member get -inf.
+__global__ void k_bond_sigma1_2d(int nfam, const int *__restrict__ fam_off, const int *__restrict__ fam_mem,
+                                 const int *__restrict__ bond, const double *__restrict__ sig, double *s1) {
+#pragma clang fp contract(off)
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfam) return;
+    const int o0 = fam_off[f], o1 = fam_off[f + 1];
+    const double *sk = sig + (size_t)fam_mem[o0] * 6;
+    s1[o0] = -INFINITY;
+    for (int q = o0 + 1; q < o1; q++) {
+        if (bond[q] != 1) {
+            s1[q] = -INFINITY;
+            continue;
+        }
+        const double *sm = sig + (size_t)fam_mem[q] * 6;
+        const double a = 0.5 * (sk[0] + sm[0]), b = 0.5 * (sk[1] + sm[1]), c = 0.5 * (sk[3] + sm[3]);
+        const double h = 0.5 * (a - b);
+        s1[q] = 0.5 * (a + b) + sqrt(h * h + c * c);
+    }
+}
+
+int dev_bond_sigma1(Ctx &c, const int *bond_dev, const double *sigma_dev, double *s1_dev, std::string &err) {
+    if (c.dim != 2) {
+        err = "the principal-stress bond criterion is two-dimensional";
+        return PDX_ERR_ARG;
+    }
+    if (c.nfam > 0) k_bond_sigma1_2d<<<(c.nfam + 127) / 128, 128, 0, c.stream>>>(c.nfam, c.fam_off, c.fam_mem, bond_dev,
+                                                                                sigma_dev, s1_dev);
+    PDX_HIP(hipGetLastError());
+    return PDX_OK;
+}
+
 int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err) {
     if (!c.K_valid) {
         int rc = dev_assemble_K(c, err);  // the family kernel leaves the bond strain weights

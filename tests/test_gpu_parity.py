@@ -214,6 +214,33 @@ def test_box_example_end_to_end_vtk(tmp_path):
     assert np.array_equal(phi, ref["phi"].astype(np.float64))
 
 
+def _quasi_case():
+    m = W.crack2d(seed=1, nx=41, ny=21, dx=1e-3)
+    m["ebc"][2] = dict(nid=m["ebc"][2]["nid"], dof=1, val=0.0, vary=True, vel=2e-6)
+    return m
+
+
+@pytest.mark.parametrize("topk,steps", [(2, 6), (1, 8)])
+def test_quasi_static_fc2_matches_oracle(topk, steps):
+    """Quasi-static load steps of an edge-cracked plate with the maximum-principal-stress
+    bond failure (FC 2, TOPK bonds per step): per step the bonds selected and broken are
+    the oracle's (exact), the load ratio and largest principal bond stress within 1e-8;
+    after the run the bond states and the local damage equal the oracle's and the DOF
+    values agree to 1e-8."""
+    m = _quasi_case()
+    mo = O.Model(m)
+    u0, info0 = mo.quasi_static_run(steps, fc=2, topk=topk, sigma_ult=3.56e7)
+    assert info0[:, 1].sum() > 0
+    cx = pdx.Context(pdx.Model(m))
+    u1, info1, st = cx.quasi_static_run(steps, pdx.solve_opts(precision=0, precond=2, tol=1e-12, max_iter=100000),
+                                        fc=2, topk=topk, sigma_ult=3.56e7)
+    assert np.array_equal(info1[:, :2], info0[:, :2]), (info1, info0)
+    assert rel(info1[:, 2:], info0[:, 2:]) <= 1e-8
+    assert np.array_equal(cx.bonds(), mo.families()["bond"])
+    assert np.array_equal(cx.local_damage(), mo.local_damage())
+    assert rel(u1, u0) <= 1e-8
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_csr_seam_solves_oracle_system(precision):
     mo = O.Model(W.plate2d())

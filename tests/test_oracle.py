@@ -501,3 +501,104 @@ def test_node_stresses_extrapolate_linear_stress(kind):
         for i in range(len(x)):
             X, Y = x[i, 0], x[i, 1]
             assert np.allclose(sig[i], D @ np.array([Y, 0, 0, X, 0, 0]), rtol=0, atol=1e-9 * np.abs(D).max())
+
+
+# ---------------------------------------------------------------------------
+# quasi-static load steps with the maximum-principal-stress bond failure (2D, FC 2)
+# ---------------------------------------------------------------------------
+
+def _quasi_case(seed=1):
+    m = W.crack2d(seed=seed, nx=41, ny=21, dx=1e-3)
+    m["ebc"][2] = dict(nid=m["ebc"][2]["nid"], dof=1, val=0.0, vary=True, vel=2e-6)
+    return m
+
+
+def test_quasi_static_first_load_step_is_the_static_solution():
+    """Step 1 of pdfemQuasiStaticSolver carries no load (the ramped EBC starts at 0) and
+    advances the EBC by vel; step 2 solves K_ff du = -K u at the new prescribed values,
+    which for a linear model is the static solution with that EBC (known answer)."""
+    import scipy.sparse.linalg as spla
+    m = _quasi_case()
+    M = O.Model(m)
+    u, info = M.quasi_static_run(2, fc=2, topk=2, sigma_ult=1e9)
+    assert info[0, 3] == 0 and info[0, 2] == 1.0 and info[1, 1] == 0
+    ms = _quasi_case()
+    ms["ebc"][2] = dict(nid=ms["ebc"][2]["nid"], dof=1, val=2e-6)
+    S = O.Model(ms)
+    K, F = S.assemble(1)
+    us = spla.spsolve(S.sparse(K).tocsc(), F)
+    eq = S.nodes()["eq"][:, :2].ravel()
+    free = eq >= 0  # the prescribed values have already advanced to the next load level
+    assert np.linalg.norm(u[free] - us[eq[free]]) <= 1e-12 * np.linalg.norm(us)
+
+
+def test_quasi_static_fc2_breaks_bond_pairs_along_cracks():
+    """The FC 2 update puts each crack segment through a bond's midpoint; the bond and its
+    reverse (same midpoint, same stress mean) are cut together, so broken bonds come in
+    pairs; bonds break, the load is held while they do, and the damage is the broken
+    fraction of each family."""
+    M = O.Model(_quasi_case())
+    u, info = M.quasi_static_run(6, fc=2, topk=2, sigma_ult=3.56e7)
+    assert info[:, 1].sum() > 0 and np.all(info[:, 0] <= 2)
+    fam = M.families()
+    off, mem, bond = fam["off"], fam["mem"], fam["bond"]
+    state = {}
+    for f in range(len(off) - 1):
+        k = mem[off[f]]
+        for q in range(off[f] + 1, off[f + 1]):
+            state[(k, mem[q])] = bond[q]
+    broken = [(a, b) for (a, b), v in state.items() if v == 0]
+    assert broken and all(state.get((b, a), 0) == 0 for a, b in broken)
+    phi = M.local_damage()
+    for f in range(len(off) - 1):
+        n = off[f + 1] - off[f] - 1
+        assert phi[mem[off[f]] - 1] == 1.0 - bond[off[f] + 1: off[f + 1]].sum() / n
+
+
+def _seg_cross(a1, a2, b1, b2):
+    """pdsolve::intersection (pdsolve.cpp:9058-9079), brute-force copy for the check."""
+    if (max(a1[0], a2[0]) < min(b1[0], b2[0]) or max(a1[1], a2[1]) < min(b1[1], b2[1]) or
+            max(b1[0], b2[0]) < min(a1[0], a2[0]) or max(b1[1], b2[1]) < min(a1[1], a2[1])):
+        return False
+    s1 = (((a1[0] - b1[0]) * (b2[1] - b1[1]) - (a1[1] - b1[1]) * (b2[0] - b1[0])) *
+          ((a2[0] - b1[0]) * (b2[1] - b1[1]) - (a2[1] - b1[1]) * (b2[0] - b1[0])))
+    s2 = (((b1[0] - a1[0]) * (a2[1] - a1[1]) - (b1[1] - a1[1]) * (a2[0] - a1[0])) *
+          ((b2[0] - a1[0]) * (a2[1] - a1[1]) - (b2[1] - a1[1]) * (a2[0] - a1[0])))
+    return not (s1 > 0 or s2 > 0)
+
+
+@pytest.mark.parametrize("field", ["syy", "sxx", "sxy"])
+def test_quasi_static_fc2_cracks_are_normal_to_the_principal_stress(field):
+    """One FC 2 update on an all-PD plate under a uniform stress (sigma_yy, sigma_xx or
+    pure shear): every bond ties, so the topk bonds kept are the first ones in family /
+    member order; each puts a crack segment of length minDelta through its midpoint,
+    normal to the principal direction (y, x, or the diagonal of the shear); the bonds cut
+    are exactly those a brute-force segment test over all bonds finds (a crack through a
+    bond's midpoint on a node row also cuts the collinear bonds of that row, as the
+    reference's segment test counts a collinear overlap as a crossing)."""
+    m = W.crack2d(seed=1, nx=15, ny=11, dx=1e-3)
+    m["cracks"] = np.zeros((0, 3, 3))
+    m["algo"] = np.ones_like(m["algo"])
+    M = O.Model(m)
+    sig = np.zeros((M.nn, 6))
+    sig[:, {"syy": 1, "sxx": 0, "sxy": 3}[field]] = 1e6
+    kept, broke, smax = M.fc2_step(sig, 5e5, 3)
+    assert kept == 3 and smax == 1e6 and broke > 0
+    fam = M.families()
+    off, mem, bond = fam["off"], fam["mem"], fam["bond"]
+    x = m["x"]
+    e = {"syy": (0.0, 1.0), "sxx": (1.0, 0.0), "sxy": (2 ** -0.5, 2 ** -0.5)}[field]
+    half = 0.5 * M.geom()["minDelta"]
+    firsts = [(f, q) for f in range(len(off) - 1) for q in range(off[f] + 1, off[f + 1])][:3]
+    segs = []
+    for f, q in firsts:
+        xc = 0.5 * (x[mem[off[f]] - 1] + x[mem[q] - 1])
+        segs.append(((xc[0] - half * e[1], xc[1] + half * e[0]), (xc[0] + half * e[1], xc[1] - half * e[0])))
+    expect = np.ones_like(bond)
+    for f in range(len(off) - 1):
+        xk = x[mem[off[f]] - 1]
+        for q in range(off[f] + 1, off[f + 1]):
+            xm = x[mem[q] - 1]
+            if any(_seg_cross(xk, xm, s0, s1) for s0, s1 in segs):
+                expect[q] = 0
+    assert np.array_equal(bond, expect)
