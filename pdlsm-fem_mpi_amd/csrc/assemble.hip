@@ -581,27 +581,34 @@ __global__ void k_ke_simplex(KeArgs A) {
     for (int c = 0; c < NEN * D; c++) out[c] = Krow[c] * V;
 }
 
-// node-centric add of the staged element rows into the node's block row, in
-// element order (the reference's summation order of FE terms into K)
+// Block-owned add of the staged element rows: one thread per (K block, row) starts from the
+// block's PD part and adds the element terms in element order (the reference's summation order
+// of FE terms into K), then writes the row once.  The contributions of block (a, c) are the
+// rows of node a in the elements that hold both nodes; the threads of one row node read
+// neighbouring stretches of the staged rows, and K is written once, contiguously.
 template <int D>
-__global__ void k_fe_gather(int nn, const int *nfe_off, const int *nfe_item, const int *fe_elem, const int *nen,
-                            const int *fe_blk, const double *S, double *K) {
+__global__ void k_fe_gather(long long nblk, const int *__restrict__ fec_off, const int *__restrict__ fec,
+                            const int *__restrict__ nfe_item, const double *__restrict__ S, double *K) {
     constexpr int ND = D == 3 ? 24 : 8, DD = D * D;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nn * D) return;
-    const int a = t / D, ii = t % D;
-    for (int q = nfe_off[a]; q < nfe_off[a + 1]; q++) {
-        const int item = nfe_item[q];
-        const int fp = item >> 3, la = item & 7;
-        const int n = nen[fe_elem[fp]];
-        const double *srow = S + (size_t)fp * ND * ND + (size_t)(D * la + ii) * ND;
-        const int *blk = &fe_blk[(size_t)fp * 64 + la * 8];
-        for (int b = 0; b < n; b++) {
-            double *kb = K + (size_t)blk[b] * DD + ii * D;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= nblk * D) return;
+    const long long q = t / D;
+    const int ii = int(t % D);
+    const int c0 = fec_off[q], c1 = fec_off[q + 1];
+    if (c0 == c1) return;
+    double *kb = K + (size_t)q * DD + ii * D;
+    double acc[D];
 #pragma unroll
-            for (int jj = 0; jj < D; jj++) kb[jj] = kb[jj] + srow[D * b + jj];
-        }
+    for (int jj = 0; jj < D; jj++) acc[jj] = kb[jj];
+    for (int c = c0; c < c1; c++) {
+        const int e = fec[c], it = nfe_item[e >> 3];
+        const int fp = it >> 3, la = it & 7, lb = e & 7;
+        const double *srow = S + (size_t)fp * ND * ND + (size_t)(D * la + ii) * ND + D * lb;
+#pragma unroll
+        for (int jj = 0; jj < D; jj++) acc[jj] = acc[jj] + srow[jj];
     }
+#pragma unroll
+    for (int jj = 0; jj < D; jj++) kb[jj] = acc[jj];
 }
 
 // ------------------------------------------------------------------------
@@ -804,8 +811,8 @@ int dev_assemble_K(Ctx &c, std::string &err) {
                 A.list = c.fe_list_b; A.nlist = c.n_fe_b;
                 k_ke_simplex<3><<<(c.n_fe_b * 12 + 127) / 128, 128, 0, c.stream>>>(A);
             }
-            k_fe_gather<3><<<(c.nn * 3 + 255) / 256, 256, 0, c.stream>>>(c.nn, c.nfe_off, c.nfe_item, c.fe_elem, c.nen,
-                                                                         c.fe_blk, c.Ke_stage, c.K);
+            k_fe_gather<3><<<(unsigned)((c.nblk * 3 + 255) / 256), 256, 0, c.stream>>>(c.nblk, c.fec_off, c.fec, c.nfe_item,
+                                                                                      c.Ke_stage, c.K);
         } else {
             if (c.n_fe_a) {
                 A.list = c.fe_list_a; A.nlist = c.n_fe_a;
@@ -815,8 +822,8 @@ int dev_assemble_K(Ctx &c, std::string &err) {
                 A.list = c.fe_list_b; A.nlist = c.n_fe_b;
                 k_ke_simplex<2><<<(c.n_fe_b * 6 + 127) / 128, 128, 0, c.stream>>>(A);
             }
-            k_fe_gather<2><<<(c.nn * 2 + 255) / 256, 256, 0, c.stream>>>(c.nn, c.nfe_off, c.nfe_item, c.fe_elem, c.nen,
-                                                                         c.fe_blk, c.Ke_stage, c.K);
+            k_fe_gather<2><<<(unsigned)((c.nblk * 2 + 255) / 256), 256, 0, c.stream>>>(c.nblk, c.fec_off, c.fec, c.nfe_item,
+                                                                                      c.Ke_stage, c.K);
         }
         PDX_HIP(hipGetLastError());
     }

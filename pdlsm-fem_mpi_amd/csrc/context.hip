@@ -158,6 +158,25 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
         UP(c->fe_list_b, lb);
         UP(c->nfe_off, cnt);
         UP(c->nfe_item, item);
+        // per K block, its FE contributions (item of the block's row node, column node's local
+        // index) in element order, for the block-owned gather
+        {
+            std::vector<int> fcnt((size_t)c->nblk + 1, 0);
+            for (int k = 0; k < m.nn; k++)
+                for (int q = cnt[k]; q < cnt[k + 1]; q++) {
+                    const int t = item[q] >> 3, a = item[q] & 7;
+                    for (int b = 0; b < m.nen[m.fe[t]]; b++) fcnt[(size_t)fe_blk[(size_t)t * 64 + a * 8 + b] + 1]++;
+                }
+            for (long long q = 0; q < c->nblk; q++) fcnt[q + 1] += fcnt[q];
+            std::vector<int> fec(std::max(fcnt[c->nblk], 1)), ffill(fcnt.begin(), fcnt.end() - 1);
+            for (int k = 0; k < m.nn; k++)
+                for (int q = cnt[k]; q < cnt[k + 1]; q++) {
+                    const int t = item[q] >> 3, a = item[q] & 7;
+                    for (int b = 0; b < m.nen[m.fe[t]]; b++) fec[ffill[fe_blk[(size_t)t * 64 + a * 8 + b]]++] = (q << 3) | b;
+                }
+            UP(c->fec_off, fcnt);
+            UP(c->fec, fec);
+        }
         // all elements (lumped mass)
         std::vector<int> acnt(m.nn + 1, 0);
         for (int e = 0; e < m.ne; e++)

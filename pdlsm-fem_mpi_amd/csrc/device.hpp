@@ -58,6 +58,7 @@ struct Ctx {
     int *fp_off = nullptr, *fp_list = nullptr, *pdbe = nullptr;
     int nfe = 0;
     int *fe_elem = nullptr, *fe_blk = nullptr, *nfe_off = nullptr, *nfe_item = nullptr;
+    int *fec_off = nullptr, *fec = nullptr;  // per K block: FE contributions (item << 3 | local column node)
     int n_fe_a = 0, n_fe_b = 0;  // FE-list positions of hex8/quad4 (a) and tet4/tri3 (b) elements
     int *fe_list_a = nullptr, *fe_list_b = nullptr;
     double *Ke_stage = nullptr;  // element stiffness staging [nfe][ND*ND]
