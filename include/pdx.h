@@ -113,7 +113,7 @@ typedef struct pdx_solve_opts {
     double tol;        /* true relative residual target, e.g. 1e-8 */
     int max_iter;      /* total inner iterations */
     double inner_tol;  /* FP32 inner relative tolerance (refinement), e.g. 1e-4 */
-    int warm_start;    /* use x as the initial guess */
+    int warm_start;    /* use x as the initial guess; Newmark: 0/1 start from a_n, 2 from 2 a_n - a_{n-1} */
     int coarse_agg;    /* two-level: aggregate edge in node spacings (0: 10 in 2D, 6 in 3D) */
 } pdx_solve_opts;
 

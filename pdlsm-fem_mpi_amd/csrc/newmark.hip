@@ -203,6 +203,12 @@ int dev_mass_on_K(Ctx &c, std::string &err) {
     return PDX_OK;
 }
 
+// x = 2 a - ap (linear extrapolation of the acceleration over the last two steps)
+__global__ void k_extrapolate(int n, const double *a, const double *ap, double *x) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[t] = 2.0 * a[t] - ap[t];
+}
+
 int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double> &dofval, double *u_h, double *v_h,
                 double *a_h, pdx_solve_stats &stats, std::string &err) {
     int rc;
@@ -211,7 +217,7 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
     const Model &m = *c.host;
     const int D = c.dim;
     const size_t n = (size_t)c.nn * D, nk = (size_t)c.dd * c.nblk;
-    double *Keff, *u, *v, *a, *a1, *UN, *y, *F, *val, *dU;
+    double *Keff, *u, *v, *a, *a1, *ap, *UN, *y, *F, *val, *dU;
     float *Keff32 = nullptr;
     double *bufs[] = {nullptr};
     (void)bufs;
@@ -220,6 +226,7 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
     PDX_HIP(hipMalloc((void **)&v, n * 8));
     PDX_HIP(hipMalloc((void **)&a, n * 8));
     PDX_HIP(hipMalloc((void **)&a1, n * 8));
+    PDX_HIP(hipMalloc((void **)&ap, n * 8));  // a_{n-1}, for the extrapolated initial guess
     PDX_HIP(hipMalloc((void **)&UN, n * 8));
     PDX_HIP(hipMalloc((void **)&y, n * 8));
     PDX_HIP(hipMalloc((void **)&F, n * 8));
@@ -236,6 +243,7 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
     PDX_HIP(hipMemsetAsync(v, 0, n * 8, c.stream));
     PDX_HIP(hipMemsetAsync(a, 0, n * 8, c.stream));
     PDX_HIP(hipMemsetAsync(a1, 0, n * 8, c.stream));
+    PDX_HIP(hipMemsetAsync(ap, 0, n * 8, c.stream));
     std::vector<double> dUh(std::max<size_t>(m.ebc.size(), 1), 0.0);
     for (size_t s = 0; s < m.ebc.size(); s++) dUh[s] = c.ebc_vary[s] ? c.dt * c.ebc_vel[s] : 0.0;
     PDX_HIP(hipMemcpyAsync(dU, dUh.data(), dUh.size() * 8, hipMemcpyHostToDevice, c.stream));
@@ -254,7 +262,12 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
                                                             c.beta, UN);
         if ((rc = dev_spmv_K(c, c.K, UN, y))) { err = "spmv failed"; return rc; }
         k_newmark_rhs<<<(th + 255) / 256, 256, 0, c.stream>>>(th, y, c.f_unit, c.n_nbc, nbcv_d, c.eq, D, F);
-        PDX_HIP(hipMemcpyAsync(a1, a, n * 8, hipMemcpyDeviceToDevice, c.stream));
+        // initial guess a_{n+1} <- a_n (warm start), or 2 a_n - a_{n-1} (warm_start 2, from the
+        // second step on)
+        if (o.warm_start >= 2 && step > 0)
+            k_extrapolate<<<(th + 255) / 256, 256, 0, c.stream>>>(th, a, ap, a1);
+        else
+            PDX_HIP(hipMemcpyAsync(a1, a, n * 8, hipMemcpyDeviceToDevice, c.stream));
         pdx_solve_stats st{};
         rc = dev_solve_mat(c, Keff, Keff32, oo, F, a1, st, err);
         stats.iterations += st.iterations;
@@ -262,7 +275,8 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
         stats.true_relres = std::max(stats.true_relres, st.true_relres);
         if (rc) break;
         k_newmark_update<<<(th + 255) / 256, 256, 0, c.stream>>>(th, c.eq, D, u, v, a, a1, c.dt, c.beta, c.gamma);
-        std::swap(a, a1);
+        std::swap(ap, a);  // ap <- a_n
+        std::swap(a, a1);  // a <- a_{n+1}
         // store displacement, then advance BCs (pdsolve.cpp:6248-6253)
         PDX_HIP(hipMemcpyAsync(uh.data(), u, n * 8, hipMemcpyDeviceToHost, c.stream));
         PDX_HIP(hipStreamSynchronize(c.stream));
@@ -293,7 +307,7 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
             if (v_h) v_h[r] = vh[(size_t)k * D + i];
             if (a_h) a_h[r] = ah[(size_t)k * D + i];
         }
-    double *fr[] = {Keff, u, v, a, a1, UN, y, F, val, dU, nbcv_d};
+    double *fr[] = {Keff, u, v, a, a1, ap, UN, y, F, val, dU, nbcv_d};
     for (double *p : fr) hipFree(p);
     if (Keff32) hipFree(Keff32);
     return rc;
