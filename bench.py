@@ -50,14 +50,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def step_bytes(m, nblk, prec):
+def column_classes(m):
+    """(classes, offsets in all classes) of the node rows grouped by the offsets of their
+    block columns from the row index."""
+    off, adj = m.adjacency()
+    ids = {}
+    for r in range(m.nn):
+        key = (adj[off[r]: off[r + 1]] - r).tobytes()
+        if key not in ids:
+            ids[key] = off[r + 1] - off[r]
+    return len(ids), int(sum(ids.values()))
+
+
+def step_bytes(m, nblk, prec, col_classes=None):
     """Algorithmic HBM bytes of one explicit step in the node-block layout:
     block values (d*d per block) + one 4-byte column per block, row offsets,
     u_n gathered once (FP64), and per DOF: read u_n, u_{n-1}, m, unit traction
-    force, write u_{n+1} (FP64)."""
+    force, write u_{n+1} (FP64).  With column pattern classes (col_classes =
+    (classes, offsets)) the columns per block are replaced by one class per row
+    and the classes' offset lists."""
     d = m.dim
     sv = 4 if prec == 1 else 8
-    return nblk * (d * d * sv + 4) + (m.nn + 1) * 4 + m.nn * d * 8 + m.nn * d * (8 * 4 + 8)
+    cols = nblk * 4 if col_classes is None else m.nn * 4 + (col_classes[0] + 1) * 4 + col_classes[1] * 4
+    return nblk * d * d * sv + cols + (m.nn + 1) * 4 + m.nn * d * 8 + m.nn * d * (8 * 4 + 8)
 
 
 def survey_step_bytes(m, prec):
@@ -312,16 +327,18 @@ def main():
     drift = float(np.linalg.norm(u32 - u64) / max(np.linalg.norm(u64), 1e-300))
     nblk = model.nadj
     prec = 1
-    bytes_step = step_bytes(model, nblk, prec)
+    lanes, nt, xm, soa, rel, tune_ms = ctxs[prec].explicit_tuning(prec)
+    classes = column_classes(model)
+    bytes_step = step_bytes(model, nblk, prec, classes if rel else None)
     achieved = bytes_step / (res[prec]["step_ms"] * 1e-3)
-    lanes, nt, xm, soa, tune_ms = ctxs[prec].explicit_tuning(prec)
     tf = lambda b: "true" if b else "false"
-    kname = f"pdx::k_explicit<{model.dim}, float, {lanes}, {tf(nt)}, {tf(xm)}, {tf(soa)}>"
+    kname = f"pdx::k_explicit<{model.dim}, float, {lanes}, {tf(nt)}, {tf(xm)}, {tf(soa)}, {tf(rel)}>"
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": measured_traffic(kname) if args.n == 60 else None,
                 "kernel": kname + " (node-block SpMV + central-difference update, FP32 values)",
                 "tuner_ms_per_form": [float(t) for t in tune_ms],
                 "bytes_per_launch": bytes_step,
+                "column_classes": {"classes": classes[0], "offsets": classes[1]},
                 "scalar_csr_bytes_per_step": survey_step_bytes(model, prec)}
     out = {"metric": METRIC, "value": res[1]["dofs"], "unit": "DOF-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res[1]["step_ms"], "higher_is_better": True, "scaling": "weak",
@@ -332,7 +349,8 @@ def main():
                       "neq": model.neq, "nnz": model.nnz, "dt": dt, "parallelism": "single GPU"},
            "roofline": roofline,
            "fp64": {"value": res[0]["dofs"], "ms_per_step": res[0]["step_ms"],
-                    "roofline_frac": step_bytes(model, nblk, 0) / (res[0]["step_ms"] * 1e-3) / HBM_PEAK},
+                    "roofline_frac": step_bytes(model, nblk, 0, classes if ctxs[0].explicit_tuning(0)[4] else None)
+                    / (res[0]["step_ms"] * 1e-3) / HBM_PEAK},
            "fp32_vs_fp64_rel": drift,
            "assembly_ms": min(asm_ms)}
     if not args.no_solve:

@@ -2,6 +2,8 @@
 // device.hpp) and the extern "C" entry points of libpdx.so (include/pdx.h).
 #include <hip/hip_runtime.h>
 
+#include <map>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -201,6 +203,27 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
         c->n_long = int(rl.size());
         UP(c->rows_short, rs);
         UP(c->rows_long, rl);
+    }
+    // column pattern classes: rows whose block columns have the same offsets from the row index
+    // (the stencil of a structured mesh, jittered or not) share one list of offsets
+    {
+        std::map<std::vector<int>, int> ids;
+        std::vector<int> cls(m.nn), coff(1, 0), crel, key;
+        for (int r = 0; r < m.nn; r++) {
+            key.clear();
+            for (int b = m.adj_off[r]; b < m.adj_off[r + 1]; b++) key.push_back(m.adj[b] - r);
+            auto it = ids.find(key);
+            if (it == ids.end()) {
+                it = ids.emplace(key, int(ids.size())).first;
+                crel.insert(crel.end(), key.begin(), key.end());
+                coff.push_back(int(crel.size()));
+            }
+            cls[r] = it->second;
+        }
+        c->n_cls = int(ids.size());
+        UP(c->cls, cls);
+        UP(c->cls_off, coff);
+        UP(c->cls_rel, crel);
     }
     // natural BCs
     {
@@ -616,15 +639,16 @@ int pdx_node_stresses_timed(pdx_ctx *pc, const double *u_all, double *ms) {
 }
 
 int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal, int *xcd_order, int *row_soa,
-                        float *ms) {
+                        int *col_classes, float *ms) {
     if (!pc || (precision != PDX_FP64 && precision != PDX_FP32)) return fail(PDX_ERR_ARG, "bad argument");
     const Ctx &c = *pc->c;
     const int v = c.tune_explicit[precision];
-    const ExplicitForm f = v >= 0 ? kExplicitForms[v] : ExplicitForm{0, false, false, false};
+    const ExplicitForm f = v >= 0 ? kExplicitForms[v] : ExplicitForm{0, false, false, false, false};
     if (lanes) *lanes = f.lanes;
     if (nontemporal) *nontemporal = f.nt;
     if (xcd_order) *xcd_order = f.xm;
     if (row_soa) *row_soa = f.soa;
+    if (col_classes) *col_classes = f.rel;
     if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + PDX_EXPLICIT_FORMS, ms);
     return PDX_OK;
 }

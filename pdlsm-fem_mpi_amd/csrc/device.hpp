@@ -28,16 +28,18 @@ struct DevBuf {
 };
 
 // explicit-step kernel forms searched by the tuner: lanes per short row, non-temporal K loads,
-// XCD-contiguous block order, row-local value order of K
+// XCD-contiguous block order, row-local value order of K, column pattern classes
 struct ExplicitForm {
     int lanes;
-    bool nt, xm, soa;
+    bool nt, xm, soa, rel;
 };
 constexpr ExplicitForm kExplicitForms[PDX_EXPLICIT_FORMS] = {
-    {8, false, false, false}, {4, false, false, false}, {16, false, false, false}, {8, true, false, false},
-    {4, true, false, false},  {16, true, false, false}, {32, false, false, false}, {32, true, false, false},
-    {16, false, true, false}, {8, false, true, false},  {8, false, true, true},    {16, false, true, true},
-    {4, false, true, true},   {32, false, true, true}};
+    {8, false, false, false, false},  {4, false, false, false, false}, {16, false, false, false, false},
+    {8, true, false, false, false},   {4, true, false, false, false},  {16, true, false, false, false},
+    {32, false, false, false, false}, {32, true, false, false, false}, {16, false, true, false, false},
+    {8, false, true, false, false},   {8, false, true, true, false},   {16, false, true, true, false},
+    {4, false, true, true, false},    {32, false, true, true, false},  {8, false, true, false, true},
+    {16, false, true, false, true},   {8, false, true, true, true},    {16, false, true, true, true}};
 
 struct Ctx {
     int device = 0;
@@ -89,6 +91,9 @@ struct Ctx {
     // K in row-local value order for the explicit step: value e of block b of the row whose blocks
     // are [b0, b0 + nb) at b0*dd + e*nb + (b - b0), so that lanes on consecutive blocks read
     // consecutive words; built on first use for the K of generation ks_gen
+    // column pattern classes of the node rows: row r's block columns are r + cls_rel[cls_off[cls[r]] + k]
+    int *cls = nullptr, *cls_off = nullptr, *cls_rel = nullptr;
+    int n_cls = 0;
     double *K64s = nullptr;
     float *K32s = nullptr;
     long long ks_gen[2] = {-1, -1};

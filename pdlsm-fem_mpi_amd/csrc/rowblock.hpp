@@ -15,9 +15,12 @@ namespace pdx {
 // from the cancellation of large displacement terms).
 // SOA: K is in row-local value order (value e of block b of a row whose blocks are [b0, b0 + nb)
 // at b0*DD + e*nb + (b - b0)): lanes on consecutive blocks then read consecutive words.
-template <int D, typename T, typename X, int G, bool DIFF, bool NT = false, bool SOA = false>
+// REL: the block columns are read as r + rel[k] from the row's column pattern class (rel points at
+// the class's offsets) instead of from adj.
+template <int D, typename T, typename X, int G, bool DIFF, bool NT = false, bool SOA = false, bool REL = false>
 __device__ inline void row_matvec(int r, int lane, const int *__restrict__ adj_off, const int *__restrict__ adj,
-                                  const T *__restrict__ K, const X *__restrict__ x, T acc[D]) {
+                                  const T *__restrict__ K, const X *__restrict__ x, T acc[D],
+                                  const int *__restrict__ rel = nullptr) {
     constexpr int DD = D * D;
 #pragma unroll
     for (int i = 0; i < D; i++) acc[i] = T(0);
@@ -28,7 +31,7 @@ __device__ inline void row_matvec(int r, int lane, const int *__restrict__ adj_o
     }
     const int b0 = adj_off[r], b1 = adj_off[r + 1], nb = b1 - b0;
     for (int b = b0 + lane; b < b1; b += G) {
-        const int c = adj[b];
+        const int c = REL ? r + rel[b - b0] : adj[b];
         T kb[DD];
 #pragma unroll
         for (int e = 0; e < DD; e++) {

@@ -29,9 +29,14 @@ namespace pdx {
 // round-robin; with XM the launch is padded to a multiple of 8 and block b works
 // on logical block (b % 8) * per + b / 8, so each XCD (and its L2) owns one
 // contiguous eighth of the rows and the gathers of u_n stay local to it.
-template <int D, typename T, int GS, bool NT, bool XM, bool SOA>
+// column pattern classes (REL forms): class of each row, offsets of each class's relative columns
+struct ColClasses {
+    const int *cls, *cls_off, *cls_rel;
+};
+
+template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL>
 __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int nbs, const int *rows_l, int nl,
-                                                  int nbt, const int *adj_off, const int *adj, const T *K,
+                                                  int nbt, const int *adj_off, const int *adj, ColClasses cc, const T *K,
                                                   const double *Un, const double *Unm1, double *Unp1,
                                                   const double *m, const double *f_unit, int n_nbc, int nn,
                                                   const double *nbc_cur, double *nbc_next, const double *nbc_inc,
@@ -76,8 +81,9 @@ __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int
             inc = ebc_inc[presc_set[code]];
         }
     }
-    if (grp == GS) row_matvec<D, T, double, GS, DIFF, NT, SOA>(r, lane, adj_off, adj, K, Un, acc);
-    else row_matvec<D, T, double, 64, DIFF, NT, SOA>(r, lane, adj_off, adj, K, Un, acc);
+    const int *rel = REL ? cc.cls_rel + cc.cls_off[cc.cls[r]] : nullptr;
+    if (grp == GS) row_matvec<D, T, double, GS, DIFF, NT, SOA, REL>(r, lane, adj_off, adj, K, Un, acc, rel);
+    else row_matvec<D, T, double, 64, DIFF, NT, SOA, REL>(r, lane, adj_off, adj, K, Un, acc, rel);
     if (act) {
         const T v = pick<D, T>(acc, lane);
         if (fr) {
@@ -156,14 +162,15 @@ static int ensure_Ks(Ctx &c, int prec, std::string &err) {
     return PDX_OK;
 }
 
-template <int D, typename T, int GS, bool NT, bool XM, bool SOA>
+template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL>
 static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double *Unm1, double *Unp1,
                               const double *nbc_cur, double *nbc_next, int startup) {
     const int nbs = (c.n_short * GS + 255) / 256, nbl = (c.n_long * 64 + 255) / 256;
     const int nbt = std::max(nbs + nbl, 1);
     const int grid = XM ? (nbt + 7) / 8 * 8 : nbt;
-    k_explicit<D, T, GS, NT, XM, SOA><<<grid, 256, 0, c.stream>>>(
-        c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, nbt, c.adj_off, c.adj, K, Un, Unm1, Unp1, c.mlump,
+    const ColClasses cc{c.cls, c.cls_off, c.cls_rel};
+    k_explicit<D, T, GS, NT, XM, SOA, REL><<<grid, 256, 0, c.stream>>>(
+        c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, nbt, c.adj_off, c.adj, cc, K, Un, Unm1, Unp1, c.mlump,
         c.f_unit, c.n_nbc, c.nn, nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup);
 }
 
@@ -171,29 +178,33 @@ static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double
 template <int D, typename T>
 static void launch_explicit(Ctx &c, int variant, const T *K, const double *Un, const double *Unm1, double *Unp1,
                             const double *nbc_cur, double *nbc_next, int startup) {
-#define PDX_EXPLICIT_FORM(V, GS, NT, XM, SOA)                                                                   \
+#define PDX_EXPLICIT_FORM(V, GS, NT, XM, SOA, REL)                                                              \
     case V:                                                                                                     \
         static_assert(kExplicitForms[V].lanes == GS && kExplicitForms[V].nt == NT && kExplicitForms[V].xm == XM && \
-                          kExplicitForms[V].soa == SOA,                                                         \
+                          kExplicitForms[V].soa == SOA && kExplicitForms[V].rel == REL,                         \
                       "form table");                                                                            \
-        launch_explicit_v<D, T, GS, NT, XM, SOA>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);                 \
+        launch_explicit_v<D, T, GS, NT, XM, SOA, REL>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);            \
         break;
     switch (variant) {
-        PDX_EXPLICIT_FORM(1, 4, false, false, false)
-        PDX_EXPLICIT_FORM(2, 16, false, false, false)
-        PDX_EXPLICIT_FORM(3, 8, true, false, false)
-        PDX_EXPLICIT_FORM(4, 4, true, false, false)
-        PDX_EXPLICIT_FORM(5, 16, true, false, false)
-        PDX_EXPLICIT_FORM(6, 32, false, false, false)
-        PDX_EXPLICIT_FORM(7, 32, true, false, false)
-        PDX_EXPLICIT_FORM(8, 16, false, true, false)
-        PDX_EXPLICIT_FORM(9, 8, false, true, false)
-        PDX_EXPLICIT_FORM(10, 8, false, true, true)
-        PDX_EXPLICIT_FORM(11, 16, false, true, true)
-        PDX_EXPLICIT_FORM(12, 4, false, true, true)
-        PDX_EXPLICIT_FORM(13, 32, false, true, true)
+        PDX_EXPLICIT_FORM(1, 4, false, false, false, false)
+        PDX_EXPLICIT_FORM(2, 16, false, false, false, false)
+        PDX_EXPLICIT_FORM(3, 8, true, false, false, false)
+        PDX_EXPLICIT_FORM(4, 4, true, false, false, false)
+        PDX_EXPLICIT_FORM(5, 16, true, false, false, false)
+        PDX_EXPLICIT_FORM(6, 32, false, false, false, false)
+        PDX_EXPLICIT_FORM(7, 32, true, false, false, false)
+        PDX_EXPLICIT_FORM(8, 16, false, true, false, false)
+        PDX_EXPLICIT_FORM(9, 8, false, true, false, false)
+        PDX_EXPLICIT_FORM(10, 8, false, true, true, false)
+        PDX_EXPLICIT_FORM(11, 16, false, true, true, false)
+        PDX_EXPLICIT_FORM(12, 4, false, true, true, false)
+        PDX_EXPLICIT_FORM(13, 32, false, true, true, false)
+        PDX_EXPLICIT_FORM(14, 8, false, true, false, true)
+        PDX_EXPLICIT_FORM(15, 16, false, true, false, true)
+        PDX_EXPLICIT_FORM(16, 8, false, true, true, true)
+        PDX_EXPLICIT_FORM(17, 16, false, true, true, true)
     default:
-        PDX_EXPLICIT_FORM(0, 8, false, false, false)
+        PDX_EXPLICIT_FORM(0, 8, false, false, false, false)
     }
 #undef PDX_EXPLICIT_FORM
 }

@@ -130,7 +130,7 @@ def test_explicit_graph_replay_matches_eager():
 @pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-5)])
 def test_explicit_forms_agree(prec, tol):
     """Explicit-step kernel forms with the same lanes per row (non-temporal
-    loads, XCD order, row-local K order) give the same trajectory bit for bit,
+    loads, XCD order, row-local K order, column pattern classes) give the same trajectory bit for bit,
     eager and in graph replay; the lanes per row change only the order of the
     row sum (rounding), checked to tol.  2D and 3D explicit cases."""
     for mesh in (_explicit_case(), W.box3d(n=12, pd_block=4)):
