@@ -361,11 +361,13 @@ def main():
         cfgs = (("plain_fp64_jacobi_bicgstab", pdx.solve_opts(precision=0, precond=1, tol=1e-8, max_iter=100000)),
                 ("fp32_ir_blockjacobi_bicgstab", pdx.solve_opts(precision=1, precond=2, tol=1e-8, max_iter=100000)),
                 ("fp64_two_level_bicgstab", pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
-                ("fp32_ir_two_level_bicgstab", pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)))
+                ("fp32_ir_two_level_bicgstab", pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
+                ("fp64_ilu0_bicgstab", pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_ILU0, tol=1e-8)),
+                ("fp32_ir_ilu0_bicgstab", pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_ILU0, tol=1e-8)))
         for name, o in cfgs:
             try:
-                # first call builds the coarse space of the two-level kinds; the
-                # second reuses it (repeated solves with one operator)
+                # first call builds the coarse space of the two-level kinds (the
+                # factor of ILU(0)); the second reuses it (repeated solves with one operator)
                 u, st0 = scx.static_solve(o)
                 u, st = scx.static_solve(o)
                 solves[name] = {"seconds": st["seconds"], "seconds_first_call": st0["seconds"],

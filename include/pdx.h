@@ -68,6 +68,7 @@ extern "C" {
 #define PDX_PREC_BLOCK_JACOBI 2 /* node d x d blocks */
 #define PDX_PREC_LEARNED 3      /* two-level, node-block smoother predicted by the graph network */
 #define PDX_PREC_TWO_LEVEL 4    /* two-level hybrid: rigid-body aggregate coarse space + node-block Jacobi */
+#define PDX_PREC_ILU0 5         /* node-block ILU(0) in multicolour order (BiCGStab only) */
 
 typedef struct pdx_model pdx_model; /* host-side data model (families, pattern) */
 typedef struct pdx_ctx pdx_ctx;     /* device-resident state on one GPU         */
@@ -115,6 +116,11 @@ typedef struct pdx_solve_opts {
     double inner_tol;  /* FP32 inner relative tolerance (refinement), e.g. 1e-4 */
     int warm_start;    /* use x as the initial guess; Newmark: 0/1 start from a_n, 2 from 2 a_n - a_{n-1} */
     int coarse_agg;    /* two-level: aggregate edge in node spacings (0: 10 in 2D, 6 in 3D) */
+    int fallback;      /* nonzero: when the solve stops short of tol (PDX_ERR_NOCONV), a preconditioner is
+                          singular, or PDX_PREC_LEARNED has no smoother for this operator, retry with
+                          BiCGStab down the chain learned / two-level -> ILU(0) -> node-block Jacobi
+                          (any other preconditioner -> node-block Jacobi) -> FP64 node-block Jacobi,
+                          from the iterate the failed attempt left when it beat zero */
 } pdx_solve_opts;
 
 typedef struct pdx_solve_stats {
@@ -123,6 +129,9 @@ typedef struct pdx_solve_stats {
     double true_relres; /* ||b - A x|| / ||b|| in FP64 */
     double seconds;
     int converged;
+    int precond_used;   /* PDX_PREC_* of the attempt that ended the solve */
+    int precision_used; /* PDX_FP64 / PDX_FP32 of that attempt */
+    int attempts;       /* 1 + fallback retries */
 } pdx_solve_stats;
 
 const char *pdx_last_error(void);
@@ -190,6 +199,15 @@ int pdx_max_eig(pdx_ctx *c, int iters, double *lmax);
 int pdx_set_dt(pdx_ctx *c, double dt);
 
 int pdx_static_solve(pdx_ctx *c, const pdx_solve_opts *o, double *u, pdx_solve_stats *st);
+
+/* The block ILU(0) factor behind PDX_PREC_ILU0, for the context's assembled K (rows and
+ * columns of prescribed DOFs removed, 1 on their diagonal).  No counterpart in the reference,
+ * whose solves are MKL PARDISO (calMatrixOperations.cpp:378-498); IKJ ILU(0) over the d x d node
+ * blocks, nodes eliminated by greedy colour (node order within a colour).  Each output may be
+ * NULL: colors [n_node], F [blocks*d*d] in pdx_model_adjacency's block order (strict lower
+ * blocks hold L, the rest U), Dinv [n_node*d*d] the inverses of U's diagonal blocks, and with r
+ * [n_node*d] z = (L U)^-1 r. */
+int pdx_ilu0(pdx_ctx *c, const double *r, double *z, int *colors, double *F, double *Dinv, int *ncolors);
 
 /* Quasi-static load steps with bond failure: pdfemQuasiStaticSolver_CSRformat
  * (pdsolve.cpp:6464-6580).  Per step: K_ff du = F_ext - K u at the current DOF values (the

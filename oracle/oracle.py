@@ -88,6 +88,12 @@ def lib():
         L.om_local_damage.argtypes = [P, P]
         L.om_fc2_step.restype = I
         L.om_fc2_step.argtypes = [P, P, D, I, P]
+        L.om_greedy_colors.restype = I
+        L.om_greedy_colors.argtypes = [I, P, P, P]
+        L.om_bilu0.restype = I
+        L.om_bilu0.argtypes = [I, I, P, P, P, P, P, P]
+        L.om_bilu0_apply.restype = None
+        L.om_bilu0_apply.argtypes = [I, I, P, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -546,3 +552,71 @@ def fb_family(dim, x, vol, delta, nlf, lam, mu, mem):
     H = np.zeros(len(mem) * dim * dim)
     _chk(lib().om_fb_family(dim, _p(x), _p(vol), _p(delta), nlf, lam, mu, len(mem), _p(mem), _p(A), _p(H)))
     return A.reshape(p, p), H.reshape(len(mem), dim * dim)
+
+
+# ---- block ILU(0) (textbook restatement; the reference solves with PARDISO) ----
+def greedy_colors(off, adj):
+    """Greedy colouring of the node graph in node order (om_greedy_colors)."""
+    off = np.ascontiguousarray(off, np.int32)
+    adj = np.ascontiguousarray(adj, np.int32)
+    col = np.zeros(len(off) - 1, np.int32)
+    nc = lib().om_greedy_colors(len(off) - 1, _p(off), _p(adj), _p(col))
+    return col, nc
+
+
+def colour_rank(col):
+    """Elimination rank: by colour, node order within a colour."""
+    order = np.lexsort((np.arange(len(col)), col))
+    rank = np.zeros(len(col), np.int32)
+    rank[order] = np.arange(len(col), dtype=np.int32)
+    return rank
+
+
+def bilu0(off, adj, Kb, rank, d):
+    """Block ILU(0) (om_bilu0): F [blocks, d, d] (strict lower L, rest U), Dinv [nn, d, d]."""
+    off = np.ascontiguousarray(off, np.int32)
+    adj = np.ascontiguousarray(adj, np.int32)
+    Kb = np.ascontiguousarray(Kb, np.float64).reshape(-1)
+    rank = np.ascontiguousarray(rank, np.int32)
+    nn = len(off) - 1
+    F = np.zeros_like(Kb)
+    Dinv = np.zeros(nn * d * d)
+    _chk(lib().om_bilu0(nn, d, _p(off), _p(adj), _p(Kb), _p(rank), _p(F), _p(Dinv)))
+    return F.reshape(-1, d, d), Dinv.reshape(nn, d, d)
+
+
+def bilu0_apply(off, adj, F, Dinv, rank, r, d):
+    """z = (L U)^-1 r with the factors of bilu0 (om_bilu0_apply)."""
+    off = np.ascontiguousarray(off, np.int32)
+    adj = np.ascontiguousarray(adj, np.int32)
+    nn = len(off) - 1
+    z = np.zeros(nn * d)
+    lib().om_bilu0_apply(nn, d, _p(off), _p(adj), _p(np.ascontiguousarray(F, np.float64).reshape(-1)),
+                         _p(np.ascontiguousarray(Dinv, np.float64).reshape(-1)),
+                         _p(np.ascontiguousarray(rank, np.int32)), _p(np.ascontiguousarray(r, np.float64)), _p(z))
+    return z
+
+
+def masked_blocks(off, adj, eq, d, ia, ja, K):
+    """Node-block form [blocks, d, d] of a free-DOF CSR matrix (ia, ja, K) on the node adjacency
+    (off, adj); rows and columns of prescribed DOFs (eq < 0) zero with 1 on their diagonal."""
+    off = np.asarray(off, np.int64)
+    adj = np.asarray(adj, np.int64)
+    eq = np.asarray(eq)[:, :d]
+    nn = len(off) - 1
+    node = np.full(int(eq.max()) + 1, -1, np.int64)
+    comp = np.full(int(eq.max()) + 1, -1, np.int64)
+    a_idx, c_idx = np.nonzero(eq >= 0)
+    node[eq[a_idx, c_idx]] = a_idx
+    comp[eq[a_idx, c_idx]] = c_idx
+    rows = np.repeat(np.arange(len(ia) - 1), np.diff(ia))
+    keys = np.repeat(np.arange(nn, dtype=np.int64), np.diff(off)) * nn + adj
+    ra, cb = node[rows], node[np.asarray(ja)]
+    slot = np.searchsorted(keys, ra * nn + cb)
+    assert np.array_equal(keys[slot], ra * nn + cb)
+    Kb = np.zeros((len(adj), d, d))
+    np.add.at(Kb, (slot, comp[rows], comp[np.asarray(ja)]), np.asarray(K))
+    diag = np.searchsorted(keys, np.arange(nn, dtype=np.int64) * (nn + 1))
+    pa, pc = np.nonzero(eq < 0)
+    Kb[diag[pa], pc, pc] = 1.0
+    return Kb

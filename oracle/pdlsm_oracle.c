@@ -2109,3 +2109,185 @@ void om_local_damage(const om_model *M, double *phi) {
         phi[M->fam_mem[M->fam_off[f]] - 1] = 1.0 - t / (n - 1.0);
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Block ILU(0) preconditioner.  The reference has no counterpart (it calls   */
+/* MKL PARDISO, calMatrixOperations.cpp:378-498); these are textbook          */
+/* restatements that pin the library's ILU(0) path:                            */
+/*  - greedy colouring of the node graph in node order;                        */
+/*  - ILU(0), IKJ form (Saad, Iterative Methods for Sparse Linear Systems,     */
+/*    2nd ed., Alg. 10.4) over d x d node blocks in a given elimination order; */
+/*  - the forward / backward solves with its factors.                          */
+/* The node-block matrix uses the node adjacency (off, adj: sorted columns,    */
+/* self included), values K[slot*d*d + i*d + j] row-major per block.           */
+/* ------------------------------------------------------------------------ */
+
+/* greedy colouring of the symmetrised node graph: node a takes the smallest
+   colour no lower-numbered neighbour holds (b < a with b in row a or a in
+   row b); returns the number of colours */
+int om_greedy_colors(int nn, const int *off, const int *adj, int *color) {
+    int *toff = (int *)xcalloc((size_t)nn + 1, sizeof(int));
+    int *tadj = (int *)xcalloc((size_t)off[nn] + 1, sizeof(int));
+    for (int a = 0; a < nn; a++)
+        for (int s = off[a]; s < off[a + 1]; s++) toff[adj[s] + 1]++;
+    for (int a = 0; a < nn; a++) toff[a + 1] += toff[a];
+    int *pos = (int *)xcalloc((size_t)nn + 1, sizeof(int));
+    memcpy(pos, toff, sizeof(int) * (size_t)nn);
+    for (int a = 0; a < nn; a++)
+        for (int s = off[a]; s < off[a + 1]; s++) tadj[pos[adj[s]]++] = a;
+    free(pos);
+    int nc = 0;
+    int *mark = (int *)xcalloc((size_t)nn + 1, sizeof(int));
+    for (int a = 0; a < nn; a++) {
+        for (int s = off[a]; s < off[a + 1]; s++)
+            if (adj[s] < a) mark[color[adj[s]]] = a + 1;
+        for (int s = toff[a]; s < toff[a + 1]; s++)
+            if (tadj[s] < a) mark[color[tadj[s]]] = a + 1;
+        int c = 0;
+        while (mark[c] == a + 1) c++;
+        color[a] = c;
+        if (c + 1 > nc) nc = c + 1;
+    }
+    free(mark);
+    free(toff);
+    free(tadj);
+    return nc;
+}
+
+/* inverse of a d x d block by cofactors (d <= 3); returns 0 when singular */
+static int block_inverse(int d, const double *A, double *X) {
+    if (d == 1) {
+        if (A[0] == 0.0) return 0;
+        X[0] = 1.0 / A[0];
+        return 1;
+    }
+    if (d == 2) {
+        double det = A[0] * A[3] - A[1] * A[2];
+        if (det == 0.0 || !isfinite(det)) return 0;
+        X[0] = A[3] / det; X[1] = -A[1] / det; X[2] = -A[2] / det; X[3] = A[0] / det;
+        return 1;
+    }
+    double c00 = A[4] * A[8] - A[5] * A[7], c01 = A[5] * A[6] - A[3] * A[8], c02 = A[3] * A[7] - A[4] * A[6];
+    double det = A[0] * c00 + A[1] * c01 + A[2] * c02;
+    if (det == 0.0 || !isfinite(det)) return 0;
+    X[0] = c00 / det;
+    X[1] = (A[2] * A[7] - A[1] * A[8]) / det;
+    X[2] = (A[1] * A[5] - A[2] * A[4]) / det;
+    X[3] = c01 / det;
+    X[4] = (A[0] * A[8] - A[2] * A[6]) / det;
+    X[5] = (A[2] * A[3] - A[0] * A[5]) / det;
+    X[6] = c02 / det;
+    X[7] = (A[1] * A[6] - A[0] * A[7]) / det;
+    X[8] = (A[0] * A[4] - A[1] * A[3]) / det;
+    return 1;
+}
+
+typedef struct { int rank, slot; } om_rs;
+static int rs_cmp(const void *a, const void *b) {
+    const om_rs *x = (const om_rs *)a, *y = (const om_rs *)b;
+    return (x->rank > y->rank) - (x->rank < y->rank);
+}
+
+/* Block ILU(0) in the elimination order rank[] (a permutation of 0..nn-1).
+   F (same layout as K) receives L's strict lower blocks (unit block diagonal
+   implied) and U's diagonal and upper blocks; Dinv[a*d*d] the inverses of
+   U's diagonal blocks.  For node i in increasing rank, for each lower
+   neighbour k in increasing rank: F_ik <- F_ik * Dinv_k, then F_ij -= F_ik * F_kj
+   for every j of row i with rank[j] > rank[k] and (k, j) in the pattern. */
+int om_bilu0(int nn, int d, const int *off, const int *adj, const double *K, const int *rank, double *F,
+             double *Dinv) {
+    const int dd = d * d;
+    memcpy(F, K, sizeof(double) * (size_t)off[nn] * dd);
+    int *node_of = (int *)xcalloc((size_t)nn, sizeof(int));
+    for (int a = 0; a < nn; a++) node_of[rank[a]] = a;
+    int maxrow = 0;
+    for (int a = 0; a < nn; a++)
+        if (off[a + 1] - off[a] > maxrow) maxrow = off[a + 1] - off[a];
+    om_rs *lo = (om_rs *)xcalloc((size_t)maxrow + 1, sizeof(om_rs));
+    int rc = OM_OK;
+    for (int t = 0; t < nn && rc == OM_OK; t++) {
+        int i = node_of[t], nl = 0, di = -1;
+        for (int s = off[i]; s < off[i + 1]; s++) {
+            if (adj[s] == i) di = s;
+            else if (rank[adj[s]] < t) { lo[nl].rank = rank[adj[s]]; lo[nl].slot = s; nl++; }
+        }
+        if (di < 0) { rc = OM_ERR_PATTERN; break; }
+        qsort(lo, (size_t)nl, sizeof(om_rs), rs_cmp);
+        for (int q = 0; q < nl; q++) {
+            int sk = lo[q].slot, k = adj[sk], rk = lo[q].rank;
+            double X[9], T[9];
+            const double *Dk = &Dinv[(size_t)k * dd];
+            double *Fik = &F[(size_t)sk * dd];
+            for (int r = 0; r < d; r++)
+                for (int c = 0; c < d; c++) {
+                    double acc = 0.0;
+                    for (int e = 0; e < d; e++) acc += Fik[r * d + e] * Dk[e * d + c];
+                    X[r * d + c] = acc;
+                }
+            memcpy(Fik, X, sizeof(double) * dd);
+            /* merge row i and row k (both sorted by node) */
+            int s = off[i], u = off[k];
+            while (s < off[i + 1] && u < off[k + 1]) {
+                if (adj[s] < adj[u]) { s++; continue; }
+                if (adj[s] > adj[u]) { u++; continue; }
+                if (rank[adj[s]] > rk) {
+                    const double *Fkj = &F[(size_t)u * dd];
+                    double *Fij = &F[(size_t)s * dd];
+                    for (int r = 0; r < d; r++)
+                        for (int c = 0; c < d; c++) {
+                            double acc = 0.0;
+                            for (int e = 0; e < d; e++) acc += X[r * d + e] * Fkj[e * d + c];
+                            T[r * d + c] = acc;
+                        }
+                    for (int e = 0; e < dd; e++) Fij[e] -= T[e];
+                }
+                s++;
+                u++;
+            }
+        }
+        if (!block_inverse(d, &F[(size_t)di * dd], &Dinv[(size_t)i * dd])) rc = OM_ERR_SINGULAR;
+    }
+    free(lo);
+    free(node_of);
+    return rc;
+}
+
+/* z = (L U)^-1 r with the factors of om_bilu0: forward solve in increasing
+   rank (y_i = r_i - sum_{rank k < rank i} F_ik y_k), backward in decreasing
+   rank (z_i = Dinv_i (y_i - sum_{rank j > rank i} F_ij z_j)); row sums in slot
+   order.  r, z: [nn*d]; z may not alias r. */
+void om_bilu0_apply(int nn, int d, const int *off, const int *adj, const double *F, const double *Dinv,
+                    const int *rank, const double *r, double *z) {
+    const int dd = d * d;
+    int *node_of = (int *)xcalloc((size_t)nn, sizeof(int));
+    for (int a = 0; a < nn; a++) node_of[rank[a]] = a;
+    for (int t = 0; t < nn; t++) {
+        int i = node_of[t];
+        double acc[3];
+        for (int r0 = 0; r0 < d; r0++) acc[r0] = r[(size_t)i * d + r0];
+        for (int s = off[i]; s < off[i + 1]; s++) {
+            int k = adj[s];
+            if (k == i || rank[k] > t) continue;
+            for (int r0 = 0; r0 < d; r0++)
+                for (int c = 0; c < d; c++) acc[r0] -= F[(size_t)s * dd + r0 * d + c] * z[(size_t)k * d + c];
+        }
+        for (int r0 = 0; r0 < d; r0++) z[(size_t)i * d + r0] = acc[r0];
+    }
+    for (int t = nn - 1; t >= 0; t--) {
+        int i = node_of[t];
+        double acc[3];
+        for (int r0 = 0; r0 < d; r0++) acc[r0] = z[(size_t)i * d + r0];
+        for (int s = off[i]; s < off[i + 1]; s++) {
+            int j = adj[s];
+            if (j == i || rank[j] < t) continue;
+            for (int r0 = 0; r0 < d; r0++)
+                for (int c = 0; c < d; c++) acc[r0] -= F[(size_t)s * dd + r0 * d + c] * z[(size_t)j * d + c];
+        }
+        for (int r0 = 0; r0 < d; r0++) {
+            double v = 0.0;
+            for (int c = 0; c < d; c++) v += Dinv[(size_t)i * dd + r0 * d + c] * acc[c];
+            z[(size_t)i * d + r0] = v;
+        }
+    }
+    free(node_of);
+}

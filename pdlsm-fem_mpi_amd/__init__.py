@@ -26,6 +26,7 @@ PDX_MODE_EXPLICIT, PDX_MODE_STATIC, PDX_MODE_QUASI, PDX_MODE_NEWMARK = 0, 1, 2, 
 PDX_FP64, PDX_FP32 = 0, 1
 PDX_KRYLOV_BICGSTAB, PDX_KRYLOV_CG = 0, 1
 PDX_PREC_NONE, PDX_PREC_JACOBI, PDX_PREC_BLOCK_JACOBI, PDX_PREC_LEARNED, PDX_PREC_TWO_LEVEL = 0, 1, 2, 3, 4
+PDX_PREC_ILU0 = 5
 FB_VARIANTS = 7  # family-batch launch forms (PDX_FB_VARIANTS)
 EXPLICIT_FORMS = 18  # explicit-step kernel forms (PDX_EXPLICIT_FORMS)
 
@@ -44,12 +45,12 @@ class MeshDesc(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("krylov", _I), ("precond", _I), ("precision", _I), ("tol", _D), ("max_iter", _I),
-                ("inner_tol", _D), ("warm_start", _I), ("coarse_agg", _I)]
+                ("inner_tol", _D), ("warm_start", _I), ("coarse_agg", _I), ("fallback", _I)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", _I), ("refinements", _I), ("true_relres", _D), ("seconds", _D),
-                ("converged", _I)]
+                ("converged", _I), ("precond_used", _I), ("precision_used", _I), ("attempts", _I)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -86,6 +87,7 @@ EXPORTS = {
     "pdx_max_eig": (_I, [_P, _I, _P]),
     "pdx_set_dt": (_I, [_P, _D]),
     "pdx_static_solve": (_I, [_P, _P, _P, _P]),
+    "pdx_ilu0": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "pdx_newmark_run": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "pdx_solve_csr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
     "pdx_last_timing": (_I, [_P, _P, _P]),
@@ -269,10 +271,10 @@ class Model:
 
 
 def solve_opts(krylov=PDX_KRYLOV_BICGSTAB, precond=PDX_PREC_JACOBI, precision=PDX_FP64, tol=1e-8, max_iter=20000,
-               inner_tol=1e-4, warm_start=False, coarse_agg=0):
+               inner_tol=1e-4, warm_start=False, coarse_agg=0, fallback=False):
     o = SolveOpts()
     o.krylov, o.precond, o.precision, o.tol, o.max_iter = krylov, precond, precision, tol, max_iter
-    o.inner_tol, o.warm_start, o.coarse_agg = inner_tol, int(warm_start), int(coarse_agg)
+    o.inner_tol, o.warm_start, o.coarse_agg, o.fallback = inner_tol, int(warm_start), int(coarse_agg), int(fallback)
     return o
 
 
@@ -372,11 +374,34 @@ class Context:
     def set_dt(self, dt):
         _chk(lib().pdx_set_dt(self.h, float(dt)))
 
-    def static_solve(self, opts=None, u0=None):
+    def ilu0(self, r=None):
+        """Block ILU(0) factor of the assembled K (pdx_ilu0): dict with colors [nn], F [blocks, d, d]
+        (strict lower blocks L, the rest U, in adjacency order), Dinv [nn, d, d], ncolors, and z =
+        (L U)^-1 r when r [nn*d] is given."""
+        m = self.model
+        d = m.dim
+        nblk = m.nadj
+        col = np.zeros(max(m.nn, 1), np.int32)
+        F = np.zeros(max(nblk, 1) * d * d)
+        Dinv = np.zeros(max(m.nn, 1) * d * d)
+        nc = C.c_int(0)
+        rr = None if r is None else np.ascontiguousarray(r, np.float64)
+        z = None if r is None else np.zeros(m.nn * d)
+        _chk(lib().pdx_ilu0(self.h, _p(rr), _p(z), _p(col), _p(F), _p(Dinv), C.byref(nc)))
+        out = dict(colors=col[: m.nn], F=F[: nblk * d * d].reshape(nblk, d, d), Dinv=Dinv[: m.nn * d * d].reshape(m.nn, d, d),
+                   ncolors=nc.value)
+        if z is not None:
+            out["z"] = z
+        return out
+
+    def static_solve(self, opts=None, u0=None, check=True):
+        """u, stats; with check=False also the status code instead of raising on failure."""
         opts = opts or solve_opts()
         u = np.zeros(max(self.model.neq, 1)) if u0 is None else np.array(u0, np.float64)
         st = SolveStats()
         rc = lib().pdx_static_solve(self.h, C.byref(opts), _p(u), C.byref(st))
+        if not check:
+            return u[: self.model.neq], st.as_dict(), rc
         _chk(rc)
         return u[: self.model.neq], st.as_dict()
 

@@ -82,6 +82,7 @@ static int csr_solve(int device, long long n, const long long *ia, const long lo
     hipFree(db);
     hipFree(dx);
     if (K32) hipFree(K32);
+    free_ilu0(c);
     if (c.red) hipFree(c.red);
     c.red = nullptr;
     for (void *p : c.owned) hipFree(p);

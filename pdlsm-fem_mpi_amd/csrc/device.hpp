@@ -119,6 +119,7 @@ struct Ctx {
     double *red_host = nullptr; // pinned mirror
     double last_total_ms = 0, last_step_ms = 0;
     void *two_level = nullptr;       // coarse space of the last two-level solve (krylov.hip)
+    void *ilu0 = nullptr;            // block ILU(0) factor of the last ILU(0) solve (ilu0.hip)
     double *learned_Minv = nullptr;  // [nn*dd] node-block smoother predicted by the graph network
     long long learned_gen = -1;      // K_gen the smoother was predicted for
     void *gnn_work = nullptr;        // graph-network inputs and activations (learned.hip)
@@ -145,6 +146,13 @@ int dev_fill(Ctx &c, double *p, double v, size_t n);
 
 int dev_spmv_K(Ctx &c, const double *K64, const double *x, double *y);
 void free_two_level(Ctx &c);
+// block ILU(0) preconditioner (ilu0.hip): factor of K64 cached per generation, applications on c.stream
+int ilu0_prepare(Ctx &c, const double *K64, bool f32, void *&out, std::string &err);
+void ilu0_apply(Ctx &c, const void *h, const double *r, double *z);
+void ilu0_apply(Ctx &c, const void *h, const float *r, float *z);
+int ilu0_colors(const void *h);
+double ilu0_setup_ms(const void *h);
+void free_ilu0(Ctx &c);
 void free_gnn_work(Ctx &c);
 int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err);
 int dev_bond_sigma1(Ctx &c, const int *bond_dev, const double *sigma_dev, double *s1_dev, std::string &err);

@@ -1,0 +1,501 @@
+// Block ILU(0) preconditioner on the node-block operator, in multicolour order.
+//
+// The reference solves its systems with MKL PARDISO (calMatrixOperations.cpp:
+// 378-498) and has no preconditioner; this is the incomplete-factorisation
+// fallback of the Krylov solves.  K is unsymmetric (mtype 11), so the
+// factorisation is ILU(0), not IC(0): the IKJ form (Saad, Iterative Methods for
+// Sparse Linear Systems, 2nd ed., Alg. 10.4) over the d x d node blocks of K,
+// keeping exactly K's block pattern.
+//
+// Order.  Nodes are coloured greedily in node order over the symmetrised node
+// graph (no two neighbours share a colour) and eliminated colour by colour,
+// nodes of one colour in node order.  A node's lower neighbours then all lie
+// in earlier colours, so
+//   * the factorisation runs one launch per colour, one thread per node row,
+//     each row reading only rows already final;
+//   * the forward solve runs the colours in increasing order and the backward
+//     solve in decreasing order, one launch per colour, G lanes per row.
+// The ordering costs iterations against natural-order ILU(0) (the 501 x 251
+// cracked plate: 812 against 420 BiCGStab iterations, Jacobi 2,247) and buys
+// ~15 launches per sweep instead of ~1,000 dependency levels.
+//
+// Layout.  The factor is computed in place on a masked copy of K (rows and
+// columns of prescribed DOFs removed, identity on their diagonal) in K's block
+// layout, then compacted per node row into a lower part (strict lower blocks in
+// elimination order) and an upper part (strict upper blocks) with their
+// column nodes, plus the inverses of the pivot blocks; the solves stream those
+// arrays in the working precision (FP64, or FP32 for the refined FP32 solves).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "../../include/pdx.h"
+#include "device.hpp"
+#include "rowblock.hpp"
+
+namespace pdx {
+
+struct Ilu0 {
+    const double *K = nullptr;  // operator this was built for
+    long long gen = -1;
+    int nn = 0, D = 0, ncol = 0;
+    std::vector<int> col_off;  // colour c: rows order[col_off[c] .. col_off[c + 1])
+    std::vector<int> color;
+    int *order = nullptr, *rank = nullptr, *lo_off = nullptr, *lo_slot = nullptr, *up_off = nullptr,
+        *up_slot = nullptr, *Lc = nullptr, *Uc = nullptr;
+    double *F = nullptr, *Dinv = nullptr, *Lv = nullptr, *Uv = nullptr;
+    float *Lv32 = nullptr, *Uv32 = nullptr, *Dinv32 = nullptr;
+    long long nlo = 0, nup = 0;
+    double setup_ms = 0;
+    std::vector<void *> owned;
+    ~Ilu0() {
+        for (void *p : owned) hipFree(p);
+    }
+};
+
+template <typename P>
+static int il_alloc(Ilu0 &t, P *&p, size_t count, std::string &err) {
+    PDX_HIP(hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(P)));
+    t.owned.push_back((void *)p);
+    return PDX_OK;
+}
+
+// ------------------------------------------------------------------------
+// kernels
+// ------------------------------------------------------------------------
+// F = K with the rows and columns of prescribed DOFs removed and 1 on their diagonal
+template <int D>
+__global__ void __launch_bounds__(256) k_ilu_mask(int nn, const int *adj_off, const int *adj, const int *eq,
+                                                  const double *K, double *F) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nn) return;
+    for (int b = adj_off[a]; b < adj_off[a + 1]; b++) {
+        const int c = adj[b];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                double v = K[(size_t)b * D * D + i * D + j];
+                if (eq[a * 3 + i] < 0 || eq[c * 3 + j] < 0) v = (c == a && i == j) ? 1.0 : 0.0;
+                F[(size_t)b * D * D + i * D + j] = v;
+            }
+    }
+}
+
+// inverse of a d x d block by cofactors; false when singular
+template <int D>
+__device__ inline bool block_inverse(const double *A, double *X) {
+#pragma clang fp contract(off)
+    if (D == 1) {
+        if (A[0] == 0.0) return false;
+        X[0] = 1.0 / A[0];
+        return true;
+    } else if (D == 2) {
+        const double det = A[0] * A[3] - A[1] * A[2];
+        if (det == 0.0 || !isfinite(det)) return false;
+        X[0] = A[3] / det; X[1] = -A[1] / det; X[2] = -A[2] / det; X[3] = A[0] / det;
+        return true;
+    } else {
+        const double c00 = A[4] * A[8] - A[5] * A[7], c01 = A[5] * A[6] - A[3] * A[8],
+                     c02 = A[3] * A[7] - A[4] * A[6];
+        const double det = A[0] * c00 + A[1] * c01 + A[2] * c02;
+        if (det == 0.0 || !isfinite(det)) return false;
+        X[0] = c00 / det;
+        X[1] = (A[2] * A[7] - A[1] * A[8]) / det;
+        X[2] = (A[1] * A[5] - A[2] * A[4]) / det;
+        X[3] = c01 / det;
+        X[4] = (A[0] * A[8] - A[2] * A[6]) / det;
+        X[5] = (A[2] * A[3] - A[0] * A[5]) / det;
+        X[6] = c02 / det;
+        X[7] = (A[1] * A[6] - A[0] * A[7]) / det;
+        X[8] = (A[0] * A[4] - A[1] * A[3]) / det;
+        return true;
+    }
+}
+
+// IKJ elimination of the rows of one colour (one thread per row).  Row i's
+// lower neighbours k (earlier colours, increasing elimination rank) are final:
+//   F_ik <- F_ik Dinv_k;  F_ij -= F_ik F_kj  for j in row i with rank j > rank k
+// and (k, j) in the pattern (merge of the two sorted rows); then Dinv_i.
+// Products and updates in the same order and without contraction as the
+// textbook restatement, so the factor is bitwise reproducible.
+template <int D>
+__global__ void __launch_bounds__(256) k_ilu_factor(int cnt, const int *rows, const int *adj_off, const int *adj,
+                                                    const int *rank, const int *lo_off, const int *lo_slot,
+                                                    double *F, double *Dinv, int *status) {
+#pragma clang fp contract(off)
+    constexpr int DD = D * D;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = rows[t];
+    const int i0 = adj_off[i], i1 = adj_off[i + 1];
+    for (int q = lo_off[i]; q < lo_off[i + 1]; q++) {
+        const int sk = lo_slot[q], k = adj[sk], rk = rank[k];
+        double X[DD];
+        double *Fik = F + (size_t)sk * DD;
+        const double *Dk = Dinv + (size_t)k * DD;
+#pragma unroll
+        for (int r = 0; r < D; r++)
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int e = 0; e < D; e++) acc += Fik[r * D + e] * Dk[e * D + c];
+                X[r * D + c] = acc;
+            }
+#pragma unroll
+        for (int e = 0; e < DD; e++) Fik[e] = X[e];
+        int s = i0, u = adj_off[k];
+        const int u1 = adj_off[k + 1];
+        while (s < i1 && u < u1) {
+            const int js = adj[s], ju = adj[u];
+            if (js < ju) { s++; continue; }
+            if (js > ju) { u++; continue; }
+            if (rank[js] > rk) {
+                const double *Fkj = F + (size_t)u * DD;
+                double *Fij = F + (size_t)s * DD;
+                double T[DD];
+#pragma unroll
+                for (int r = 0; r < D; r++)
+#pragma unroll
+                    for (int c = 0; c < D; c++) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int e = 0; e < D; e++) acc += X[r * D + e] * Fkj[e * D + c];
+                        T[r * D + c] = acc;
+                    }
+#pragma unroll
+                for (int e = 0; e < DD; e++) Fij[e] -= T[e];
+            }
+            s++;
+            u++;
+        }
+    }
+    int lo = i0, hi = i1 - 1, di = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        if (adj[mid] == i) { di = mid; break; }
+        if (adj[mid] < i) lo = mid + 1; else hi = mid - 1;
+    }
+    double A[DD], Xi[DD];
+    if (di < 0) { atomicOr(status, 2); return; }
+#pragma unroll
+    for (int e = 0; e < DD; e++) A[e] = F[(size_t)di * DD + e];
+    if (!block_inverse<D>(A, Xi)) { atomicOr(status, 1); return; }
+#pragma unroll
+    for (int e = 0; e < DD; e++) Dinv[(size_t)i * DD + e] = Xi[e];
+}
+
+// compact the factor's strict lower / upper blocks of every row (listed slots) in precision V
+template <int DD, typename V>
+__global__ void k_ilu_compact(long long n, const int *slot, const int *adj, const double *F, V *val, int *col) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+        const int s = slot[q];
+        col[q] = adj[s];
+#pragma unroll
+        for (int e = 0; e < DD; e++) val[q * DD + e] = V(F[(size_t)s * DD + e]);
+    }
+}
+
+// forward solve of one colour: z_i = r_i - sum_lower L_ik z_k
+template <int D, typename V, int G>
+__global__ void __launch_bounds__(256) k_ilu_fwd(int cnt, const int *rows, const int *off, const int *col,
+                                                 const V *Lv, const V *r, V *z) {
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    if (g >= cnt) return;  // whole groups leave together
+    const int i = rows[g];
+    V acc[D];
+#pragma unroll
+    for (int e = 0; e < D; e++) acc[e] = V(0);
+    for (int b = off[i] + lane; b < off[i + 1]; b += G) {
+        const int c = col[b];
+        V zc[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) zc[j] = z[(size_t)c * D + j];
+#pragma unroll
+        for (int e = 0; e < D; e++)
+#pragma unroll
+            for (int j = 0; j < D; j++) acc[e] += Lv[(size_t)b * D * D + e * D + j] * zc[j];
+    }
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1)
+#pragma unroll
+        for (int e = 0; e < D; e++) acc[e] += __shfl_xor(acc[e], s, G);
+    if (lane < D) z[(size_t)i * D + lane] = r[(size_t)i * D + lane] - pick<D, V>(acc, lane);
+}
+
+// backward solve of one colour: z_i = Dinv_i (z_i - sum_upper U_ij z_j)
+template <int D, typename V, int G>
+__global__ void __launch_bounds__(256) k_ilu_bwd(int cnt, const int *rows, const int *off, const int *col,
+                                                 const V *Uv, const V *Dinv, V *z) {
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    if (g >= cnt) return;
+    const int i = rows[g];
+    V yi[D], acc[D];
+#pragma unroll
+    for (int e = 0; e < D; e++) {
+        yi[e] = z[(size_t)i * D + e];
+        acc[e] = V(0);
+    }
+    for (int b = off[i] + lane; b < off[i + 1]; b += G) {
+        const int c = col[b];
+        V zc[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) zc[j] = z[(size_t)c * D + j];
+#pragma unroll
+        for (int e = 0; e < D; e++)
+#pragma unroll
+            for (int j = 0; j < D; j++) acc[e] += Uv[(size_t)b * D * D + e * D + j] * zc[j];
+    }
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1)
+#pragma unroll
+        for (int e = 0; e < D; e++) acc[e] += __shfl_xor(acc[e], s, G);
+    if (lane < D) {
+        V v = V(0);
+#pragma unroll
+        for (int j = 0; j < D; j++) v += Dinv[(size_t)i * D * D + lane * D + j] * (yi[j] - acc[j]);
+        z[(size_t)i * D + lane] = v;
+    }
+}
+
+// ------------------------------------------------------------------------
+// host set-up
+// ------------------------------------------------------------------------
+// greedy colouring in node order over the symmetrised node graph
+static int greedy_colors(int nn, const std::vector<int> &off, const std::vector<int> &adj, std::vector<int> &color) {
+    std::vector<int> toff(nn + 1, 0), tadj(adj.size());
+    for (int a = 0; a < nn; a++)
+        for (int s = off[a]; s < off[a + 1]; s++) toff[adj[s] + 1]++;
+    for (int a = 0; a < nn; a++) toff[a + 1] += toff[a];
+    {
+        std::vector<int> pos(toff.begin(), toff.end() - 1);
+        for (int a = 0; a < nn; a++)
+            for (int s = off[a]; s < off[a + 1]; s++) tadj[pos[adj[s]]++] = a;
+    }
+    color.assign(nn, 0);
+    std::vector<int> mark;
+    int nc = 0;
+    for (int a = 0; a < nn; a++) {
+        auto see = [&](int b) {
+            if (b >= a) return;
+            if ((int)mark.size() <= color[b]) mark.resize(color[b] + 1, 0);
+            mark[color[b]] = a + 1;
+        };
+        for (int s = off[a]; s < off[a + 1]; s++) see(adj[s]);
+        for (int s = toff[a]; s < toff[a + 1]; s++) see(tadj[s]);
+        int c = 0;
+        while (c < (int)mark.size() && mark[c] == a + 1) c++;
+        color[a] = c;
+        nc = std::max(nc, c + 1);
+    }
+    return nc;
+}
+
+template <int D>
+static int build_ilu0(Ctx &c, const double *K64, Ilu0 &t, std::string &err) {
+    const int nn = c.nn;
+    std::vector<int> off(nn + 1);
+    PDX_HIP(hipMemcpy(off.data(), c.adj_off, (nn + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> adj(std::max(off[nn], 1));
+    PDX_HIP(hipMemcpy(adj.data(), c.adj, (size_t)off[nn] * sizeof(int), hipMemcpyDeviceToHost));
+    t.nn = nn;
+    t.D = D;
+    t.ncol = greedy_colors(nn, off, adj, t.color);
+    // elimination order: by colour, node order within a colour
+    t.col_off.assign(t.ncol + 1, 0);
+    for (int a = 0; a < nn; a++) t.col_off[t.color[a] + 1]++;
+    for (int q = 0; q < t.ncol; q++) t.col_off[q + 1] += t.col_off[q];
+    std::vector<int> order(nn), rank(nn);
+    {
+        std::vector<int> pos(t.col_off.begin(), t.col_off.end() - 1);
+        for (int a = 0; a < nn; a++) order[pos[t.color[a]]++] = a;
+        for (int q = 0; q < nn; q++) rank[order[q]] = q;
+    }
+    // per row: strict lower slots by increasing rank, strict upper slots in slot order
+    std::vector<int> lo_off(nn + 1, 0), up_off(nn + 1, 0), lo_slot, up_slot;
+    lo_slot.reserve(off[nn] / 2 + 1);
+    up_slot.reserve(off[nn] / 2 + 1);
+    std::vector<std::pair<int, int>> tmp;
+    for (int a = 0; a < nn; a++) {
+        tmp.clear();
+        bool diag = false;
+        for (int s = off[a]; s < off[a + 1]; s++) {
+            const int b = adj[s];
+            if (b == a) diag = true;
+            else if (rank[b] < rank[a]) tmp.push_back({rank[b], s});
+            else up_slot.push_back(s);
+        }
+        if (!diag) {
+            err = "ILU(0) needs every diagonal block in the pattern (row " + std::to_string(a) + ")";
+            return PDX_ERR_PATTERN;
+        }
+        std::sort(tmp.begin(), tmp.end());
+        for (auto &p : tmp) lo_slot.push_back(p.second);
+        lo_off[a + 1] = (int)lo_slot.size();
+        up_off[a + 1] = (int)up_slot.size();
+    }
+    t.nlo = (long long)lo_slot.size();
+    t.nup = (long long)up_slot.size();
+    constexpr int DD = D * D;
+    int rc;
+    auto up_i = [&](int *&d, const std::vector<int> &h) -> int {
+        int r = il_alloc(t, d, h.size(), err);
+        if (r) return r;
+        if (!h.empty()) PDX_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+        return PDX_OK;
+    };
+    if ((rc = up_i(t.order, order)) || (rc = up_i(t.rank, rank)) || (rc = up_i(t.lo_off, lo_off)) ||
+        (rc = up_i(t.lo_slot, lo_slot)) || (rc = up_i(t.up_off, up_off)) || (rc = up_i(t.up_slot, up_slot)) ||
+        (rc = il_alloc(t, t.F, (size_t)off[nn] * DD, err)) || (rc = il_alloc(t, t.Dinv, (size_t)nn * DD, err)) ||
+        (rc = il_alloc(t, t.Lv, (size_t)t.nlo * DD, err)) || (rc = il_alloc(t, t.Uv, (size_t)t.nup * DD, err)) ||
+        (rc = il_alloc(t, t.Lc, (size_t)t.nlo, err)) || (rc = il_alloc(t, t.Uc, (size_t)t.nup, err)))
+        return rc;
+    int *status;
+    if ((rc = il_alloc(t, status, 1, err))) return rc;
+    PDX_HIP(hipMemsetAsync(status, 0, sizeof(int), c.stream));
+    k_ilu_mask<D><<<(nn + 255) / 256, 256, 0, c.stream>>>(nn, c.adj_off, c.adj, c.eq, K64, t.F);
+    for (int q = 0; q < t.ncol; q++) {
+        const int cnt = t.col_off[q + 1] - t.col_off[q];
+        if (cnt == 0) continue;
+        k_ilu_factor<D><<<(cnt + 255) / 256, 256, 0, c.stream>>>(cnt, t.order + t.col_off[q], c.adj_off, c.adj,
+                                                                t.rank, t.lo_off, t.lo_slot, t.F, t.Dinv, status);
+    }
+    PDX_HIP(hipGetLastError());
+    int hs = 0;
+    PDX_HIP(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PDX_HIP(hipStreamSynchronize(c.stream));
+    if (hs & 2) { err = "ILU(0): diagonal block missing"; return PDX_ERR_PATTERN; }
+    if (hs & 1) { err = "ILU(0): zero pivot block (singular factor)"; return PDX_ERR_SINGULAR; }
+    k_ilu_compact<DD, double><<<2048, 256, 0, c.stream>>>(t.nlo, t.lo_slot, c.adj, t.F, t.Lv, t.Lc);
+    k_ilu_compact<DD, double><<<2048, 256, 0, c.stream>>>(t.nup, t.up_slot, c.adj, t.F, t.Uv, t.Uc);
+    PDX_HIP(hipGetLastError());
+    return PDX_OK;
+}
+
+template <typename A, typename B>
+__global__ void k_ilu_cast(size_t n, const A *a, B *b) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = B(a[i]);
+}
+
+// ILU(0) of operator K64 (cached on the context for its generation); f32: also the FP32 copies
+int ilu0_prepare(Ctx &c, const double *K64, bool f32, void *&out, std::string &err) {
+    Ilu0 *t = (Ilu0 *)c.ilu0;
+    if (!(t && t->K == K64 && t->gen == c.K_gen)) {
+        delete t;
+        c.ilu0 = nullptr;
+        t = new Ilu0();
+        auto t0 = std::chrono::steady_clock::now();
+        int rc = c.dim == 3   ? build_ilu0<3>(c, K64, *t, err)
+                 : c.dim == 2 ? build_ilu0<2>(c, K64, *t, err)
+                              : build_ilu0<1>(c, K64, *t, err);
+        if (rc) {
+            delete t;
+            return rc;
+        }
+        PDX_HIP(hipStreamSynchronize(c.stream));
+        t->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        t->K = K64;
+        t->gen = c.K_gen;
+        c.ilu0 = t;
+    }
+    if (f32 && !t->Dinv32) {
+        const size_t dd = (size_t)c.dd;
+        int rc;
+        if ((rc = il_alloc(*t, t->Lv32, t->nlo * dd, err)) || (rc = il_alloc(*t, t->Uv32, t->nup * dd, err)) ||
+            (rc = il_alloc(*t, t->Dinv32, (size_t)c.nn * dd, err)))
+            return rc;
+        k_ilu_cast<double, float><<<2048, 256, 0, c.stream>>>(t->nlo * dd, t->Lv, t->Lv32);
+        k_ilu_cast<double, float><<<2048, 256, 0, c.stream>>>(t->nup * dd, t->Uv, t->Uv32);
+        k_ilu_cast<double, float><<<2048, 256, 0, c.stream>>>((size_t)c.nn * dd, t->Dinv, t->Dinv32);
+        PDX_HIP(hipGetLastError());
+    }
+    out = t;
+    return PDX_OK;
+}
+
+template <int D, typename V>
+static void apply_t(Ctx &c, const Ilu0 &t, const V *Lv, const V *Uv, const V *Dinv, const V *r, V *z) {
+    constexpr int G = D == 3 ? 8 : (D == 2 ? 4 : 8);
+    for (int q = 0; q < t.ncol; q++) {
+        const int cnt = t.col_off[q + 1] - t.col_off[q];
+        if (cnt == 0) continue;
+        k_ilu_fwd<D, V, G><<<(cnt * G + 255) / 256, 256, 0, c.stream>>>(cnt, t.order + t.col_off[q], t.lo_off, t.Lc,
+                                                                        Lv, r, z);
+    }
+    for (int q = t.ncol - 1; q >= 0; q--) {
+        const int cnt = t.col_off[q + 1] - t.col_off[q];
+        if (cnt == 0) continue;
+        k_ilu_bwd<D, V, G><<<(cnt * G + 255) / 256, 256, 0, c.stream>>>(cnt, t.order + t.col_off[q], t.up_off, t.Uc,
+                                                                        Uv, Dinv, z);
+    }
+}
+
+// z = (L U)^-1 r on the context's stream (r, z full node-major vectors, z != r)
+void ilu0_apply(Ctx &c, const void *h, const double *r, double *z) {
+    const Ilu0 &t = *(const Ilu0 *)h;
+    if (c.dim == 3) apply_t<3, double>(c, t, t.Lv, t.Uv, t.Dinv, r, z);
+    else if (c.dim == 2) apply_t<2, double>(c, t, t.Lv, t.Uv, t.Dinv, r, z);
+    else apply_t<1, double>(c, t, t.Lv, t.Uv, t.Dinv, r, z);
+}
+
+void ilu0_apply(Ctx &c, const void *h, const float *r, float *z) {
+    const Ilu0 &t = *(const Ilu0 *)h;
+    if (c.dim == 3) apply_t<3, float>(c, t, t.Lv32, t.Uv32, t.Dinv32, r, z);
+    else if (c.dim == 2) apply_t<2, float>(c, t, t.Lv32, t.Uv32, t.Dinv32, r, z);
+    else apply_t<1, float>(c, t, t.Lv32, t.Uv32, t.Dinv32, r, z);
+}
+
+int ilu0_colors(const void *h) { return ((const Ilu0 *)h)->ncol; }
+double ilu0_setup_ms(const void *h) { return ((const Ilu0 *)h)->setup_ms; }
+
+void free_ilu0(Ctx &c) {
+    delete (Ilu0 *)c.ilu0;
+    c.ilu0 = nullptr;
+}
+
+int set_error(int rc, const std::string &msg);
+
+}  // namespace pdx
+
+using namespace pdx;
+
+// The factor of the context's assembled K (masked to the free DOFs) and, with r
+// and z, one application z = (L U)^-1 r in FP64.
+extern "C" int pdx_ilu0(pdx_ctx *pc, const double *r, double *z, int *colors, double *F, double *Dinv,
+                        int *ncolors) {
+    if (!pc) return set_error(PDX_ERR_ARG, "null argument");
+    Ctx &c = *pc->c;
+    std::string err;
+    double *dr = nullptr, *dz = nullptr;
+    auto run = [&]() -> int {
+        if (!c.K_valid) { err = "K not assembled (call pdx_assemble first)"; return PDX_ERR_ARG; }
+        void *h;
+        int rc = ilu0_prepare(c, c.K, false, h, err);
+        if (rc) return rc;
+        const Ilu0 &t = *(const Ilu0 *)h;
+        const size_t n = (size_t)c.nn * c.dim;
+        if (ncolors) *ncolors = t.ncol;
+        if (colors) std::copy(t.color.begin(), t.color.end(), colors);
+        if (F) PDX_HIP(hipMemcpy(F, t.F, (size_t)c.nblk * c.dd * sizeof(double), hipMemcpyDeviceToHost));
+        if (Dinv) PDX_HIP(hipMemcpy(Dinv, t.Dinv, (size_t)c.nn * c.dd * sizeof(double), hipMemcpyDeviceToHost));
+        if (r && z) {
+            PDX_HIP(hipMalloc((void **)&dr, n * sizeof(double)));
+            PDX_HIP(hipMalloc((void **)&dz, n * sizeof(double)));
+            PDX_HIP(hipMemcpyAsync(dr, r, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+            ilu0_apply(c, h, dr, dz);
+            PDX_HIP(hipGetLastError());
+            PDX_HIP(hipMemcpyAsync(z, dz, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        }
+        PDX_HIP(hipStreamSynchronize(c.stream));
+        return PDX_OK;
+    };
+    int rc = run();
+    if (dr) hipFree(dr);
+    if (dz) hipFree(dz);
+    return rc ? set_error(rc, err) : PDX_OK;
+}

@@ -11,7 +11,11 @@
 //     whose coarse space P holds the rigid-body modes of geometric node
 //     aggregates (cells of the mesh split into K-connected components, so a
 //     crack separates them) and whose smoother S is node-block Jacobi or the
-//     node-block smoother predicted by the graph network (learned.hip).
+//     node-block smoother predicted by the graph network (learned.hip);
+//   * node-block ILU(0) in multicolour order (ilu0.hip);
+//   * an optional fallback chain (learned / two-level -> ILU(0) -> node-block
+//     Jacobi -> FP64 node-block Jacobi) when a solve stops short or a
+//     preconditioner is singular.
 //     The coarse matrix is inverted once per operator on the device (blocked
 //     Gauss-Jordan, FP64); each application restricts inside the vector
 //     update kernels, multiplies by the dense inverse, and prolongates with
@@ -169,7 +173,7 @@ __global__ void __launch_bounds__(256) k_bicg_p(int nn, const V *r, V *p, const 
             const size_t o = (size_t)a * D + i;
             p[o] = first ? r[o] : V(r[o] + bv * (p[o] - om * v[o]));
         }
-        apply_blk<D, V>(Minv, p, ph, a);
+        if (Minv) apply_blk<D, V>(Minv, p, ph, a);
     }
 }
 
@@ -184,7 +188,7 @@ __global__ void __launch_bounds__(256) k_bicg_s(int nn, const V *r, const V *v, 
             const size_t o = (size_t)a * D + i;
             s[o] = r[o] - al * v[o];
         }
-        apply_blk<D, V>(Minv, s, sh, a);
+        if (Minv) apply_blk<D, V>(Minv, s, sh, a);
     }
 }
 
@@ -914,6 +918,7 @@ struct KrylovWs {
     const TwoLevel *tl = nullptr;
     const V *Pn = nullptr, *kp = nullptr, *Ainv = nullptr;
     const V *RS = nullptr;  // block row sums (difference-form SpMV) or null
+    const void *ilu = nullptr;  // ILU(0) factor (ilu0.hip) or null
 };
 
 template <typename V>
@@ -964,6 +969,13 @@ static void bicg_iter(Ctx &c, const T *K, KrylovWs<V> &w, double *red, double *s
         k_coarse_mv<V><<<gc, 256, 0, c.stream>>>(t.nc, t.ld, w.Ainv, w.rc, w.yc);
         k_prolong<D, NM, V><<<gn, 256, 0, c.stream>>>(c.nn, w.Pn, t.node_agg, t.kp_off, t.kp_agg, w.kp, w.Minv, w.s,
                                                       w.yc, w.sh);
+    } else if (w.ilu) {
+        k_bicg_p<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.r, w.p, w.v, w.ph, (const V *)nullptr, rho, sc,
+                                                        it == 0);
+        ilu0_apply(c, w.ilu, w.p, w.ph);
+        spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.ph, w.v, w.rh, rv, (const V *)nullptr, nullptr);
+        k_bicg_s<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.r, w.v, w.s, w.sh, (const V *)nullptr, rho, rv, sc);
+        ilu0_apply(c, w.ilu, w.s, w.sh);
     } else {
         k_bicg_p<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.r, w.p, w.v, w.ph, w.Minv, rho, sc, it == 0);
         spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.ph, w.v, w.rh, rv, (const V *)nullptr, nullptr);
@@ -1127,6 +1139,12 @@ static int build_precond(Ctx &c, const double *K64, int kind, V *Minv, std::stri
 // preconditioner of a workspace: node-block part and, for the two-level kinds, the coarse operator
 template <int D, typename V>
 static int setup_precond(Ctx &c, const double *K64, const pdx_solve_opts &o, KrylovWs<V> &w, std::string &err) {
+    if (o.precond == PDX_PREC_ILU0) {
+        void *h;
+        int rc = ilu0_prepare(c, K64, sizeof(V) == 4, h, err);
+        w.ilu = rc ? nullptr : h;
+        return rc;
+    }
     const bool two = o.precond == PDX_PREC_TWO_LEVEL || o.precond == PDX_PREC_LEARNED;
     int rc = build_precond<D, V>(c, K64, two && o.precond != PDX_PREC_LEARNED ? PDX_PREC_BLOCK_JACOBI : o.precond,
                                  w.Minv, err);
@@ -1165,6 +1183,9 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
     double bnorm = std::sqrt(host_sum(c, part, NPART, err));
     st.iterations = 0;
     st.refinements = 0;
+    st.seconds = 0;
+    st.true_relres = 1.0;
+    st.converged = 0;
     if (bnorm == 0.0) {
         PDX_HIP(hipMemsetAsync(x_dev, 0, n * 8, c.stream));
         PDX_HIP(hipStreamSynchronize(c.stream));
@@ -1251,22 +1272,70 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
     return PDX_OK;
 }
 
+static int solve_once(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
+                      double *x_dev, pdx_solve_stats &st, std::string &err) {
+    if (c.dim == 3) return solve_masked<3>(c, K64, K32, o, b_dev, x_dev, st, err);
+    if (c.dim == 2) return solve_masked<2>(c, K64, K32, o, b_dev, x_dev, st, err);
+    return solve_masked<1>(c, K64, K32, o, b_dev, x_dev, st, err);
+}
+
+// the attempt after o in the fallback chain (learned / two-level -> ILU(0) -> node-block Jacobi ->
+// FP64 node-block Jacobi, all with BiCGStab); false at its end
+static bool next_fallback(pdx_solve_opts &o) {
+    o.krylov = PDX_KRYLOV_BICGSTAB;
+    if (o.precond == PDX_PREC_LEARNED || o.precond == PDX_PREC_TWO_LEVEL) {
+        o.precond = PDX_PREC_ILU0;
+        return true;
+    }
+    if (o.precond != PDX_PREC_BLOCK_JACOBI) {
+        o.precond = PDX_PREC_BLOCK_JACOBI;
+        return true;
+    }
+    if (o.precision == PDX_FP32) {
+        o.precision = PDX_FP64;
+        return true;
+    }
+    return false;
+}
+
 // solve with an explicit node-block matrix (K, K_eff or a 1x1-block CSR)
 int dev_solve_mat(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
                   double *x_dev, pdx_solve_stats &st, std::string &err) {
     if (o.krylov != PDX_KRYLOV_BICGSTAB && o.krylov != PDX_KRYLOV_CG) { err = "unknown Krylov method"; return PDX_ERR_ARG; }
-    if (o.krylov == PDX_KRYLOV_CG && (o.precond == PDX_PREC_TWO_LEVEL || o.precond == PDX_PREC_LEARNED)) {
-        err = "CG needs a symmetric preconditioner (none, Jacobi, block Jacobi); the hybrid two-level one is not";
+    if (o.krylov == PDX_KRYLOV_CG &&
+        (o.precond == PDX_PREC_TWO_LEVEL || o.precond == PDX_PREC_LEARNED || o.precond == PDX_PREC_ILU0)) {
+        err = "CG needs a symmetric preconditioner (none, Jacobi, block Jacobi); two-level and ILU(0) are not";
         return PDX_ERR_ARG;
     }
-    if (o.precond < PDX_PREC_NONE || o.precond > PDX_PREC_TWO_LEVEL) { err = "unknown preconditioner"; return PDX_ERR_ARG; }
-    if (c.dim == 3) return solve_masked<3>(c, K64, K32, o, b_dev, x_dev, st, err);
-    if (c.dim == 2) return solve_masked<2>(c, K64, K32, o, b_dev, x_dev, st, err);
-    if (o.precond == PDX_PREC_TWO_LEVEL || o.precond == PDX_PREC_LEARNED) {
+    if (o.precond < PDX_PREC_NONE || o.precond > PDX_PREC_ILU0) { err = "unknown preconditioner"; return PDX_ERR_ARG; }
+    if (c.dim == 1 && (o.precond == PDX_PREC_TWO_LEVEL || o.precond == PDX_PREC_LEARNED)) {
         err = "two-level preconditioners need the node-block operator of a model (use pdx_static_solve)";
         return PDX_ERR_ARG;
     }
-    return solve_masked<1>(c, K64, K32, o, b_dev, x_dev, st, err);
+    pdx_solve_opts a = o;
+    int iters = 0, refs = 0;
+    double secs = 0;
+    for (int attempt = 1;; attempt++) {
+        int rc = solve_once(c, K64, K32, a, b_dev, x_dev, st, err);
+        iters += st.iterations;
+        refs += st.refinements;
+        secs += st.seconds;
+        st.iterations = iters;
+        st.refinements = refs;
+        st.seconds = secs;
+        st.precond_used = a.precond;
+        st.precision_used = a.precision;
+        st.attempts = attempt;
+        // a learned preconditioner with no smoother predicted for this operator also falls back
+        const bool learned_missing = rc == PDX_ERR_ARG && a.precond == PDX_PREC_LEARNED &&
+                                     (!c.learned_Minv || c.learned_gen != c.K_gen);
+        if (rc == PDX_OK || !o.fallback || (rc != PDX_ERR_NOCONV && rc != PDX_ERR_SINGULAR && !learned_missing))
+            return rc;
+        // restart from the iterate the failed attempt left, unless it is no better than zero
+        const bool keep = rc == PDX_ERR_NOCONV && std::isfinite(st.true_relres) && st.true_relres < 1.0;
+        if (!next_fallback(a)) return rc;
+        a.warm_start = keep ? 1 : 0;
+    }
 }
 
 int dev_solve(Ctx &c, const pdx_solve_opts &o, const double *b_dev, double *x_dev, pdx_solve_stats &st,

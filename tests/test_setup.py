@@ -141,7 +141,7 @@ def test_python_constants_match_the_header():
     hdr = open(os.path.join(ROOT, "include", "pdx.h")).read()
     defs = dict(re.findall(r"#define\s+(PDX_[A-Z0-9_]+)\s+(-?\d+)", hdr))
     for name in ("PDX_FP64", "PDX_FP32", "PDX_KRYLOV_BICGSTAB", "PDX_KRYLOV_CG", "PDX_PREC_NONE",
-                 "PDX_PREC_JACOBI", "PDX_PREC_BLOCK_JACOBI", "PDX_PREC_LEARNED", "PDX_PREC_TWO_LEVEL"):
+                 "PDX_PREC_JACOBI", "PDX_PREC_BLOCK_JACOBI", "PDX_PREC_LEARNED", "PDX_PREC_TWO_LEVEL", "PDX_PREC_ILU0"):
         if name in defs:
             assert getattr(pdx, name) == int(defs[name]), name
     assert pdx.FB_VARIANTS == int(defs["PDX_FB_VARIANTS"])
