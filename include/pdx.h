@@ -121,6 +121,8 @@ typedef struct pdx_solve_opts {
                           BiCGStab down the chain learned / two-level -> ILU(0) -> node-block Jacobi
                           (any other preconditioner -> node-block Jacobi) -> FP64 node-block Jacobi,
                           from the iterate the failed attempt left when it beat zero */
+    int no_graph;      /* nonzero: launch every Krylov iteration's kernels one by one instead of
+                          replaying captured hipGraphs of 10 iterations (same results, bit for bit) */
 } pdx_solve_opts;
 
 typedef struct pdx_solve_stats {

@@ -288,6 +288,7 @@ void ctx_free(Ctx *c) {
     if (c->stress_stage) hipFree(c->stress_stage);
     free_two_level(*c);
     free_ilu0(*c);
+    free_krylov_ws(*c);
     free_gnn_work(*c);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

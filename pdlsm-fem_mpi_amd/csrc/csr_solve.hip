@@ -83,6 +83,7 @@ static int csr_solve(int device, long long n, const long long *ia, const long lo
     hipFree(dx);
     if (K32) hipFree(K32);
     free_ilu0(c);
+    free_krylov_ws(c);
     if (c.red) hipFree(c.red);
     c.red = nullptr;
     for (void *p : c.owned) hipFree(p);

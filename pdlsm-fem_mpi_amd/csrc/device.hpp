@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -120,6 +121,7 @@ struct Ctx {
     double last_total_ms = 0, last_step_ms = 0;
     void *two_level = nullptr;       // coarse space of the last two-level solve (krylov.hip)
     void *ilu0 = nullptr;            // block ILU(0) factor of the last ILU(0) solve (ilu0.hip)
+    void *krylov_ws[2] = {nullptr, nullptr};  // Krylov workspaces (FP64, FP32) and their graphs (krylov.hip)
     double *learned_Minv = nullptr;  // [nn*dd] node-block smoother predicted by the graph network
     long long learned_gen = -1;      // K_gen the smoother was predicted for
     void *gnn_work = nullptr;        // graph-network inputs and activations (learned.hip)
@@ -128,6 +130,12 @@ struct Ctx {
     // helpers
     std::vector<void *> owned;
 };
+
+// process-wide serial numbers of rebuilt device objects (a rebuild may reuse freed addresses)
+inline long long next_build_serial() {
+    static std::atomic<long long> n{0};
+    return ++n;
+}
 
 constexpr int RED_PARTS = 512;  // blocks of every reducing kernel
 constexpr int RED_SLOTS = 16;
@@ -151,8 +159,10 @@ int ilu0_prepare(Ctx &c, const double *K64, bool f32, void *&out, std::string &e
 void ilu0_apply(Ctx &c, const void *h, const double *r, double *z);
 void ilu0_apply(Ctx &c, const void *h, const float *r, float *z);
 int ilu0_colors(const void *h);
+long long ilu0_serial(const void *h);
 double ilu0_setup_ms(const void *h);
 void free_ilu0(Ctx &c);
+void free_krylov_ws(Ctx &c);
 void free_gnn_work(Ctx &c);
 int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err);
 int dev_bond_sigma1(Ctx &c, const int *bond_dev, const double *sigma_dev, double *s1_dev, std::string &err);

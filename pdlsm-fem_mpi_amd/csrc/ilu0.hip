@@ -50,6 +50,7 @@ struct Ilu0 {
     float *Lv32 = nullptr, *Uv32 = nullptr, *Dinv32 = nullptr;
     long long nlo = 0, nup = 0;
     double setup_ms = 0;
+    long long serial = next_build_serial();
     std::vector<void *> owned;
     ~Ilu0() {
         for (void *p : owned) hipFree(p);
@@ -451,6 +452,7 @@ void ilu0_apply(Ctx &c, const void *h, const float *r, float *z) {
 }
 
 int ilu0_colors(const void *h) { return ((const Ilu0 *)h)->ncol; }
+long long ilu0_serial(const void *h) { return ((const Ilu0 *)h)->serial; }
 double ilu0_setup_ms(const void *h) { return ((const Ilu0 *)h)->setup_ms; }
 
 void free_ilu0(Ctx &c) {

@@ -668,6 +668,7 @@ struct TwoLevel {
     double *Pn = nullptr, *kp = nullptr, *Ainv = nullptr;  // FP64
     float *Pn32 = nullptr, *kp32 = nullptr, *Ainv32 = nullptr;
     double setup_ms = 0;
+    long long serial = next_build_serial();  // distinguishes rebuilds that reuse freed addresses
     std::vector<void *> owned;
     ~TwoLevel() {
         for (void *p : owned) hipFree(p);
@@ -909,11 +910,31 @@ void free_two_level(Ctx &c) {
 // ------------------------------------------------------------------------
 // host: BiCGStab
 // ------------------------------------------------------------------------
+// device pointers a captured chunk of iterations reads; a change of any of them re-captures
+struct GraphSig {
+    const void *K = nullptr, *RS = nullptr, *tl = nullptr, *ilu = nullptr, *Pn = nullptr, *kp = nullptr,
+               *Ainv = nullptr, *rc = nullptr, *yc = nullptr, *red = nullptr;
+    long long K_gen = -1, tl_serial = -1, ilu_serial = -1;  // launch shapes follow these objects
+    int krylov = -1;
+    bool same(const GraphSig &o) const {
+        return K == o.K && RS == o.RS && tl == o.tl && ilu == o.ilu && Pn == o.Pn && kp == o.kp && Ainv == o.Ainv &&
+               rc == o.rc && yc == o.yc && red == o.red && K_gen == o.K_gen && tl_serial == o.tl_serial &&
+               ilu_serial == o.ilu_serial && krylov == o.krylov;
+    }
+};
+
+// Krylov workspace of one precision, kept on the context across solves so that the
+// captured iteration graphs are replayed by every later solve with the same operator
 template <typename V>
 struct KrylovWs {
     V *x = nullptr, *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *s = nullptr, *t = nullptr,
       *ph = nullptr, *sh = nullptr, *xb = nullptr, *Minv = nullptr;
     double *rc = nullptr, *yc = nullptr;
+    size_t rc_cap = 0;
+    V *RSbuf = nullptr;  // FP32 block row sums of the operator (FP32 workspaces)
+    // chunks of CHUNK iterations: the first (iteration 0 starts a direction) and any later one
+    hipGraphExec_t g_first = nullptr, g_next = nullptr;
+    GraphSig sig;
     // two-level operator (null for the block-diagonal preconditioners)
     const TwoLevel *tl = nullptr;
     const V *Pn = nullptr, *kp = nullptr, *Ainv = nullptr;
@@ -934,10 +955,108 @@ static int ws_alloc(Ctx &c, KrylovWs<V> &w, std::string &err) {
 }
 
 template <typename V>
+static void ws_drop_graphs(KrylovWs<V> &w) {
+    if (w.g_first) hipGraphExecDestroy(w.g_first);
+    if (w.g_next) hipGraphExecDestroy(w.g_next);
+    w.g_first = w.g_next = nullptr;
+}
+
+template <typename V>
 static void ws_free(KrylovWs<V> &w) {
-    void *ptrs[] = {w.x, w.r, w.rh, w.p, w.v, w.s, w.t, w.ph, w.sh, w.xb, w.Minv, w.rc, w.yc, (void *)w.RS};
+    ws_drop_graphs(w);
+    void *ptrs[] = {w.x, w.r, w.rh, w.p, w.v, w.s, w.t, w.ph, w.sh, w.xb, w.Minv, w.rc, w.yc, (void *)w.RSbuf};
     for (auto p : ptrs)
         if (p) hipFree(p);
+}
+
+// the context's workspace of precision V (allocated on first use); preconditioner fields cleared
+template <typename V>
+static int ws_get(Ctx &c, KrylovWs<V> *&out, std::string &err) {
+    void *&slot = c.krylov_ws[sizeof(V) == 4 ? 1 : 0];
+    if (!slot) {
+        auto *w = new KrylovWs<V>();
+        int rc = ws_alloc(c, *w, err);
+        if (rc) {
+            ws_free(*w);
+            delete w;
+            return rc;
+        }
+        slot = w;
+    }
+    out = (KrylovWs<V> *)slot;
+    out->tl = nullptr;
+    out->ilu = nullptr;
+    out->Pn = out->kp = out->Ainv = nullptr;
+    out->RS = nullptr;
+    return PDX_OK;
+}
+
+void free_krylov_ws(Ctx &c) {
+    if (c.krylov_ws[0]) {
+        ws_free(*(KrylovWs<double> *)c.krylov_ws[0]);
+        delete (KrylovWs<double> *)c.krylov_ws[0];
+    }
+    if (c.krylov_ws[1]) {
+        ws_free(*(KrylovWs<float> *)c.krylov_ws[1]);
+        delete (KrylovWs<float> *)c.krylov_ws[1];
+    }
+    c.krylov_ws[0] = c.krylov_ws[1] = nullptr;
+}
+
+constexpr int CHUNK = 10;  // iterations between host residual checks
+
+// Run n iterations body(it), body(it + 1), ...  A full chunk (n == CHUNK, it a multiple of
+// CHUNK) replays a captured hipGraph: the first chunk differs from the later ones only by
+// iteration 0, and every later chunk launches the same kernels with the same arguments
+// (the iteration index enters through its parity only).  A change of operator or
+// preconditioner pointers re-captures.
+template <typename V, typename Body>
+static int run_chunk(Ctx &c, KrylovWs<V> &w, const GraphSig &sig, bool use_graph, int it, int n, Body &&body,
+                     std::string &err) {
+    if (!use_graph || n != CHUNK) {
+        for (int k = 0; k < n; k++) body(it + k);
+        return PDX_OK;
+    }
+    if (!w.sig.same(sig)) {
+        ws_drop_graphs(w);
+        w.sig = sig;
+    }
+    hipGraphExec_t &g = it == 0 ? w.g_first : w.g_next;
+    if (!g) {
+        hipGraph_t gr = nullptr;
+        PDX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < n; k++) body(it + k);
+        PDX_HIP(hipStreamEndCapture(c.stream, &gr));
+        hipError_t e = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (e != hipSuccess) {
+            g = nullptr;
+            err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+            return PDX_ERR_HIP;
+        }
+    }
+    PDX_HIP(hipGraphLaunch(g, c.stream));
+    return PDX_OK;
+}
+
+template <typename V>
+static GraphSig make_sig(Ctx &c, const void *K, const KrylovWs<V> &w, int krylov) {
+    GraphSig g;
+    g.K = K;
+    g.RS = w.RS;
+    g.tl = w.tl;
+    g.ilu = w.ilu;
+    g.Pn = w.Pn;
+    g.kp = w.kp;
+    g.Ainv = w.Ainv;
+    g.rc = w.rc;
+    g.yc = w.yc;
+    g.red = c.red;
+    g.K_gen = c.K_gen;
+    g.tl_serial = w.tl ? w.tl->serial : -1;
+    g.ilu_serial = w.ilu ? ilu0_serial(w.ilu) : -1;
+    g.krylov = krylov;
+    return g;
 }
 
 template <int D, typename T, typename V>
@@ -993,7 +1112,7 @@ static void bicg_iter(Ctx &c, const T *K, KrylovWs<V> &w, double *red, double *s
 // run with that iterate restored.
 template <int D, typename T, typename V>
 static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, double tol, int max_iter,
-                    int *iters, double *relres, std::string &err) {
+                    bool use_graph, int *iters, double *relres, std::string &err) {
     const size_t n = (size_t)c.nn * c.dim;
     double *red = c.red, *sc = c.red + (size_t)RED_SLOTS * NPART;
     double *y;
@@ -1011,11 +1130,15 @@ static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm
     double rr0 = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
     double rel = std::sqrt(rr0) / bnorm;
     int it = 0, since_best = 0;
-    const int chunk = 10;
     double best = rel;
     PDX_HIP(hipMemcpyAsync(w.xb, w.x, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
+    const GraphSig sig = make_sig(c, K, w, PDX_KRYLOV_BICGSTAB);
     while (rel > tol && it < max_iter) {
-        for (int k = 0; k < chunk && it < max_iter; k++, it++) bicg_iter<D, T, V>(c, K, w, red, sc, it);
+        const int nrun = std::min(CHUNK, max_iter - it);
+        int rc = run_chunk(c, w, sig, use_graph, it, nrun,
+                           [&](int i) { bicg_iter<D, T, V>(c, K, w, red, sc, i); }, err);
+        if (rc) return rc;
+        it += nrun;
         PDX_HIP(hipGetLastError());
         const double rr = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
         rel = std::sqrt(rr) / bnorm;
@@ -1045,8 +1168,8 @@ static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm
 // same stopping rules and best-iterate restore as bicgstab().  Workspace: p = w.p,
 // q = w.v, z = w.s.
 template <int D, typename T, typename V>
-static int pcg(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, double tol, int max_iter, int *iters,
-               double *relres, std::string &err) {
+static int pcg(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, double tol, int max_iter,
+               bool use_graph, int *iters, double *relres, std::string &err) {
     const size_t n = (size_t)c.nn * c.dim;
     double *red = c.red;
     double *y;
@@ -1067,19 +1190,24 @@ static int pcg(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, dou
     int it = 0, since_best = 0;
     double best = rel;
     PDX_HIP(hipMemcpyAsync(w.xb, w.x, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
+    const GraphSig sig = make_sig(c, K, w, PDX_KRYLOV_CG);
+    auto body = [&](int i) {
+        double *rho = red + (size_t)((i & 1) ? P_RHO1 : P_RHO0) * NPART;
+        // the other slot holds the previous rho until this step's k_cg_xr overwrites it
+        double *rho_new = red + (size_t)((i & 1) ? P_RHO0 : P_RHO1) * NPART;
+        double *rho_prev = rho_new;
+        k_cg_p<V><<<RED_PARTS, 256, 0, c.stream>>>(n, w.p, w.s, rho, rho_prev, i == 0);
+        spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.p, w.v, w.p, red + (size_t)P_RV * NPART, (const V *)nullptr,
+                          nullptr);
+        k_cg_xr<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.x, w.r, w.s, w.p, w.v, w.Minv, rho,
+                                                      red + (size_t)P_RV * NPART, rho_new,
+                                                      red + (size_t)P_RR * NPART);
+    };
     while (rel > tol && it < max_iter) {
-        for (int k = 0; k < 10 && it < max_iter; k++, it++) {
-            double *rho = red + (size_t)((it & 1) ? P_RHO1 : P_RHO0) * NPART;
-            // the other slot holds the previous rho until this step's k_cg_xr overwrites it
-            double *rho_new = red + (size_t)((it & 1) ? P_RHO0 : P_RHO1) * NPART;
-            double *rho_prev = rho_new;
-            k_cg_p<V><<<RED_PARTS, 256, 0, c.stream>>>(n, w.p, w.s, rho, rho_prev, it == 0);
-            spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.p, w.v, w.p, red + (size_t)P_RV * NPART, (const V *)nullptr,
-                              nullptr);
-            k_cg_xr<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.x, w.r, w.s, w.p, w.v, w.Minv, rho,
-                                                          red + (size_t)P_RV * NPART, rho_new,
-                                                          red + (size_t)P_RR * NPART);
-        }
+        const int nrun = std::min(CHUNK, max_iter - it);
+        int rc = run_chunk(c, w, sig, use_graph, it, nrun, body, err);
+        if (rc) return rc;
+        it += nrun;
         PDX_HIP(hipGetLastError());
         const double rr = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
         rel = std::sqrt(rr) / bnorm;
@@ -1152,8 +1280,15 @@ static int setup_precond(Ctx &c, const double *K64, const pdx_solve_opts &o, Kry
     TwoLevel *t;
     if ((rc = get_two_level(c, K64, o.coarse_agg, t, err))) return rc;
     w.tl = t;
-    PDX_HIP(hipMalloc((void **)&w.rc, (size_t)t->ld * 8));
-    PDX_HIP(hipMalloc((void **)&w.yc, (size_t)t->ld * 8));
+    if ((size_t)t->ld > w.rc_cap) {
+        if (w.rc) hipFree(w.rc);
+        if (w.yc) hipFree(w.yc);
+        w.rc = w.yc = nullptr;
+        w.rc_cap = 0;
+        PDX_HIP(hipMalloc((void **)&w.rc, (size_t)t->ld * 8));
+        PDX_HIP(hipMalloc((void **)&w.yc, (size_t)t->ld * 8));
+        w.rc_cap = (size_t)t->ld;
+    }
     if (sizeof(V) == 4) {
         if ((rc = ensure_two_level_f32(c, *t, err))) return rc;
         w.Pn = (const V *)t->Pn32;
@@ -1198,18 +1333,21 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
     double rel = 1.0;
     double *y;
     PDX_HIP(hipMalloc((void **)&y, n * 8));
+    const bool use_graph = !o.no_graph;
     if (o.precision == PDX_FP64) {
-        KrylovWs<double> w;
-        if ((rc = ws_alloc(c, w, err)) || (rc = setup_precond<D, double>(c, K64, o, w, err))) { ws_free(w); hipFree(y); return rc; }
+        KrylovWs<double> *wp;
+        if ((rc = ws_get(c, wp, err)) || (rc = setup_precond<D, double>(c, K64, o, *wp, err))) { hipFree(y); return rc; }
+        KrylovWs<double> &w = *wp;
         PDX_HIP(hipMemcpyAsync(w.x, x_dev, n * 8, hipMemcpyDeviceToDevice, c.stream));
         double prev = INFINITY;
         for (int restart = 0; restart < 8; restart++) {
             int it = 0;
             double rr;
             rc = o.krylov == PDX_KRYLOV_CG
-                     ? pcg<D, double, double>(c, K64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total, &it, &rr, err)
-                     : bicgstab<D, double, double>(c, K64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total, &it, &rr,
-                                                   err);
+                     ? pcg<D, double, double>(c, K64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total, use_graph,
+                                              &it, &rr, err)
+                     : bicgstab<D, double, double>(c, K64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total,
+                                                   use_graph, &it, &rr, err);
             total += it;
             if (rc) break;
             spmv_dot<D, double, double>(c, K64, nullptr, w.x, y, (const double *)nullptr, nullptr,
@@ -1220,16 +1358,16 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
             prev = rel;
         }
         PDX_HIP(hipMemcpyAsync(x_dev, w.x, n * 8, hipMemcpyDeviceToDevice, c.stream));
-        ws_free(w);
     } else {
         if (!K32) { err = "FP32 matrix missing"; hipFree(y); return PDX_ERR_ARG; }
-        KrylovWs<float> w;
-        float *b32, *RS32;
+        KrylovWs<float> *wp;
+        float *b32;
         PDX_HIP(hipMalloc((void **)&b32, n * 4));
-        if ((rc = ws_alloc(c, w, err)) || (rc = setup_precond<D, float>(c, K64, o, w, err))) { ws_free(w); hipFree(y); hipFree(b32); return rc; }
-        PDX_HIP(hipMalloc((void **)&RS32, (size_t)c.nn * c.dd * 4));
-        k_block_rowsum<D, float><<<(c.nn + 255) / 256, 256, 0, c.stream>>>(c.nn, c.adj_off, K64, RS32);
-        w.RS = RS32;
+        if ((rc = ws_get(c, wp, err)) || (rc = setup_precond<D, float>(c, K64, o, *wp, err))) { hipFree(y); hipFree(b32); return rc; }
+        KrylovWs<float> &w = *wp;
+        if (!w.RSbuf) PDX_HIP(hipMalloc((void **)&w.RSbuf, (size_t)c.nn * c.dd * 4));
+        k_block_rowsum<D, float><<<(c.nn + 255) / 256, 256, 0, c.stream>>>(c.nn, c.adj_off, K64, w.RSbuf);
+        w.RS = w.RSbuf;
         const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-4;
         double prev = INFINITY;
         int stalls = 0;
@@ -1246,14 +1384,14 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
             int it = 0;
             double rr;
             rc = o.krylov == PDX_KRYLOV_CG
-                     ? pcg<D, float, float>(c, K32, w, b32, rel * bnorm, inner, o.max_iter - total, &it, &rr, err)
-                     : bicgstab<D, float, float>(c, K32, w, b32, rel * bnorm, inner, o.max_iter - total, &it, &rr,
-                                                 err);
+                     ? pcg<D, float, float>(c, K32, w, b32, rel * bnorm, inner, o.max_iter - total, use_graph, &it,
+                                            &rr, err)
+                     : bicgstab<D, float, float>(c, K32, w, b32, rel * bnorm, inner, o.max_iter - total, use_graph,
+                                                 &it, &rr, err);
             total += it;
             if (rc) break;
             k_add_f2d<<<2048, 256, 0, c.stream>>>(n, w.x, x_dev);
         }
-        ws_free(w);
         hipFree(b32);
     }
     hipFree(y);
