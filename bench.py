@@ -50,6 +50,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def stress_bytes(m):
+    """Algorithmic bytes of one nodal-stress recovery: DOF values and coordinates read once,
+    the element connectivity, the bond strain weights and member ids of the families, the
+    node-to-family and node-to-element-corner maps, and the nodal stresses written.  The
+    element corner stresses staged between the element and node kernels are excluded."""
+    fam = m.families()
+    nmem, nfam = len(fam["mem"]), m.nfam
+    nfe = int((m.nodes()["algo"] == 2).sum())
+    nen = 8 if m.dim == 3 else 4
+    return (m.nn * m.dim * 8 + m.nn * 3 * 8 + nfe * nen * 4 + nmem * (m.dim * 8 + 4) + (nfam + 1) * 4
+            + m.nn * 4 + (m.nn + 1) * 4 + nfe * nen * 4 + m.nn * 6 * 8)
+
+
 def column_classes(m):
     """(classes, offsets in all classes) of the node rows grouped by the offsets of their
     block columns from the row index."""
@@ -277,6 +290,8 @@ def main():
     ap.add_argument("--no-newmark", action="store_true", help="skip the 100^3 penny-crack Newmark run")
     ap.add_argument("--fb-only", action="store_true",
                     help="only run the 2M-node family batch and print its figures (for kernel profiling)")
+    ap.add_argument("--stress-only", action="store_true",
+                    help="only time the nodal stress recovery of the box (for kernel profiling)")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only time a streaming read and a copy (practical HBM bandwidth) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -296,6 +311,16 @@ def main():
     sys.path.insert(0, PKG)
     import workloads as W
 
+    if args.stress_only:
+        model = pdx.Model(W.box3d(n=args.n))
+        cx = pdx.Context(model)
+        cx.assemble_timed()
+        uall = np.random.default_rng(0).uniform(-1e-6, 1e-6, model.nn * model.dim)
+        ms = [cx.node_stresses_timed(uall) for _ in range(max(args.steps, 1))]
+        sb = stress_bytes(model)
+        print(json.dumps({"stress_recovery": {"ms": min(ms), "alg_bytes": sb,
+                                              "GBps": sb / (min(ms) * 1e-3) / 1e9}}), flush=True)
+        return
     if args.fb_only:
         print(json.dumps({"family_batch": family_batch_bench(pdx, W)}), flush=True)
         return
@@ -325,6 +350,16 @@ def main():
     u64, _ = ctxs[0].explicit_run(0)
     u32, _ = ctxs[1].explicit_run(0)
     drift = float(np.linalg.norm(u32 - u64) / max(np.linalg.norm(u64), 1e-300))
+    # per-step nodal stress recovery (the reference recovers stresses after every explicit step,
+    # pdsolve.cpp:6062), timed on its own: PD strains from the bond weights K's assembly left,
+    # FE Gauss-point stresses extrapolated to the corners, node averages
+    uall = np.random.default_rng(0).uniform(-1e-6, 1e-6, model.nn * model.dim)
+    st_ms = min(cx.node_stresses_timed(uall) for _ in range(5))
+    sb = stress_bytes(model)
+    stress = {"workload": "nodal stresses of the same box (synthetic DOF values)", "ms": st_ms,
+              "alg_bytes": sb, "GBps": sb / (st_ms * 1e-3) / 1e9, "roofline_frac": sb / (st_ms * 1e-3) / HBM_PEAK,
+              "staged_bytes": int((model.nodes()["algo"] == 2).sum()) * 8 * 6 * 8 * 2}
+    log(f"stress recovery {st_ms * 1e3:.1f} us ({stress['GBps']:.0f} GB/s algorithmic)")
     nblk = model.nadj
     prec = 1
     lanes, nt, xm, soa, rel, tune_ms = ctxs[prec].explicit_tuning(prec)
@@ -352,6 +387,7 @@ def main():
                     "roofline_frac": step_bytes(model, nblk, 0, classes if ctxs[0].explicit_tuning(0)[4] else None)
                     / (res[0]["step_ms"] * 1e-3) / HBM_PEAK},
            "fp32_vs_fp64_rel": drift,
+           "stress_recovery": stress,
            "assembly_ms": min(asm_ms)}
     if not args.no_solve:
         sm = W.crack2d(seed=0)

@@ -424,8 +424,19 @@ __device__ inline void btdb_row2(const double (*g)[2], int nnod, int la, int ii,
 // Jacobian, its partial-pivot LU and dN/dX of all nodes into LDS, then one
 // thread per (element, Ke row) accumulates B^T D B w detJ over the Gauss
 // points in the reference loop order.
+// Staged element matrices keep the upper triangle of node blocks only (Ke is symmetric): the
+// d x d block of node pair a <= b of an element with at most NB = ND / d nodes is contiguous
+// at (pair * d + i) * d + j, pair = a NB - a (a - 1) / 2 + b - a, NP = NB (NB + 1) / 2 blocks
+// per element.  The block-owned gather reads a lower block as the transpose of its partner.
+__host__ __device__ constexpr int ke_pairs(int D, int ND) { return (ND / D) * (ND / D + 1) / 2; }
+__host__ __device__ inline int ke_pair(int NB, int a, int b) { return a * NB - a * (a - 1) / 2 + (b - a); }
+__device__ inline size_t ke_up(int D, int ND, int row, int col) {
+    return (size_t)ke_pair(ND / D, row / D, col / D) * (D * D) + (row % D) * D + col % D;
+}
+
 struct KeArgs {
     const double *x, *Dm;
+    double lam, mu;  // isotropic D: D_01 and D_33
     const int *econ, *nen, *fe_elem;
     const int *list;  // FE-list positions of the elements of this type
     int nlist;
@@ -433,43 +444,60 @@ struct KeArgs {
     int *status;
 };
 
+// dN/dX of the 8 nodes and detJ at Gauss point g of element list entry idx
+// (pdfemEleBrick8N::eleStiffMatFEM's order: r outer, q, p inner), into g8o / *wdet; the
+// parent derivatives are formed again per node for the solves instead of being kept live
+__device__ inline void hex8_gauss_point(const KeArgs &A, int idx, int g, double (*g8o)[3], double *wdet) {
+    const double gp = 0.57735026918962573;
+    const double gpt[2] = {-gp, gp};
+    const int e = A.fe_elem[A.list[idx]];
+    const double p = gpt[g & 1], qq = gpt[(g >> 1) & 1], r = gpt[g >> 2];
+    auto dN = [&](int i, double d[3]) {
+        d[0] = 1.0 / 8 * c_hp[i] * (1 + qq * c_hq[i]) * (1 + r * c_hr[i]);
+        d[1] = 1.0 / 8 * c_hq[i] * (1 + p * c_hp[i]) * (1 + r * c_hr[i]);
+        d[2] = 1.0 / 8 * c_hr[i] * (1 + p * c_hp[i]) * (1 + qq * c_hq[i]);
+    };
+    double J[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) J[i] = 0.0;
+    // every J entry sums over the nodes in node order, as the reference's cc-outer loop does
+#pragma unroll 2
+    for (int i = 0; i < 8; i++) {
+        const int j = A.econ[(size_t)e * 8 + i];
+        double d[3], xi[3];
+        dN(i, d);
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) xi[cc] = A.x[(size_t)j * 3 + cc];
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+            J[0 * 3 + cc] = d[0] * xi[cc] + J[0 * 3 + cc];
+            J[1 * 3 + cc] = d[1] * xi[cc] + J[1 * 3 + cc];
+            J[2 * 3 + cc] = d[2] * xi[cc] + J[2 * 3 + cc];
+        }
+    }
+    const double detJ = det3(J);
+    int piv[3];
+    if (!lu3_factor(J, piv)) atomicOr(A.status, 2);
+#pragma unroll 2
+    for (int b = 0; b < 8; b++) {
+        double d[3], o[3];
+        dN(b, d);
+        lu3_solve(J, piv, d, o);
+#pragma unroll
+        for (int c = 0; c < 3; c++) g8o[b][c] = o[c];
+    }
+    *wdet = 1.0 * 1.0 * 1.0 * detJ;
+}
+
 constexpr int HEX_EPB = 8;  // 8 elements x 24 rows = 192 threads
 __global__ void __launch_bounds__(HEX_EPB * 24) k_ke_hex8(KeArgs A) {
     __shared__ double sg[HEX_EPB][8][8][3];
     __shared__ double sw[HEX_EPB][8];
     const int t = threadIdx.x;
     const int e0 = blockIdx.x * HEX_EPB;
-    const double gp = 0.57735026918962573;
-    const double gpt[2] = {-gp, gp};
     if (t < HEX_EPB * 8) {
         const int el = t / 8, g = t % 8, idx = e0 + el;
-        if (idx < A.nlist) {
-            const int e = A.fe_elem[A.list[idx]];
-            double xN[8][3];
-            for (int b = 0; b < 8; b++)
-                for (int c = 0; c < 3; c++) xN[b][c] = A.x[(size_t)A.econ[(size_t)e * 8 + b] * 3 + c];
-            // Gauss point order of pdfemEleBrick8N::eleStiffMatFEM: r outer, q, p inner
-            const double p = gpt[g & 1], qq = gpt[(g >> 1) & 1], r = gpt[g >> 2];
-            double dN[8][3], J[9];
-            for (int i = 0; i < 8; i++) {
-                dN[i][0] = 1.0 / 8 * c_hp[i] * (1 + qq * c_hq[i]) * (1 + r * c_hr[i]);
-                dN[i][1] = 1.0 / 8 * c_hq[i] * (1 + p * c_hp[i]) * (1 + r * c_hr[i]);
-                dN[i][2] = 1.0 / 8 * c_hr[i] * (1 + p * c_hp[i]) * (1 + qq * c_hq[i]);
-            }
-            for (int i = 0; i < 9; i++) J[i] = 0.0;
-            for (int cc = 0; cc < 3; cc++)
-                for (int i = 0; i < 8; i++) {
-                    J[0 * 3 + cc] = dN[i][0] * xN[i][cc] + J[0 * 3 + cc];
-                    J[1 * 3 + cc] = dN[i][1] * xN[i][cc] + J[1 * 3 + cc];
-                    J[2 * 3 + cc] = dN[i][2] * xN[i][cc] + J[2 * 3 + cc];
-                }
-            const double detJ = det3(J);
-            double g8[8][3];
-            if (!lu3_solve8(J, dN, g8)) atomicOr(A.status, 2);
-            for (int b = 0; b < 8; b++)
-                for (int c = 0; c < 3; c++) sg[el][g][b][c] = g8[b][c];
-            sw[el][g] = 1.0 * 1.0 * 1.0 * detJ;
-        }
+        if (idx < A.nlist) hex8_gauss_point(A, idx, g, sg[el][g], &sw[el][g]);
     }
     __syncthreads();
     const int el = t / 24, row = t % 24, idx = e0 + el;
@@ -478,8 +506,55 @@ __global__ void __launch_bounds__(HEX_EPB * 24) k_ke_hex8(KeArgs A) {
     double Krow[24];
     for (int c = 0; c < 24; c++) Krow[c] = 0.0;
     for (int g = 0; g < 8; g++) btdb_row3(sg[el][g], 8, la, ii, A.Dm, sw[el][g], Krow);
-    double *out = A.S + (size_t)A.list[idx] * 576 + row * 24;
-    for (int c = 0; c < 24; c++) out[c] = Krow[c];
+    double *out = A.S + (size_t)A.list[idx] * (ke_pairs(3, 24) * 9);
+    for (int c = 3 * la; c < 24; c++) out[ke_up(3, 24, row, c)] = Krow[c];
+}
+
+// Hex8 with an isotropic D (the reference's D, pdsolve.cpp:37-94, always is): per Gauss point
+// the node block B_a^T D B_b w detJ = lam (w g_a) g_b^T + mu g_b (w g_a)^T + mu (w g_a . g_b) I,
+// so one thread per (element, node pair a <= b) accumulates its 3 x 3 block over the 8 Gauss
+// points (27 FMA each instead of the 324 of three full B^T D B rows) and writes the block and,
+// Ke being symmetric, its transpose.
+constexpr int HEXI_EPB = 16;  // 16 elements: 128 set-up threads, 16 x 36 = 576 block threads
+__global__ void __launch_bounds__(HEXI_EPB * 36) k_ke_hex8_iso(KeArgs A) {
+    __shared__ double sg[HEXI_EPB][8][8][3];
+    __shared__ double sw[HEXI_EPB][8];
+    const int t = threadIdx.x;
+    const int e0 = blockIdx.x * HEXI_EPB;
+    if (t < HEXI_EPB * 8) {
+        const int el = t / 8, g = t % 8, idx = e0 + el;
+        if (idx < A.nlist) hex8_gauss_point(A, idx, g, sg[el][g], &sw[el][g]);
+    }
+    __syncthreads();
+    const int el = t / 36, idx = e0 + el;
+    if (idx >= A.nlist) return;
+    int a = 0, rem = t % 36;
+    while (rem >= 8 - a) {
+        rem -= 8 - a;
+        a++;
+    }
+    const int b = a + rem;
+    const double lam = A.lam, mu = A.mu;
+    double k[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) k[e] = 0.0;
+#pragma unroll 2
+    for (int g = 0; g < 8; g++) {
+        const double w = sw[el][g];
+        const double wa[3] = {w * sg[el][g][a][0], w * sg[el][g][a][1], w * sg[el][g][a][2]};
+        const double gb[3] = {sg[el][g][b][0], sg[el][g][b][1], sg[el][g][b][2]};
+        const double dot = wa[0] * gb[0] + wa[1] * gb[1] + wa[2] * gb[2];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) k[i * 3 + j] += lam * wa[i] * gb[j] + mu * gb[i] * wa[j];
+#pragma unroll
+        for (int i = 0; i < 3; i++) k[i * 4] += mu * dot;
+    }
+    // pair index t % 36 enumerates a <= b in ke_pair order: one contiguous block per thread
+    double *out = A.S + (size_t)A.list[idx] * (ke_pairs(3, 24) * 9) + (size_t)(t % 36) * 9;
+#pragma unroll
+    for (int e = 0; e < 9; e++) out[e] = k[e];
 }
 
 constexpr int QUAD_EPB = 32;  // 32 elements x 8 rows = 256 threads
@@ -524,8 +599,8 @@ __global__ void __launch_bounds__(QUAD_EPB * 8) k_ke_quad4(KeArgs A) {
     double Krow[8];
     for (int c = 0; c < 8; c++) Krow[c] = 0.0;
     for (int g = 0; g < 4; g++) btdb_row2(sg[el][g], 4, la, ii, A.Dm, sw[el][g], Krow);
-    double *out = A.S + (size_t)A.list[idx] * 64 + row * 8;
-    for (int c = 0; c < 8; c++) out[c] = Krow[c];
+    double *out = A.S + (size_t)A.list[idx] * (ke_pairs(2, 8) * 4);
+    for (int c = 2 * la; c < 8; c++) out[ke_up(2, 8, row, c)] = Krow[c];
 }
 
 // tri3 / tet4: constant B, Ke = (Bt D B) V (pdfemEleTri3N.cpp:33-47, :185-210;
@@ -577,19 +652,20 @@ __global__ void k_ke_simplex(KeArgs A) {
         for (int i = 0; i < 3; i++) { g[i][0] = bb[i] / V2; g[i][1] = cc[i] / V2; }
         btdb_row2(g, 3, la, ii, A.Dm, 1.0, Krow);
     }
-    double *out = A.S + (size_t)A.list[idx] * ND * ND + row * ND;
-    for (int c = 0; c < NEN * D; c++) out[c] = Krow[c] * V;
+    double *out = A.S + (size_t)A.list[idx] * (ke_pairs(D, ND) * D * D);
+    for (int c = D * la; c < NEN * D; c++) out[ke_up(D, ND, row, c)] = Krow[c] * V;
 }
 
-// Block-owned add of the staged element rows: one thread per (K block, row) starts from the
+// Block-owned add of the staged element blocks: one thread per (K block, row) starts from the
 // block's PD part and adds the element terms in element order (the reference's summation order
 // of FE terms into K), then writes the row once.  The contributions of block (a, c) are the
-// rows of node a in the elements that hold both nodes; the threads of one row node read
-// neighbouring stretches of the staged rows, and K is written once, contiguously.
+// staged blocks of the elements that hold both nodes, listed as (staged block index << 1 |
+// transposed); the d row threads of a K block read one contiguous run of d*d values per
+// contribution, and K is written once, contiguously.
 template <int D>
 __global__ void k_fe_gather(long long nblk, const int *__restrict__ fec_off, const int *__restrict__ fec,
-                            const int *__restrict__ nfe_item, const double *__restrict__ S, double *K) {
-    constexpr int ND = D == 3 ? 24 : 8, DD = D * D;
+                            const unsigned char *__restrict__ pd_bits, const double *__restrict__ S, double *K) {
+    constexpr int DD = D * D;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= nblk * D) return;
     const long long q = t / D;
@@ -597,15 +673,21 @@ __global__ void k_fe_gather(long long nblk, const int *__restrict__ fec_off, con
     const int c0 = fec_off[q], c1 = fec_off[q + 1];
     if (c0 == c1) return;
     double *kb = K + (size_t)q * DD + ii * D;
+    // blocks without a PD part start from zero instead of reading K
+    const bool pd = (pd_bits[q >> 3] >> (q & 7)) & 1;
     double acc[D];
 #pragma unroll
-    for (int jj = 0; jj < D; jj++) acc[jj] = kb[jj];
+    for (int jj = 0; jj < D; jj++) acc[jj] = pd ? kb[jj] : 0.0;
     for (int c = c0; c < c1; c++) {
-        const int e = fec[c], it = nfe_item[e >> 3];
-        const int fp = it >> 3, la = it & 7, lb = e & 7;
-        const double *srow = S + (size_t)fp * ND * ND + (size_t)(D * la + ii) * ND + D * lb;
+        const int f = fec[c];
+        const double *blk = S + (size_t)(f >> 1) * DD;
+        if (f & 1) {
 #pragma unroll
-        for (int jj = 0; jj < D; jj++) acc[jj] = acc[jj] + srow[jj];
+            for (int jj = 0; jj < D; jj++) acc[jj] = acc[jj] + blk[jj * D + ii];
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < D; jj++) acc[jj] = acc[jj] + blk[ii * D + jj];
+        }
     }
 #pragma unroll
     for (int jj = 0; jj < D; jj++) kb[jj] = acc[jj];
@@ -766,7 +848,7 @@ int dev_assemble_K(Ctx &c, std::string &err) {
     int *status = nullptr;
     PDX_HIP(hipMallocAsync((void **)&status, sizeof(int), c.stream));
     PDX_HIP(hipMemsetAsync(status, 0, sizeof(int), c.stream));
-    if (dev_fill(c, c.K, 0.0, (size_t)c.dd * c.nblk)) { err = "fill failed"; return PDX_ERR_HIP; }
+    if (c.k_needs_fill && dev_fill(c, c.K, 0.0, (size_t)c.dd * c.nblk)) { err = "fill failed"; return PDX_ERR_HIP; }
     if (c.nfam > 0) {
         FamArgs A;
         A.x = c.x; A.nv = c.nv; A.fam_node = c.fam_node; A.fam_off = c.fam_off; A.fam_mem = c.fam_mem;
@@ -798,20 +880,39 @@ int dev_assemble_K(Ctx &c, std::string &err) {
     }
     if (c.nfe > 0) {
         const int ND = c.dim == 3 ? 24 : 8;
-        if (!c.Ke_stage) PDX_HIP(hipMalloc((void **)&c.Ke_stage, (size_t)c.nfe * ND * ND * sizeof(double)));
+        if (!c.Ke_stage)
+            PDX_HIP(hipMalloc((void **)&c.Ke_stage, (size_t)c.nfe * ke_pairs(c.dim, ND) * c.dd * sizeof(double)));
         KeArgs A;
         A.x = c.x; A.Dm = c.Dm; A.econ = c.econ; A.nen = c.nen; A.fe_elem = c.fe_elem; A.S = c.Ke_stage;
         A.status = status;
+        // isotropic 3D D (lam + 2 mu on the normal diagonal, lam off it, mu on the shear diagonal)
+        bool iso = false;
+        A.lam = A.mu = 0.0;
+        if (c.dim == 3 && c.host) {
+            const double *Dh = c.host->D;
+            const double lam = Dh[1], mu = Dh[21], tol = 1e-14 * (std::fabs(lam) + std::fabs(mu));
+            iso = true;
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++) {
+                    const double want = i < 3 && j < 3 ? lam + (i == j ? 2 * mu : 0.0) : (i == j ? mu : 0.0);
+                    if (std::fabs(Dh[i * 6 + j] - want) > tol) iso = false;
+                }
+            A.lam = lam;
+            A.mu = mu;
+        }
         if (c.dim == 3) {
             if (c.n_fe_a) {
                 A.list = c.fe_list_a; A.nlist = c.n_fe_a;
-                k_ke_hex8<<<(c.n_fe_a + HEX_EPB - 1) / HEX_EPB, HEX_EPB * 24, 0, c.stream>>>(A);
+                if (iso)
+                    k_ke_hex8_iso<<<(c.n_fe_a + HEXI_EPB - 1) / HEXI_EPB, HEXI_EPB * 36, 0, c.stream>>>(A);
+                else
+                    k_ke_hex8<<<(c.n_fe_a + HEX_EPB - 1) / HEX_EPB, HEX_EPB * 24, 0, c.stream>>>(A);
             }
             if (c.n_fe_b) {
                 A.list = c.fe_list_b; A.nlist = c.n_fe_b;
                 k_ke_simplex<3><<<(c.n_fe_b * 12 + 127) / 128, 128, 0, c.stream>>>(A);
             }
-            k_fe_gather<3><<<(unsigned)((c.nblk * 3 + 255) / 256), 256, 0, c.stream>>>(c.nblk, c.fec_off, c.fec, c.nfe_item,
+            k_fe_gather<3><<<(unsigned)((c.nblk * 3 + 255) / 256), 256, 0, c.stream>>>(c.nblk, c.fec_off, c.fec, c.pd_bits,
                                                                                       c.Ke_stage, c.K);
         } else {
             if (c.n_fe_a) {
@@ -822,7 +923,7 @@ int dev_assemble_K(Ctx &c, std::string &err) {
                 A.list = c.fe_list_b; A.nlist = c.n_fe_b;
                 k_ke_simplex<2><<<(c.n_fe_b * 6 + 127) / 128, 128, 0, c.stream>>>(A);
             }
-            k_fe_gather<2><<<(unsigned)((c.nblk * 2 + 255) / 256), 256, 0, c.stream>>>(c.nblk, c.fec_off, c.fec, c.nfe_item,
+            k_fe_gather<2><<<(unsigned)((c.nblk * 2 + 255) / 256), 256, 0, c.stream>>>(c.nblk, c.fec_off, c.fec, c.pd_bits,
                                                                                       c.Ke_stage, c.K);
         }
         PDX_HIP(hipGetLastError());

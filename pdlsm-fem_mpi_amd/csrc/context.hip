@@ -103,6 +103,10 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
     UP(c->fam_mem, m.fam_mem);
     UP(c->bond, m.bond);
     UP(c->fam_blk, fam_blk);
+    // K blocks the family kernel stores (every member block of every family, broken bonds included)
+    std::vector<unsigned char> pd_bits(((size_t)c->nblk + 7) / 8, 0);
+    for (int b : fam_blk) pd_bits[(size_t)b >> 3] |= (unsigned char)(1u << (b & 7));
+    UP(c->pd_bits, pd_bits);
     UP(c->fam_v, m.fam_v);
     UP(c->fam_delta, m.fam_delta);
     // PD-boundary entities incident to each family's node, in entity order
@@ -160,8 +164,7 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
         UP(c->fe_list_b, lb);
         UP(c->nfe_off, cnt);
         UP(c->nfe_item, item);
-        // per K block, its FE contributions (item of the block's row node, column node's local
-        // index) in element order, for the block-owned gather
+        // per K block, its FE contributions in element order, for the block-owned gather
         {
             std::vector<int> fcnt((size_t)c->nblk + 1, 0);
             for (int k = 0; k < m.nn; k++)
@@ -170,14 +173,31 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
                     for (int b = 0; b < m.nen[m.fe[t]]; b++) fcnt[(size_t)fe_blk[(size_t)t * 64 + a * 8 + b] + 1]++;
                 }
             for (long long q = 0; q < c->nblk; q++) fcnt[q + 1] += fcnt[q];
+            // contributions as staged block indices (FE-list position, local row node, local column node)
+            const int NB = m.dim == 3 ? 8 : 4, NP = NB * (NB + 1) / 2;
+            if ((long long)m.fe.size() * NP * 2 > 0x7fffffffLL) {
+                err = "too many FE elements for 32-bit staged block indices";
+                return PDX_ERR_LIMIT;
+            }
             std::vector<int> fec(std::max(fcnt[c->nblk], 1)), ffill(fcnt.begin(), fcnt.end() - 1);
             for (int k = 0; k < m.nn; k++)
                 for (int q = cnt[k]; q < cnt[k + 1]; q++) {
                     const int t = item[q] >> 3, a = item[q] & 7;
-                    for (int b = 0; b < m.nen[m.fe[t]]; b++) fec[ffill[fe_blk[(size_t)t * 64 + a * 8 + b]]++] = (q << 3) | b;
+                    for (int b = 0; b < m.nen[m.fe[t]]; b++) {
+                        // upper staged block (min, max) of the node pair; transposed when a > b
+                        const int lo = std::min(a, b), hi = std::max(a, b);
+                        const int pair = lo * NB - lo * (lo - 1) / 2 + (hi - lo);
+                        fec[ffill[fe_blk[(size_t)t * 64 + a * 8 + b]]++] = ((t * NP + pair) << 1) | (a > b ? 1 : 0);
+                    }
                 }
             UP(c->fec_off, fcnt);
             UP(c->fec, fec);
+            // a block neither stored by a family nor gathered from elements keeps the fill's zero;
+            // without such blocks the assembly needs no fill of K
+            long long bare = 0;
+            for (long long q = 0; q < c->nblk; q++)
+                if (!((pd_bits[(size_t)q >> 3] >> (q & 7)) & 1) && fcnt[q + 1] == fcnt[q]) bare++;
+            c->k_needs_fill = bare > 0;
         }
         // all elements (lumped mass)
         std::vector<int> acnt(m.nn + 1, 0);
@@ -286,6 +306,8 @@ void ctx_free(Ctx *c) {
     if (c->learned_Minv) hipFree(c->learned_Minv);
     if (c->fam_c) hipFree(c->fam_c);
     if (c->stress_stage) hipFree(c->stress_stage);
+    if (c->stress_famid) hipFree(c->stress_famid);
+    if (c->stress_status) hipFree(c->stress_status);
     free_two_level(*c);
     free_ilu0(*c);
     free_krylov_ws(*c);
@@ -624,9 +646,10 @@ int pdx_node_stresses_timed(pdx_ctx *pc, const double *u_all, double *ms) {
         PDX_HIP(hipEventCreate(&e0));
         PDX_HIP(hipEventCreate(&e1));
         PDX_HIP(hipEventRecord(e0, c.stream));
-        rc = dev_node_stresses(c, du, ds, err);
+        rc = dev_node_stresses(c, du, ds, err, false);
         PDX_HIP(hipEventRecord(e1, c.stream));
         PDX_HIP(hipEventSynchronize(e1));
+        if (!rc) rc = stress_status_check(c, err);
         float t = 0;
         PDX_HIP(hipEventElapsedTime(&t, e0, e1));
         hipEventDestroy(e0);

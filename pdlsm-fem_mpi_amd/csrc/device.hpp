@@ -61,7 +61,9 @@ struct Ctx {
     int *fp_off = nullptr, *fp_list = nullptr, *pdbe = nullptr;
     int nfe = 0;
     int *fe_elem = nullptr, *fe_blk = nullptr, *nfe_off = nullptr, *nfe_item = nullptr;
-    int *fec_off = nullptr, *fec = nullptr;  // per K block: FE contributions (item << 3 | local column node)
+    int *fec_off = nullptr, *fec = nullptr;  // per K block: FE contributions (staged block << 1 | transposed)
+    unsigned char *pd_bits = nullptr;        // per K block: 1 if a family kernel stores it
+    bool k_needs_fill = true;                // some K block is written by neither assembly kernel
     int n_fe_a = 0, n_fe_b = 0;  // FE-list positions of hex8/quad4 (a) and tet4/tri3 (b) elements
     int *fe_list_a = nullptr, *fe_list_b = nullptr;
     double *Ke_stage = nullptr;  // element stiffness staging [nfe][ND*ND]
@@ -126,6 +128,8 @@ struct Ctx {
     long long learned_gen = -1;      // K_gen the smoother was predicted for
     void *gnn_work = nullptr;        // graph-network inputs and activations (learned.hip)
     double *stress_stage = nullptr;  // [nfe*8*6 + nfam*6] element corner stresses, family stresses
+    int *stress_famid = nullptr;     // [nn] family id of every node (-1: none)
+    int *stress_status = nullptr;    // singular-Jacobian flag of the stress recovery
 
     // helpers
     std::vector<void *> owned;
@@ -164,7 +168,9 @@ double ilu0_setup_ms(const void *h);
 void free_ilu0(Ctx &c);
 void free_krylov_ws(Ctx &c);
 void free_gnn_work(Ctx &c);
-int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err);
+// nodal stresses of u_dev; check: read the singular-Jacobian flag (synchronises the stream)
+int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err, bool check = true);
+int stress_status_check(Ctx &c, std::string &err);
 int dev_bond_sigma1(Ctx &c, const int *bond_dev, const double *sigma_dev, double *s1_dev, std::string &err);
 double two_level_setup_ms(Ctx &c);
 int set_error(int rc, const std::string &msg);

@@ -24,32 +24,24 @@
 namespace pdx {
 
 // ---- PD family nodes ------------------------------------------------------
+// One wavefront per family: the lanes stride over the members m >= 1 accumulating the strain
+// terms C_m u_m and the weight sum; after a wave reduction the self block C_0 = -sum_m C_m
+// (calPDNodeStresses, pdsolve.cpp:9081-9166) adds C_0 u_self, and sigma = D eps.
 template <int D>
-__global__ void k_stress_pd(int nfam, const int *fam_off, const int *fam_mem, const double *fam_c, const double *u,
-                            const double *Dm, double *sig_fam) {
+__global__ void __launch_bounds__(256) k_stress_pd(int nfam, const int *fam_off, const int *fam_mem,
+                                                   const double *fam_c, const double *u, const double *Dm,
+                                                   double *sig_fam) {
 #pragma clang fp contract(off)
     constexpr int NV = D == 3 ? 6 : 3;
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nfam) return;
+    const int f = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (f >= nfam) return;  // whole waves leave together
     const int off = fam_off[f], n = fam_off[f + 1] - off;
-    // block 0 of C: the negative sums of the member weights, in member order
-    double c0[D];
-#pragma unroll
-    for (int i = 0; i < D; i++) c0[i] = 0.0;
-    for (int m = 1; m < n; m++)
-#pragma unroll
-        for (int i = 0; i < D; i++) c0[i] = -fam_c[(size_t)(off + m) * D + i] + c0[i];
-    double eps[NV];
+    double eps[NV], cs[D];
 #pragma unroll
     for (int r = 0; r < NV; r++) eps[r] = 0.0;
-    for (int m = 0; m < n; m++) {
-        const int j = fam_mem[off + m];
-        double c[D], uj[D];
 #pragma unroll
-        for (int i = 0; i < D; i++) {
-            c[i] = m == 0 ? c0[i] : fam_c[(size_t)(off + m) * D + i];
-            uj[i] = u[(size_t)j * D + i];
-        }
+    for (int i = 0; i < D; i++) cs[i] = 0.0;
+    auto add = [&](const double *c, const double *uj) {
         // row r of C times the member's DOFs, DOF-minor (zero entries skipped)
         if constexpr (D == 2) {
             eps[0] = c[0] * uj[0] + eps[0];
@@ -67,14 +59,40 @@ __global__ void k_stress_pd(int nfam, const int *fam_off, const int *fam_mem, co
             eps[5] = c[2] * uj[0] + eps[5];
             eps[5] = c[0] * uj[2] + eps[5];
         }
-    }
-    double *out = sig_fam + (size_t)f * 6;
+    };
+    for (int m = 1 + lane; m < n; m += 64) {
+        const int j = fam_mem[off + m];
+        double c[D], uj[D];
 #pragma unroll
-    for (int i = 0; i < NV; i++) {
+        for (int i = 0; i < D; i++) {
+            c[i] = fam_c[(size_t)(off + m) * D + i];
+            uj[i] = u[(size_t)j * D + i];
+            cs[i] = cs[i] + c[i];
+        }
+        add(c, uj);
+    }
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+#pragma unroll
+        for (int r = 0; r < NV; r++) eps[r] += __shfl_xor(eps[r], s, 64);
+#pragma unroll
+        for (int i = 0; i < D; i++) cs[i] += __shfl_xor(cs[i], s, 64);
+    }
+    {
+        const int j = fam_mem[off];
+        double c[D], uj[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            c[i] = -cs[i];
+            uj[i] = u[(size_t)j * D + i];
+        }
+        add(c, uj);
+    }
+    if (lane < NV) {
         double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < NV; k++) t = Dm[i * NV + k] * eps[k] + t;
-        out[i] = t;
+        for (int k = 0; k < NV; k++) t = Dm[lane * NV + k] * eps[k] + t;
+        sig_fam[(size_t)f * 6 + lane] = t;
     }
 }
 
@@ -85,6 +103,8 @@ __constant__ double c_shr[8] = {-1., -1., -1., -1., 1., 1., 1., 1.};
 
 struct StressFeArgs {
     const double *x, *u, *Dm;
+    double lam, mu;  // isotropic 3D D: D_01 and D_33 (iso != 0)
+    int iso;
     const int *econ, *nen, *fe_elem, *list;
     int nlist;
     double L8[64], L4[16];
@@ -92,59 +112,98 @@ struct StressFeArgs {
     int *status;
 };
 
-// hex8: 32 elements per block; phase 1 one thread per (element, Gauss point)
-// (pdfemEleBrick8N::eleFitStresses points p[i], q[i], r[i], :136-151), phase 2
-// one thread per (element, corner): Nsigma = L sigma_gp
+// hex8: 32 elements per block.  Phase 0, one thread per (element, node): the node's
+// coordinates and DOF values into LDS.  Phase 1, one thread per (element, Gauss point)
+// (pdfemEleBrick8N::eleFitStresses points p[i], q[i], r[i], :136-151): J = sum_i x_i dN_i^T
+// and H = sum_i u_i dN_i^T over the parent derivatives, the displacement gradient from
+// J^T grad(u) = H (one partial-pivot LU, three right-hand sides, instead of a solve per
+// shape function), the engineering strains and sigma = D eps (lam tr(eps) + 2 mu eps,
+// mu gamma for an isotropic D).  Phase 2, one thread per (element, corner):
+// Nsigma = L sigma_gp.
 constexpr int SH_EPB = 32;
 __global__ void __launch_bounds__(SH_EPB * 8) k_stress_hex8(StressFeArgs A) {
 #pragma clang fp contract(off)
+    __shared__ double sxu[SH_EPB][8][6];
     __shared__ double sgp[SH_EPB][8][6];
     const int el = threadIdx.x / 8, g = threadIdx.x % 8, idx = blockIdx.x * SH_EPB + el;
     const double a = 1.0 / sqrt(3.0);
     if (idx < A.nlist) {
         const int e = A.fe_elem[A.list[idx]];
-        double xN[8][3], Ue[24];
-        for (int b = 0; b < 8; b++) {
-            const int j = A.econ[(size_t)e * 8 + b];
-            for (int c = 0; c < 3; c++) {
-                xN[b][c] = A.x[(size_t)j * 3 + c];
-                Ue[3 * b + c] = A.u[(size_t)j * 3 + c];
-            }
+        const int j = A.econ[(size_t)e * 8 + g];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            sxu[el][g][c] = A.x[(size_t)j * 3 + c];
+            sxu[el][g][3 + c] = A.u[(size_t)j * 3 + c];
         }
+    }
+    __syncthreads();
+    if (idx < A.nlist) {
         const double p = a * c_shp[g], q = a * c_shq[g], r = a * c_shr[g];
-        double dN[8][3], J[9];
+        // M = [J | H]: row k (parent direction), columns x, y, z then ux, uy, uz
+        double M[3][6];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int c = 0; c < 6; c++) M[k][c] = 0.0;
+#pragma unroll 2
         for (int i = 0; i < 8; i++) {
-            dN[i][0] = 1.0 / 8 * c_shp[i] * (1 + q * c_shq[i]) * (1 + r * c_shr[i]);
-            dN[i][1] = 1.0 / 8 * c_shq[i] * (1 + p * c_shp[i]) * (1 + r * c_shr[i]);
-            dN[i][2] = 1.0 / 8 * c_shr[i] * (1 + p * c_shp[i]) * (1 + q * c_shq[i]);
-        }
-        for (int i = 0; i < 9; i++) J[i] = 0.0;
-        for (int cc = 0; cc < 3; cc++)
-            for (int i = 0; i < 8; i++) {
-                J[0 * 3 + cc] = dN[i][0] * xN[i][cc] + J[0 * 3 + cc];
-                J[1 * 3 + cc] = dN[i][1] * xN[i][cc] + J[1 * 3 + cc];
-                J[2 * 3 + cc] = dN[i][2] * xN[i][cc] + J[2 * 3 + cc];
+            const double d0 = 1.0 / 8 * c_shp[i] * (1 + q * c_shq[i]) * (1 + r * c_shr[i]);
+            const double d1 = 1.0 / 8 * c_shq[i] * (1 + p * c_shp[i]) * (1 + r * c_shr[i]);
+            const double d2 = 1.0 / 8 * c_shr[i] * (1 + p * c_shp[i]) * (1 + q * c_shq[i]);
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                const double v = sxu[el][i][c];
+                M[0][c] = d0 * v + M[0][c];
+                M[1][c] = d1 * v + M[1][c];
+                M[2][c] = d2 * v + M[2][c];
             }
-        double gr[8][3];
-        if (!lu3_solve8(J, dN, gr)) atomicOr(A.status, 2);
-        // eps = B Ue over the 24 columns (node-major, DOF-minor; zeros skipped)
-        double eps[6] = {0, 0, 0, 0, 0, 0};
-        for (int i = 0; i < 8; i++) {
-            const double ux = Ue[3 * i], uy = Ue[3 * i + 1], uz = Ue[3 * i + 2];
-            eps[0] = gr[i][0] * ux + eps[0];
-            eps[1] = gr[i][1] * uy + eps[1];
-            eps[2] = gr[i][2] * uz + eps[2];
-            eps[3] = gr[i][1] * ux + eps[3];
-            eps[3] = gr[i][0] * uy + eps[3];
-            eps[4] = gr[i][2] * uy + eps[4];
-            eps[4] = gr[i][1] * uz + eps[4];
-            eps[5] = gr[i][2] * ux + eps[5];
-            eps[5] = gr[i][0] * uz + eps[5];
         }
-        for (int i = 0; i < 6; i++) {
-            double t = 0.0;
-            for (int k = 0; k < 6; k++) t = A.Dm[i * 6 + k] * eps[k] + t;
-            sgp[el][g][i] = t;
+        // J G = H with G[d][c] = d u_c / d x_d: partial-pivot elimination on [J | H]
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            int pv = k;
+            double mx = fabs(M[k][k]);
+#pragma unroll
+            for (int i = k + 1; i < 3; i++)
+                if (fabs(M[i][k]) > mx) { mx = fabs(M[i][k]); pv = i; }
+#pragma unroll
+            for (int i = k + 1; i < 3; i++)
+                if (pv == i)
+#pragma unroll
+                    for (int c = 0; c < 6; c++) { const double t = M[k][c]; M[k][c] = M[i][c]; M[i][c] = t; }
+            if (M[k][k] == 0.0) ok = false;
+#pragma unroll
+            for (int i = k + 1; i < 3; i++) {
+                const double l = M[i][k] / M[k][k];
+#pragma unroll
+                for (int c = k; c < 6; c++) M[i][c] = M[i][c] - l * M[k][c];
+            }
+        }
+        if (!ok) atomicOr(A.status, 2);
+        double G[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int i = 2; i >= 0; i--) {
+                double t = M[i][3 + c];
+#pragma unroll
+                for (int k = i + 1; k < 3; k++) t = t - M[i][k] * G[k][c];
+                G[i][c] = t / M[i][i];
+            }
+        const double eps[6] = {G[0][0], G[1][1], G[2][2], G[1][0] + G[0][1], G[2][1] + G[1][2], G[2][0] + G[0][2]};
+        if (A.iso) {
+            const double lt = A.lam * (eps[0] + eps[1] + eps[2]);
+#pragma unroll
+            for (int i = 0; i < 3; i++) sgp[el][g][i] = lt + 2.0 * A.mu * eps[i];
+#pragma unroll
+            for (int i = 3; i < 6; i++) sgp[el][g][i] = A.mu * eps[i];
+        } else {
+            for (int i = 0; i < 6; i++) {
+                double t = 0.0;
+                for (int k = 0; k < 6; k++) t = A.Dm[i * 6 + k] * eps[k] + t;
+                sgp[el][g][i] = t;
+            }
         }
     }
     __syncthreads();
@@ -318,17 +377,34 @@ __global__ void k_stress_nodes(int nn, const int *famid, const double *sig_fam, 
             for (int i = 0; i < 6; i++) s[i] = sig_fam[(size_t)f * 6 + i];
         }
     }
-    for (int q = nfe_off[a]; q < nfe_off[a + 1]; q++) {
-        const int item = nfe_item[q];
-        const double *v = stage + ((size_t)(item >> 3) * 8 + (item & 7)) * 6;
-        if constexpr (D == 2) {
-            s[0] = s[0] + v[0];
-            s[1] = s[1] + v[1];
-            s[3] = s[3] + v[3];
-        } else {
-            for (int i = 0; i < 6; i++) s[i] = s[i] + v[i];
+    // incident element corners in FE order, four at a time: all loads of a batch are issued
+    // before the batch is summed (same order of the sum)
+    const int q1 = nfe_off[a + 1];
+    for (int q0 = nfe_off[a]; q0 < q1; q0 += 4) {
+        double v[4][6];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (q0 + k < q1) {
+                const int item = nfe_item[q0 + k];
+                const double *src = stage + ((size_t)(item >> 3) * 8 + (item & 7)) * 6;
+#pragma unroll
+                for (int i = 0; i < 6; i++) v[k][i] = src[i];
+            }
         }
-        count++;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (q0 + k < q1) {
+                if constexpr (D == 2) {
+                    s[0] = s[0] + v[k][0];
+                    s[1] = s[1] + v[k][1];
+                    s[3] = s[3] + v[k][3];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 6; i++) s[i] = s[i] + v[k][i];
+                }
+                count++;
+            }
+        }
     }
     if (count > 0)
         for (int i = 0; i < 6; i++) s[i] = s[i] / count;
@@ -372,7 +448,16 @@ int dev_bond_sigma1(Ctx &c, const int *bond_dev, const double *sigma_dev, double
     return PDX_OK;
 }
 
-int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err) {
+// the singular-Jacobian flag of the last recovery (synchronises the stream)
+int stress_status_check(Ctx &c, std::string &err) {
+    int hs = 0;
+    PDX_HIP(hipMemcpyAsync(&hs, c.stress_status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PDX_HIP(hipStreamSynchronize(c.stream));
+    if (hs) { err = "singular element Jacobian in the stress recovery"; return PDX_ERR_SINGULAR; }
+    return PDX_OK;
+}
+
+int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err, bool check) {
     if (!c.K_valid) {
         int rc = dev_assemble_K(c, err);  // the family kernel leaves the bond strain weights
         if (rc) return rc;
@@ -383,19 +468,40 @@ int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::strin
         PDX_HIP(hipMalloc((void **)&c.stress_stage, ((size_t)std::max(c.nfe, 1) * 48 + (size_t)std::max(c.nfam, 1) * 6) *
                                                         sizeof(double)));
     double *stage = c.stress_stage, *sig_fam = c.stress_stage + (size_t)std::max(c.nfe, 1) * 48;
-    int *status;
-    PDX_HIP(hipMalloc((void **)&status, sizeof(int)));
+    if (!c.stress_famid) {
+        // node -> family id, and the singular-Jacobian flag, kept on the device for per-step use
+        PDX_HIP(hipMalloc((void **)&c.stress_famid, (size_t)std::max(c.nn, 1) * sizeof(int)));
+        PDX_HIP(hipMalloc((void **)&c.stress_status, sizeof(int)));
+        std::vector<int> h(c.nn);
+        for (int k = 0; k < c.nn; k++) h[k] = m.famid[k];
+        PDX_HIP(hipMemcpy(c.stress_famid, h.data(), (size_t)c.nn * sizeof(int), hipMemcpyHostToDevice));
+    }
+    int *status = c.stress_status, *famid = c.stress_famid;
     PDX_HIP(hipMemsetAsync(status, 0, sizeof(int), c.stream));
     if (c.nfam > 0) {
-        if (D == 3) k_stress_pd<3><<<(c.nfam + 127) / 128, 128, 0, c.stream>>>(c.nfam, c.fam_off, c.fam_mem, c.fam_c,
-                                                                             u_dev, c.Dm, sig_fam);
-        else k_stress_pd<2><<<(c.nfam + 127) / 128, 128, 0, c.stream>>>(c.nfam, c.fam_off, c.fam_mem, c.fam_c, u_dev,
-                                                                       c.Dm, sig_fam);
+        if (D == 3) k_stress_pd<3><<<(c.nfam + 3) / 4, 256, 0, c.stream>>>(c.nfam, c.fam_off, c.fam_mem, c.fam_c,
+                                                                           u_dev, c.Dm, sig_fam);
+        else k_stress_pd<2><<<(c.nfam + 3) / 4, 256, 0, c.stream>>>(c.nfam, c.fam_off, c.fam_mem, c.fam_c, u_dev,
+                                                                     c.Dm, sig_fam);
     }
     if (c.nfe > 0) {
         StressFeArgs A;
         A.x = c.x; A.u = u_dev; A.Dm = c.Dm; A.econ = c.econ; A.nen = c.nen; A.fe_elem = c.fe_elem;
         A.stage = stage; A.status = status;
+        A.iso = 0;
+        A.lam = A.mu = 0.0;
+        if (D == 3) {
+            const double *Dh = m.D;
+            const double lam = Dh[1], mu = Dh[21], tol = 1e-14 * (std::fabs(lam) + std::fabs(mu));
+            A.iso = 1;
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++) {
+                    const double want = i < 3 && j < 3 ? lam + (i == j ? 2 * mu : 0.0) : (i == j ? mu : 0.0);
+                    if (std::fabs(Dh[i * 6 + j] - want) > tol) A.iso = 0;
+                }
+            A.lam = lam;
+            A.mu = mu;
+        }
         // extrapolation matrices L (pdsolve.cpp:9176-9196, :9254-9277)
         {
             const double pI[8] = {-1.0, 1., 1., -1., -1., 1., 1., -1.}, qI[8] = {-1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0},
@@ -422,26 +528,15 @@ int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::strin
             else k_stress_simplex<2><<<(c.n_fe_b + 127) / 128, 128, 0, c.stream>>>(A);
         }
     }
-    int *famid;
-    PDX_HIP(hipMalloc((void **)&famid, (size_t)c.nn * sizeof(int)));
     {
-        std::vector<int> h(c.nn);
-        for (int k = 0; k < c.nn; k++) h[k] = m.famid[k];
-        PDX_HIP(hipMemcpyAsync(famid, h.data(), (size_t)c.nn * sizeof(int), hipMemcpyHostToDevice, c.stream));
         const int nb = (c.nn + 255) / 256;
         if (D == 3) k_stress_nodes<3><<<nb, 256, 0, c.stream>>>(c.nn, famid, sig_fam, c.nfe_off, c.nfe_item, stage, m.nu,
                                                                0, sigma_dev);
         else k_stress_nodes<2><<<nb, 256, 0, c.stream>>>(c.nn, famid, sig_fam, c.nfe_off, c.nfe_item, stage, m.nu,
                                                          m.pro_type == 2, sigma_dev);
         PDX_HIP(hipGetLastError());
-        int hs = 0;
-        PDX_HIP(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        PDX_HIP(hipStreamSynchronize(c.stream));
-        hipFree(famid);
-        hipFree(status);
-        if (hs) { err = "singular element Jacobian in the stress recovery"; return PDX_ERR_SINGULAR; }
     }
-    return PDX_OK;
+    return check ? stress_status_check(c, err) : PDX_OK;
 }
 
 }  // namespace pdx
