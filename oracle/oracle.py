@@ -29,10 +29,12 @@ OM_ERRORS = {-1: "bad argument", -2: "singular matrix", -3: "CSR slot missing", 
 
 def build(force: bool = False) -> str:
     """Compile the C restatement with gcc (no FMA contraction, like the
-    reference's x86-64 -O2 build which has no FMA instructions)."""
+    reference's x86-64 -O2 build which has no FMA instructions; OpenMP only for the
+    opt-in threaded explicit force of the CPU baseline)."""
     src = os.path.join(_HERE, "pdlsm_oracle.c")
     if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(_LIB_PATH) < os.path.getmtime(src):
-        subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", _LIB_PATH, src, "-lm"])
+        subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-o", _LIB_PATH,
+                               src, "-lm"])
     return _LIB_PATH
 
 
@@ -88,6 +90,8 @@ def lib():
         L.om_local_damage.argtypes = [P, P]
         L.om_fc2_step.restype = I
         L.om_fc2_step.argtypes = [P, P, D, I, P]
+        L.om_set_threads.restype = None
+        L.om_set_threads.argtypes = [I]
         L.om_greedy_colors.restype = I
         L.om_greedy_colors.argtypes = [I, P, P, P]
         L.om_bilu0.restype = I

@@ -1416,9 +1416,9 @@ static void scatter(const om_model *M, scat_ctx *S, int row, int node_c, int j, 
 }
 
 /* pdsolve::assembleInterWorkPD_CSRformat (pdsolve.cpp:2524-2730) */
-static int assemble_pd(const om_model *M, scat_ctx *S) {
+static int assemble_pd_range(const om_model *M, scat_ctx *S, int f0, int f1) {
     int d = M->dim;
-    for (int f = 0; f < M->nfam; f++) {
+    for (int f = f0; f < f1; f++) {
         int n = M->fam_off[f + 1] - M->fam_off[f], w = d * n;
         int k = M->fam_node[f];
         double dvk = M->fam_v[f];
@@ -1469,9 +1469,9 @@ static int norm_mat_area(const int con[4], double xN[4][3], double *N /*3x6*/) {
 
 /* pdsolve::assemblePDBEwork_CSRformat 2D part (pdsolve.cpp:4049-4253) and
  * assemblePDBEwork3D_CSRformat (:2732-2915) */
-static int assemble_pdbe(const om_model *M, scat_ctx *S) {
+static int assemble_pdbe_range(const om_model *M, scat_ctx *S, int p0, int p1) {
     int d = M->dim;
-    for (int p = 0; p < M->npdbe; p++) {
+    for (int p = p0; p < p1; p++) {
         const int *con = &M->pdbe[p * 4];
         if (d == 2) {
             int a = con[0] - 1, b = con[1] - 1;
@@ -1529,10 +1529,10 @@ static int assemble_pdbe(const om_model *M, scat_ctx *S) {
 }
 
 /* pdsolve::assembleSEDbyFEM_CSRformat (pdsolve.cpp:4882-5007) */
-static int assemble_fe(const om_model *M, scat_ctx *S) {
+static int assemble_fe_range(const om_model *M, scat_ctx *S, int t0, int t1) {
     int d = M->dim;
     double Ke[576];
-    for (int t = 0; t < M->nfe; t++) {
+    for (int t = t0; t < t1; t++) {
         int e = M->fe[t], n = M->nen[e], nd;
         int r = ele_stiff(M, e, Ke, &nd);
         if (r) return r;
@@ -1628,15 +1628,59 @@ static void external_force(const om_model *M, double *F) {
 static int g_skip = 0;
 void om_set_skip(int flags) { g_skip = flags; }
 
+/* Threads of the explicit internal force (om_set_threads; 1 = the serial restatement).
+ * With P > 1 threads the explicit force (no K) is computed as the reference's P MPI ranks
+ * do (pdsolve.cpp:2529-2531, :4887-4889, :5831-5840): thread r takes the contiguous slices
+ * [r N / P, (r + 1) N / P) of the families, the PD-boundary entities and the FE elements
+ * into a private F, and the P vectors are summed in rank order (the MPI_Allreduce).  Used
+ * for the timed CPU baseline on all cores; the checks use the serial path. */
+static int g_threads = 1;
+void om_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
+static int assemble_pd(const om_model *M, scat_ctx *S) { return assemble_pd_range(M, S, 0, M->nfam); }
+static int assemble_pdbe(const om_model *M, scat_ctx *S) { return assemble_pdbe_range(M, S, 0, M->npdbe); }
+static int assemble_fe(const om_model *M, scat_ctx *S) { return assemble_fe_range(M, S, 0, M->nfe); }
+
+static int explicit_force_threads(const om_model *M, double *F) {
+    const int P = g_threads, neq = M->neq;
+    double *part = (double *)xcalloc((size_t)P * (neq > 0 ? neq : 1), sizeof(double));
+    int err = OM_OK;
+#pragma omp parallel for num_threads(P) schedule(static, 1)
+    for (int r = 0; r < P; r++) {
+        scat_ctx S = {0, 0, NULL, NULL, part + (size_t)r * neq, 0};
+        int e = OM_OK;
+        long long P_ = P;
+        if (!(g_skip & 4)) e = assemble_pd_range(M, &S, (int)(r * (long long)M->nfam / P_), (int)((r + 1) * (long long)M->nfam / P_));
+        if (!e && !(g_skip & 1))
+            e = assemble_pdbe_range(M, &S, (int)(r * (long long)M->npdbe / P_), (int)((r + 1) * (long long)M->npdbe / P_));
+        if (!e && !(g_skip & 2)) e = assemble_fe_range(M, &S, (int)(r * (long long)M->nfe / P_), (int)((r + 1) * (long long)M->nfe / P_));
+        if (e) {
+#pragma omp critical
+            err = e;
+        }
+    }
+    for (int i = 0; i < neq; i++) {
+        double t = 0.0;
+        for (int r = 0; r < P; r++) t = t + part[(size_t)r * neq + i];
+        F[i] = t;
+    }
+    free(part);
+    return err;
+}
+
 int om_assemble(om_model *M, int mode, int newmark, const double *UN, double *Ku, double *F, int with_ext) {
     scat_ctx S = {mode, newmark, UN, Ku, F, 0};
     if (Ku) memset(Ku, 0, sizeof(double) * M->nnz);
     memset(F, 0, sizeof(double) * M->neq);
     if (!Ku && (mode != 0 || newmark)) return OM_ERR_ARG;
     int e;
-    if (!(g_skip & 4) && (e = assemble_pd(M, &S))) return e;
-    if (!(g_skip & 1) && (e = assemble_pdbe(M, &S))) return e;
-    if (!(g_skip & 2) && (e = assemble_fe(M, &S))) return e;
+    if (g_threads > 1 && mode == 0 && !newmark && !Ku) {
+        if ((e = explicit_force_threads(M, F))) return e;
+    } else {
+        if (!(g_skip & 4) && (e = assemble_pd(M, &S))) return e;
+        if (!(g_skip & 1) && (e = assemble_pdbe(M, &S))) return e;
+        if (!(g_skip & 2) && (e = assemble_fe(M, &S))) return e;
+    }
     if (with_ext) external_force(M, F);
     return OM_OK;
 }

@@ -602,3 +602,19 @@ def test_quasi_static_fc2_cracks_are_normal_to_the_principal_stress(field):
             if any(_seg_cross(xk, xm, s0, s1) for s0, s1 in segs):
                 expect[q] = 0
     assert np.array_equal(bond, expect)
+
+
+def test_threaded_explicit_force_matches_serial():
+    """The CPU baseline's threaded explicit force (the reference's rank slices as threads,
+    F summed in rank order) gives the serial trajectory to rounding, for thread counts that
+    do and do not divide the family, boundary-entity and element counts."""
+    mesh = W.box3d(n=10, pd_block=4)
+    mesh["dt"] = 2e-8
+    u1, _ = O.Model(mesh).explicit_run(6)
+    for p in (3, 8):
+        O.lib().om_set_threads(p)
+        try:
+            up, _ = O.Model(mesh).explicit_run(6)
+        finally:
+            O.lib().om_set_threads(1)
+        assert np.linalg.norm(up - u1) <= 1e-13 * np.linalg.norm(u1)
