@@ -75,17 +75,40 @@ def column_classes(m):
     return len(ids), int(sum(ids.values()))
 
 
-def step_bytes(m, nblk, prec, col_classes=None):
+def step_bytes(m, nblk, prec, col_classes=None, n_own=None):
     """Algorithmic HBM bytes of one explicit step in the node-block layout:
     block values (d*d per block) + one 4-byte column per block, row offsets,
     u_n gathered once (FP64), and per DOF: read u_n, u_{n-1}, m, unit traction
     force, write u_{n+1} (FP64).  With column pattern classes (col_classes =
     (classes, offsets)) the columns per block are replaced by one class per row
-    and the classes' offset lists."""
+    and the classes' offset lists.  With mirrored storage (n_own own blocks) only
+    the own blocks' values are read, plus one 4-byte source index per block."""
     d = m.dim
     sv = 4 if prec == 1 else 8
     cols = nblk * 4 if col_classes is None else m.nn * 4 + (col_classes[0] + 1) * 4 + col_classes[1] * 4
-    return nblk * d * d * sv + cols + (m.nn + 1) * 4 + m.nn * d * 8 + m.nn * d * (8 * 4 + 8)
+    vals = nblk * d * d * sv if n_own is None else n_own * d * d * sv + nblk * 4
+    return vals + cols + (m.nn + 1) * 4 + m.nn * d * 8 + m.nn * d * (8 * 4 + 8)
+
+
+def own_blocks(m):
+    """Own blocks of the mirrored K storage: all blocks but the lower blocks (r, c) whose two
+    orientations no family stores (those equal their partner's transpose)."""
+    off, adj = m.adjacency()
+    fam = m.families()
+    nn = m.nn
+    rows = np.repeat(np.arange(nn, dtype=np.int64), np.diff(off))
+    fn = fam["node"].astype(np.int64)
+    pdkeys = np.sort(np.repeat(fn, np.diff(fam["off"])) * nn + fam["mem"].astype(np.int64))
+
+    def stored(keys):
+        if len(pdkeys) == 0:
+            return np.zeros(len(keys), bool)
+        i = np.minimum(np.searchsorted(pdkeys, keys), len(pdkeys) - 1)
+        return pdkeys[i] == keys
+
+    adj64 = adj.astype(np.int64)
+    mirrored = (adj64 < rows) & ~stored(rows * nn + adj64) & ~stored(adj64 * nn + rows)
+    return int(len(adj) - mirrored.sum())
 
 
 def survey_step_bytes(m, prec):
@@ -377,18 +400,20 @@ def main():
     log(f"stress recovery {st_ms * 1e3:.1f} us ({stress['GBps']:.0f} GB/s algorithmic)")
     nblk = model.nadj
     prec = 1
-    lanes, nt, xm, soa, rel, tune_ms = ctxs[prec].explicit_tuning(prec)
+    lanes, nt, xm, soa, rel, sym, tune_ms = ctxs[prec].explicit_tuning(prec)
     classes = column_classes(model)
-    bytes_step = step_bytes(model, nblk, prec, classes if rel else None)
+    n_own = own_blocks(model)
+    bytes_step = step_bytes(model, nblk, prec, classes if rel else None, n_own if sym else None)
     achieved = bytes_step / (res[prec]["step_ms"] * 1e-3)
     tf = lambda b: "true" if b else "false"
-    kname = f"pdx::k_explicit<{model.dim}, float, {lanes}, {tf(nt)}, {tf(xm)}, {tf(soa)}, {tf(rel)}>"
+    kname = f"pdx::k_explicit<{model.dim}, float, {lanes}, {tf(nt)}, {tf(xm)}, {tf(soa)}, {tf(rel)}, {tf(sym)}>"
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": measured_traffic(kname) if args.n == 60 else None,
                 "kernel": kname + " (node-block SpMV + central-difference update, FP32 values)",
                 "tuner_ms_per_form": [float(t) for t in tune_ms],
                 "bytes_per_launch": bytes_step,
                 "column_classes": {"classes": classes[0], "offsets": classes[1]},
+                "mirrored_storage": {"own_blocks": n_own, "blocks": nblk},
                 "scalar_csr_bytes_per_step": survey_step_bytes(model, prec)}
     out = {"metric": METRIC, "value": res[1]["dofs"], "unit": "DOF-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res[1]["step_ms"], "higher_is_better": True, "scaling": "weak",
@@ -399,7 +424,8 @@ def main():
                       "neq": model.neq, "nnz": model.nnz, "dt": dt, "parallelism": "single GPU"},
            "roofline": roofline,
            "fp64": {"value": res[0]["dofs"], "ms_per_step": res[0]["step_ms"],
-                    "roofline_frac": step_bytes(model, nblk, 0, classes if ctxs[0].explicit_tuning(0)[4] else None)
+                    "roofline_frac": step_bytes(model, nblk, 0, classes if ctxs[0].explicit_tuning(0)[4] else None,
+                                         n_own if ctxs[0].explicit_tuning(0)[5] else None)
                     / (res[0]["step_ms"] * 1e-3) / HBM_PEAK},
            "fp32_vs_fp64_rel": drift,
            "stress_recovery": stress,

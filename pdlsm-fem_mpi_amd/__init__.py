@@ -28,7 +28,7 @@ PDX_KRYLOV_BICGSTAB, PDX_KRYLOV_CG = 0, 1
 PDX_PREC_NONE, PDX_PREC_JACOBI, PDX_PREC_BLOCK_JACOBI, PDX_PREC_LEARNED, PDX_PREC_TWO_LEVEL = 0, 1, 2, 3, 4
 PDX_PREC_ILU0 = 5
 FB_VARIANTS = 7  # family-batch launch forms (PDX_FB_VARIANTS)
-EXPLICIT_FORMS = 18  # explicit-step kernel forms (PDX_EXPLICIT_FORMS)
+EXPLICIT_FORMS = 20  # explicit-step kernel forms (PDX_EXPLICIT_FORMS)
 
 _P, _I, _D, _LL = C.c_void_p, C.c_int, C.c_double, C.c_longlong
 
@@ -92,7 +92,7 @@ EXPORTS = {
     "pdx_newmark_run": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "pdx_solve_csr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
     "pdx_last_timing": (_I, [_P, _P, _P]),
-    "pdx_explicit_tuning": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "pdx_explicit_tuning": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "pdx_explicit_set_form": (_I, [_P, _I, _I]),
     "pdx_node_stresses": (_I, [_P, _P, _P]),
     "pdx_node_stresses_timed": (_I, [_P, _P, _P]),
@@ -361,12 +361,13 @@ class Context:
     def explicit_tuning(self, precision):
         """The explicit-step form the tuner chose: (lanes per short row, non-temporal K
         loads, XCD-contiguous block order, row-local K value order, column pattern
-        classes, ms per form)."""
-        lanes, nt, xm, soa, rel = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        classes, mirrored lower blocks, ms per form)."""
+        lanes, nt, xm, soa, rel, sym = (C.c_int(0) for _ in range(6))
         ms = np.zeros(EXPLICIT_FORMS, np.float32)
         _chk(lib().pdx_explicit_tuning(self.h, precision, C.byref(lanes), C.byref(nt), C.byref(xm), C.byref(soa),
-                                       C.byref(rel), _p(ms)))
-        return lanes.value, bool(nt.value), bool(xm.value), bool(soa.value), bool(rel.value), ms
+                                       C.byref(rel), C.byref(sym), _p(ms)))
+        return (lanes.value, bool(nt.value), bool(xm.value), bool(soa.value), bool(rel.value), bool(sym.value),
+                ms)
 
     def max_eig(self, iters=100):
         v = C.c_double(0)

@@ -107,6 +107,30 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
     std::vector<unsigned char> pd_bits(((size_t)c->nblk + 7) / 8, 0);
     for (int b : fam_blk) pd_bits[(size_t)b >> 3] |= (unsigned char)(1u << (b & 7));
     UP(c->pd_bits, pd_bits);
+    // mirrored storage for the explicit step: a lower block (r, c) none of whose two orientations
+    // a family stores is a sum of element blocks only, over the same elements in the same order
+    // as its upper partner (c, r) with every element block transposed, so it equals the partner's
+    // transpose exactly and is read from the partner's storage
+    {
+        auto pd = [&](long long b) { return (pd_bits[(size_t)b >> 3] >> (b & 7)) & 1; };
+        std::vector<int> ksrc((size_t)std::max<long long>(c->nblk, 1)), own_slot, partner((size_t)c->nblk, -1);
+        own_slot.reserve((size_t)c->nblk);
+        for (int r = 0; r < m.nn; r++)
+            for (int b = m.adj_off[r]; b < m.adj_off[r + 1]; b++) {
+                const int col = m.adj[b];
+                if (col < r && !pd(b)) {
+                    const int tb = find_block(m, col, r);
+                    if (tb >= 0 && !pd(tb)) { partner[b] = tb; continue; }
+                }
+                ksrc[b] = (int)own_slot.size();
+                own_slot.push_back(b);
+            }
+        for (long long b = 0; b < c->nblk; b++)
+            if (partner[b] >= 0) ksrc[b] = ~ksrc[partner[b]];  // partners are upper blocks, always own
+        c->n_own = (long long)own_slot.size();
+        UP(c->ksrc, ksrc);
+        UP(c->kown_slot, own_slot);
+    }
     UP(c->fam_v, m.fam_v);
     UP(c->fam_delta, m.fam_delta);
     // PD-boundary entities incident to each family's node, in entity order
@@ -305,6 +329,8 @@ void ctx_free(Ctx *c) {
     if (c->red) hipFree(c->red);
     if (c->learned_Minv) hipFree(c->learned_Minv);
     if (c->fam_c) hipFree(c->fam_c);
+    if (c->Kown32) hipFree(c->Kown32);
+    if (c->Kown64) hipFree(c->Kown64);
     if (c->stress_stage) hipFree(c->stress_stage);
     if (c->stress_famid) hipFree(c->stress_famid);
     if (c->stress_status) hipFree(c->stress_status);
@@ -664,16 +690,17 @@ int pdx_node_stresses_timed(pdx_ctx *pc, const double *u_all, double *ms) {
 }
 
 int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal, int *xcd_order, int *row_soa,
-                        int *col_classes, float *ms) {
+                        int *col_classes, int *mirrored, float *ms) {
     if (!pc || (precision != PDX_FP64 && precision != PDX_FP32)) return fail(PDX_ERR_ARG, "bad argument");
     const Ctx &c = *pc->c;
     const int v = c.tune_explicit[precision];
-    const ExplicitForm f = v >= 0 ? kExplicitForms[v] : ExplicitForm{0, false, false, false, false};
+    const ExplicitForm f = v >= 0 ? kExplicitForms[v] : ExplicitForm{0, false, false, false, false, false};
     if (lanes) *lanes = f.lanes;
     if (nontemporal) *nontemporal = f.nt;
     if (xcd_order) *xcd_order = f.xm;
     if (row_soa) *row_soa = f.soa;
     if (col_classes) *col_classes = f.rel;
+    if (mirrored) *mirrored = f.sym;
     if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + PDX_EXPLICIT_FORMS, ms);
     return PDX_OK;
 }

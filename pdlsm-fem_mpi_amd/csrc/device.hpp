@@ -29,18 +29,23 @@ struct DevBuf {
 };
 
 // explicit-step kernel forms searched by the tuner: lanes per short row, non-temporal K loads,
-// XCD-contiguous block order, row-local value order of K, column pattern classes
+// XCD-contiguous block order, row-local value order of K, column pattern classes, mirrored
+// lower blocks read from their transposed partners
 struct ExplicitForm {
     int lanes;
-    bool nt, xm, soa, rel;
+    bool nt, xm, soa, rel, sym;
 };
 constexpr ExplicitForm kExplicitForms[PDX_EXPLICIT_FORMS] = {
-    {8, false, false, false, false},  {4, false, false, false, false}, {16, false, false, false, false},
-    {8, true, false, false, false},   {4, true, false, false, false},  {16, true, false, false, false},
-    {32, false, false, false, false}, {32, true, false, false, false}, {16, false, true, false, false},
-    {8, false, true, false, false},   {8, false, true, true, false},   {16, false, true, true, false},
-    {4, false, true, true, false},    {32, false, true, true, false},  {8, false, true, false, true},
-    {16, false, true, false, true},   {8, false, true, true, true},    {16, false, true, true, true}};
+    {8, false, false, false, false, false},  {4, false, false, false, false, false},
+    {16, false, false, false, false, false}, {8, true, false, false, false, false},
+    {4, true, false, false, false, false},   {16, true, false, false, false, false},
+    {32, false, false, false, false, false}, {32, true, false, false, false, false},
+    {16, false, true, false, false, false},  {8, false, true, false, false, false},
+    {8, false, true, true, false, false},    {16, false, true, true, false, false},
+    {4, false, true, true, false, false},    {32, false, true, true, false, false},
+    {8, false, true, false, true, false},    {16, false, true, false, true, false},
+    {8, false, true, true, true, false},     {16, false, true, true, true, false},
+    {8, false, true, false, true, true},     {16, false, true, false, true, true}};
 
 struct Ctx {
     int device = 0;
@@ -97,6 +102,14 @@ struct Ctx {
     // column pattern classes of the node rows: row r's block columns are r + cls_rel[cls_off[cls[r]] + k]
     int *cls = nullptr, *cls_off = nullptr, *cls_rel = nullptr;
     int n_cls = 0;
+    // mirrored storage of K for the explicit step: the "own" blocks (diagonal, upper, and lower
+    // blocks with a PD part in them or in their partner) in row order, and per K block its own
+    // index, or ~(own index of its partner) for a lower block read as the partner's transpose
+    int *ksrc = nullptr, *kown_slot = nullptr;
+    long long n_own = 0;
+    double *Kown64 = nullptr;
+    float *Kown32 = nullptr;
+    long long kown_gen[2] = {-1, -1};
     double *K64s = nullptr;
     float *K32s = nullptr;
     long long ks_gen[2] = {-1, -1};

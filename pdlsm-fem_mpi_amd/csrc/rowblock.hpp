@@ -17,10 +17,13 @@ namespace pdx {
 // at b0*DD + e*nb + (b - b0)): lanes on consecutive blocks then read consecutive words.
 // REL: the block columns are read as r + rel[k] from the row's column pattern class (rel points at
 // the class's offsets) instead of from adj.
-template <int D, typename T, typename X, int G, bool DIFF, bool NT = false, bool SOA = false, bool REL = false>
+// SYM: K holds the own blocks only; src[b] is block b's own index, or ~(own index of its
+// partner) for a lower block that equals its partner's transpose.
+template <int D, typename T, typename X, int G, bool DIFF, bool NT = false, bool SOA = false, bool REL = false,
+          bool SYM = false>
 __device__ inline void row_matvec(int r, int lane, const int *__restrict__ adj_off, const int *__restrict__ adj,
                                   const T *__restrict__ K, const X *__restrict__ x, T acc[D],
-                                  const int *__restrict__ rel = nullptr) {
+                                  const int *__restrict__ rel = nullptr, const int *__restrict__ src = nullptr) {
     constexpr int DD = D * D;
 #pragma unroll
     for (int i = 0; i < D; i++) acc[i] = T(0);
@@ -33,10 +36,23 @@ __device__ inline void row_matvec(int r, int lane, const int *__restrict__ adj_o
     for (int b = b0 + lane; b < b1; b += G) {
         const int c = REL ? r + rel[b - b0] : adj[b];
         T kb[DD];
+        if constexpr (SYM) {
+            const int s = src[b];
+            const bool tr = s < 0;
+            const T *kp = K + (size_t)(tr ? ~s : s) * DD;
+            T raw[DD];
 #pragma unroll
-        for (int e = 0; e < DD; e++) {
-            const size_t k = SOA ? (size_t)b0 * DD + (size_t)e * nb + (b - b0) : (size_t)b * DD + e;
-            kb[e] = NT ? __builtin_nontemporal_load(&K[k]) : K[k];
+            for (int e = 0; e < DD; e++) raw[e] = kp[e];
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) kb[i * D + j] = tr ? raw[j * D + i] : raw[i * D + j];
+        } else {
+#pragma unroll
+            for (int e = 0; e < DD; e++) {
+                const size_t k = SOA ? (size_t)b0 * DD + (size_t)e * nb + (b - b0) : (size_t)b * DD + e;
+                kb[e] = NT ? __builtin_nontemporal_load(&K[k]) : K[k];
+            }
         }
         T xv[D];
 #pragma unroll

@@ -32,9 +32,10 @@ namespace pdx {
 // column pattern classes (REL forms): class of each row, offsets of each class's relative columns
 struct ColClasses {
     const int *cls, *cls_off, *cls_rel;
+    const int *ksrc;  // mirrored forms: own index of every block (~partner's for a mirrored one)
 };
 
-template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL>
+template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL, bool SYM>
 __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int nbs, const int *rows_l, int nl,
                                                   int nbt, const int *adj_off, const int *adj, ColClasses cc, const T *K,
                                                   const double *Un, const double *Unm1, double *Unp1,
@@ -82,8 +83,8 @@ __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int
         }
     }
     const int *rel = REL ? cc.cls_rel + cc.cls_off[cc.cls[r]] : nullptr;
-    if (grp == GS) row_matvec<D, T, double, GS, DIFF, NT, SOA, REL>(r, lane, adj_off, adj, K, Un, acc, rel);
-    else row_matvec<D, T, double, 64, DIFF, NT, SOA, REL>(r, lane, adj_off, adj, K, Un, acc, rel);
+    if (grp == GS) row_matvec<D, T, double, GS, DIFF, NT, SOA, REL, SYM>(r, lane, adj_off, adj, K, Un, acc, rel, cc.ksrc);
+    else row_matvec<D, T, double, 64, DIFF, NT, SOA, REL, SYM>(r, lane, adj_off, adj, K, Un, acc, rel, cc.ksrc);
     if (act) {
         const T v = pick<D, T>(acc, lane);
         if (fr) {
@@ -143,6 +144,35 @@ __global__ void k_row_soa(int nn, int dd, const int *__restrict__ adj_off, const
     }
 }
 
+// own blocks of K (mirrored storage) in precision T
+template <typename T>
+__global__ void k_gather_own(long long n_own, int dd, const int *__restrict__ slot, const double *__restrict__ K,
+                             T *Kown) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n_own * dd;
+         v += (long long)gridDim.x * blockDim.x) {
+        const long long q = v / dd;
+        Kown[v] = T(K[(size_t)slot[q] * dd + (v - q * dd)]);
+    }
+}
+
+// the mirrored-storage copy of K for the explicit step, rebuilt when K has been assembled again
+static int ensure_Kown(Ctx &c, int prec, std::string &err) {
+    const int pi = prec == PDX_FP32 ? 1 : 0;
+    if (!c.K_valid) { err = "K not assembled"; return PDX_ERR_ARG; }
+    if (c.kown_gen[pi] == c.K_gen) return PDX_OK;
+    const size_t n = (size_t)c.dd * std::max<long long>(c.n_own, 1);
+    if (pi) {
+        if (!c.Kown32) PDX_HIP(hipMalloc((void **)&c.Kown32, sizeof(float) * n));
+        k_gather_own<float><<<4096, 256, 0, c.stream>>>(c.n_own, c.dd, c.kown_slot, c.K, c.Kown32);
+    } else {
+        if (!c.Kown64) PDX_HIP(hipMalloc((void **)&c.Kown64, sizeof(double) * n));
+        k_gather_own<double><<<4096, 256, 0, c.stream>>>(c.n_own, c.dd, c.kown_slot, c.K, c.Kown64);
+    }
+    PDX_HIP(hipGetLastError());
+    c.kown_gen[pi] = c.K_gen;
+    return PDX_OK;
+}
+
 // the row-local copy of K for the explicit step, rebuilt when K has been assembled again
 static int ensure_Ks(Ctx &c, int prec, std::string &err) {
     const int pi = prec == PDX_FP32 ? 1 : 0;
@@ -162,14 +192,14 @@ static int ensure_Ks(Ctx &c, int prec, std::string &err) {
     return PDX_OK;
 }
 
-template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL>
+template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL, bool SYM>
 static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double *Unm1, double *Unp1,
                               const double *nbc_cur, double *nbc_next, int startup) {
     const int nbs = (c.n_short * GS + 255) / 256, nbl = (c.n_long * 64 + 255) / 256;
     const int nbt = std::max(nbs + nbl, 1);
     const int grid = XM ? (nbt + 7) / 8 * 8 : nbt;
-    const ColClasses cc{c.cls, c.cls_off, c.cls_rel};
-    k_explicit<D, T, GS, NT, XM, SOA, REL><<<grid, 256, 0, c.stream>>>(
+    const ColClasses cc{c.cls, c.cls_off, c.cls_rel, c.ksrc};
+    k_explicit<D, T, GS, NT, XM, SOA, REL, SYM><<<grid, 256, 0, c.stream>>>(
         c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, nbt, c.adj_off, c.adj, cc, K, Un, Unm1, Unp1, c.mlump,
         c.f_unit, c.n_nbc, c.nn, nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup);
 }
@@ -178,46 +208,49 @@ static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double
 template <int D, typename T>
 static void launch_explicit(Ctx &c, int variant, const T *K, const double *Un, const double *Unm1, double *Unp1,
                             const double *nbc_cur, double *nbc_next, int startup) {
-#define PDX_EXPLICIT_FORM(V, GS, NT, XM, SOA, REL)                                                              \
+#define PDX_EXPLICIT_FORM(V, GS, NT, XM, SOA, REL, SYM)                                                         \
     case V:                                                                                                     \
         static_assert(kExplicitForms[V].lanes == GS && kExplicitForms[V].nt == NT && kExplicitForms[V].xm == XM && \
-                          kExplicitForms[V].soa == SOA && kExplicitForms[V].rel == REL,                         \
+                          kExplicitForms[V].soa == SOA && kExplicitForms[V].rel == REL &&                       \
+                          kExplicitForms[V].sym == SYM,                                                         \
                       "form table");                                                                            \
-        launch_explicit_v<D, T, GS, NT, XM, SOA, REL>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);            \
+        launch_explicit_v<D, T, GS, NT, XM, SOA, REL, SYM>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);       \
         break;
     switch (variant) {
-        PDX_EXPLICIT_FORM(1, 4, false, false, false, false)
-        PDX_EXPLICIT_FORM(2, 16, false, false, false, false)
-        PDX_EXPLICIT_FORM(3, 8, true, false, false, false)
-        PDX_EXPLICIT_FORM(4, 4, true, false, false, false)
-        PDX_EXPLICIT_FORM(5, 16, true, false, false, false)
-        PDX_EXPLICIT_FORM(6, 32, false, false, false, false)
-        PDX_EXPLICIT_FORM(7, 32, true, false, false, false)
-        PDX_EXPLICIT_FORM(8, 16, false, true, false, false)
-        PDX_EXPLICIT_FORM(9, 8, false, true, false, false)
-        PDX_EXPLICIT_FORM(10, 8, false, true, true, false)
-        PDX_EXPLICIT_FORM(11, 16, false, true, true, false)
-        PDX_EXPLICIT_FORM(12, 4, false, true, true, false)
-        PDX_EXPLICIT_FORM(13, 32, false, true, true, false)
-        PDX_EXPLICIT_FORM(14, 8, false, true, false, true)
-        PDX_EXPLICIT_FORM(15, 16, false, true, false, true)
-        PDX_EXPLICIT_FORM(16, 8, false, true, true, true)
-        PDX_EXPLICIT_FORM(17, 16, false, true, true, true)
+        PDX_EXPLICIT_FORM(1, 4, false, false, false, false, false)
+        PDX_EXPLICIT_FORM(2, 16, false, false, false, false, false)
+        PDX_EXPLICIT_FORM(3, 8, true, false, false, false, false)
+        PDX_EXPLICIT_FORM(4, 4, true, false, false, false, false)
+        PDX_EXPLICIT_FORM(5, 16, true, false, false, false, false)
+        PDX_EXPLICIT_FORM(6, 32, false, false, false, false, false)
+        PDX_EXPLICIT_FORM(7, 32, true, false, false, false, false)
+        PDX_EXPLICIT_FORM(8, 16, false, true, false, false, false)
+        PDX_EXPLICIT_FORM(9, 8, false, true, false, false, false)
+        PDX_EXPLICIT_FORM(10, 8, false, true, true, false, false)
+        PDX_EXPLICIT_FORM(11, 16, false, true, true, false, false)
+        PDX_EXPLICIT_FORM(12, 4, false, true, true, false, false)
+        PDX_EXPLICIT_FORM(13, 32, false, true, true, false, false)
+        PDX_EXPLICIT_FORM(14, 8, false, true, false, true, false)
+        PDX_EXPLICIT_FORM(15, 16, false, true, false, true, false)
+        PDX_EXPLICIT_FORM(16, 8, false, true, true, true, false)
+        PDX_EXPLICIT_FORM(17, 16, false, true, true, true, false)
+        PDX_EXPLICIT_FORM(18, 8, false, true, false, true, true)
+        PDX_EXPLICIT_FORM(19, 16, false, true, false, true, true)
     default:
-        PDX_EXPLICIT_FORM(0, 8, false, false, false, false)
+        PDX_EXPLICIT_FORM(0, 8, false, false, false, false, false)
     }
 #undef PDX_EXPLICIT_FORM
 }
 
 static void launch_explicit_any(Ctx &c, int prec, int variant, const double *Un, const double *Unm1, double *Unp1,
                                 const double *nbc_cur, double *nbc_next, int startup) {
-    const bool soa = kExplicitForms[variant].soa;
+    const bool soa = kExplicitForms[variant].soa, sym = kExplicitForms[variant].sym;
     if (prec == PDX_FP32) {
-        const float *K = soa ? c.K32s : c.K32;
+        const float *K = sym ? c.Kown32 : soa ? c.K32s : c.K32;
         if (c.dim == 3) launch_explicit<3, float>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
         else launch_explicit<2, float>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
     } else {
-        const double *K = soa ? c.K64s : c.K;
+        const double *K = sym ? c.Kown64 : soa ? c.K64s : c.K;
         if (c.dim == 3) launch_explicit<3, double>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
         else launch_explicit<2, double>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
     }
@@ -288,6 +321,7 @@ int explicit_prepare(Ctx &c, int prec, std::string &err) {
         if (e) return e;
     }
     if (int e = ensure_Ks(c, prec, err)) return e;
+    if (int e = ensure_Kown(c, prec, err)) return e;
     if (!c.mlump_valid) {
         int e = dev_lumped_mass(c, err);
         if (e) return e;
