@@ -208,6 +208,32 @@ def measured_traffic(kernel):
     return k["traffic_bytes"] if k else None
 
 
+def gnn_bench(pdx, cx, model):
+    """Inference of the learned smoother's graph network (4 message-passing layers, width 32) on
+    the node graph of a model, FP32 and BF16-activation MFMA, timed with HIP events (layers,
+    head and smoother; the network inputs are built once per K).  Algorithmic bytes: per layer
+    the activations read and written once, the edge weights and adjacency once; input layer,
+    head and smoother likewise.  FLOPs: per layer 2 N F (2F) + 2 E F, plus input and head."""
+    net = pdx.GNN(model.dim, seed=5)
+    w = net.params()
+    rng = np.random.default_rng(0)
+    w[:] = rng.normal(size=w.size).astype(np.float32) * 0.1
+    net.set_params(w)
+    N, E, F, L, DD, GIN = model.nn, model.nadj, 32, 4, model.dim ** 2, 8
+    flops = L * (2 * N * F * 2 * F + 2 * E * F) + 2 * N * GIN * F + 2 * N * (F + DD) * DD
+    out = {"workload": f"node graph of the {model.nn}-node model, {E} edges, 4 layers of width 32",
+           "flops": flops}
+    for name, bf16, sa, peak_flops in (("fp32", False, 4, 157.3e12), ("bf16_mfma", True, 2, 2.5e15)):
+        net.infer(cx, bf16=bf16)
+        ms = min(net.infer(cx, bf16=bf16)[1] for _ in range(5))
+        nbytes = (L * (2 * N * F * sa + E * 8 + N * 4) + N * GIN * 4 + N * F * sa
+                  + N * F * sa + N * DD * 4 * 2 + N * DD * (4 + 4 + 8))
+        out[name] = {"ms": ms, "alg_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
+                     "hbm_frac": nbytes / (ms * 1e-3) / HBM_PEAK, "TFLOPs": flops / (ms * 1e-3) / 1e12,
+                     "compute_frac": flops / (ms * 1e-3) / peak_flops}
+    return out
+
+
 def static_plate_bench(pdx, W):
     """The reference's CPU-runnable static case: a 2D plane-stress plate 100 x 50 mm
     (201 x 101 nodes, 40,299 eq) with a central PDLSM strip under displacement-controlled
@@ -454,6 +480,11 @@ def main():
                 solves[name] = {"error": str(e)}
             log(name, solves[name])
         out["solve_to_1e-8"] = {"workload": f"crack2d 501x251 seed 0, neq {smod.neq}", **solves}
+        try:
+            out["gnn_inference"] = gnn_bench(pdx, scx, smod)
+        except Exception as e:  # reported, not hidden
+            out["gnn_inference"] = {"error": str(e)}
+        log("gnn inference", out["gnn_inference"])
     if not args.no_solve:
         out["static_plate2d"] = static_plate_bench(pdx, W)
         log("static plate2d", out["static_plate2d"])

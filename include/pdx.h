@@ -265,6 +265,10 @@ int pdx_gnn_inputs(pdx_ctx *c, int n_probes, unsigned long long probe_seed, int 
                    float *Aaa, float *B, float *f, float *y);
 /* predict the smoother for the context's current K (used by PDX_PREC_LEARNED) */
 int pdx_gnn_apply(pdx_ctx *c, const pdx_gnn *g);
+/* The same prediction with FP32 (bf16 = 0) or BF16-activation MFMA (bf16 != 0,
+ * v_mfma_f32_16x16x32_bf16, FP32 accumulation) inference; M [n_node*d*d] (or NULL) receives the
+ * smoother, ms (or NULL) the inference time.  Also makes it the context's learned smoother. */
+int pdx_gnn_infer(pdx_ctx *c, const pdx_gnn *g, int bf16, double *M, double *ms);
 
 /* ---- batch of PD families (a node cloud, every node a family) ------------
  * The family kernel of the reference -- inflFunc, shapTens2D/3D, vec_gd2D/3D

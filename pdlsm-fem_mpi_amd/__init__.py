@@ -105,6 +105,7 @@ EXPORTS = {
     "pdx_gnn_loss_grad": (_I, [_P, _P, _I, C.c_ulonglong, _I, _P, _P]),
     "pdx_gnn_inputs": (_I, [_P, _I, C.c_ulonglong, _I, _P, _P, _P, _P, _P, _P]),
     "pdx_gnn_apply": (_I, [_P, _P]),
+    "pdx_gnn_infer": (_I, [_P, _P, _I, _P, _P]),
     "pdx_fb_create": (_I, [_I, _I, _LL, _P, _P, _P, _D, _D, _D, _P]),
     "pdx_fb_destroy": (None, [_P]),
     "pdx_fb_sizes": (_I, [_P, _P, _P, _P]),
@@ -494,6 +495,15 @@ class GNN:
 
     def apply(self, cx: "Context"):
         _chk(lib().pdx_gnn_apply(cx.h, self.h))
+
+    def infer(self, cx: "Context", bf16: bool = False):
+        """Smoother [nn, d, d] predicted for cx's K (FP32, or BF16 MFMA inference) and the
+        inference time in ms; it becomes cx's learned smoother."""
+        m = cx.model
+        M = np.zeros(m.nn * m.dim * m.dim)
+        ms = C.c_double()
+        _chk(lib().pdx_gnn_infer(cx.h, self.h, int(bool(bf16)), _p(M), C.byref(ms)))
+        return M.reshape(m.nn, m.dim, m.dim), ms.value
 
 
 def gnn_inputs(cx: "Context", n_probes: int = 4, probe_seed: int = 0, coarse_agg: int = 0) -> dict:

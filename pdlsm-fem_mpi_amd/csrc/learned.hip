@@ -387,6 +387,7 @@ struct GnnWork {
     float *X = nullptr, *cw = nullptr, *Aaa = nullptr, *B = nullptr;
     int *csc_off = nullptr, *csc_edge = nullptr, *csc_row = nullptr;
     float *H[GL + 1] = {}, *HM[GL] = {}, *U0 = nullptr, *U[GL] = {}, *O = nullptr;
+    __bf16 *Hb[GL + 1] = {};  // BF16 activations of the MFMA inference (allocated on first use)
     float *f = nullptr, *y = nullptr, *R = nullptr;
     float *dO = nullptr, *dH = nullptr, *dH2 = nullptr, *dM = nullptr, *dU = nullptr;
     double *part = nullptr, *lpart = nullptr;
@@ -539,6 +540,97 @@ static int gnn_probes(Ctx &c, GnnWork &g, int P, unsigned long long seed, int ag
     return rc;
 }
 
+// ------------------------------------------------------------------------
+// BF16 MFMA inference
+// ------------------------------------------------------------------------
+// Layers on v_mfma_f32_16x16x32_bf16 (A 16x32, B 32x16, D 16x16 FP32; lane l holds
+// A[l%16][(l/16)*8 + v], B[(l/16)*8 + v][l%16], v < 8, and D[(l/16)*4 + v][l%16], v < 4, the
+// layout checked on gfx950).  One wavefront takes 16 nodes: A = the nodes' [h ; sum_b c_ab h_b]
+// (two 32-wide K chunks), B = W^T (two 16-wide output tiles): 4 MFMAs per 16 nodes.  The
+// activations are stored in BF16 (the gathers of the aggregation read half the bytes); sums,
+// bias, ReLU and the residual are FP32.
+typedef __bf16 gnn_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float gnn_f4 __attribute__((ext_vector_type(4)));
+
+// h0 = relu(W_in x + b_in) in BF16, one thread per (node, feature)
+__global__ void k_dense_in_bf16(int nn, const float *X, const float *W, const float *b, __bf16 *H) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn * GF) return;
+    const int a = t / GF, f = t % GF;
+    float s = b[f];
+#pragma unroll
+    for (int k = 0; k < GIN; k++) s += W[f * GIN + k] * X[(size_t)a * GIN + k];
+    H[t] = (__bf16)(s > 0.f ? s : 0.f);
+}
+
+__global__ void __launch_bounds__(256) k_gnn_layer_mfma(int nn, const int *__restrict__ adj_off,
+                                                        const int *__restrict__ adj, const float *__restrict__ cw,
+                                                        const __bf16 *__restrict__ H, const float *__restrict__ W,
+                                                        const float *__restrict__ bias, __bf16 *__restrict__ Hn) {
+    const int l = threadIdx.x & 63;
+    const int n0 = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 16;
+    if (n0 >= nn) return;  // whole wavefronts leave together
+    const int q = l >> 4, node = n0 + (l & 15);
+    gnn_bf16x8 bw[2][2];  // [K chunk][output tile]: B[k][j] = W[j][k]
+#pragma unroll
+    for (int kc = 0; kc < 2; kc++)
+#pragma unroll
+        for (int jt = 0; jt < 2; jt++)
+#pragma unroll
+            for (int v = 0; v < 8; v++) bw[kc][jt][v] = (__bf16)W[(jt * 16 + (l & 15)) * (2 * GF) + kc * GF + q * 8 + v];
+    gnn_bf16x8 ah, am;
+    float mv[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) mv[v] = 0.f;
+    if (node < nn) {
+#pragma unroll
+        for (int v = 0; v < 8; v++) ah[v] = H[(size_t)node * GF + q * 8 + v];
+        for (int b = adj_off[node]; b < adj_off[node + 1]; b++) {
+            const float w = cw[b];
+            const __bf16 *hb = H + (size_t)adj[b] * GF + q * 8;
+#pragma unroll
+            for (int v = 0; v < 8; v++) mv[v] += w * float(hb[v]);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 8; v++) ah[v] = (__bf16)0.f;
+    }
+#pragma unroll
+    for (int v = 0; v < 8; v++) am[v] = (__bf16)mv[v];
+    gnn_f4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
+    d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bw[0][0], d0, 0, 0, 0);
+    d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bw[1][0], d0, 0, 0, 0);
+    d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bw[0][1], d1, 0, 0, 0);
+    d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bw[1][1], d1, 0, 0, 0);
+    const int j0 = l & 15, j1 = 16 + (l & 15);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int nd = n0 + q * 4 + v;
+        if (nd < nn) {
+            const float o0 = d0[v] + bias[j0], o1 = d1[v] + bias[j1];
+            Hn[(size_t)nd * GF + j0] = (__bf16)((o0 > 0.f ? o0 : 0.f) + float(H[(size_t)nd * GF + j0]));
+            Hn[(size_t)nd * GF + j1] = (__bf16)((o1 > 0.f ? o1 : 0.f) + float(H[(size_t)nd * GF + j1]));
+        }
+    }
+}
+
+// O = W_o [H ; A_aa] + b_o from BF16 H; one thread per (node, output)
+template <int D>
+__global__ void k_head_bf16(int nn, const __bf16 *H, const float *Aaa, const float *Wo, const float *bo, float *O) {
+    constexpr int DD = D * D, NH = GF + DD;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn * DD) return;
+    const int a = t / DD, k = t % DD;
+    const float *w = Wo + (size_t)k * NH;
+    float s = bo[k];
+    for (int q = 0; q < GF; q++) s += w[q] * float(H[(size_t)a * GF + q]);
+#pragma unroll
+    for (int q = 0; q < DD; q++) s += w[GF + q] * Aaa[(size_t)a * DD + q];
+    O[t] = s;
+}
+
+static int gnn_forward_bf16(Ctx &c, GnnWork &g, const pdx_gnn &net, std::string &err);
+
 static void gnn_forward(Ctx &c, GnnWork &g, const pdx_gnn &net) {
     const int nn = g.nn;
     const GnnLayout &L = net.lay;
@@ -552,6 +644,25 @@ static void gnn_forward(Ctx &c, GnnWork &g, const pdx_gnn &net) {
     const int nh = (nn * L.DD + 255) / 256;
     if (g.D == 3) k_head<3><<<nh, 256, 0, c.stream>>>(nn, g.H[GL], g.Aaa, w + L.w_o, w + L.b_o, g.O);
     else k_head<2><<<nh, 256, 0, c.stream>>>(nn, g.H[GL], g.Aaa, w + L.w_o, w + L.b_o, g.O);
+}
+
+static int gnn_forward_bf16(Ctx &c, GnnWork &g, const pdx_gnn &net, std::string &err) {
+    const int nn = g.nn;
+    const GnnLayout &L = net.lay;
+    const float *w = net.w;
+    int rc;
+    for (int l = 0; l <= GL; l++)
+        if (!g.Hb[l] && (rc = walloc(g.owned, g.Hb[l], (size_t)nn * GF, err))) return rc;
+    k_dense_in_bf16<<<(nn * GF + 255) / 256, 256, 0, c.stream>>>(nn, g.X, w + L.w_in, w + L.b_in, g.Hb[0]);
+    const int waves = (nn + 15) / 16, blocks = (waves + 3) / 4;
+    for (int l = 0; l < GL; l++)
+        k_gnn_layer_mfma<<<blocks, 256, 0, c.stream>>>(nn, c.adj_off, c.adj, g.cw, g.Hb[l], w + L.w_l[l], w + L.b_l[l],
+                                                       g.Hb[l + 1]);
+    const int nh = (nn * L.DD + 255) / 256;
+    if (g.D == 3) k_head_bf16<3><<<nh, 256, 0, c.stream>>>(nn, g.Hb[GL], g.Aaa, w + L.w_o, w + L.b_o, g.O);
+    else k_head_bf16<2><<<nh, 256, 0, c.stream>>>(nn, g.Hb[GL], g.Aaa, w + L.w_o, w + L.b_o, g.O);
+    PDX_HIP(hipGetLastError());
+    return PDX_OK;
 }
 
 // loss of the current weights and its gradient into net.g
@@ -754,6 +865,44 @@ int pdx_gnn_apply(pdx_ctx *pc, const pdx_gnn *g) {
     else k_smoother<2><<<nb, 256, 0, c.stream>>>(c.nn, w->O, w->B, c.eq, c.learned_Minv);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c.stream) != hipSuccess)
         return gfail(PDX_ERR_HIP, "pdx_gnn_apply: kernel failed");
+    c.learned_gen = c.K_gen;
+    return PDX_OK;
+}
+
+// Inference into the learned smoother of the context's current K, FP32 (bf16 == 0, the kernels
+// of pdx_gnn_apply) or BF16 activations on MFMA (bf16 != 0); M [n_node*d*d] (or NULL) gets the
+// smoother, ms (or NULL) the inference time (layers, head and smoother; HIP events).
+int pdx_gnn_infer(pdx_ctx *pc, const pdx_gnn *g, int bf16, double *M, double *ms) {
+    std::string err;
+    int rc = check_pair(pc, g, err);
+    if (rc) return gfail(rc, err);
+    Ctx &c = *pc->c;
+    hipSetDevice(c.device);
+    GnnWork *w;
+    if ((rc = gnn_work(c, w, err))) return gfail(rc, err);
+    if (!c.learned_Minv && hipMalloc((void **)&c.learned_Minv, (size_t)c.nn * c.dd * 8) != hipSuccess)
+        return gfail(PDX_ERR_HIP, "alloc failed");
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return gfail(PDX_ERR_HIP, "event");
+    hipEventRecord(e0, c.stream);
+    if (bf16) {
+        if ((rc = gnn_forward_bf16(c, *w, *g, err))) return gfail(rc, err);
+    } else {
+        gnn_forward(c, *w, *g);
+    }
+    const int nb = (c.nn + 255) / 256;
+    if (c.dim == 3) k_smoother<3><<<nb, 256, 0, c.stream>>>(c.nn, w->O, w->B, c.eq, c.learned_Minv);
+    else k_smoother<2><<<nb, 256, 0, c.stream>>>(c.nn, w->O, w->B, c.eq, c.learned_Minv);
+    hipEventRecord(e1, c.stream);
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+        return gfail(PDX_ERR_HIP, "pdx_gnn_infer: kernel failed");
+    float t = 0.f;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (ms) *ms = t;
+    if (M && hipMemcpy(M, c.learned_Minv, (size_t)c.nn * c.dd * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return gfail(PDX_ERR_HIP, "copy failed");
     c.learned_gen = c.K_gen;
     return PDX_OK;
 }

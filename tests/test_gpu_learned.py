@@ -96,3 +96,47 @@ def test_learned_gain_on_held_out_seed():
     assert np.linalg.norm(F - A @ u) / np.linalg.norm(F) <= 1.2e-8
     print("held-out iterations: two-level block Jacobi", s_tl["iterations"], "learned", s_ln["iterations"])
     assert s_ln["iterations"] <= s_tl["iterations"], (s_ln, s_tl)
+
+
+@pytest.mark.parametrize("dim,bf16", [(2, False), (2, True), (3, False), (3, True)])
+def test_inference_matches_oracle_smoother(dim, bf16):
+    """The predicted smoother against the oracle's float64 network on the library's own inputs:
+    FP32 inference within 1e-5, BF16-activation MFMA inference within 1e-2 (norm-wise), and the
+    BF16 result is not the FP32 one (its own kernels ran)."""
+    mesh = _crack(6) if dim == 2 else W.box3d(n=12, pd_block=6)
+    cx = _ctx(mesh)
+    net = pdx.GNN(dim, seed=9)
+    rng = np.random.default_rng(13)
+    w = net.params()
+    lay = G.layout(dim)
+    o, s = lay["w_o"]
+    w[o:o + int(np.prod(s)) + dim * dim] = rng.normal(size=int(np.prod(s)) + dim * dim) * 0.05
+    net.set_params(w)
+    inp = pdx.gnn_inputs(cx, n_probes=1, probe_seed=0, coarse_agg=4 if dim == 3 else 8)
+    ref = np.asarray(G.smoother(w, inp, dim), np.float64)
+    M, ms = net.infer(cx, bf16=bf16)
+    err = np.linalg.norm(M - ref) / np.linalg.norm(ref)
+    print(f"dim {dim} bf16 {bf16}: smoother rel err {err:.2e}, {ms:.3f} ms", flush=True)
+    assert err <= (1e-2 if bf16 else 1e-5), err
+    if bf16:
+        M32, _ = net.infer(cx, bf16=False)
+        assert not np.array_equal(M, M32)
+
+
+def test_bf16_inference_solve():
+    """A learned solve with the smoother from BF16 MFMA inference reaches the 1e-8 true residual
+    in as many iterations (within 2) as with FP32 inference."""
+    mesh = _crack(7)
+    cx = _ctx(mesh)
+    net = pdx.GNN(2, seed=4)
+    net.train(cx, steps=10, n_probes=8, probe_seed=1, lr=3e-3, coarse_agg=8)
+    mo = O.Model(mesh)
+    K, F = mo.assemble(1)
+    A = mo.sparse(K)
+    its = []
+    for bf16 in (False, True):
+        net.infer(cx, bf16=bf16)
+        u, st = cx.static_solve(pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_LEARNED, tol=1e-8, coarse_agg=8))
+        assert st["converged"] and np.linalg.norm(F - A @ u) / np.linalg.norm(F) <= 1.2e-8
+        its.append(st["iterations"])
+    assert abs(its[1] - its[0]) <= 2, its
