@@ -208,6 +208,13 @@ def measured_traffic(kernel):
     return k["traffic_bytes"] if k else None
 
 
+KRYLOV_NO_GRAPH = False  # --krylov-no-graph: launch Krylov iterations one by one (profiling runs)
+
+
+def solve_opts(pdx, **kw):
+    return pdx.solve_opts(no_graph=KRYLOV_NO_GRAPH, **kw)
+
+
 def gnn_bench(pdx, cx, model):
     """Inference of the learned smoother's graph network (4 message-passing layers, width 32) on
     the node graph of a model, FP32 and BF16-activation MFMA, timed with HIP events (layers,
@@ -247,8 +254,8 @@ def static_plate_bench(pdx, W):
     asm_ms = min(cx.assemble_timed() for _ in range(3))
     res = {"workload": f"plate2d 201x101 nodes, PD strip, neq {mp.neq}, nnz {mp.nnz}", "host_setup_s": setup_s,
            "assembly_ms": asm_ms}
-    for name, o in (("fp64_two_level_bicgstab", pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
-                    ("plain_fp64_jacobi_bicgstab", pdx.solve_opts(precision=0, precond=1, tol=1e-8, max_iter=100000))):
+    for name, o in (("fp64_two_level_bicgstab", solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
+                    ("plain_fp64_jacobi_bicgstab", solve_opts(pdx, precision=0, precond=1, tol=1e-8, max_iter=100000))):
         try:
             u, st0 = cx.static_solve(o)
             u, st = cx.static_solve(o)
@@ -281,11 +288,11 @@ def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
     setup_s = time.time() - t0
     out = {"workload": f"penny3d {n}^3 nodes, PD 30^3, dt = 10 dt_crit = {dt:.3e} s, {steps} Newmark steps",
            "neq": mp.neq, "host_setup_s": setup_s}
-    runs = {"plain_fp64_blockjacobi": pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
+    runs = {"plain_fp64_blockjacobi": solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
                                                      max_iter=100000),
-            "fp32_ir_blockjacobi": pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
+            "fp32_ir_blockjacobi": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
                                                   max_iter=100000),
-            "fp32_ir_two_level": pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8,
+            "fp32_ir_two_level": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8,
                                                 max_iter=100000, coarse_agg=coarse_agg)}
     us = {}
     for name, opts in runs.items():
@@ -354,6 +361,9 @@ def main():
     ap.add_argument("--no-newmark", action="store_true", help="skip the 100^3 penny-crack Newmark run")
     ap.add_argument("--fb-only", action="store_true",
                     help="only run the 2M-node family batch and print its figures (for kernel profiling)")
+    ap.add_argument("--krylov-no-graph", action="store_true",
+                    help="launch the Krylov iterations' kernels one by one instead of replaying captured graphs "
+                         "(same results; for kernel-trace profiling)")
     ap.add_argument("--stress-only", action="store_true",
                     help="only time the nodal stress recovery of the box (for kernel profiling)")
     ap.add_argument("--hbm-probe", action="store_true",
@@ -361,6 +371,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
+    global KRYLOV_NO_GRAPH
+    KRYLOV_NO_GRAPH = args.krylov_no_graph
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -461,12 +473,12 @@ def main():
         smod = pdx.Model(sm)
         scx = pdx.Context(smod)
         solves = {}
-        cfgs = (("plain_fp64_jacobi_bicgstab", pdx.solve_opts(precision=0, precond=1, tol=1e-8, max_iter=100000)),
-                ("fp32_ir_blockjacobi_bicgstab", pdx.solve_opts(precision=1, precond=2, tol=1e-8, max_iter=100000)),
-                ("fp64_two_level_bicgstab", pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
-                ("fp32_ir_two_level_bicgstab", pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
-                ("fp64_ilu0_bicgstab", pdx.solve_opts(precision=0, precond=pdx.PDX_PREC_ILU0, tol=1e-8)),
-                ("fp32_ir_ilu0_bicgstab", pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_ILU0, tol=1e-8)))
+        cfgs = (("plain_fp64_jacobi_bicgstab", solve_opts(pdx, precision=0, precond=1, tol=1e-8, max_iter=100000)),
+                ("fp32_ir_blockjacobi_bicgstab", solve_opts(pdx, precision=1, precond=2, tol=1e-8, max_iter=100000)),
+                ("fp64_two_level_bicgstab", solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
+                ("fp32_ir_two_level_bicgstab", solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
+                ("fp64_ilu0_bicgstab", solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_ILU0, tol=1e-8)),
+                ("fp32_ir_ilu0_bicgstab", solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_ILU0, tol=1e-8)))
         for name, o in cfgs:
             try:
                 # first call builds the coarse space of the two-level kinds (the
