@@ -352,14 +352,23 @@ __global__ void __launch_bounds__(256) k_bicg_s_agg(int na, const int *agg_off, 
     }
 }
 
-// yc = Ainv rc (dense, nc x nc, row stride ld); one wave per row
+// yc = Ainv rc (dense, nc x nc, row stride ld, a multiple of 64); one wave per row, 16-byte
+// loads of the row (ld keeps them inside the matrix), FP64 sums
 template <typename V>
 __global__ void __launch_bounds__(256) k_coarse_mv(int nc, int ld, const V *Ainv, const double *rc, double *yc) {
+    constexpr int VEC = 16 / sizeof(V);
+    typedef V vec_t __attribute__((ext_vector_type(VEC)));
     const int lane = threadIdx.x & 63;
     const int nw = (gridDim.x * blockDim.x) >> 6;
     for (int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < nc; row += nw) {
+        const V *a = Ainv + (size_t)row * ld;
         double s = 0.0;
-        for (int j = lane; j < nc; j += 64) s += double(Ainv[(size_t)row * ld + j]) * rc[j];
+        for (int j0 = lane * VEC; j0 < nc; j0 += 64 * VEC) {
+            const vec_t v = *(const vec_t *)(a + j0);
+#pragma unroll
+            for (int u = 0; u < VEC; u++)
+                if (j0 + u < nc) s += double(v[u]) * rc[j0 + u];
+        }
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
         if (lane == 0) yc[row] = s;
     }
@@ -938,6 +947,7 @@ struct KrylovWs {
     // two-level operator (null for the block-diagonal preconditioners)
     const TwoLevel *tl = nullptr;
     const V *Pn = nullptr, *kp = nullptr, *Ainv = nullptr;
+    const float *Ainv32 = nullptr;  // the coarse inverse applied by both precisions
     const V *RS = nullptr;  // block row sums (difference-form SpMV) or null
     const void *ilu = nullptr;  // ILU(0) factor (ilu0.hip) or null
 };
@@ -987,6 +997,7 @@ static int ws_get(Ctx &c, KrylovWs<V> *&out, std::string &err) {
     out->tl = nullptr;
     out->ilu = nullptr;
     out->Pn = out->kp = out->Ainv = nullptr;
+    out->Ainv32 = nullptr;
     out->RS = nullptr;
     return PDX_OK;
 }
@@ -1048,7 +1059,7 @@ static GraphSig make_sig(Ctx &c, const void *K, const KrylovWs<V> &w, int krylov
     g.ilu = w.ilu;
     g.Pn = w.Pn;
     g.kp = w.kp;
-    g.Ainv = w.Ainv;
+    g.Ainv = w.Ainv32;
     g.rc = w.rc;
     g.yc = w.yc;
     g.red = c.red;
@@ -1079,13 +1090,13 @@ static void bicg_iter(Ctx &c, const T *K, KrylovWs<V> &w, double *red, double *s
         const int gc = std::min(2048, (t.nc + 3) / 4), gn = (c.nn + 255) / 256;
         k_bicg_p_agg<D, NM, V><<<RED_PARTS, 256, 0, c.stream>>>(t.na, t.agg_off, t.agg_node, w.r, w.p, w.v, w.Pn,
                                                                 rho, sc, it == 0, w.rc);
-        k_coarse_mv<V><<<gc, 256, 0, c.stream>>>(t.nc, t.ld, w.Ainv, w.rc, w.yc);
+        k_coarse_mv<float><<<gc, 256, 0, c.stream>>>(t.nc, t.ld, w.Ainv32, w.rc, w.yc);
         k_prolong<D, NM, V><<<gn, 256, 0, c.stream>>>(c.nn, w.Pn, t.node_agg, t.kp_off, t.kp_agg, w.kp, w.Minv, w.p,
                                                       w.yc, w.ph);
         spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.ph, w.v, w.rh, rv, (const V *)nullptr, nullptr);
         k_bicg_s_agg<D, NM, V><<<RED_PARTS, 256, 0, c.stream>>>(t.na, t.agg_off, t.agg_node, w.r, w.v, w.s, w.Pn,
                                                                 rho, rv, sc, w.rc);
-        k_coarse_mv<V><<<gc, 256, 0, c.stream>>>(t.nc, t.ld, w.Ainv, w.rc, w.yc);
+        k_coarse_mv<float><<<gc, 256, 0, c.stream>>>(t.nc, t.ld, w.Ainv32, w.rc, w.yc);
         k_prolong<D, NM, V><<<gn, 256, 0, c.stream>>>(c.nn, w.Pn, t.node_agg, t.kp_off, t.kp_agg, w.kp, w.Minv, w.s,
                                                       w.yc, w.sh);
     } else if (w.ilu) {
@@ -1289,8 +1300,11 @@ static int setup_precond(Ctx &c, const double *K64, const pdx_solve_opts &o, Kry
         PDX_HIP(hipMalloc((void **)&w.yc, (size_t)t->ld * 8));
         w.rc_cap = (size_t)t->ld;
     }
+    // the coarse inverse is applied in FP32 in both precisions: it only shapes the preconditioner
+    // (the Krylov iteration and its residual keep the working precision), at half the bytes
+    if ((rc = ensure_two_level_f32(c, *t, err))) return rc;
+    w.Ainv32 = t->Ainv32;
     if (sizeof(V) == 4) {
-        if ((rc = ensure_two_level_f32(c, *t, err))) return rc;
         w.Pn = (const V *)t->Pn32;
         w.kp = (const V *)t->kp32;
         w.Ainv = (const V *)t->Ainv32;
