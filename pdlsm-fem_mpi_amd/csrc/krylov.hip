@@ -45,15 +45,21 @@ int ensure_K32(Ctx &c, std::string &err);
 // algebraically identical; the block rows of a stiffness sum to ~0 (rigid
 // translations), so in FP32 this avoids the cancellation of the diagonal
 // against the neighbours for smooth vectors.
-template <int D, typename T, typename V, int G, bool DIFF>
-__device__ inline void spmv_dot_rows(const int *rows, int nrows, int first_group, int groups, const int *adj_off,
-                                     const int *adj, const T *K, const T *RS, const V *x, V *y, const int *eq,
-                                     const V *w0, const V *w1, double &d0, double &d1) {
+// Rows [lb * per, (lb + 1) * per) of the list go to logical block lb; the G-lane groups of the
+// block stride through them.  With REL the block columns come from the row's column pattern
+// class (cls, cls_off, cls_rel) instead of adj.
+template <int D, typename T, typename V, int G, bool DIFF, bool REL>
+__device__ inline void spmv_dot_rows(const int *rows, int nrows, int lb, int nblocks, const int *adj_off,
+                                     const int *adj, const int *cls, const int *cls_off, const int *cls_rel,
+                                     const T *K, const T *RS, const V *x, V *y, const int *eq, const V *w0,
+                                     const V *w1, double &d0, double &d1) {
     const int lane = threadIdx.x % G;
-    for (int gid = first_group; gid < nrows; gid += groups) {
+    const int per = (nrows + nblocks - 1) / nblocks, g0 = lb * per, g1 = min(nrows, g0 + per);
+    for (int gid = g0 + (int)threadIdx.x / G; gid < g1; gid += (int)blockDim.x / G) {
         const int r = rows[gid];
         T acc[D];
-        row_matvec<D, T, V, G, DIFF>(r, lane, adj_off, adj, K, x, acc);
+        const int *rel = REL ? cls_rel + cls_off[cls[r]] : nullptr;
+        row_matvec<D, T, V, G, DIFF, false, false, REL>(r, lane, adj_off, adj, K, x, acc, rel);
         if (lane < D) {
             T v = pick<D, T>(acc, lane);
             if (DIFF) {
@@ -69,20 +75,28 @@ __device__ inline void spmv_dot_rows(const int *rows, int nrows, int first_group
     }
 }
 
-template <int D, typename T, typename V>
+// The hardware deals blocks to the 8 XCDs round-robin: logical block (b % 8) * (grid / 8) + b / 8
+// gives each XCD (and its L2) one contiguous eighth of the rows, so its gathers of x stay local.
+template <int D, typename T, typename V, bool REL>
 __global__ void __launch_bounds__(SPMV_THREADS) k_spmv_dot(const int *rows_s, int ns, const int *rows_l, int nl,
-                                                  const int *adj_off, const int *adj, const T *K, const T *RS,
+                                                  const int *adj_off, const int *adj, const int *cls,
+                                                  const int *cls_off, const int *cls_rel, const T *K, const T *RS,
                                                   const V *x, V *y, const int *eq, const V *w0, double *part0,
                                                   const V *w1, double *part1) {
     __shared__ double sm[17];
     double d0 = 0.0, d1 = 0.0;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    const int nb = gridDim.x;
+    const int lb = nb % 8 == 0 ? (int)(blockIdx.x % 8) * (nb / 8) + (int)blockIdx.x / 8 : (int)blockIdx.x;
     if (RS) {
-        spmv_dot_rows<D, T, V, 8, true>(rows_s, ns, t / 8, nt / 8, adj_off, adj, K, RS, x, y, eq, w0, w1, d0, d1);
-        spmv_dot_rows<D, T, V, 64, true>(rows_l, nl, t / 64, nt / 64, adj_off, adj, K, RS, x, y, eq, w0, w1, d0, d1);
+        spmv_dot_rows<D, T, V, 8, true, REL>(rows_s, ns, lb, nb, adj_off, adj, cls, cls_off, cls_rel, K, RS, x, y, eq,
+                                             w0, w1, d0, d1);
+        spmv_dot_rows<D, T, V, 64, true, REL>(rows_l, nl, lb, nb, adj_off, adj, cls, cls_off, cls_rel, K, RS, x, y,
+                                              eq, w0, w1, d0, d1);
     } else {
-        spmv_dot_rows<D, T, V, 8, false>(rows_s, ns, t / 8, nt / 8, adj_off, adj, K, RS, x, y, eq, w0, w1, d0, d1);
-        spmv_dot_rows<D, T, V, 64, false>(rows_l, nl, t / 64, nt / 64, adj_off, adj, K, RS, x, y, eq, w0, w1, d0, d1);
+        spmv_dot_rows<D, T, V, 8, false, REL>(rows_s, ns, lb, nb, adj_off, adj, cls, cls_off, cls_rel, K, RS, x, y,
+                                              eq, w0, w1, d0, d1);
+        spmv_dot_rows<D, T, V, 64, false, REL>(rows_l, nl, lb, nb, adj_off, adj, cls, cls_off, cls_rel, K, RS, x, y,
+                                               eq, w0, w1, d0, d1);
     }
     if (part0) {
         d0 = block_sum(d0, sm);
@@ -1073,8 +1087,14 @@ static GraphSig make_sig(Ctx &c, const void *K, const KrylovWs<V> &w, int krylov
 template <int D, typename T, typename V>
 static void spmv_dot(Ctx &c, const T *K, const T *RS, const V *x, V *y, const V *w0, double *p0, const V *w1,
                      double *p1) {
-    k_spmv_dot<D, T, V><<<RED_PARTS, SPMV_THREADS, 0, c.stream>>>(c.rows_short, c.n_short, c.rows_long, c.n_long, c.adj_off,
-                                                          c.adj, K, RS, x, y, c.eq, w0, p0, w1, p1);
+    if (c.cls)
+        k_spmv_dot<D, T, V, true><<<RED_PARTS, SPMV_THREADS, 0, c.stream>>>(
+            c.rows_short, c.n_short, c.rows_long, c.n_long, c.adj_off, c.adj, c.cls, c.cls_off, c.cls_rel, K, RS, x,
+            y, c.eq, w0, p0, w1, p1);
+    else
+        k_spmv_dot<D, T, V, false><<<RED_PARTS, SPMV_THREADS, 0, c.stream>>>(
+            c.rows_short, c.n_short, c.rows_long, c.n_long, c.adj_off, c.adj, nullptr, nullptr, nullptr, K, RS, x, y,
+            c.eq, w0, p0, w1, p1);
 }
 
 // one BiCGStab iteration (it = iteration index, parity selects the rho slot)
