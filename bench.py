@@ -241,6 +241,48 @@ def gnn_bench(pdx, cx, model):
     return out
 
 
+def learned_bench(pdx, W, train_seeds=range(8), held_seeds=(448, 449, 450, 451), epochs=4, steps=10):
+    """The metric's learned-vs-plain comparison on held-out cracked plates: a network trained
+    on the GPU (the library's forward, backward and Adam kernels) on 121 x 61 plates of the
+    training seeds, applied with BF16 MFMA inference to 501 x 251 plates of held-out seeds,
+    and solved to a 1e-8 true residual in FP64 and in FP32 with FP64 refinement, beside
+    two-level without the network and plain FP64 Jacobi."""
+    t0 = time.time()
+    net = pdx.GNN(2, seed=3)
+    ctxs = []
+    for s in train_seeds:
+        cx = pdx.Context(pdx.Model(W.crack2d(seed=s, nx=121, ny=61, dx=0.1 / 120)))
+        cx.assemble(pdx.PDX_MODE_STATIC)
+        ctxs.append(cx)
+    for epoch in range(epochs):
+        for k, cx in enumerate(ctxs):
+            net.train(cx, steps=steps, n_probes=8, probe_seed=100 * epoch + k, lr=3e-3, coarse_agg=8)
+    out = {"train": f"{len(ctxs)} seeds ({min(train_seeds)}-{max(train_seeds)}) at 121x61, {epochs} epochs x "
+                    f"{steps} Adam steps", "train_s": time.time() - t0, "held_out": {}}
+    cfgs = (("plain_fp64_jacobi", dict(precision=0, precond=1, tol=1e-8, max_iter=100000)),
+            ("fp64_two_level", dict(precision=0, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
+            ("fp32_ir_two_level", dict(precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
+            ("fp64_learned", dict(precision=0, precond=pdx.PDX_PREC_LEARNED, tol=1e-8)),
+            ("fp32_ir_learned", dict(precision=1, precond=pdx.PDX_PREC_LEARNED, tol=1e-8)))
+    for s in held_seeds:
+        cx = pdx.Context(pdx.Model(W.crack2d(seed=s)))
+        cx.assemble(pdx.PDX_MODE_STATIC)
+        _, infer_ms = net.infer(cx, bf16=True)
+        row = {"bf16_inference_ms": infer_ms}
+        for name, kw in cfgs:
+            try:
+                cx.static_solve(solve_opts(pdx, **kw))
+                _, st = cx.static_solve(solve_opts(pdx, **kw))
+                row[name] = {"seconds": st["seconds"], "iterations": st["iterations"], "true_relres": st["true_relres"]}
+            except Exception as e:  # reported, not hidden
+                row[name] = {"error": str(e)}
+        if all("seconds" in row[k] for k in ("plain_fp64_jacobi", "fp32_ir_learned", "fp64_learned")):
+            row["speedup_fp32_ir_learned_vs_plain"] = row["plain_fp64_jacobi"]["seconds"] / row["fp32_ir_learned"]["seconds"]
+            row["speedup_fp64_learned_vs_plain"] = row["plain_fp64_jacobi"]["seconds"] / row["fp64_learned"]["seconds"]
+        out["held_out"][f"seed {s}"] = row
+    return out
+
+
 def static_plate_bench(pdx, W):
     """The reference's CPU-runnable static case: a 2D plane-stress plate 100 x 50 mm
     (201 x 101 nodes, 40,299 eq) with a central PDLSM strip under displacement-controlled
@@ -497,6 +539,11 @@ def main():
         except Exception as e:  # reported, not hidden
             out["gnn_inference"] = {"error": str(e)}
         log("gnn inference", out["gnn_inference"])
+        try:
+            out["learned_vs_plain"] = learned_bench(pdx, W)
+        except Exception as e:  # reported, not hidden
+            out["learned_vs_plain"] = {"error": str(e)}
+        log("learned vs plain", out["learned_vs_plain"])
     if not args.no_solve:
         out["static_plate2d"] = static_plate_bench(pdx, W)
         log("static plate2d", out["static_plate2d"])
