@@ -46,6 +46,56 @@ __constant__ int c_pe2[5][2] = {{1, 0}, {0, 1}, {2, 0}, {0, 2}, {1, 1}};
 template <int D>
 __device__ inline double a_term(int e, double x, double y, double z, double om, double vm);
 
+// the products of a_term with the exponents of entry e fixed per lane: multiplications by a
+// selected factor, 1.0 where a_term's loops would stop (exact), so the bits are the same and the
+// lanes do not diverge
+struct ATermExp {
+    int ex, ey, ez;
+    double half;
+};
+template <int D>
+__device__ inline ATermExp a_term_exp(int e) {
+    ATermExp t;
+    if constexpr (D == 3) {
+        const int r = e / 9, c = e % 9;
+        t.ex = c_pe3[r][0] + c_pe3[c][0];
+        t.ey = c_pe3[r][1] + c_pe3[c][1];
+        t.ez = c_pe3[r][2] + c_pe3[c][2];
+        t.half = (c >= 3 && c < 6) ? 0.5 : 1.0;
+    } else {
+        const int r = e / 5, c = e % 5;
+        t.ex = c_pe2[r][0] + c_pe2[c][0];
+        t.ey = c_pe2[r][1] + c_pe2[c][1];
+        t.ez = 0;
+        t.half = (c == 2 || c == 3) ? 0.5 : 1.0;
+    }
+    return t;
+}
+
+template <int D>
+__device__ inline double a_term_sel(int e, const ATermExp &q, double x, double y, double z, double om, double vm) {
+#pragma clang fp contract(off)
+    if constexpr (D == 3) {
+        double t = 1.0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) t = t * (q.ex > i ? x : 1.0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) t = t * (q.ey > i ? y : 1.0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) t = t * (q.ez > i ? z : 1.0);
+        t = t * q.half;
+        return (om * vm) * t;
+    } else {
+        if (e == 0) return ((om * x) * x) * vm;
+        double t = om * vm;
+#pragma unroll
+        for (int i = 0; i < 4; i++) t = t * (q.ex > i ? x : 1.0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) t = t * (q.ey > i ? y : 1.0);
+        return t * q.half;
+    }
+}
+
 template <>
 __device__ inline double a_term<3>(int e, double x, double y, double z, double om, double vm) {
 #pragma clang fp contract(off)
@@ -83,7 +133,8 @@ struct FamArgs {
     double *fam_c;  // [members * d] strain weights mu_km omega g V of every bond (kept for stresses)
     long long nblk;
     int *status;
-    int mf;  // LDS capacity in members
+    int mf;     // LDS capacity in members
+    int maxpb;  // LDS capacity in PD-boundary entities per family (3D N.D cache)
 };
 
 // normMatArea (pdsolve.cpp:2917-2967): area-weighted face normal matrix 3x6
@@ -134,6 +185,35 @@ __device__ inline void c_block(const double c[D], double Cb[Dm<D>::NV * D]) {
         Cb[4 * 3 + 1] = c[2]; Cb[4 * 3 + 2] = c[1];
         Cb[5 * 3 + 0] = c[2]; Cb[5 * 3 + 2] = c[0];
     }
+}
+
+// 3D: Nm . D (3 x 6) of PD-boundary entity p into ND; a degenerate face sets the status and
+// leaves ND zero (the caller reports the error)
+__device__ inline void pdbe_nd3(const FamArgs &A, int p, double *ND) {
+#pragma clang fp contract(off)
+    double Nm[18];
+    if (!norm_mat_area(A.x, &A.pdbe[p * 4], Nm)) {
+        atomicOr(A.status, 4);
+        for (int i = 0; i < 18; i++) ND[i] = 0.0;
+        return;
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 6; j++) {
+            double t = 0.0;
+            for (int k = 0; k < 6; k++) t = Nm[i * 6 + k] * A.Dm[k * 6 + j] + t;
+            ND[i * 6 + j] = t;
+        }
+}
+
+// 3D: (Nm . D) . C with the cached Nm . D, added to val (3 x 3)
+__device__ inline void add_pdbe_nd3(const double *ND, const double *Cb, double *val) {
+#pragma clang fp contract(off)
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double t = 0.0;
+            for (int k = 0; k < 6; k++) t = ND[i * 6 + k] * Cb[k * 3 + j] + t;
+            val[i * 3 + j] = val[i * 3 + j] + t;
+        }
 }
 
 // boundary traction term of one PD-boundary entity for one C block, added to val (D x D)
@@ -199,7 +279,8 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
     double *svm = som + mf;         // [mf] member volume
     double *sgv = svm + mf;         // [mf*DD] G_m * V_m
     double *sc = sgv + (size_t)mf * DD;  // [mf*D]  mu_km * om * g * V_m
-    double *sA = sc + (size_t)mf * D;    // [P*P]
+    double *sND = sc + (size_t)mf * D;   // [maxpb*18] 3D: Nm . D of the family's PD-boundary entities
+    double *sA = sND + (D == 3 ? (size_t)A.maxpb * 18 : 0);  // [P*P]
     int *spiv = (int *)(sA + P * P);     // [P] + flag
     if (n > mf) {
         if (lane == 0) atomicOr(A.status, 8);
@@ -220,8 +301,9 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
     __syncthreads();
     // shape tensor, one entry per lane, members in reference order
     for (int e = lane; e < P * P; e += 64) {
+        const ATermExp q = a_term_exp<D>(e);
         double acc = 0.0;
-        for (int m = 1; m < n; m++) acc = a_term<D>(e, sx[m], sy[m], sz[m], som[m], svm[m]) + acc;
+        for (int m = 1; m < n; m++) acc = a_term_sel<D>(e, q, sx[m], sy[m], sz[m], som[m], svm[m]) + acc;
         sA[e] = acc;
     }
     __syncthreads();
@@ -259,6 +341,12 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
     const double cdv = A.fam_v[f];
     const double lam = A.lam, mu = A.mu;
     const int p0 = A.fp_off[f], p1 = A.fp_off[f + 1];
+    if constexpr (D == 3) {
+        // Nm . D of every incident PD-boundary entity, once per family instead of once per member
+        // (the same products in the same order, so the same bits)
+        for (int q = p0 + lane; q < p1; q += 64) pdbe_nd3(A, A.fp_list[q], sND + (size_t)(q - p0) * 18);
+        __syncthreads();
+    }
     const size_t nb = (size_t)A.nblk;
     for (int m = 1 + lane; m < n; m += 64) {
         const double x0 = sx[m], x1 = sy[m], x2 = sz[m];
@@ -326,7 +414,10 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
         if (p1 > p0) {
             double Cb[Dm<D>::NV * D];
             c_block<D>(c, Cb);
-            for (int q = p0; q < p1; q++) add_pdbe_term<D>(A, A.fp_list[q], Cb, val);
+            for (int q = p0; q < p1; q++) {
+                if constexpr (D == 3) add_pdbe_nd3(sND + (size_t)(q - p0) * 18, Cb, val);
+                else add_pdbe_term<D>(A, A.fp_list[q], Cb, val);
+            }
         }
         const size_t blk = (size_t)A.fam_blk[off + m];
 #pragma unroll
@@ -350,7 +441,10 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
             }
             double Cb[Dm<D>::NV * D];
             c_block<D>(cs, Cb);
-            for (int q = p0; q < p1; q++) add_pdbe_term<D>(A, A.fp_list[q], Cb, val);
+            for (int q = p0; q < p1; q++) {
+                if constexpr (D == 3) add_pdbe_nd3(sND + (size_t)(q - p0) * 18, Cb, val);
+                else add_pdbe_term<D>(A, A.fp_list[q], Cb, val);
+            }
         }
         A.K[(size_t)A.fam_blk[off] * DD + e] = val[e];
     }
@@ -839,9 +933,10 @@ int dev_fill(Ctx &c, double *p, double v, size_t n) {
     return hipGetLastError() == hipSuccess ? PDX_OK : PDX_ERR_HIP;
 }
 
-static size_t family_lds_bytes(int D, int mf) {
+static size_t family_lds_bytes(int D, int mf, int maxpb) {
     const int P = D == 2 ? 5 : 9, DD = D * D;
-    return (size_t)mf * (5 + DD + D) * sizeof(double) + (size_t)P * P * sizeof(double) + (P + 2) * sizeof(int);
+    return (size_t)mf * (5 + DD + D) * sizeof(double) + (D == 3 ? (size_t)maxpb * 18 * sizeof(double) : 0) +
+           (size_t)P * P * sizeof(double) + (P + 2) * sizeof(int);
 }
 
 int dev_assemble_K(Ctx &c, std::string &err) {
@@ -856,6 +951,7 @@ int dev_assemble_K(Ctx &c, std::string &err) {
         A.fp_off = c.fp_off; A.fp_list = c.fp_list; A.pdbe = c.pdbe; A.Dm = c.Dm;
         A.lam = c.lambda; A.mu = c.mu; A.nlf = c.nlf; A.K = c.K; A.nblk = c.nblk; A.status = status;
         A.mf = c.max_family;
+        A.maxpb = c.max_pdbe_per_family;
         if (!c.fam_c) {
             int nmem = 0;
             PDX_HIP(hipMemcpy(&nmem, c.fam_off + c.nfam, sizeof(int), hipMemcpyDeviceToHost));
@@ -863,7 +959,7 @@ int dev_assemble_K(Ctx &c, std::string &err) {
             PDX_HIP(hipMemsetAsync(c.fam_c, 0, std::max<size_t>((size_t)nmem * c.dim, 1) * sizeof(double), c.stream));
         }
         A.fam_c = c.fam_c;
-        size_t lds = family_lds_bytes(c.dim, A.mf);
+        size_t lds = family_lds_bytes(c.dim, A.mf, A.maxpb);
         if (lds > 160 * 1024) {
             err = "family too large for LDS (" + std::to_string(c.max_family) + " members)";
             hipFreeAsync(status, c.stream);

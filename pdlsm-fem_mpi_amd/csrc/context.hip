@@ -154,6 +154,8 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
         }
         UP(c->fp_off, off);
         UP(c->fp_list, lst);
+        c->max_pdbe_per_family = 0;
+        for (int f = 0; f < m.nfam; f++) c->max_pdbe_per_family = std::max(c->max_pdbe_per_family, off[f + 1] - off[f]);
         std::vector<int> pd(m.pdbe);
         UP(c->pdbe, pd);
     }

@@ -64,6 +64,7 @@ struct Ctx {
     double *fam_v = nullptr, *fam_delta = nullptr;
     double *fam_c = nullptr;  // [members*d] strain weights of the bonds (written by the family kernel)
     int *fp_off = nullptr, *fp_list = nullptr, *pdbe = nullptr;
+    int max_pdbe_per_family = 0;  // most PD-boundary entities incident to one family
     int nfe = 0;
     int *fe_elem = nullptr, *fe_blk = nullptr, *nfe_off = nullptr, *nfe_item = nullptr;
     int *fec_off = nullptr, *fec = nullptr;  // per K block: FE contributions (staged block << 1 | transposed)
