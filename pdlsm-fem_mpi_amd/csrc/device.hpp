@@ -138,6 +138,8 @@ struct Ctx {
     void *two_level = nullptr;       // coarse space of the last two-level solve (krylov.hip)
     void *ilu0 = nullptr;            // block ILU(0) factor of the last ILU(0) solve (ilu0.hip)
     void *krylov_ws[2] = {nullptr, nullptr};  // Krylov workspaces (FP64, FP32) and their graphs (krylov.hip)
+    double *solve_y = nullptr;   // FP64 residual of the refinement / restart loop [nn*d]
+    float *solve_b32 = nullptr;  // FP32 right-hand side of the refined inner solves [nn*d]
     double *learned_Minv = nullptr;  // [nn*dd] node-block smoother predicted by the graph network
     long long learned_gen = -1;      // K_gen the smoother was predicted for
     void *gnn_work = nullptr;        // graph-network inputs and activations (learned.hip)

@@ -955,6 +955,8 @@ struct KrylovWs {
     double *rc = nullptr, *yc = nullptr;
     size_t rc_cap = 0;
     V *RSbuf = nullptr;  // FP32 block row sums of the operator (FP32 workspaces)
+    double *y64 = nullptr;  // FP64 scratch vector of the start-up residual
+    int *status = nullptr;  // singular-block flag of the block-diagonal preconditioners
     // chunks of CHUNK iterations: the first (iteration 0 starts a direction) and any later one
     hipGraphExec_t g_first = nullptr, g_next = nullptr;
     GraphSig sig;
@@ -975,6 +977,8 @@ static int ws_alloc(Ctx &c, KrylovWs<V> &w, std::string &err) {
         PDX_HIP(hipMemsetAsync(*pp, 0, n * sizeof(V), c.stream));
     }
     PDX_HIP(hipMalloc((void **)&w.Minv, (size_t)c.nn * c.dd * sizeof(V)));
+    PDX_HIP(hipMalloc((void **)&w.y64, n * sizeof(double)));
+    PDX_HIP(hipMalloc((void **)&w.status, sizeof(int)));
     return PDX_OK;
 }
 
@@ -988,7 +992,8 @@ static void ws_drop_graphs(KrylovWs<V> &w) {
 template <typename V>
 static void ws_free(KrylovWs<V> &w) {
     ws_drop_graphs(w);
-    void *ptrs[] = {w.x, w.r, w.rh, w.p, w.v, w.s, w.t, w.ph, w.sh, w.xb, w.Minv, w.rc, w.yc, (void *)w.RSbuf};
+    void *ptrs[] = {w.x, w.r, w.rh, w.p, w.v, w.s, w.t, w.ph, w.sh, w.xb, w.Minv, w.rc, w.yc, (void *)w.RSbuf,
+                    w.y64, w.status};
     for (auto p : ptrs)
         if (p) hipFree(p);
 }
@@ -1017,6 +1022,10 @@ static int ws_get(Ctx &c, KrylovWs<V> *&out, std::string &err) {
 }
 
 void free_krylov_ws(Ctx &c) {
+    if (c.solve_y) hipFree(c.solve_y);
+    if (c.solve_b32) hipFree(c.solve_b32);
+    c.solve_y = nullptr;
+    c.solve_b32 = nullptr;
     if (c.krylov_ws[0]) {
         ws_free(*(KrylovWs<double> *)c.krylov_ws[0]);
         delete (KrylovWs<double> *)c.krylov_ws[0];
@@ -1146,8 +1155,7 @@ static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm
                     bool use_graph, int *iters, double *relres, std::string &err) {
     const size_t n = (size_t)c.nn * c.dim;
     double *red = c.red, *sc = c.red + (size_t)RED_SLOTS * NPART;
-    double *y;
-    PDX_HIP(hipMalloc((void **)&y, n * sizeof(double)));
+    double *y = w.y64;
     {
         V *tmp = w.t;
         spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.x, tmp, (const V *)nullptr, nullptr, (const V *)nullptr, nullptr);
@@ -1157,7 +1165,6 @@ static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm
         PDX_HIP(hipMemsetAsync(w.p, 0, n * sizeof(V), c.stream));
         PDX_HIP(hipMemsetAsync(w.v, 0, n * sizeof(V), c.stream));
     }
-    hipFree(y);
     double rr0 = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
     double rel = std::sqrt(rr0) / bnorm;
     int it = 0, since_best = 0;
@@ -1203,12 +1210,10 @@ static int pcg(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, dou
                bool use_graph, int *iters, double *relres, std::string &err) {
     const size_t n = (size_t)c.nn * c.dim;
     double *red = c.red;
-    double *y;
-    PDX_HIP(hipMalloc((void **)&y, n * sizeof(double)));
+    double *y = w.y64;
     spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.x, w.t, (const V *)nullptr, nullptr, (const V *)nullptr, nullptr);
     k_copy<V, double><<<2048, 256, 0, c.stream>>>(n, w.t, y);
     k_resid<V, V><<<RED_PARTS, 256, 0, c.stream>>>(n, b, y, w.r, nullptr, red + (size_t)P_RR * NPART);
-    hipFree(y);
     // z = Minv r and rho0 = (r, z) through a zero step
     PDX_HIP(hipMemsetAsync(w.p, 0, n * sizeof(V), c.stream));
     PDX_HIP(hipMemsetAsync(w.v, 0, n * sizeof(V), c.stream));
@@ -1263,9 +1268,7 @@ static int pcg(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, dou
 }
 
 template <int D, typename V>
-static int build_precond(Ctx &c, const double *K64, int kind, V *Minv, std::string &err) {
-    int *status;
-    PDX_HIP(hipMalloc((void **)&status, sizeof(int)));
+static int build_precond(Ctx &c, const double *K64, int kind, V *Minv, int *status, std::string &err) {
     PDX_HIP(hipMemsetAsync(status, 0, sizeof(int), c.stream));
     if (kind == PDX_PREC_NONE) {
         std::vector<V> h((size_t)c.nn * c.dd, V(0));
@@ -1278,7 +1281,6 @@ static int build_precond(Ctx &c, const double *K64, int kind, V *Minv, std::stri
         PDX_HIP(hipStreamSynchronize(c.stream));
     } else if (kind == PDX_PREC_LEARNED) {
         if (!c.learned_Minv || c.learned_gen != c.K_gen) {
-            hipFree(status);
             err = "no learned smoother for the current operator (run pdx_gnn_apply first)";
             return PDX_ERR_ARG;
         }
@@ -1290,7 +1292,6 @@ static int build_precond(Ctx &c, const double *K64, int kind, V *Minv, std::stri
     int hs = 0;
     PDX_HIP(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     PDX_HIP(hipStreamSynchronize(c.stream));
-    hipFree(status);
     if (hs) { err = "singular diagonal block in the preconditioner"; return PDX_ERR_SINGULAR; }
     return PDX_OK;
 }
@@ -1306,7 +1307,7 @@ static int setup_precond(Ctx &c, const double *K64, const pdx_solve_opts &o, Kry
     }
     const bool two = o.precond == PDX_PREC_TWO_LEVEL || o.precond == PDX_PREC_LEARNED;
     int rc = build_precond<D, V>(c, K64, two && o.precond != PDX_PREC_LEARNED ? PDX_PREC_BLOCK_JACOBI : o.precond,
-                                 w.Minv, err);
+                                 w.Minv, w.status, err);
     if (rc || !two) return rc;
     TwoLevel *t;
     if ((rc = get_two_level(c, K64, o.coarse_agg, t, err))) return rc;
@@ -1365,12 +1366,13 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
     if (!o.warm_start) PDX_HIP(hipMemsetAsync(x_dev, 0, n * 8, c.stream));
     int total = 0, refine = 0, rc = PDX_OK;
     double rel = 1.0;
-    double *y;
-    PDX_HIP(hipMalloc((void **)&y, n * 8));
+    // FP64 residual vector and FP32 right-hand side, kept on the context across solves
+    if (!c.solve_y) PDX_HIP(hipMalloc((void **)&c.solve_y, n * 8));
+    double *y = c.solve_y;
     const bool use_graph = !o.no_graph;
     if (o.precision == PDX_FP64) {
         KrylovWs<double> *wp;
-        if ((rc = ws_get(c, wp, err)) || (rc = setup_precond<D, double>(c, K64, o, *wp, err))) { hipFree(y); return rc; }
+        if ((rc = ws_get(c, wp, err)) || (rc = setup_precond<D, double>(c, K64, o, *wp, err))) return rc;
         KrylovWs<double> &w = *wp;
         PDX_HIP(hipMemcpyAsync(w.x, x_dev, n * 8, hipMemcpyDeviceToDevice, c.stream));
         double prev = INFINITY;
@@ -1393,11 +1395,11 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
         }
         PDX_HIP(hipMemcpyAsync(x_dev, w.x, n * 8, hipMemcpyDeviceToDevice, c.stream));
     } else {
-        if (!K32) { err = "FP32 matrix missing"; hipFree(y); return PDX_ERR_ARG; }
+        if (!K32) { err = "FP32 matrix missing"; return PDX_ERR_ARG; }
         KrylovWs<float> *wp;
-        float *b32;
-        PDX_HIP(hipMalloc((void **)&b32, n * 4));
-        if ((rc = ws_get(c, wp, err)) || (rc = setup_precond<D, float>(c, K64, o, *wp, err))) { hipFree(y); hipFree(b32); return rc; }
+        if (!c.solve_b32) PDX_HIP(hipMalloc((void **)&c.solve_b32, n * 4));
+        float *b32 = c.solve_b32;
+        if ((rc = ws_get(c, wp, err)) || (rc = setup_precond<D, float>(c, K64, o, *wp, err))) return rc;
         KrylovWs<float> &w = *wp;
         if (!w.RSbuf) PDX_HIP(hipMalloc((void **)&w.RSbuf, (size_t)c.nn * c.dd * 4));
         k_block_rowsum<D, float><<<(c.nn + 255) / 256, 256, 0, c.stream>>>(c.nn, c.adj_off, K64, w.RSbuf);
@@ -1426,9 +1428,7 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
             if (rc) break;
             k_add_f2d<<<2048, 256, 0, c.stream>>>(n, w.x, x_dev);
         }
-        hipFree(b32);
     }
-    hipFree(y);
     PDX_HIP(hipStreamSynchronize(c.stream));
     st.iterations = total;
     st.refinements = refine;
@@ -1600,8 +1600,12 @@ int two_level_probes(Ctx &c, const double *K64, int agg, const double *z, double
 
 // node-block Jacobi inverse of K (FP64, [nn][d*d], prescribed rows/columns zero)
 int block_jacobi_inverse(Ctx &c, const double *K64, double *Minv, std::string &err) {
-    if (c.dim == 3) return build_precond<3, double>(c, K64, PDX_PREC_BLOCK_JACOBI, Minv, err);
-    return build_precond<2, double>(c, K64, PDX_PREC_BLOCK_JACOBI, Minv, err);
+    int *status;
+    PDX_HIP(hipMalloc((void **)&status, sizeof(int)));
+    const int rc = c.dim == 3 ? build_precond<3, double>(c, K64, PDX_PREC_BLOCK_JACOBI, Minv, status, err)
+                              : build_precond<2, double>(c, K64, PDX_PREC_BLOCK_JACOBI, Minv, status, err);
+    hipFree(status);
+    return rc;
 }
 
 double two_level_setup_ms(Ctx &c) {
