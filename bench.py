@@ -314,11 +314,12 @@ def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
     nodes, bottom fixed, top traction ramped linearly to 5 MPa over the 50 steps,
     beta = 1/4, gamma = 1/2, dt = 10x the explicit critical step, a_{n+1}
     warm-started; each step solves (M + beta dt^2 K) a = F - K U_N to a 1e-8 true
-    residual, three ways: plain (FP64 node-block-Jacobi BiCGStab), FP32
-    BiCGStab with FP64 refinement and node-block Jacobi, and the same with the
-    two-level hybrid preconditioner (aggregates of coarse_agg^3 nodes, built
-    once per run).  The mixed runs' displacements are compared with the plain
-    run's."""
+    residual: plain (FP64 node-block-Jacobi BiCGStab), FP32 BiCGStab with FP64
+    refinement and node-block Jacobi, the same with the two-level hybrid
+    preconditioner (aggregates of coarse_agg^3 nodes, built once per run), and
+    the two block-Jacobi runs again with the SpMVs on mirrored storage (same
+    results bit for bit).  The other runs' displacements are compared with the
+    plain run's."""
     t0 = time.time()
     m0 = W.penny3d(n=n)
     cx0 = pdx.Context(pdx.Model(m0))
@@ -335,7 +336,11 @@ def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
             "fp32_ir_blockjacobi": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
                                                   max_iter=100000),
             "fp32_ir_two_level": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8,
-                                                max_iter=100000, coarse_agg=coarse_agg)}
+                                                max_iter=100000, coarse_agg=coarse_agg),
+            "plain_fp64_blockjacobi_mirrored": solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_BLOCK_JACOBI,
+                                                              tol=1e-8, max_iter=100000, mirrored=True),
+            "fp32_ir_blockjacobi_mirrored": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_BLOCK_JACOBI,
+                                                           tol=1e-8, max_iter=100000, mirrored=True)}
     us = {}
     for name, opts in runs.items():
         cx = pdx.Context(mp)
@@ -350,6 +355,8 @@ def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
         out[name] = {"seconds": wall, "ms_per_step": wall / steps * 1e3, "krylov_iterations": st["iterations"],
                      "refinements": st["refinements"], "last_true_relres": st["true_relres"],
                      "dof_steps_per_s": mp.neq * steps / wall, "u_max": float(np.abs(u).max())}
+        if st["mirrored_used"]:
+            out[name]["mirrored_storage"] = True
         del cx
     ref = us.get("plain_fp64_blockjacobi")
     for name in us:
@@ -520,7 +527,11 @@ def main():
                 ("fp64_two_level_bicgstab", solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
                 ("fp32_ir_two_level_bicgstab", solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
                 ("fp64_ilu0_bicgstab", solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_ILU0, tol=1e-8)),
-                ("fp32_ir_ilu0_bicgstab", solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_ILU0, tol=1e-8)))
+                ("fp32_ir_ilu0_bicgstab", solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_ILU0, tol=1e-8)),
+                ("plain_fp64_jacobi_bicgstab_mirrored", solve_opts(pdx, precision=0, precond=1, tol=1e-8,
+                                                                   max_iter=100000, mirrored=True)),
+                ("fp64_two_level_bicgstab_mirrored", solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_TWO_LEVEL,
+                                                                tol=1e-8, mirrored=True)))
         for name, o in cfgs:
             try:
                 # first call builds the coarse space of the two-level kinds (the
@@ -530,6 +541,8 @@ def main():
                 solves[name] = {"seconds": st["seconds"], "seconds_first_call": st0["seconds"],
                                 "iterations": st["iterations"], "refinements": st["refinements"],
                                 "true_relres": st["true_relres"]}
+                if st["mirrored_used"]:
+                    solves[name]["mirrored_storage"] = True
             except Exception as e:  # reported, not hidden
                 solves[name] = {"error": str(e)}
             log(name, solves[name])

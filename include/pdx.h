@@ -123,6 +123,11 @@ typedef struct pdx_solve_opts {
                           from the iterate the failed attempt left when it beat zero */
     int no_graph;      /* nonzero: launch every Krylov iteration's kernels one by one instead of
                           replaying captured hipGraphs of 10 iterations (same results, bit for bit) */
+    int mirrored;      /* nonzero (2D / 3D models): the SpMVs read the operator from mirrored storage,
+                          each lower block that no family stores in either orientation read as its
+                          upper partner's transpose, once a device check has found every such block
+                          equal to that transpose bit for bit (else full storage; same results, bit
+                          for bit, either way) */
 } pdx_solve_opts;
 
 typedef struct pdx_solve_stats {
@@ -134,6 +139,7 @@ typedef struct pdx_solve_stats {
     int precond_used;   /* PDX_PREC_* of the attempt that ended the solve */
     int precision_used; /* PDX_FP64 / PDX_FP32 of that attempt */
     int attempts;       /* 1 + fallback retries */
+    int mirrored_used;  /* 1 when the SpMVs read mirrored storage (pdx_solve_opts.mirrored) */
 } pdx_solve_stats;
 
 const char *pdx_last_error(void);

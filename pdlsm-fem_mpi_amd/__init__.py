@@ -46,12 +46,13 @@ class MeshDesc(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("krylov", _I), ("precond", _I), ("precision", _I), ("tol", _D), ("max_iter", _I),
                 ("inner_tol", _D), ("warm_start", _I), ("coarse_agg", _I), ("fallback", _I),
-                ("no_graph", _I)]
+                ("no_graph", _I), ("mirrored", _I)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", _I), ("refinements", _I), ("true_relres", _D), ("seconds", _D),
-                ("converged", _I), ("precond_used", _I), ("precision_used", _I), ("attempts", _I)]
+                ("converged", _I), ("precond_used", _I), ("precision_used", _I), ("attempts", _I),
+                ("mirrored_used", _I)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -273,11 +274,11 @@ class Model:
 
 
 def solve_opts(krylov=PDX_KRYLOV_BICGSTAB, precond=PDX_PREC_JACOBI, precision=PDX_FP64, tol=1e-8, max_iter=20000,
-               inner_tol=1e-4, warm_start=False, coarse_agg=0, fallback=False, no_graph=False):
+               inner_tol=1e-4, warm_start=False, coarse_agg=0, fallback=False, no_graph=False, mirrored=False):
     o = SolveOpts()
     o.krylov, o.precond, o.precision, o.tol, o.max_iter = krylov, precond, precision, tol, max_iter
     o.inner_tol, o.warm_start, o.coarse_agg, o.fallback = inner_tol, int(warm_start), int(coarse_agg), int(fallback)
-    o.no_graph = int(no_graph)
+    o.no_graph, o.mirrored = int(no_graph), int(mirrored)
     return o
 
 

@@ -13,8 +13,6 @@
 
 namespace pdx {
 
-int dev_solve_mat(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
-                  double *x_dev, pdx_solve_stats &st, std::string &err);
 int to_f32(Ctx &c, const double *K64, float *&K32, std::string &err);
 int set_error(int rc, const std::string &msg);
 

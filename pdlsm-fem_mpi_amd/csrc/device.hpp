@@ -111,6 +111,8 @@ struct Ctx {
     double *Kown64 = nullptr;
     float *Kown32 = nullptr;
     long long kown_gen[2] = {-1, -1};
+    long long kown_exact_gen = -1;  // K generation whose mirrored blocks were checked exact
+    bool kown_exact = false;
     double *K64s = nullptr;
     float *K32s = nullptr;
     long long ks_gen[2] = {-1, -1};
@@ -173,6 +175,21 @@ int dev_spmv(Ctx &c, const double *x, double *y, int prec, bool mask_free);
 int dev_fill(Ctx &c, double *p, double v, size_t n);
 
 int dev_spmv_K(Ctx &c, const double *K64, const double *x, double *y);
+// Own-block copies (mirrored storage, ksrc order) of one node-block operator for the Krylov
+// SpMVs; a null member means that precision reads the full operator
+struct Mirrored {
+    const double *K64 = nullptr;
+    const float *K32 = nullptr;
+};
+// exact: every mirrored lower block of A (dd*nblk values, K's block order) equals its partner's
+// transpose bit for bit (checked on the device; synchronises the stream)
+int check_mirrored(Ctx &c, const double *A, bool &exact, std::string &err);
+// own64 / own32 (either may be null; dd*n_own values) <- the own blocks of A
+int gather_own(Ctx &c, const double *A, double *own64, float *own32, std::string &err);
+// mirrored copies of the context's K for a solve in precision prec (empty when not exact)
+int kown_for_solve(Ctx &c, int prec, Mirrored &m, std::string &err);
+int dev_solve_mat(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
+                  double *x_dev, pdx_solve_stats &st, std::string &err, const Mirrored *mir = nullptr);
 void free_two_level(Ctx &c);
 // block ILU(0) preconditioner (ilu0.hip): factor of K64 cached per generation, applications on c.stream
 int ilu0_prepare(Ctx &c, const double *K64, bool f32, void *&out, std::string &err);

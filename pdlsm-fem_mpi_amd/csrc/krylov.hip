@@ -47,19 +47,21 @@ int ensure_K32(Ctx &c, std::string &err);
 // against the neighbours for smooth vectors.
 // Rows [lb * per, (lb + 1) * per) of the list go to logical block lb; the G-lane groups of the
 // block stride through them.  With REL the block columns come from the row's column pattern
-// class (cls, cls_off, cls_rel) instead of adj.
-template <int D, typename T, typename V, int G, bool DIFF, bool REL>
+// class (cls, cls_off, cls_rel) instead of adj.  With SYM, K holds only the own blocks of the
+// mirrored storage and src[b] is block b's own index, or ~(its partner's) for a lower block read
+// as its partner's transpose (the same values, so the same products in the same order).
+template <int D, typename T, typename V, int G, bool DIFF, bool REL, bool SYM>
 __device__ inline void spmv_dot_rows(const int *rows, int nrows, int lb, int nblocks, const int *adj_off,
                                      const int *adj, const int *cls, const int *cls_off, const int *cls_rel,
-                                     const T *K, const T *RS, const V *x, V *y, const int *eq, const V *w0,
-                                     const V *w1, double &d0, double &d1) {
+                                     const int *src, const T *K, const T *RS, const V *x, V *y, const int *eq,
+                                     const V *w0, const V *w1, double &d0, double &d1) {
     const int lane = threadIdx.x % G;
     const int per = (nrows + nblocks - 1) / nblocks, g0 = lb * per, g1 = min(nrows, g0 + per);
     for (int gid = g0 + (int)threadIdx.x / G; gid < g1; gid += (int)blockDim.x / G) {
         const int r = rows[gid];
         T acc[D];
         const int *rel = REL ? cls_rel + cls_off[cls[r]] : nullptr;
-        row_matvec<D, T, V, G, DIFF, false, false, REL>(r, lane, adj_off, adj, K, x, acc, rel);
+        row_matvec<D, T, V, G, DIFF, false, false, REL, SYM>(r, lane, adj_off, adj, K, x, acc, rel, src);
         if (lane < D) {
             T v = pick<D, T>(acc, lane);
             if (DIFF) {
@@ -77,26 +79,26 @@ __device__ inline void spmv_dot_rows(const int *rows, int nrows, int lb, int nbl
 
 // The hardware deals blocks to the 8 XCDs round-robin: logical block (b % 8) * (grid / 8) + b / 8
 // gives each XCD (and its L2) one contiguous eighth of the rows, so its gathers of x stay local.
-template <int D, typename T, typename V, bool REL>
+template <int D, typename T, typename V, bool REL, bool SYM>
 __global__ void __launch_bounds__(SPMV_THREADS) k_spmv_dot(const int *rows_s, int ns, const int *rows_l, int nl,
                                                   const int *adj_off, const int *adj, const int *cls,
-                                                  const int *cls_off, const int *cls_rel, const T *K, const T *RS,
-                                                  const V *x, V *y, const int *eq, const V *w0, double *part0,
-                                                  const V *w1, double *part1) {
+                                                  const int *cls_off, const int *cls_rel, const int *src,
+                                                  const T *K, const T *RS, const V *x, V *y, const int *eq,
+                                                  const V *w0, double *part0, const V *w1, double *part1) {
     __shared__ double sm[17];
     double d0 = 0.0, d1 = 0.0;
     const int nb = gridDim.x;
     const int lb = nb % 8 == 0 ? (int)(blockIdx.x % 8) * (nb / 8) + (int)blockIdx.x / 8 : (int)blockIdx.x;
     if (RS) {
-        spmv_dot_rows<D, T, V, 8, true, REL>(rows_s, ns, lb, nb, adj_off, adj, cls, cls_off, cls_rel, K, RS, x, y, eq,
-                                             w0, w1, d0, d1);
-        spmv_dot_rows<D, T, V, 64, true, REL>(rows_l, nl, lb, nb, adj_off, adj, cls, cls_off, cls_rel, K, RS, x, y,
-                                              eq, w0, w1, d0, d1);
+        spmv_dot_rows<D, T, V, 8, true, REL, SYM>(rows_s, ns, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
+                                                  RS, x, y, eq, w0, w1, d0, d1);
+        spmv_dot_rows<D, T, V, 64, true, REL, SYM>(rows_l, nl, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
+                                                   RS, x, y, eq, w0, w1, d0, d1);
     } else {
-        spmv_dot_rows<D, T, V, 8, false, REL>(rows_s, ns, lb, nb, adj_off, adj, cls, cls_off, cls_rel, K, RS, x, y,
-                                              eq, w0, w1, d0, d1);
-        spmv_dot_rows<D, T, V, 64, false, REL>(rows_l, nl, lb, nb, adj_off, adj, cls, cls_off, cls_rel, K, RS, x, y,
-                                               eq, w0, w1, d0, d1);
+        spmv_dot_rows<D, T, V, 8, false, REL, SYM>(rows_s, ns, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
+                                                   RS, x, y, eq, w0, w1, d0, d1);
+        spmv_dot_rows<D, T, V, 64, false, REL, SYM>(rows_l, nl, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
+                                                    RS, x, y, eq, w0, w1, d0, d1);
     }
     if (part0) {
         d0 = block_sum(d0, sm);
@@ -936,12 +938,12 @@ void free_two_level(Ctx &c) {
 // device pointers a captured chunk of iterations reads; a change of any of them re-captures
 struct GraphSig {
     const void *K = nullptr, *RS = nullptr, *tl = nullptr, *ilu = nullptr, *Pn = nullptr, *kp = nullptr,
-               *Ainv = nullptr, *rc = nullptr, *yc = nullptr, *red = nullptr;
+               *Ainv = nullptr, *rc = nullptr, *yc = nullptr, *red = nullptr, *src = nullptr;
     long long K_gen = -1, tl_serial = -1, ilu_serial = -1;  // launch shapes follow these objects
     int krylov = -1;
     bool same(const GraphSig &o) const {
         return K == o.K && RS == o.RS && tl == o.tl && ilu == o.ilu && Pn == o.Pn && kp == o.kp && Ainv == o.Ainv &&
-               rc == o.rc && yc == o.yc && red == o.red && K_gen == o.K_gen && tl_serial == o.tl_serial &&
+               rc == o.rc && yc == o.yc && red == o.red && src == o.src && K_gen == o.K_gen && tl_serial == o.tl_serial &&
                ilu_serial == o.ilu_serial && krylov == o.krylov;
     }
 };
@@ -966,6 +968,7 @@ struct KrylovWs {
     const float *Ainv32 = nullptr;  // the coarse inverse applied by both precisions
     const V *RS = nullptr;  // block row sums (difference-form SpMV) or null
     const void *ilu = nullptr;  // ILU(0) factor (ilu0.hip) or null
+    const int *src = nullptr;  // own index map when the iterations read mirrored storage, else null
 };
 
 template <typename V>
@@ -1086,6 +1089,7 @@ static GraphSig make_sig(Ctx &c, const void *K, const KrylovWs<V> &w, int krylov
     g.rc = w.rc;
     g.yc = w.yc;
     g.red = c.red;
+    g.src = w.src;
     g.K_gen = c.K_gen;
     g.tl_serial = w.tl ? w.tl->serial : -1;
     g.ilu_serial = w.ilu ? ilu0_serial(w.ilu) : -1;
@@ -1093,17 +1097,25 @@ static GraphSig make_sig(Ctx &c, const void *K, const KrylovWs<V> &w, int krylov
     return g;
 }
 
+// y = mask(A x) with fused dot partials; A is K (all blocks) or, with src, the own blocks of
+// its mirrored storage (node-block operators only)
 template <int D, typename T, typename V>
-static void spmv_dot(Ctx &c, const T *K, const T *RS, const V *x, V *y, const V *w0, double *p0, const V *w1,
-                     double *p1) {
-    if (c.cls)
-        k_spmv_dot<D, T, V, true><<<RED_PARTS, SPMV_THREADS, 0, c.stream>>>(
-            c.rows_short, c.n_short, c.rows_long, c.n_long, c.adj_off, c.adj, c.cls, c.cls_off, c.cls_rel, K, RS, x,
-            y, c.eq, w0, p0, w1, p1);
-    else
-        k_spmv_dot<D, T, V, false><<<RED_PARTS, SPMV_THREADS, 0, c.stream>>>(
-            c.rows_short, c.n_short, c.rows_long, c.n_long, c.adj_off, c.adj, nullptr, nullptr, nullptr, K, RS, x, y,
-            c.eq, w0, p0, w1, p1);
+static void spmv_dot(Ctx &c, const T *K, const int *src, const T *RS, const V *x, V *y, const V *w0, double *p0,
+                     const V *w1, double *p1) {
+#define PDX_SPMV_DOT(REL, SYM)                                                                                 \
+    k_spmv_dot<D, T, V, REL, SYM><<<RED_PARTS, SPMV_THREADS, 0, c.stream>>>(                                  \
+        c.rows_short, c.n_short, c.rows_long, c.n_long, c.adj_off, c.adj, REL ? c.cls : nullptr,              \
+        REL ? c.cls_off : nullptr, REL ? c.cls_rel : nullptr, src, K, RS, x, y, c.eq, w0, p0, w1, p1)
+    if constexpr (D > 1) {
+        if (src) {
+            if (c.cls) PDX_SPMV_DOT(true, true);
+            else PDX_SPMV_DOT(false, true);
+            return;
+        }
+    }
+    if (c.cls) PDX_SPMV_DOT(true, false);
+    else PDX_SPMV_DOT(false, false);
+#undef PDX_SPMV_DOT
 }
 
 // one BiCGStab iteration (it = iteration index, parity selects the rho slot)
@@ -1122,7 +1134,7 @@ static void bicg_iter(Ctx &c, const T *K, KrylovWs<V> &w, double *red, double *s
         k_coarse_mv<float><<<gc, 256, 0, c.stream>>>(t.nc, t.ld, w.Ainv32, w.rc, w.yc);
         k_prolong<D, NM, V><<<gn, 256, 0, c.stream>>>(c.nn, w.Pn, t.node_agg, t.kp_off, t.kp_agg, w.kp, w.Minv, w.p,
                                                       w.yc, w.ph);
-        spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.ph, w.v, w.rh, rv, (const V *)nullptr, nullptr);
+        spmv_dot<D, T, V>(c, K, w.src, (const T *)w.RS, w.ph, w.v, w.rh, rv, (const V *)nullptr, nullptr);
         k_bicg_s_agg<D, NM, V><<<RED_PARTS, 256, 0, c.stream>>>(t.na, t.agg_off, t.agg_node, w.r, w.v, w.s, w.Pn,
                                                                 rho, rv, sc, w.rc);
         k_coarse_mv<float><<<gc, 256, 0, c.stream>>>(t.nc, t.ld, w.Ainv32, w.rc, w.yc);
@@ -1132,15 +1144,15 @@ static void bicg_iter(Ctx &c, const T *K, KrylovWs<V> &w, double *red, double *s
         k_bicg_p<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.r, w.p, w.v, w.ph, (const V *)nullptr, rho, sc,
                                                         it == 0);
         ilu0_apply(c, w.ilu, w.p, w.ph);
-        spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.ph, w.v, w.rh, rv, (const V *)nullptr, nullptr);
+        spmv_dot<D, T, V>(c, K, w.src, (const T *)w.RS, w.ph, w.v, w.rh, rv, (const V *)nullptr, nullptr);
         k_bicg_s<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.r, w.v, w.s, w.sh, (const V *)nullptr, rho, rv, sc);
         ilu0_apply(c, w.ilu, w.s, w.sh);
     } else {
         k_bicg_p<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.r, w.p, w.v, w.ph, w.Minv, rho, sc, it == 0);
-        spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.ph, w.v, w.rh, rv, (const V *)nullptr, nullptr);
+        spmv_dot<D, T, V>(c, K, w.src, (const T *)w.RS, w.ph, w.v, w.rh, rv, (const V *)nullptr, nullptr);
         k_bicg_s<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.r, w.v, w.s, w.sh, w.Minv, rho, rv, sc);
     }
-    spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.sh, w.t, w.s, ts, w.t, tt);
+    spmv_dot<D, T, V>(c, K, w.src, (const T *)w.RS, w.sh, w.t, w.s, ts, w.t, tt);
     k_bicg_x<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.x, w.r, w.rh, w.ph, w.sh, w.s, w.t, rho, ts, tt, sc,
                                                     rho_next, rr);
 }
@@ -1158,7 +1170,7 @@ static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm
     double *y = w.y64;
     {
         V *tmp = w.t;
-        spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.x, tmp, (const V *)nullptr, nullptr, (const V *)nullptr, nullptr);
+        spmv_dot<D, T, V>(c, K, w.src, (const T *)w.RS, w.x, tmp, (const V *)nullptr, nullptr, (const V *)nullptr, nullptr);
         k_copy<V, double><<<2048, 256, 0, c.stream>>>(n, tmp, y);
         k_resid<V, V><<<RED_PARTS, 256, 0, c.stream>>>(n, b, y, w.r, w.rh, red + (size_t)P_RR * NPART);
         k_dot<V><<<RED_PARTS, 256, 0, c.stream>>>(w.rh, w.r, n, red + (size_t)P_RHO0 * NPART);
@@ -1211,7 +1223,7 @@ static int pcg(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, dou
     const size_t n = (size_t)c.nn * c.dim;
     double *red = c.red;
     double *y = w.y64;
-    spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.x, w.t, (const V *)nullptr, nullptr, (const V *)nullptr, nullptr);
+    spmv_dot<D, T, V>(c, K, w.src, (const T *)w.RS, w.x, w.t, (const V *)nullptr, nullptr, (const V *)nullptr, nullptr);
     k_copy<V, double><<<2048, 256, 0, c.stream>>>(n, w.t, y);
     k_resid<V, V><<<RED_PARTS, 256, 0, c.stream>>>(n, b, y, w.r, nullptr, red + (size_t)P_RR * NPART);
     // z = Minv r and rho0 = (r, z) through a zero step
@@ -1233,7 +1245,7 @@ static int pcg(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, dou
         double *rho_new = red + (size_t)((i & 1) ? P_RHO0 : P_RHO1) * NPART;
         double *rho_prev = rho_new;
         k_cg_p<V><<<RED_PARTS, 256, 0, c.stream>>>(n, w.p, w.s, rho, rho_prev, i == 0);
-        spmv_dot<D, T, V>(c, K, (const T *)w.RS, w.p, w.v, w.p, red + (size_t)P_RV * NPART, (const V *)nullptr,
+        spmv_dot<D, T, V>(c, K, w.src, (const T *)w.RS, w.p, w.v, w.p, red + (size_t)P_RV * NPART, (const V *)nullptr,
                           nullptr);
         k_cg_xr<D, V><<<RED_PARTS, 256, 0, c.stream>>>(c.nn, w.x, w.r, w.s, w.p, w.v, w.Minv, rho,
                                                       red + (size_t)P_RV * NPART, rho_new,
@@ -1339,9 +1351,11 @@ static int setup_precond(Ctx &c, const double *K64, const pdx_solve_opts &o, Kry
 
 // Solve K_ff x = b in FP64 (plain) or FP32 inner + FP64 refinement.
 // b_dev: full DOF vector (double, prescribed zero).  x_dev: in/out (double).
+// mir: own-block copies of the operator the SpMVs read instead of K64 / K32 (the
+// preconditioner set-up and the row sums still read K64).
 template <int D>
 static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
-                        double *x_dev, pdx_solve_stats &st, std::string &err) {
+                        double *x_dev, pdx_solve_stats &st, std::string &err, const Mirrored &mir) {
     auto t0 = std::chrono::steady_clock::now();
     const size_t n = (size_t)c.nn * c.dim;
     if (!c.red) {
@@ -1356,6 +1370,7 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
     st.seconds = 0;
     st.true_relres = 1.0;
     st.converged = 0;
+    st.mirrored_used = D > 1 && mir.K64 && (o.precision == PDX_FP64 || mir.K32) ? 1 : 0;
     if (bnorm == 0.0) {
         PDX_HIP(hipMemsetAsync(x_dev, 0, n * 8, c.stream));
         PDX_HIP(hipStreamSynchronize(c.stream));
@@ -1370,23 +1385,28 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
     if (!c.solve_y) PDX_HIP(hipMalloc((void **)&c.solve_y, n * 8));
     double *y = c.solve_y;
     const bool use_graph = !o.no_graph;
+    // operator of the SpMVs: FP64 (residuals, FP64 iterations) and FP32 (FP32 iterations)
+    const int *src64 = D > 1 && mir.K64 ? c.ksrc : nullptr, *src32 = D > 1 && mir.K32 ? c.ksrc : nullptr;
+    const double *A64 = src64 ? mir.K64 : K64;
+    const float *A32 = src32 ? mir.K32 : K32;
     if (o.precision == PDX_FP64) {
         KrylovWs<double> *wp;
         if ((rc = ws_get(c, wp, err)) || (rc = setup_precond<D, double>(c, K64, o, *wp, err))) return rc;
         KrylovWs<double> &w = *wp;
+        w.src = src64;
         PDX_HIP(hipMemcpyAsync(w.x, x_dev, n * 8, hipMemcpyDeviceToDevice, c.stream));
         double prev = INFINITY;
         for (int restart = 0; restart < 8; restart++) {
             int it = 0;
             double rr;
             rc = o.krylov == PDX_KRYLOV_CG
-                     ? pcg<D, double, double>(c, K64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total, use_graph,
+                     ? pcg<D, double, double>(c, A64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total, use_graph,
                                               &it, &rr, err)
-                     : bicgstab<D, double, double>(c, K64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total,
+                     : bicgstab<D, double, double>(c, A64, w, b_dev, bnorm, o.tol * 0.5, o.max_iter - total,
                                                    use_graph, &it, &rr, err);
             total += it;
             if (rc) break;
-            spmv_dot<D, double, double>(c, K64, nullptr, w.x, y, (const double *)nullptr, nullptr,
+            spmv_dot<D, double, double>(c, A64, src64, nullptr, w.x, y, (const double *)nullptr, nullptr,
                                         (const double *)nullptr, nullptr);
             k_resid<double, double><<<RED_PARTS, 256, 0, c.stream>>>(n, b_dev, y, w.t, nullptr, part);
             rel = std::sqrt(host_sum(c, part, NPART, err)) / bnorm;
@@ -1401,6 +1421,7 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
         float *b32 = c.solve_b32;
         if ((rc = ws_get(c, wp, err)) || (rc = setup_precond<D, float>(c, K64, o, *wp, err))) return rc;
         KrylovWs<float> &w = *wp;
+        w.src = src32;
         if (!w.RSbuf) PDX_HIP(hipMalloc((void **)&w.RSbuf, (size_t)c.nn * c.dd * 4));
         k_block_rowsum<D, float><<<(c.nn + 255) / 256, 256, 0, c.stream>>>(c.nn, c.adj_off, K64, w.RSbuf);
         w.RS = w.RSbuf;
@@ -1409,7 +1430,7 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
         int stalls = 0;
         for (refine = 0; refine < 60; refine++) {
             // FP64 residual r = b - K x
-            spmv_dot<D, double, double>(c, K64, nullptr, x_dev, y, (const double *)nullptr, nullptr,
+            spmv_dot<D, double, double>(c, A64, src64, nullptr, x_dev, y, (const double *)nullptr, nullptr,
                                         (const double *)nullptr, nullptr);
             k_resid<float, double><<<RED_PARTS, 256, 0, c.stream>>>(n, b_dev, y, b32, nullptr, part);
             rel = std::sqrt(host_sum(c, part, NPART, err)) / bnorm;
@@ -1420,9 +1441,9 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
             int it = 0;
             double rr;
             rc = o.krylov == PDX_KRYLOV_CG
-                     ? pcg<D, float, float>(c, K32, w, b32, rel * bnorm, inner, o.max_iter - total, use_graph, &it,
+                     ? pcg<D, float, float>(c, A32, w, b32, rel * bnorm, inner, o.max_iter - total, use_graph, &it,
                                             &rr, err)
-                     : bicgstab<D, float, float>(c, K32, w, b32, rel * bnorm, inner, o.max_iter - total, use_graph,
+                     : bicgstab<D, float, float>(c, A32, w, b32, rel * bnorm, inner, o.max_iter - total, use_graph,
                                                  &it, &rr, err);
             total += it;
             if (rc) break;
@@ -1445,10 +1466,10 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
 }
 
 static int solve_once(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
-                      double *x_dev, pdx_solve_stats &st, std::string &err) {
-    if (c.dim == 3) return solve_masked<3>(c, K64, K32, o, b_dev, x_dev, st, err);
-    if (c.dim == 2) return solve_masked<2>(c, K64, K32, o, b_dev, x_dev, st, err);
-    return solve_masked<1>(c, K64, K32, o, b_dev, x_dev, st, err);
+                      double *x_dev, pdx_solve_stats &st, std::string &err, const Mirrored &mir) {
+    if (c.dim == 3) return solve_masked<3>(c, K64, K32, o, b_dev, x_dev, st, err, mir);
+    if (c.dim == 2) return solve_masked<2>(c, K64, K32, o, b_dev, x_dev, st, err, mir);
+    return solve_masked<1>(c, K64, K32, o, b_dev, x_dev, st, err, mir);
 }
 
 // the attempt after o in the fallback chain (learned / two-level -> ILU(0) -> node-block Jacobi ->
@@ -1472,7 +1493,7 @@ static bool next_fallback(pdx_solve_opts &o) {
 
 // solve with an explicit node-block matrix (K, K_eff or a 1x1-block CSR)
 int dev_solve_mat(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
-                  double *x_dev, pdx_solve_stats &st, std::string &err) {
+                  double *x_dev, pdx_solve_stats &st, std::string &err, const Mirrored *mir) {
     if (o.krylov != PDX_KRYLOV_BICGSTAB && o.krylov != PDX_KRYLOV_CG) { err = "unknown Krylov method"; return PDX_ERR_ARG; }
     if (o.krylov == PDX_KRYLOV_CG &&
         (o.precond == PDX_PREC_TWO_LEVEL || o.precond == PDX_PREC_LEARNED || o.precond == PDX_PREC_ILU0)) {
@@ -1488,7 +1509,7 @@ int dev_solve_mat(Ctx &c, const double *K64, const float *K32, const pdx_solve_o
     int iters = 0, refs = 0;
     double secs = 0;
     for (int attempt = 1;; attempt++) {
-        int rc = solve_once(c, K64, K32, a, b_dev, x_dev, st, err);
+        int rc = solve_once(c, K64, K32, a, b_dev, x_dev, st, err, mir ? *mir : Mirrored{});
         iters += st.iterations;
         refs += st.refinements;
         secs += st.seconds;
@@ -1515,7 +1536,9 @@ int dev_solve(Ctx &c, const pdx_solve_opts &o, const double *b_dev, double *x_de
     if (!c.K_valid) { err = "K not assembled"; return PDX_ERR_ARG; }
     int rc;
     if (o.precision == PDX_FP32 && (rc = ensure_K32(c, err))) return rc;
-    return dev_solve_mat(c, c.K, o.precision == PDX_FP32 ? c.K32 : nullptr, o, b_dev, x_dev, st, err);
+    Mirrored mir;
+    if (o.mirrored && c.dim > 1 && (rc = kown_for_solve(c, o.precision, mir, err))) return rc;
+    return dev_solve_mat(c, c.K, o.precision == PDX_FP32 ? c.K32 : nullptr, o, b_dev, x_dev, st, err, &mir);
 }
 
 // ------------------------------------------------------------------------
@@ -1571,12 +1594,12 @@ static int probe_impl(Ctx &c, const double *K64, TwoLevel &t, const double *z, d
     for (int p = 0; p < nprobe; p++) {
         const double *zp = z + p * n;
         double *fp = f + p * n, *yp = y + p * n;
-        spmv_dot<D, double, double>(c, K64, nullptr, zp, kz, (const double *)nullptr, nullptr, (const double *)nullptr,
+        spmv_dot<D, double, double>(c, K64, nullptr, nullptr, zp, kz, (const double *)nullptr, nullptr, (const double *)nullptr,
                                     nullptr);
         k_restrict_agg<D, NM><<<RED_PARTS, 256, 0, c.stream>>>(t.na, t.agg_off, t.agg_node, kz, t.Pn, rc);
         k_coarse_mv<double><<<gc, 256, 0, c.stream>>>(t.nc, t.ld, t.Ainv, rc, yc);
         k_sub_prolong<D, NM><<<(c.nn + 255) / 256, 256, 0, c.stream>>>(c.nn, t.Pn, t.node_agg, yc, zp, fp);
-        spmv_dot<D, double, double>(c, K64, nullptr, fp, yp, (const double *)nullptr, nullptr, (const double *)nullptr,
+        spmv_dot<D, double, double>(c, K64, nullptr, nullptr, fp, yp, (const double *)nullptr, nullptr, (const double *)nullptr,
                                     nullptr);
     }
     PDX_HIP(hipGetLastError());

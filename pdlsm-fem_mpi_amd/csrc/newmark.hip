@@ -17,8 +17,6 @@
 
 namespace pdx {
 
-int dev_solve_mat(Ctx &c, const double *K64, const float *K32, const pdx_solve_opts &o, const double *b_dev,
-                  double *x_dev, pdx_solve_stats &st, std::string &err);
 int to_f32(Ctx &c, const double *K64, float *&K32, std::string &err);
 
 struct MassArgs {
@@ -234,6 +232,23 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
     PDX_HIP(hipMalloc((void **)&dU, std::max<size_t>(m.ebc.size(), 1) * 8));
     k_keff<<<4096, 256, 0, c.stream>>>(nk, c.K, c.Mk, c.beta, c.dt, Keff);
     if (o.precision == PDX_FP32 && (rc = to_f32(c, Keff, Keff32, err))) return rc;
+    // mirrored storage of K_eff for the SpMVs, when its mirrored blocks are exact transposes
+    // (K's are by construction; the consistent mass's are checked with them)
+    Mirrored mir;
+    double *own64 = nullptr;
+    float *own32 = nullptr;
+    if (o.mirrored && c.dim > 1) {
+        bool exact = false;
+        if ((rc = check_mirrored(c, Keff, exact, err))) return rc;
+        if (exact) {
+            const size_t no = (size_t)c.dd * std::max<long long>(c.n_own, 1);
+            PDX_HIP(hipMalloc((void **)&own64, no * 8));
+            if (o.precision == PDX_FP32) PDX_HIP(hipMalloc((void **)&own32, no * 4));
+            if ((rc = gather_own(c, Keff, own64, own32, err))) return rc;
+            mir.K64 = own64;
+            mir.K32 = own32;
+        }
+    }
     // u from DOF values (free), v = a = 0 (pdsolve.cpp:6207-6220)
     std::vector<double> uh(n, 0.0);
     for (int k = 0; k < c.nn; k++)
@@ -269,10 +284,11 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
         else
             PDX_HIP(hipMemcpyAsync(a1, a, n * 8, hipMemcpyDeviceToDevice, c.stream));
         pdx_solve_stats st{};
-        rc = dev_solve_mat(c, Keff, Keff32, oo, F, a1, st, err);
+        rc = dev_solve_mat(c, Keff, Keff32, oo, F, a1, st, err, &mir);
         stats.iterations += st.iterations;
         stats.refinements += st.refinements;
         stats.true_relres = std::max(stats.true_relres, st.true_relres);
+        stats.mirrored_used = st.mirrored_used;
         if (rc) break;
         k_newmark_update<<<(th + 255) / 256, 256, 0, c.stream>>>(th, c.eq, D, u, v, a, a1, c.dt, c.beta, c.gamma);
         std::swap(ap, a);  // ap <- a_n
@@ -310,6 +326,8 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
     double *fr[] = {Keff, u, v, a, a1, ap, UN, y, F, val, dU, nbcv_d};
     for (double *p : fr) hipFree(p);
     if (Keff32) hipFree(Keff32);
+    if (own64) hipFree(own64);
+    if (own32) hipFree(own32);
     return rc;
 }
 

@@ -155,6 +155,47 @@ __global__ void k_gather_own(long long n_own, int dd, const int *__restrict__ sl
     }
 }
 
+// *bad = 1 when a mirrored lower block (ksrc < 0) differs from its partner's transpose in any bit
+__global__ void k_check_mirror(long long nblk, int D, const int *__restrict__ ksrc, const int *__restrict__ slot,
+                               const double *__restrict__ A, int *bad) {
+    const int dd = D * D;
+    for (long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x; b < nblk;
+         b += (long long)gridDim.x * blockDim.x) {
+        const int s = ksrc[b];
+        if (s >= 0) continue;
+        const double *lo = A + (size_t)b * dd, *up = A + (size_t)slot[~s] * dd;
+        bool same = true;
+        for (int i = 0; i < D; i++)
+            for (int j = 0; j < D; j++)
+                same = same && __double_as_longlong(lo[i * D + j]) == __double_as_longlong(up[j * D + i]);
+        if (!same) *bad = 1;
+    }
+}
+
+int check_mirrored(Ctx &c, const double *A, bool &exact, std::string &err) {
+    exact = false;
+    if (!c.ksrc || c.dim < 2) return PDX_OK;
+    int *bad, h = 0;
+    PDX_HIP(hipMalloc((void **)&bad, sizeof(int)));
+    PDX_HIP(hipMemsetAsync(bad, 0, sizeof(int), c.stream));
+    k_check_mirror<<<4096, 256, 0, c.stream>>>(c.nblk, c.dim, c.ksrc, c.kown_slot, A, bad);
+    PDX_HIP(hipGetLastError());
+    PDX_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PDX_HIP(hipStreamSynchronize(c.stream));
+    hipFree(bad);
+    exact = h == 0;
+    (void)err;
+    return PDX_OK;
+}
+
+int gather_own(Ctx &c, const double *A, double *own64, float *own32, std::string &err) {
+    if (!c.ksrc) { err = "no mirrored storage map"; return PDX_ERR_ARG; }
+    if (own64) k_gather_own<double><<<4096, 256, 0, c.stream>>>(c.n_own, c.dd, c.kown_slot, A, own64);
+    if (own32) k_gather_own<float><<<4096, 256, 0, c.stream>>>(c.n_own, c.dd, c.kown_slot, A, own32);
+    PDX_HIP(hipGetLastError());
+    return PDX_OK;
+}
+
 // the mirrored-storage copy of K for the explicit step, rebuilt when K has been assembled again
 static int ensure_Kown(Ctx &c, int prec, std::string &err) {
     const int pi = prec == PDX_FP32 ? 1 : 0;
@@ -170,6 +211,25 @@ static int ensure_Kown(Ctx &c, int prec, std::string &err) {
     }
     PDX_HIP(hipGetLastError());
     c.kown_gen[pi] = c.K_gen;
+    return PDX_OK;
+}
+
+int kown_for_solve(Ctx &c, int prec, Mirrored &m, std::string &err) {
+    m = Mirrored{};
+    if (!c.K_valid) { err = "K not assembled"; return PDX_ERR_ARG; }
+    int rc;
+    if (c.kown_exact_gen != c.K_gen) {
+        if ((rc = check_mirrored(c, c.K, c.kown_exact, err))) return rc;
+        c.kown_exact_gen = c.K_gen;
+    }
+    if (!c.kown_exact) return PDX_OK;
+    // the FP64 copy serves the true residuals of both precisions
+    if ((rc = ensure_Kown(c, PDX_FP64, err))) return rc;
+    m.K64 = c.Kown64;
+    if (prec == PDX_FP32) {
+        if ((rc = ensure_Kown(c, PDX_FP32, err))) return rc;
+        m.K32 = c.Kown32;
+    }
     return PDX_OK;
 }
 
