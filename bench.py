@@ -155,6 +155,95 @@ def cpu_baseline(budget_s=15.0):
             "single_core": {"value": v1, "cores": 1, "sample": f"{what}, 1 thread, start-up + {s1} steps, {d1:.1f} s"}}
 
 
+def _mkl_rt():
+    """The MKL single dynamic library, if this host has one (None otherwise)."""
+    import ctypes
+    import ctypes.util
+    names = [ctypes.util.find_library("mkl_rt")]
+    for pre in (sys.prefix, "/opt/conda"):
+        names += [os.path.join(pre, "lib", "libmkl_rt.so.1"), os.path.join(pre, "lib", "libmkl_rt.so")]
+    for nm in names:
+        if nm:
+            try:
+                return ctypes.CDLL(nm), nm
+            except OSError:
+                pass
+    return None, None
+
+
+def direct_solve(path):
+    """CPU sparse direct solve of the system saved at path (ia, ja, a, b; zero-based CSR), as the
+    reference solves it: MKL PARDISO, real unsymmetric (mtype 11), phases 11 / 22 / 33, with the
+    reference's iparm (calMatrixOperations.cpp:277-301: METIS ordering, 2 refinement steps,
+    1e-13 pivot perturbation, MPS scaling and weighted matching, zero-based indices) and the
+    sequential threading layer it links (one core).  Without an MKL runtime: SciPy SuperLU
+    (COLAMD).  Prints one JSON line.  Runs in a child process of the bench, on the CPU only."""
+    import ctypes as C
+    z = np.load(path)
+    ia, ja = np.ascontiguousarray(z["ia"], np.int64), np.ascontiguousarray(z["ja"], np.int64)
+    a, b = np.ascontiguousarray(z["a"], np.float64), np.ascontiguousarray(z["b"], np.float64)
+    n = len(b)
+    x = np.zeros(n)
+    lib, name = _mkl_rt()
+    res = {"neq": n, "nnz": int(len(a))}
+    t0 = time.time()
+    if lib is not None:
+        pt = (C.c_void_p * 64)()
+        iparm = np.zeros(64, np.int64)
+        for k, v in {0: 1, 1: 2, 7: 2, 9: 13, 10: 1, 12: 1, 17: -1, 18: -1, 34: 1}.items():
+            iparm[k] = v
+        L = C.c_longlong
+        one, mtype, msglvl, idum, err, nn = L(1), L(11), L(0), L(0), L(0), L(n)
+        ptr = lambda v: v.ctypes.data_as(C.c_void_p)
+        phases = {}
+        for ph in (11, 22, 33, -1):
+            t1 = time.time()
+            lib.pardiso_64(pt, C.byref(one), C.byref(one), C.byref(mtype), C.byref(L(ph)), C.byref(nn), ptr(a),
+                           ptr(ia), ptr(ja), C.byref(idum), C.byref(one), ptr(iparm), C.byref(msglvl), ptr(b),
+                           ptr(x), C.byref(err))
+            phases[str(ph)] = time.time() - t1
+            if err.value != 0:
+                res["error"] = f"PARDISO phase {ph} error {err.value}"
+                break
+        res.update(solver="MKL PARDISO (pardiso_64, sequential)", library=name, factor_nnz=int(iparm[17]),
+                   seconds_per_phase={k: phases[k] for k in ("11", "22", "33") if k in phases})
+        seconds = sum(phases.get(k, 0.0) for k in ("11", "22", "33"))
+    else:
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+        A = sp.csr_matrix((a, ja, ia), shape=(n, n)).tocsc()
+        lu = spla.splu(A, permc_spec="COLAMD")
+        x = lu.solve(b)
+        seconds = time.time() - t0
+        res.update(solver="SciPy SuperLU (COLAMD); no MKL runtime on this host", factor_nnz=int(lu.nnz))
+    Ax = np.zeros(n)
+    for r in range(0, n, 65536):  # b - A x in CSR without SciPy
+        e = min(n, r + 65536)
+        rows = np.repeat(np.arange(r, e), np.diff(ia[r: e + 1]))
+        np.add.at(Ax, rows, a[ia[r]: ia[e]] * x[ja[ia[r]: ia[e]]])
+    res.update(seconds=seconds, cores=1, true_relres=float(np.linalg.norm(b - Ax) / np.linalg.norm(b)))
+    print(json.dumps(res), flush=True)
+
+
+def direct_solve_child(ia, ja, a, b, timeout=300):
+    """direct_solve in a child process (a fault in the library cannot end the bench)."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "system.npz")
+        np.savez(path, ia=ia, ja=ja, a=a, b=b)
+        env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL", MKL_NUM_THREADS="1")
+        try:
+            p = subprocess.run([sys.executable, os.path.abspath(__file__), "--direct-solve", path], env=env,
+                               capture_output=True, text=True, timeout=timeout)
+        except subprocess.TimeoutExpired:
+            return {"error": f"no result within {timeout} s"}
+        lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+        if p.returncode != 0 or not lines:
+            return {"error": f"exit {p.returncode}: {p.stderr[-300:]}"}
+        return json.loads(lines[-1])
+
+
 def copy_probe(mib):
     """Five device-to-device copies of a known size: under rocprofv3 --pmc the
     FETCH_SIZE / WRITE_SIZE of this copy kernel against mib MiB read and written
@@ -419,7 +508,12 @@ def main():
                     help="only time a streaming read and a copy (practical HBM bandwidth) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (HBM counter calibration) and exit")
+    ap.add_argument("--direct-solve", default="", metavar="NPZ",
+                    help="only solve the CSR system in NPZ on the CPU (PARDISO, else SuperLU) and exit")
     args = ap.parse_args()
+    if args.direct_solve:
+        direct_solve(args.direct_solve)
+        return
     global KRYLOV_NO_GRAPH
     KRYLOV_NO_GRAPH = args.krylov_no_graph
     if args.copy_probe:
@@ -547,6 +641,16 @@ def main():
                 solves[name] = {"error": str(e)}
             log(name, solves[name])
         out["solve_to_1e-8"] = {"workload": f"crack2d 501x251 seed 0, neq {smod.neq}", **solves}
+        if not args.no_cpu:
+            # the same system solved on one host core as the reference solves it (sparse direct)
+            K, F = scx.assemble(pdx.PDX_MODE_STATIC)
+            ia, ja = smod.csr()
+            cpu = direct_solve_child(ia, ja, K, F)
+            gpu = solves.get("fp64_two_level_bicgstab", {}).get("seconds")
+            if gpu and "seconds" in cpu:
+                cpu["gpu_fp64_two_level_speedup"] = cpu["seconds"] / gpu
+            out["solve_to_1e-8"]["cpu_direct_solve"] = cpu
+            log("cpu direct solve", cpu)
         try:
             out["gnn_inference"] = gnn_bench(pdx, scx, smod)
         except Exception as e:  # reported, not hidden
