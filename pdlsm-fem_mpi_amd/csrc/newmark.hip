@@ -48,12 +48,13 @@ __device__ inline double det3m(const double J[9]) {
 }
 
 // element mass rows (pdfemEleQuad4N.cpp:361-387, pdfemEleBrick8N.cpp:554-582,
-// pdfemEleTri3N.cpp:49-70, pdfemEleTetra4N.cpp:55-76), node-centric
+// pdfemEleTri3N.cpp:49-70, pdfemEleTetra4N.cpp:55-76), node-centric: one thread per node
+// forms its row of every incident element once and adds it to the d diagonal slots of the
+// node's blocks (the mass is the same for every DOF direction)
 template <int D>
 __global__ void k_mass_rows(MassArgs A) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.nn * D) return;
-    const int a = t / D, ii = t % D;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A.nn) return;
     const double g = 0.57735026918962573, gp[2] = {-g, g};
     const size_t nb = (size_t)A.nblk;
     for (int q = A.nfe_off[a]; q < A.nfe_off[a + 1]; q++) {
@@ -118,10 +119,11 @@ __global__ void k_mass_rows(MassArgs A) {
             for (int b = 0; b < 4; b++) row[b] = b == la ? fac * 2 : fac;
         }
         const int *blk = &A.fe_blk[(size_t)fp * 64 + la * 8];
-        for (int b = 0; b < n; b++) {
-            const size_t idx = (size_t)blk[b] * D * D + ii * D + ii;
-            A.M[idx] = A.M[idx] + row[b];
-        }
+        for (int b = 0; b < n; b++)
+            for (int ii = 0; ii < D; ii++) {
+                const size_t idx = (size_t)blk[b] * D * D + ii * D + ii;
+                A.M[idx] = A.M[idx] + row[b];
+            }
     }
 }
 
@@ -190,7 +192,7 @@ int dev_mass_on_K(Ctx &c, std::string &err) {
     A.x = c.x; A.econ = c.econ; A.nen = c.nen; A.fe_elem = c.fe_elem; A.fe_blk = c.fe_blk; A.nfe_off = c.nfe_off;
     A.nfe_item = c.nfe_item; A.fam_node = c.fam_node; A.fam_blk = c.fam_blk; A.fam_off = c.fam_off; A.eq = c.eq;
     A.fam_v = c.fam_v; A.rho = c.rho; A.M = c.Mk; A.nblk = c.nblk; A.nn = c.nn; A.nfam = c.nfam;
-    int th = c.nn * c.dim;
+    const int th = c.nn;
     if (c.nfe) {
         if (c.dim == 3) k_mass_rows<3><<<(th + 127) / 128, 128, 0, c.stream>>>(A);
         else k_mass_rows<2><<<(th + 127) / 128, 128, 0, c.stream>>>(A);

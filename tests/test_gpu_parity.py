@@ -72,8 +72,9 @@ def test_assembly_modes_match_oracle():
     assert row_block_rel(K1, K0, ia) <= 1e-12 and rel(F1, F0) <= 1e-12
 
 
-def test_masses_match_oracle():
-    mesh = W.box3d(n=12, pd_block=6)
+@pytest.mark.parametrize("name", ["box3d", "plate2d"])
+def test_masses_match_oracle(name):
+    mesh = W.box3d(n=12, pd_block=6) if name == "box3d" else W.plate2d(nx=41, ny=21, dx=1e-3, pd_x=(0.015, 0.025))
     mo, mp, cx = _pair(mesh)
     assert rel(cx.lumped_mass(), mo.lumped_mass()) <= 1e-14
     assert rel(cx.mass_on_K(), mo.mass_on_K()) <= 1e-14
