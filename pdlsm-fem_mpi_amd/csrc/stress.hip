@@ -119,13 +119,27 @@ struct StressFeArgs {
 // J^T grad(u) = H (one partial-pivot LU, three right-hand sides, instead of a solve per
 // shape function), the engineering strains and sigma = D eps (lam tr(eps) + 2 mu eps,
 // mu gamma for an isotropic D).  Phase 2, one thread per (element, corner):
-// Nsigma = L sigma_gp.
+// Nsigma = L sigma_gp into LDS; the block then writes its elements' 8 x 6 values with
+// 16-byte stores over one contiguous range when its list slots are consecutive (the list
+// is increasing, so a span of count - 1 means consecutive), else one corner per thread.
 constexpr int SH_EPB = 32;
+
+// The hardware deals workgroups to the 8 XCDs round-robin; with a grid that is a multiple of 8,
+// logical block (b % 8) * (grid / 8) + b / 8 gives each XCD (and its L2) one contiguous eighth
+// of the range, so corners and nodes shared with the next layers of the mesh stay in one L2.
+__device__ inline int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    return nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+inline int xcd_grid(int blocks) { return (blocks + 7) / 8 * 8; }
+
 __global__ void __launch_bounds__(SH_EPB * 8) k_stress_hex8(StressFeArgs A) {
 #pragma clang fp contract(off)
-    __shared__ double sxu[SH_EPB][8][6];
+    __shared__ __align__(16) double sxu[SH_EPB][8][6];
     __shared__ double sgp[SH_EPB][8][6];
-    const int el = threadIdx.x / 8, g = threadIdx.x % 8, idx = blockIdx.x * SH_EPB + el;
+    const int lb = xcd_block();
+    if (lb * SH_EPB >= A.nlist) return;  // padding blocks of the XCD-rounded grid (whole block)
+    const int el = threadIdx.x / 8, g = threadIdx.x % 8, idx = lb * SH_EPB + el;
     const double a = 1.0 / sqrt(3.0);
     if (idx < A.nlist) {
         const int e = A.fe_elem[A.list[idx]];
@@ -207,13 +221,23 @@ __global__ void __launch_bounds__(SH_EPB * 8) k_stress_hex8(StressFeArgs A) {
         }
     }
     __syncthreads();
-    if (idx >= A.nlist) return;
-    const int node = g;  // phase 2: thread (el, corner)
-    double *out = A.stage + ((size_t)A.list[idx] * 8 + node) * 6;
-    for (int i = 0; i < 6; i++) {
-        double t = 0.0;
-        for (int k = 0; k < 8; k++) t = A.L8[node * 8 + k] * sgp[el][k][i] + t;
-        out[i] = t;
+    const int node = g;  // phase 2: thread (el, corner); sxu is free again
+    if (idx < A.nlist)
+        for (int i = 0; i < 6; i++) {
+            double t = 0.0;
+            for (int k = 0; k < 8; k++) t = A.L8[node * 8 + k] * sgp[el][k][i] + t;
+            sxu[el][node][i] = t;
+        }
+    __syncthreads();
+    const int idx0 = lb * SH_EPB, nel = min(SH_EPB, A.nlist - idx0);
+    const int first = A.list[idx0];
+    if (A.list[idx0 + nel - 1] - first == nel - 1) {
+        double2 *dst = reinterpret_cast<double2 *>(A.stage + (size_t)first * 48);
+        const double2 *src = reinterpret_cast<const double2 *>(&sxu[0][0][0]);
+        for (int v = threadIdx.x; v < nel * 24; v += blockDim.x) dst[v] = src[v];
+    } else if (idx < A.nlist) {
+        double *out = A.stage + ((size_t)A.list[idx] * 8 + node) * 6;
+        for (int i = 0; i < 6; i++) out[i] = sxu[el][node][i];
     }
 }
 
@@ -362,7 +386,7 @@ template <int D>
 __global__ void k_stress_nodes(int nn, const int *famid, const double *sig_fam, const int *nfe_off,
                                const int *nfe_item, const double *stage, double nu, int plane_strain, double *sigma) {
 #pragma clang fp contract(off)
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = xcd_block() * blockDim.x + threadIdx.x;
     if (a >= nn) return;
     double s[6] = {0, 0, 0, 0, 0, 0};
     int count = 0;
@@ -386,9 +410,14 @@ __global__ void k_stress_nodes(int nn, const int *famid, const double *sig_fam, 
         for (int k = 0; k < 4; k++) {
             if (q0 + k < q1) {
                 const int item = nfe_item[q0 + k];
-                const double *src = stage + ((size_t)(item >> 3) * 8 + (item & 7)) * 6;
+                // 48-byte corner records: 16-byte aligned
+                const double2 *src = reinterpret_cast<const double2 *>(stage + ((size_t)(item >> 3) * 8 + (item & 7)) * 6);
 #pragma unroll
-                for (int i = 0; i < 6; i++) v[k][i] = src[i];
+                for (int i = 0; i < 3; i++) {
+                    const double2 w = src[i];
+                    v[k][2 * i] = w.x;
+                    v[k][2 * i + 1] = w.y;
+                }
             }
         }
 #pragma unroll
@@ -518,7 +547,7 @@ int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::strin
         if (c.n_fe_a > 0) {
             A.list = c.fe_list_a;
             A.nlist = c.n_fe_a;
-            if (D == 3) k_stress_hex8<<<(c.n_fe_a + SH_EPB - 1) / SH_EPB, SH_EPB * 8, 0, c.stream>>>(A);
+            if (D == 3) k_stress_hex8<<<xcd_grid((c.n_fe_a + SH_EPB - 1) / SH_EPB), SH_EPB * 8, 0, c.stream>>>(A);
             else k_stress_quad4<<<(c.n_fe_a + SQ_EPB - 1) / SQ_EPB, SQ_EPB * 4, 0, c.stream>>>(A);
         }
         if (c.n_fe_b > 0) {
@@ -529,7 +558,7 @@ int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::strin
         }
     }
     {
-        const int nb = (c.nn + 255) / 256;
+        const int nb = xcd_grid((c.nn + 255) / 256);
         if (D == 3) k_stress_nodes<3><<<nb, 256, 0, c.stream>>>(c.nn, famid, sig_fam, c.nfe_off, c.nfe_item, stage, m.nu,
                                                                0, sigma_dev);
         else k_stress_nodes<2><<<nb, 256, 0, c.stream>>>(c.nn, famid, sig_fam, c.nfe_off, c.nfe_item, stage, m.nu,
