@@ -160,6 +160,17 @@ __global__ void k_predictor(int nn, int D, const int *eq, const int *presc_set, 
     else UN[t] = val[t] + ebc_dU[presc_set[code]];
 }
 
+// prescribed entries of the ramped essential BCs take their sets' current values
+__global__ void k_ebc_values(int n, int D, const int *eq, const int *presc_set, const int *vary, const double *ebc_val,
+                             double *val) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int code = (t / D) * 3 + t % D;
+    if (eq[code] >= 0) return;
+    const int s = presc_set[code];
+    if (s >= 0 && vary[s]) val[t] = ebc_val[s];
+}
+
 // F = F_ext - K U_N on free rows (y = masked K U_N)
 __global__ void k_newmark_rhs(int n, const double *y, const double *f_unit, int n_nbc, const double *nbc_val,
                               const int *eq, int D, double *F) {
@@ -264,16 +275,26 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
     std::vector<double> dUh(std::max<size_t>(m.ebc.size(), 1), 0.0);
     for (size_t s = 0; s < m.ebc.size(); s++) dUh[s] = c.ebc_vary[s] ? c.dt * c.ebc_vel[s] : 0.0;
     PDX_HIP(hipMemcpyAsync(dU, dUh.data(), dUh.size() * 8, hipMemcpyHostToDevice, c.stream));
-    double *nbcv_d;
+    double *nbcv_d, *ebcv_d;
+    int *ebc_vary_d;
+    const size_t n_ebc = m.ebc.size();
     PDX_HIP(hipMalloc((void **)&nbcv_d, std::max(c.n_nbc, 1) * 8));
+    PDX_HIP(hipMalloc((void **)&ebcv_d, std::max<size_t>(n_ebc, 1) * 8));
+    PDX_HIP(hipMalloc((void **)&ebc_vary_d, std::max<size_t>(n_ebc, 1) * 4));
+    bool any_vary = false;
+    if (n_ebc) {
+        PDX_HIP(hipMemcpyAsync(ebc_vary_d, c.ebc_vary.data(), n_ebc * 4, hipMemcpyHostToDevice, c.stream));
+        for (size_t s = 0; s < n_ebc; s++) any_vary = any_vary || c.ebc_vary[s];
+    }
+    // DOF values (prescribed entries are what the predictor reads) stay on the device; the
+    // ramped prescribed entries are advanced there after every step
+    PDX_HIP(hipMemcpyAsync(val, dofval.data(), n * 8, hipMemcpyHostToDevice, c.stream));
     stats = pdx_solve_stats{};
     auto t0 = std::chrono::steady_clock::now();
     pdx_solve_opts oo = o;
     oo.warm_start = 1;
     const int th = (int)n;
     for (int step = 0; step < nsteps; step++) {
-        // current prescribed values (host mirror of the DOF values)
-        PDX_HIP(hipMemcpyAsync(val, dofval.data(), n * 8, hipMemcpyHostToDevice, c.stream));
         if (c.n_nbc) PDX_HIP(hipMemcpyAsync(nbcv_d, c.nbc_val.data(), c.n_nbc * 8, hipMemcpyHostToDevice, c.stream));
         k_predictor<<<(th + 255) / 256, 256, 0, c.stream>>>(c.nn, D, c.eq, c.presc_set, u, v, a, val, dU, c.dt,
                                                             c.beta, UN);
@@ -295,22 +316,28 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
         k_newmark_update<<<(th + 255) / 256, 256, 0, c.stream>>>(th, c.eq, D, u, v, a, a1, c.dt, c.beta, c.gamma);
         std::swap(ap, a);  // ap <- a_n
         std::swap(a, a1);  // a <- a_{n+1}
-        // store displacement, then advance BCs (pdsolve.cpp:6248-6253)
-        PDX_HIP(hipMemcpyAsync(uh.data(), u, n * 8, hipMemcpyDeviceToHost, c.stream));
-        PDX_HIP(hipStreamSynchronize(c.stream));
-        for (int k = 0; k < c.nn; k++)
-            for (int i = 0; i < D; i++) {
-                int ps = m.presc_set[(size_t)k * 3 + i];
-                if (ps < 0) dofval[(size_t)k * D + i] = uh[(size_t)k * D + i];
-            }
+        // advance BCs (pdsolve.cpp:6248-6253); the displacement is stored once, after the last
+        // step (every host array below was last read by copies that the solve's
+        // synchronisations have completed)
         for (size_t s = 0; s < m.ebc.size(); s++)
             if (c.ebc_vary[s]) {
                 c.ebc_val[s] = c.ebc_val[s] + c.dt * c.ebc_vel[s];
                 for (int k : m.ebc[s].nid) dofval[(size_t)k * D + m.ebc[s].dof] = c.ebc_val[s];
             }
+        if (any_vary) {
+            PDX_HIP(hipMemcpyAsync(ebcv_d, c.ebc_val.data(), n_ebc * 8, hipMemcpyHostToDevice, c.stream));
+            k_ebc_values<<<(th + 255) / 256, 256, 0, c.stream>>>(th, D, c.eq, c.presc_set, ebc_vary_d, ebcv_d, val);
+        }
         for (int s = 0; s < c.n_nbc; s++)
             if (c.nbc_vary[s]) c.nbc_val[s] = c.nbc_val[s] + c.dt * c.nbc_vel[s];
     }
+    // store the displacement of the last completed step in the DOF values
+    // (storeDisplacementResult, pdsolve.cpp:5656-5672)
+    PDX_HIP(hipMemcpyAsync(uh.data(), u, n * 8, hipMemcpyDeviceToHost, c.stream));
+    PDX_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < c.nn; k++)
+        for (int i = 0; i < D; i++)
+            if (m.presc_set[(size_t)k * 3 + i] < 0) dofval[(size_t)k * D + i] = uh[(size_t)k * D + i];
     stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     stats.converged = rc == PDX_OK;
     std::vector<double> vh(n), ah(n);
@@ -325,8 +352,9 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
             if (v_h) v_h[r] = vh[(size_t)k * D + i];
             if (a_h) a_h[r] = ah[(size_t)k * D + i];
         }
-    double *fr[] = {Keff, u, v, a, a1, ap, UN, y, F, val, dU, nbcv_d};
+    double *fr[] = {Keff, u, v, a, a1, ap, UN, y, F, val, dU, nbcv_d, ebcv_d};
     for (double *p : fr) hipFree(p);
+    hipFree(ebc_vary_d);
     if (Keff32) hipFree(Keff32);
     if (own64) hipFree(own64);
     if (own32) hipFree(own32);
