@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <vector>
 
 #include "../../include/pdx.h"
 #include "device.hpp"
@@ -230,21 +231,35 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
     const size_t n = (size_t)c.nn * D, nk = (size_t)c.dd * c.nblk;
     double *Keff, *u, *v, *a, *a1, *ap, *UN, *y, *F, *val, *dU;
     float *Keff32 = nullptr;
-    double *bufs[] = {nullptr};
-    (void)bufs;
-    PDX_HIP(hipMalloc((void **)&Keff, nk * 8));
-    PDX_HIP(hipMalloc((void **)&u, n * 8));
-    PDX_HIP(hipMalloc((void **)&v, n * 8));
-    PDX_HIP(hipMalloc((void **)&a, n * 8));
-    PDX_HIP(hipMalloc((void **)&a1, n * 8));
-    PDX_HIP(hipMalloc((void **)&ap, n * 8));  // a_{n-1}, for the extrapolated initial guess
-    PDX_HIP(hipMalloc((void **)&UN, n * 8));
-    PDX_HIP(hipMalloc((void **)&y, n * 8));
-    PDX_HIP(hipMalloc((void **)&F, n * 8));
-    PDX_HIP(hipMalloc((void **)&val, n * 8));
-    PDX_HIP(hipMalloc((void **)&dU, std::max<size_t>(m.ebc.size(), 1) * 8));
+    // every device buffer of the run, freed on every exit
+    struct Owned {
+        std::vector<void *> p;
+        ~Owned() {
+            for (void *q : p) hipFree(q);
+        }
+    } owned;
+    auto dalloc = [&owned](void **pp, size_t bytes) {
+        const hipError_t e = hipMalloc(pp, bytes);
+        if (e == hipSuccess) owned.p.push_back(*pp);
+        return e;
+    };
+    PDX_HIP(dalloc((void **)&Keff, nk * 8));
+    PDX_HIP(dalloc((void **)&u, n * 8));
+    PDX_HIP(dalloc((void **)&v, n * 8));
+    PDX_HIP(dalloc((void **)&a, n * 8));
+    PDX_HIP(dalloc((void **)&a1, n * 8));
+    PDX_HIP(dalloc((void **)&ap, n * 8));  // a_{n-1}, for the extrapolated initial guess
+    PDX_HIP(dalloc((void **)&UN, n * 8));
+    PDX_HIP(dalloc((void **)&y, n * 8));
+    PDX_HIP(dalloc((void **)&F, n * 8));
+    PDX_HIP(dalloc((void **)&val, n * 8));
+    PDX_HIP(dalloc((void **)&dU, std::max<size_t>(m.ebc.size(), 1) * 8));
     k_keff<<<4096, 256, 0, c.stream>>>(nk, c.K, c.Mk, c.beta, c.dt, Keff);
-    if (o.precision == PDX_FP32 && (rc = to_f32(c, Keff, Keff32, err))) return rc;
+    if (o.precision == PDX_FP32) {
+        rc = to_f32(c, Keff, Keff32, err);
+        if (Keff32) owned.p.push_back(Keff32);
+        if (rc) return rc;
+    }
     // mirrored storage of K_eff for the SpMVs, when its mirrored blocks are exact transposes
     // (K's are by construction; the consistent mass's are checked with them)
     Mirrored mir;
@@ -255,8 +270,8 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
         if ((rc = check_mirrored(c, Keff, exact, err))) return rc;
         if (exact) {
             const size_t no = (size_t)c.dd * std::max<long long>(c.n_own, 1);
-            PDX_HIP(hipMalloc((void **)&own64, no * 8));
-            if (o.precision == PDX_FP32) PDX_HIP(hipMalloc((void **)&own32, no * 4));
+            PDX_HIP(dalloc((void **)&own64, no * 8));
+            if (o.precision == PDX_FP32) PDX_HIP(dalloc((void **)&own32, no * 4));
             if ((rc = gather_own(c, Keff, own64, own32, err))) return rc;
             mir.K64 = own64;
             mir.K32 = own32;
@@ -278,9 +293,9 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
     double *nbcv_d, *ebcv_d;
     int *ebc_vary_d;
     const size_t n_ebc = m.ebc.size();
-    PDX_HIP(hipMalloc((void **)&nbcv_d, std::max(c.n_nbc, 1) * 8));
-    PDX_HIP(hipMalloc((void **)&ebcv_d, std::max<size_t>(n_ebc, 1) * 8));
-    PDX_HIP(hipMalloc((void **)&ebc_vary_d, std::max<size_t>(n_ebc, 1) * 4));
+    PDX_HIP(dalloc((void **)&nbcv_d, std::max(c.n_nbc, 1) * 8));
+    PDX_HIP(dalloc((void **)&ebcv_d, std::max<size_t>(n_ebc, 1) * 8));
+    PDX_HIP(dalloc((void **)&ebc_vary_d, std::max<size_t>(n_ebc, 1) * 4));
     bool any_vary = false;
     if (n_ebc) {
         PDX_HIP(hipMemcpyAsync(ebc_vary_d, c.ebc_vary.data(), n_ebc * 4, hipMemcpyHostToDevice, c.stream));
@@ -352,12 +367,6 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
             if (v_h) v_h[r] = vh[(size_t)k * D + i];
             if (a_h) a_h[r] = ah[(size_t)k * D + i];
         }
-    double *fr[] = {Keff, u, v, a, a1, ap, UN, y, F, val, dU, nbcv_d, ebcv_d};
-    for (double *p : fr) hipFree(p);
-    hipFree(ebc_vary_d);
-    if (Keff32) hipFree(Keff32);
-    if (own64) hipFree(own64);
-    if (own32) hipFree(own32);
     return rc;
 }
 
