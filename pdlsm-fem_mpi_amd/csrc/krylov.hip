@@ -1042,15 +1042,17 @@ void free_krylov_ws(Ctx &c) {
 
 constexpr int CHUNK = 10;  // iterations between host residual checks
 
-// Run n iterations body(it), body(it + 1), ...  A full chunk (n == CHUNK, it a multiple of
-// CHUNK) replays a captured hipGraph: the first chunk differs from the later ones only by
+// Run n iterations body(it), body(it + 1), ...  A full chunk (n == CHUNK) from an even
+// iteration replays a captured hipGraph: the first chunk differs from the later ones only by
 // iteration 0, and every later chunk launches the same kernels with the same arguments
 // (the iteration index enters through its parity only).  A change of operator or
 // preconditioner pointers re-captures.
 template <typename V, typename Body>
 static int run_chunk(Ctx &c, KrylovWs<V> &w, const GraphSig &sig, bool use_graph, int it, int n, Body &&body,
                      std::string &err) {
-    if (!use_graph || n != CHUNK) {
+    // the graphs hold CHUNK iterations from an even start (iteration parity selects the rho
+    // slots): any other chunk runs as plain launches
+    if (!use_graph || n != CHUNK || (it & 1)) {
         for (int k = 0; k < n; k++) body(it + k);
         return PDX_OK;
     }
@@ -1074,6 +1076,20 @@ static int run_chunk(Ctx &c, KrylovWs<V> &w, const GraphSig &sig, bool use_graph
     }
     PDX_HIP(hipGraphLaunch(g, c.stream));
     return PDX_OK;
+}
+
+// Iterations of the next chunk: a full chunk, or, once the last chunk has lowered the residual,
+// the count its mean rate predicts to reach tol, plus one (a chunk shorter than CHUNK runs as
+// plain launches); a solve then stops within a few iterations of its tolerance instead of up
+// to CHUNK - 1 past it.  Depends only on the residuals, so graph and plain launches agree.
+static int next_chunk(double rel, double rel_prev, int prev_n, double tol, int left) {
+    int n = std::min(CHUNK, left);
+    if (prev_n > 0 && rel_prev > 0.0 && rel < rel_prev && rel > tol) {
+        const double rate = std::log(rel / rel_prev) / prev_n;  // < 0
+        const double need = std::log(tol / rel) / rate;  // > 0
+        if (need + 1.0 < n) n = std::max(1, (int)std::ceil(need) + 1);
+    }
+    return n;
 }
 
 template <typename V>
@@ -1183,14 +1199,18 @@ static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm
     double best = rel;
     PDX_HIP(hipMemcpyAsync(w.xb, w.x, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
     const GraphSig sig = make_sig(c, K, w, PDX_KRYLOV_BICGSTAB);
+    double rel_prev = 0.0;
+    int prev_n = 0;
     while (rel > tol && it < max_iter) {
-        const int nrun = std::min(CHUNK, max_iter - it);
+        const int nrun = next_chunk(rel, rel_prev, prev_n, tol, max_iter - it);
         int rc = run_chunk(c, w, sig, use_graph, it, nrun,
                            [&](int i) { bicg_iter<D, T, V>(c, K, w, red, sc, i); }, err);
         if (rc) return rc;
         it += nrun;
         PDX_HIP(hipGetLastError());
         const double rr = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
+        rel_prev = rel;
+        prev_n = nrun;
         rel = std::sqrt(rr) / bnorm;
         if (!std::isfinite(rr) || rel > 1e3 * best) {
             // lost accuracy: fall back to the best iterate
@@ -1251,13 +1271,17 @@ static int pcg(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, dou
                                                       red + (size_t)P_RV * NPART, rho_new,
                                                       red + (size_t)P_RR * NPART);
     };
+    double rel_prev = 0.0;
+    int prev_n = 0;
     while (rel > tol && it < max_iter) {
-        const int nrun = std::min(CHUNK, max_iter - it);
+        const int nrun = next_chunk(rel, rel_prev, prev_n, tol, max_iter - it);
         int rc = run_chunk(c, w, sig, use_graph, it, nrun, body, err);
         if (rc) return rc;
         it += nrun;
         PDX_HIP(hipGetLastError());
         const double rr = host_sum(c, red + (size_t)P_RR * NPART, NPART, err);
+        rel_prev = rel;
+        prev_n = nrun;
         rel = std::sqrt(rr) / bnorm;
         if (!std::isfinite(rr) || rel > 1e3 * best) {
             PDX_HIP(hipMemcpyAsync(w.x, w.xb, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
