@@ -575,7 +575,7 @@ def main():
     # pdsolve.cpp:6062), timed on its own: PD strains from the bond weights K's assembly left,
     # FE Gauss-point stresses extrapolated to the corners, node averages
     uall = np.random.default_rng(0).uniform(-1e-6, 1e-6, model.nn * model.dim)
-    st_ms = min(cx.node_stresses_timed(uall) for _ in range(5))
+    st_ms = min(cx.node_stresses_timed(uall) for _ in range(20))
     sb = stress_bytes(model)
     stress = {"workload": "nodal stresses of the same box (synthetic DOF values)", "ms": st_ms,
               "alg_bytes": sb, "GBps": sb / (st_ms * 1e-3) / 1e9, "roofline_frac": sb / (st_ms * 1e-3) / HBM_PEAK,
