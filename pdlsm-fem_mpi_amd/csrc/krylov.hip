@@ -1222,7 +1222,7 @@ static int bicgstab(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm
             best = rel;
             since_best = 0;
             PDX_HIP(hipMemcpyAsync(w.xb, w.x, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
-        } else if (++since_best >= 40) {
+        } else if ((since_best += nrun) >= 40 * CHUNK) {  // iterations without a 5% gain
             // stagnation at the rounding floor of the working precision
             PDX_HIP(hipMemcpyAsync(w.x, w.xb, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
             rel = best;
@@ -1292,7 +1292,7 @@ static int pcg(Ctx &c, const T *K, KrylovWs<V> &w, const V *b, double bnorm, dou
             best = rel;
             since_best = 0;
             PDX_HIP(hipMemcpyAsync(w.xb, w.x, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
-        } else if (++since_best >= 40) {
+        } else if ((since_best += nrun) >= 40 * CHUNK) {  // iterations without a 5% gain
             PDX_HIP(hipMemcpyAsync(w.x, w.xb, n * sizeof(V), hipMemcpyDeviceToDevice, c.stream));
             rel = best;
             break;
