@@ -105,7 +105,28 @@ typedef struct pdx_mesh_desc {
     double dt, beta, gamma;
     double sigma_ult, kic; /* material strength and toughness (mesh file) */
     int fc, topk;          /* failure criterion and bonds per step (FC / TOPK commands) */
+    /* run settings of the command file; zero is the reference's default (datModel.cpp:17-56) */
+    int lumped_mass;       /* LUMPMASS ON (1) / OFF (0, default): explicit driver's mass */
+    int newmark;           /* NEWMARK ON (1) / OFF (0, default) */
+    int save_freq;         /* SETSOLVING token 3; <= 0 means 1 */
+    int vtk_ascii;         /* VTKFORMAT ASCII (1) / BINARY (0, default) */
 } pdx_mesh_desc;
+
+/* Settings of a model read from a command file (fioFiles::excuteCMD, fioFile.cpp:69-462) or
+ * given in a pdx_mesh_desc, as pdx_model_settings reports them. */
+typedef struct pdx_settings {
+    int solver_flag;   /* SOLVER: 0 DYNAMIC, 1 STATIC, 2 QUASI-STATIC, -1 none (pdx_model_create) */
+    int newmark;       /* NEWMARK ON|OFF: with solver_flag 0, Newmark (1) or central difference (0) */
+    int lumped_mass;   /* LUMPMASS ON|OFF */
+    int num_steps;     /* SETSOLVING token 2 */
+    int save_freq;     /* SETSOLVING token 3: result files every save_freq steps and at the last */
+    int vtk_binary;    /* VTKFORMAT: 1 BINARY (default), 0 ASCII */
+    int fc, topk;      /* FC flag [TopK], TOPK */
+    int n_rf;          /* RF: number of EBC set ids (the first 16 in rf_ids) */
+    int rf_ids[16];
+    double dt, beta, gamma, horizon_factor, nlf;
+    double sigma_ult, kic;  /* material strength and toughness of the mesh file */
+} pdx_settings;
 
 typedef struct pdx_solve_opts {
     int krylov;        /* PDX_KRYLOV_* */
@@ -149,6 +170,13 @@ const char *pdx_version(void);
 int pdx_model_create(const pdx_mesh_desc *desc, pdx_model **out);
 int pdx_model_read(const char *cmd_file, pdx_model **out, int *solver_flag, int *num_steps);
 void pdx_model_destroy(pdx_model *m);
+/* the model's run settings: what fioFiles::excuteCMD (fioFile.cpp:69-462) leaves in datModel
+ * (solver flag, NEWMARK, LUMPMASS, SETSOLVING, FC / TOPK, RF, VTKFORMAT) */
+int pdx_model_settings(const pdx_model *m, pdx_settings *out);
+/* Pattern of the explicit driver's consistent mass, pdsolve::setCSRIndexes_gloMassMat
+ * (pdsolve.cpp:5674-5817): zero-based int64 CSR over the free DOFs, sorted columns; rows of
+ * pure PD nodes hold their diagonal only.  ia [neq+1] and ja [*nnz] may be NULL (size query). */
+int pdx_model_mass_csr(const pdx_model *m, long long *ia, long long *ja, long long *nnz);
 /* Result file of the whole mesh in the reference's legacy VTK layout, byte for byte:
  * writeReslutsTOTAL_vtk_Binary / _ASCII (fioFile.cpp:562-769), chosen as its VTKFORMAT
  * command does (binary: big-endian).  u [n_node*3] DOF values (UZ 0 in 2D), sigma [n_node*6]
@@ -190,18 +218,33 @@ int pdx_lumped_mass(pdx_ctx *c, double *m);
 int pdx_assemble_timed(pdx_ctx *c, double *ms);
 /* consistent FE mass + PD lumped mass on the K pattern, [nnz] */
 int pdx_mass_on_K(pdx_ctx *c, double *Mv);
+/* the same mass on its own pattern (pdx_model_mass_csr), [nnz of that pattern]:
+ * assembleElemassMatFEM_CSRformat (pdsolve.cpp:5154-5210) + assembleMassMatPD_CSRformat
+ * (:4568-4595) as the explicit driver builds it (:5923-5935) */
+int pdx_mass_csr(pdx_ctx *c, double *Mv);
 
 /* DOF values (free and prescribed, [n_node*dim]) used by PDX_MODE_EXPLICIT and
  * PDX_MODE_QUASI assembly and by pdx_static_solve for the prescribed part;
  * the reference keeps them in pdDof::cd_u (pdDof.h:18). */
 int pdx_set_dof_values(pdx_ctx *c, const double *u_all);
 
-/* Explicit central difference with lumped mass (the reference driver's
- * start-up step u1 = u0 + dt^2/2 a0 on the first call, then nsteps loop
- * steps).  u_out/u_prev: [neq] host (may be NULL). */
+/* Explicit central difference, pdfemDynamicSolver_CSRformat (pdsolve.cpp:5842-6090): the
+ * start-up step u1 = u0 + dt^2/2 a0 on the first call, then nsteps loop steps, each advancing
+ * the BCs before the force.  The model's LUMPMASS selects the mass: ON, a = F / m
+ * (timeIntegration_LM :5113-5132); OFF (the reference's default), M a = F with the consistent
+ * mass of pdx_mass_csr, solved on the device by PCG to a 1e-13 relative residual
+ * (timeIntegration_NLM :5134-5152; the reference factorises M with PARDISO).  With FC 2 in 2D
+ * every stored step ends with the nodal stresses and the bond failure of failureProcess
+ * (:6062-6064, pdx_step_log); FC values other than 0 and 2 return PDX_ERR_ARG.
+ * u_out/u_prev: [neq] host (may be NULL). */
 int pdx_explicit_run(pdx_ctx *c, int nsteps, int precision, double *u_out, double *u_prev);
-/* Same loop on device-resident state only, no host copies (for timing). */
+/* Same loop on device-resident state only, no host copies (for timing); lumped mass without
+ * failure replays captured 6-step graphs when use_graph is set. */
 int pdx_explicit_steps(pdx_ctx *c, int nsteps, int precision, int use_graph);
+/* Per stored step of the dynamic drivers with FC 2 since the context was created (the
+ * explicit start-up step included): bonds kept, bonds broken, largest principal bond stress;
+ * info [*n * 3] (or NULL for the count). */
+int pdx_step_log(const pdx_ctx *c, double *info, int *n);
 /* Largest eigenvalue of M^-1 K (power iteration) -> critical dt = 2/sqrt(lmax) */
 int pdx_max_eig(pdx_ctx *c, int iters, double *lmax);
 int pdx_set_dt(pdx_ctx *c, double dt);
@@ -244,6 +287,11 @@ int pdx_local_damage(pdx_ctx *c, double *phi);
 int pdx_node_stresses(pdx_ctx *c, const double *u_all, double *sigma);
 /* the same on the device only, timed (ms of one recovery, inputs resident) */
 int pdx_node_stresses_timed(pdx_ctx *c, const double *u_all, double *ms);
+/* Newmark (acceleration form), pdfemDynamicNewmarkSolver_CSRformat (pdsolve.cpp:6092-6290): per
+ * step the predictor U_N (ramped prescribed values advanced by dt * velocity), F = F_ext - K U_N,
+ * (M + beta dt^2 K) a_{n+1} = F by the Krylov solve of o (warm-started from a_n), the update, then
+ * the BCs advance; with FC 2 in 2D the nodal stresses and the bond failure follow (:6258-6261,
+ * pdx_step_log) and K is assembled again when bonds broke.  u, v, a: [neq] (may be NULL). */
 int pdx_newmark_run(pdx_ctx *c, int nsteps, const pdx_solve_opts *o, double *u, double *v, double *a,
                     pdx_solve_stats *st);
 

@@ -68,6 +68,8 @@ def lib():
         L.om_lumped_mass.argtypes = [P, P]
         L.om_mass_on_K.restype = I
         L.om_mass_on_K.argtypes = [P, P]
+        L.om_mass_csr.restype = I
+        L.om_mass_csr.argtypes = [P, P, P, P, P]
         L.om_advance_bc.argtypes = [P]
         L.om_store_u.argtypes = [P, P]
         L.om_get_values.argtypes = [P, P]
@@ -260,6 +262,18 @@ class Model:
         _chk(lib().om_mass_on_K(self.h, _p(Mv)))
         return Mv
 
+    def mass_csr(self):
+        """(ia, ja, values) of the explicit driver's consistent mass on its own pattern
+        (setCSRIndexes_gloMassMat pdsolve.cpp:5674-5817, assembleElemassMatFEM_CSRformat
+        :5154-5210, assembleMassMatPD_CSRformat :4568-4595)."""
+        nz = C.c_longlong(0)
+        _chk(lib().om_mass_csr(self.h, None, None, None, C.byref(nz)))
+        ia = np.zeros(self.neq + 1, np.int64)
+        ja = np.zeros(max(nz.value, 1), np.int64)
+        vals = np.zeros(max(nz.value, 1))
+        _chk(lib().om_mass_csr(self.h, _p(ia), _p(ja), _p(vals), C.byref(nz)))
+        return ia, ja[: nz.value], vals[: nz.value]
+
     def advance_bc(self):
         lib().om_advance_bc(self.h)
 
@@ -327,6 +341,66 @@ class Model:
         _chk(lib().om_explicit_run(self.h, nsteps, _p(u), _p(up)))
         return u, up
 
+    def _fc2_after_store(self, fc, topk, sigma_ult):
+        """calGlobalNodeStresses + failureProcess as the dynamic drivers call them after storing
+        the step's displacement (pdsolve.cpp:6060-6064, :6258-6261): FC 2 in 2D only (3D
+        returns at once, :6594-6597).  Returns (bonds kept, bonds broken, largest s1)."""
+        if fc != 2 or self.dim != 2:
+            return 0, 0, 0.0
+        sig = self.node_stresses(self.values())
+        return self.fc2_step(sig, sigma_ult, topk)
+
+    def dynamic_run(self, nsteps, lumped=None, fc=None, topk=None, sigma_ult=None):
+        """pdfemDynamicSolver_CSRformat (pdsolve.cpp:5842-6090), single rank: the start-up step
+        u1 = u0 + dt^2/2 a0 (:5979-6017) and nsteps loop steps, each advancing the BCs before the
+        force (setVaryEssentialBC / setVaryNaturalBC), then
+          lumped mass:     u_{n+1} = 2 u_n + dt^2 F/m - u_{n-1}    (timeIntegration_LM :5113-5132)
+          consistent mass: M a = F (PARDISO in the reference, SuperLU here), then
+                           u_{n+1} = 2 u_n + dt^2 a - u_{n-1}       (timeIntegration_NLM :5134-5152)
+        and after every stored step the nodal stresses and the FC 2 bond update (2D).
+        lumped / fc / topk / sigma_ult default to the model's LUMPMASS, FC, TOPK and strength.
+        Returns u_{nsteps+1}, u_{nsteps} (free DOFs) and per step (kept, broken, largest s1)."""
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+        lumped = self.mesh.get("lumped_mass", False) if lumped is None else lumped
+        fc = self.mesh.get("fc", 0) if fc is None else fc
+        topk = self.mesh.get("topk", 0) if topk is None else topk
+        sigma_ult = self.mesh_sigma_ult if sigma_ult is None else sigma_ult
+        dt = self.mesh.get("dt", 1.0)
+        eq = self.nodes()["eq"][:, : self.dim].ravel()
+        vals = self.values()
+        un = np.zeros(self.neq)
+        un[eq[eq >= 0]] = vals[eq >= 0]
+        if lumped:
+            m = self.lumped_mass()
+            accel = lambda F: F / m
+        else:
+            ia, ja, mv = self.mass_csr()
+            lu = spla.splu(sp.csr_matrix((mv, ja, ia), shape=(self.neq, self.neq)).tocsc())
+            accel = lu.solve
+        info = np.zeros((nsteps + 1, 3))
+        self.advance_bc()
+        _, F = self.assemble(0)
+        if lumped:
+            unp1 = F / m * (0.5 * dt * dt) + un
+        else:
+            unp1 = accel(F) * (0.5 * dt * dt) + un
+        self.store_u(unp1)
+        info[0] = self._fc2_after_store(fc, topk, sigma_ult)
+        unm1 = un
+        for n in range(1, nsteps + 1):
+            unm1, un = un, unp1
+            self.advance_bc()
+            _, F = self.assemble(0)
+            if lumped:
+                tA = dt * dt * F / m
+            else:
+                tA = dt * dt * accel(F)
+            unp1 = un * 2.0 + tA - unm1
+            self.store_u(unp1)
+            info[n] = self._fc2_after_store(fc, topk, sigma_ult)
+        return unp1, un, info
+
     def newmark_UN(self, v, a):
         UN = np.zeros(self.nn * self.dim)
         lib().om_newmark_UN(self.h, _p(np.ascontiguousarray(v)), _p(np.ascontiguousarray(a)), _p(UN))
@@ -352,9 +426,11 @@ def static_solve(model: Model):
     return u, K, F
 
 
-def newmark_run(model: Model, nsteps: int):
+def newmark_run(model: Model, nsteps: int, fc=0, topk=0, sigma_ult=None, info=None):
     """pdfemDynamicNewmarkSolver_CSRformat (pdsolve.cpp:6092-6290) with SuperLU.
-    Returns the free-DOF displacement, velocity and acceleration after nsteps."""
+    Returns the free-DOF displacement, velocity and acceleration after nsteps.  With fc == 2
+    (2D) every step ends with the nodal stresses and the FC 2 bond update at the advanced BC
+    values (:6250-6261); info (or None) receives per step (kept, broken, largest s1)."""
     import scipy.sparse.linalg as spla
     neq, dt, beta, gamma = model.neq, model.mesh.get("dt", 1.0), model.mesh.get("beta", 0.25), model.mesh.get("gamma", 0.5)
     ia, ja = model.csr()
@@ -376,6 +452,9 @@ def newmark_run(model: Model, nsteps: int):
         model.store_u(u)
         model.advance_bc()
         a = anp1
+        r = model._fc2_after_store(fc, topk, model.mesh_sigma_ult if sigma_ult is None else sigma_ult)
+        if info is not None:
+            info.append(r)
     return u, v, a
 
 
@@ -401,7 +480,7 @@ def read_lis(path: str) -> dict:
     sdim = nxt()
     pro = int(nxt())
     E, nu, rho = float(nxt()), float(nxt()), float(nxt())
-    nxt(), nxt()  # KIc, sigult
+    kic, sigma_ult = float(nxt()), float(nxt())
     dim = 3 if sdim == "3D" else 2
     nn, ne = int(nxt()), int(nxt())
     x = np.zeros((nn, 3))
@@ -428,7 +507,8 @@ def read_lis(path: str) -> dict:
             nen[i] = 4 if c[3] != c[2] else 3
     # the reader skips one header line before each section (getline x2 after a token)
     rest = _sections_after_elements(lines)
-    return dict(title=title, dim=dim, proType=pro, E=E, nu=nu, rho=rho, x=x, econ=econ, nen=nen, algo=algo, **rest)
+    return dict(title=title, dim=dim, proType=pro, E=E, nu=nu, rho=rho, x=x, econ=econ, nen=nen, algo=algo,
+                kic=kic, sigma_ult=sigma_ult, **rest)
 
 
 def _sections_after_elements(lines):
@@ -485,7 +565,7 @@ def read_case(cmd_path: str) -> dict:
     base = os.path.dirname(os.path.abspath(cmd_path))
     m = None
     opts = dict(fac=3.0, nlf=0.33, me=0.0, dt=1.0, nsteps=1, save=1, gamma=0.5, beta=0.25, solver=-1,
-                newmark=False, lumped=False, fc=0)
+                newmark=False, lumped=False, fc=0, topk=0, vtk_binary=True, rf=[])
     pend = []
     with open(cmd_path) as f:
         for line in f:
@@ -506,12 +586,21 @@ def read_case(cmd_path: str) -> dict:
             opts["fac"], opts["nlf"] = float(tk[4]), float(tk[5])
             if len(tk) > 6:
                 opts["me"] = float(tk[6])
-        elif k == "NEWMARK":
-            opts["newmark"] = tk[1] == "ON"
+        elif k == "NEWMARK" and len(tk) > 1:
+            if tk[1] in ("ON", "OFF"):  # anything else is ignored (fioFile.cpp:151-181)
+                opts["newmark"] = tk[1] == "ON"
             if len(tk) > 3:
                 opts["gamma"], opts["beta"] = float(tk[2]), float(tk[3])
-        elif k == "LUMPMASS":
-            opts["lumped"] = tk[1] == "ON"
+        elif k == "LUMPMASS" and len(tk) > 1:
+            if tk[1] in ("ON", "OFF"):  # fioFile.cpp:208-230
+                opts["lumped"] = tk[1] == "ON"
+        elif k == "VTKFORMAT" and len(tk) > 1:
+            if tk[1] in ("ASCII", "BINARY"):  # fioFile.cpp:422-444
+                opts["vtk_binary"] = tk[1] == "BINARY"
+        elif k == "RF" and len(tk) > 1:
+            opts["rf"] += [int(v) for v in tk[1:]]
+        elif k == "TOPK" and len(tk) > 1:
+            opts["topk"] = int(tk[1])
         elif k == "EBC":
             m["ebc"][int(tk[1])]["val"] = float(tk[2])
         elif k == "VEBC":
@@ -524,8 +613,11 @@ def read_case(cmd_path: str) -> dict:
             m["nbc"][int(tk[1])]["vel"] = float(tk[2])
         elif k == "FC":
             opts["fc"] = int(tk[1])
+            if opts["fc"] == 2 and len(tk) > 2:
+                opts["topk"] = int(tk[2])
     m.update(fac=opts["fac"], nlf=opts["nlf"], me=opts["me"], dt=opts["dt"], beta=opts["beta"],
-             gamma=opts["gamma"])
+             gamma=opts["gamma"], lumped_mass=opts["lumped"], newmark=opts["newmark"], fc=opts["fc"],
+             topk=opts["topk"], save_freq=opts["save"], vtk_binary=opts["vtk_binary"])
     m["opts"] = opts
     return m
 

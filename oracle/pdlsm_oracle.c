@@ -1743,6 +1743,120 @@ int om_mass_on_K(const om_model *M, double *Mv) {
     return OM_OK;
 }
 
+/* Consistent mass of the explicit driver on its own pattern: pdsolve::setCSRIndexes_gloMassMat
+ * (pdsolve.cpp:5674-5817) -- per node, the family's own node (:5698-5703) and the nodes of
+ * every FE element (algo 2, element order, :5705-5723), deduplicated and sorted (:5725-5737);
+ * rows of pure PD nodes (type 2) hold their diagonal only (:5764-5781), other rows every
+ * free column of those nodes (:5742-5762); zero-based int64 ia / ja -- then
+ * assembleElemassMatFEM_CSRformat (:5154-5210) and assembleMassMatPD_CSRformat (:4568-4595)
+ * with the slot search of findCSRIndexOfMat (:1773-1802) on that pattern.
+ * Call with ia == NULL for the size: *nnz_out gets the entry count.  vals may be NULL. */
+int om_mass_csr(const om_model *M, long long *ia, long long *ja, double *vals, long long *nnz_out) {
+    int nn = M->nn, d = M->dim;
+    int *cnt = xcalloc(nn + 1, sizeof(int));
+    for (int f = 0; f < M->nfam; f++) cnt[M->fam_node[f]] += 1;
+    for (int e = 0; e < M->ne; e++)
+        if (M->algo[e] == 2)
+            for (int a = 0; a < M->nen[e]; a++) cnt[M->econ[e * 8 + a] - 1] += M->nen[e];
+    int *off = xcalloc(nn + 1, sizeof(int));
+    for (int k = 0; k < nn; k++) off[k + 1] = off[k] + cnt[k];
+    int *adj = xcalloc((size_t)off[nn] + 1, sizeof(int)), *fill = xcalloc(nn + 1, sizeof(int));
+    for (int f = 0; f < M->nfam; f++) {
+        int k = M->fam_node[f];
+        adj[off[k] + fill[k]++] = k + 1;
+    }
+    for (int e = 0; e < M->ne; e++) {
+        if (M->algo[e] != 2) continue;
+        for (int a = 0; a < M->nen[e]; a++) {
+            int k = M->econ[e * 8 + a] - 1;
+            for (int b = 0; b < M->nen[e]; b++) adj[off[k] + fill[k]++] = M->econ[e * 8 + b];
+        }
+    }
+    for (int k = 0; k < nn; k++) {
+        int *a = &adj[off[k]], n = cnt[k], u = 0;
+        qsort(a, n, sizeof(int), icmp);
+        for (int i = 0; i < n; i++) if (i == 0 || a[i] != a[i - 1]) a[u++] = a[i];
+        fill[k] = u;
+    }
+    long long *lia = xcalloc(M->neq + 2, sizeof(long long)), nz = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        long long p = 0;
+        for (int k = 0; k < nn; k++)
+            for (int i = 0; i < d; i++) {
+                int r = M->eq[k * 3 + i];
+                if (r == -1) continue;
+                if (M->ntype[k] == 2) {
+                    if (pass) ja[p] = r;
+                    p++;
+                    if (!pass) lia[r + 1]++;
+                    continue;
+                }
+                for (int t = 0; t < fill[k]; t++)
+                    for (int j = 0; j < d; j++) {
+                        int c = M->eq[(adj[off[k] + t] - 1) * 3 + j];
+                        if (c == -1) continue;
+                        if (pass) ja[p] = c;
+                        p++;
+                        if (!pass) lia[r + 1]++;
+                    }
+            }
+        if (!pass) {
+            nz = p;
+            for (int i = 0; i < M->neq; i++) lia[i + 1] = lia[i] + lia[i + 1];
+            if (nnz_out) *nnz_out = nz;
+            if (!ia) break;
+            memcpy(ia, lia, sizeof(long long) * (M->neq + 1));
+        }
+    }
+    int rc = OM_OK;
+    if (ia && vals) {
+        double Me[576];
+        memset(vals, 0, sizeof(double) * nz);
+        for (int t = 0; t < M->nfe && rc == OM_OK; t++) {
+            int e = M->fe[t], n = M->nen[e], nd;
+            ele_mass(M, e, Me, &nd);
+            for (int a = 0; a < n; a++)
+                for (int ii = 0; ii < d; ii++) {
+                    int row = M->eq[(M->econ[e * 8 + a] - 1) * 3 + ii];
+                    if (row == -1) continue;
+                    for (int b = 0; b < n; b++)
+                        for (int jj = 0; jj < d; jj++) {
+                            int col = M->eq[(M->econ[e * 8 + b] - 1) * 3 + jj];
+                            if (col == -1) continue;
+                            long long lo = ia[row], hi = ia[row + 1], s = -1;
+                            while (lo < hi) {
+                                long long mid = (lo + hi) / 2;
+                                if (col == ja[mid]) { s = mid; break; }
+                                else if (col < ja[mid]) hi = mid;
+                                else lo = mid + 1;
+                            }
+                            if (s < 0) { rc = OM_ERR_PATTERN; continue; }
+                            vals[s] = vals[s] + Me[(d * a + ii) * nd + d * b + jj];
+                        }
+                }
+        }
+        for (int f = 0; f < M->nfam && rc == OM_OK; f++) {
+            int k = M->fam_node[f];
+            double dv = M->fam_v[f];
+            for (int i = 0; i < d; i++) {
+                int r = M->eq[k * 3 + i];
+                if (r == -1) continue;
+                long long lo = ia[r], hi = ia[r + 1], s = -1;
+                while (lo < hi) {
+                    long long mid = (lo + hi) / 2;
+                    if (r == ja[mid]) { s = mid; break; }
+                    else if (r < ja[mid]) hi = mid;
+                    else lo = mid + 1;
+                }
+                if (s < 0) { rc = OM_ERR_PATTERN; break; }
+                vals[s] = vals[s] + dv * M->rho;
+            }
+        }
+    }
+    free(cnt); free(off); free(adj); free(fill); free(lia);
+    return rc;
+}
+
 /* pdsolve::setVaryEssentialBC (rati = 1, non-quasi-static branch, pdsolve.cpp:5263-5312)
  * and setVaryNaturalBC (:5314-5329) */
 void om_advance_bc(om_model *M) {

@@ -1,7 +1,6 @@
 """Python binding of libpdx.so, the MI355X PDLSM-FEM hot-path library.
 
-The package directory name contains a hyphen, so load it with
-``pdx = load_package()`` from ``pdlsm_fem_loader`` or import it by path:
+The package directory name contains a hyphen, so import it by path:
 
     import importlib.util, sys
     spec = importlib.util.spec_from_file_location(
@@ -40,7 +39,19 @@ class MeshDesc(C.Structure):
                 ("ebc_val", _P), ("ebc_vel", _P), ("ebc_vary", _P), ("n_nbc", _I), ("nbc_off", _P),
                 ("nbc_con", _P), ("nbc_val", _P), ("nbc_vel", _P), ("nbc_vary", _P), ("n_crack", _I),
                 ("crack", _P), ("horizon_factor", _D), ("nlf", _D), ("me", _D), ("dt", _D), ("beta", _D),
-                ("gamma", _D), ("sigma_ult", _D), ("kic", _D), ("fc", _I), ("topk", _I)]
+                ("gamma", _D), ("sigma_ult", _D), ("kic", _D), ("fc", _I), ("topk", _I),
+                ("lumped_mass", _I), ("newmark", _I), ("save_freq", _I), ("vtk_ascii", _I)]
+
+
+class Settings(C.Structure):
+    _fields_ = [("solver_flag", _I), ("newmark", _I), ("lumped_mass", _I), ("num_steps", _I), ("save_freq", _I),
+                ("vtk_binary", _I), ("fc", _I), ("topk", _I), ("n_rf", _I), ("rf_ids", _I * 16), ("dt", _D),
+                ("beta", _D), ("gamma", _D), ("horizon_factor", _D), ("nlf", _D), ("sigma_ult", _D), ("kic", _D)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "rf_ids"}
+        d["rf_ids"] = [self.rf_ids[i] for i in range(min(self.n_rf, 16))]
+        return d
 
 
 class SolveOpts(C.Structure):
@@ -70,6 +81,10 @@ EXPORTS = {
     "pdx_ctx_bonds": (_I, [_P, _P]),
     "pdx_local_damage": (_I, [_P, _P]),
     "pdx_model_destroy": (None, [_P]),
+    "pdx_model_settings": (_I, [_P, _P]),
+    "pdx_model_mass_csr": (_I, [_P, _P, _P, _P]),
+    "pdx_mass_csr": (_I, [_P, _P]),
+    "pdx_step_log": (_I, [_P, _P, _P]),
     "pdx_model_sizes": (_I, [_P, _P]),
     "pdx_model_csr": (_I, [_P, _P, _P]),
     "pdx_model_adjacency": (_I, [_P, _P, _P]),
@@ -188,6 +203,9 @@ def mesh_desc(mesh: dict):
     d.dt, d.beta, d.gamma = float(mesh.get("dt", 1.0)), float(mesh.get("beta", 0.25)), float(mesh.get("gamma", 0.5))
     d.sigma_ult, d.kic = float(mesh.get("sigma_ult", 0.0)), float(mesh.get("kic", 0.0))
     d.fc, d.topk = int(mesh.get("fc", 0)), int(mesh.get("topk", 0))
+    # run settings: a dict without them takes the reference's defaults (consistent mass, no Newmark)
+    d.lumped_mass, d.newmark = int(bool(mesh.get("lumped_mass", False))), int(bool(mesh.get("newmark", False)))
+    d.save_freq, d.vtk_ascii = int(mesh.get("save_freq", 1)), int(not mesh.get("vtk_binary", True))
     return d, keep
 
 
@@ -272,6 +290,22 @@ class Model:
         _chk(lib().pdx_model_values(self.h, _p(u)))
         return u
 
+    def settings(self) -> dict:
+        """Run settings of the command file (solver flag, NEWMARK, LUMPMASS, SETSOLVING, FC, RF,
+        VTKFORMAT) or of the mesh dict."""
+        st = Settings()
+        _chk(lib().pdx_model_settings(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def mass_csr(self):
+        """(ia, ja) of the explicit driver's consistent-mass pattern (setCSRIndexes_gloMassMat)."""
+        nz = C.c_longlong(0)
+        _chk(lib().pdx_model_mass_csr(self.h, None, None, C.byref(nz)))
+        ia = np.zeros(self.neq + 1, np.int64)
+        ja = np.zeros(max(nz.value, 1), np.int64)
+        _chk(lib().pdx_model_mass_csr(self.h, _p(ia), _p(ja), C.byref(nz)))
+        return ia, ja[: nz.value]
+
 
 def solve_opts(krylov=PDX_KRYLOV_BICGSTAB, precond=PDX_PREC_JACOBI, precision=PDX_FP64, tol=1e-8, max_iter=20000,
                inner_tol=1e-4, warm_start=False, coarse_agg=0, fallback=False, no_graph=False, mirrored=False):
@@ -325,6 +359,22 @@ class Context:
         out = np.zeros(max(self.model.nnz, 1))
         _chk(lib().pdx_mass_on_K(self.h, _p(out)))
         return out[: self.model.nnz]
+
+    def mass_csr(self):
+        """Values of the consistent mass on the model's mass pattern (Model.mass_csr)."""
+        ia, ja = self.model.mass_csr()
+        out = np.zeros(max(len(ja), 1))
+        _chk(lib().pdx_mass_csr(self.h, _p(out)))
+        return out[: len(ja)]
+
+    def step_log(self):
+        """Per stored step of the dynamic drivers with FC 2: [n, 3] bonds kept, bonds broken,
+        largest principal bond stress."""
+        n = C.c_int(0)
+        _chk(lib().pdx_step_log(self.h, None, C.byref(n)))
+        out = np.zeros(max(n.value, 1) * 3)
+        _chk(lib().pdx_step_log(self.h, _p(out), C.byref(n)))
+        return out[: n.value * 3].reshape(-1, 3)
 
     def explicit_run(self, nsteps, precision=PDX_FP64):
         u = np.zeros(max(self.model.neq, 1))

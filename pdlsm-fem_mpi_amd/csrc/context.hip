@@ -15,8 +15,12 @@
 
 namespace pdx {
 
-int explicit_prepare(Ctx &c, int prec, std::string &err);
-int dev_explicit_steps(Ctx &c, int nsteps, int prec, int use_graph, std::string &err);
+int dev_explicit_steps(Ctx &c, int nsteps, int prec, int use_graph, bool lumped,
+                       const std::function<int(std::string &)> &after_step, std::string &err);
+int dev_explicit_dofvals(Ctx &c, const double **out, std::string &err);
+int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double> &dofval, double *u_h, double *v_h,
+                double *a_h, pdx_solve_stats &stats, const std::function<int(const double *, std::string &)> &after_step,
+                std::string &err);
 int dev_explicit_state(Ctx &c, double *u_out, double *u_prev, std::string &err);
 int dev_max_eig(Ctx &c, int iters, double *lmax, std::string &err);
 int dev_solve(Ctx &c, const pdx_solve_opts &o, const double *b_dev, double *x_dev, pdx_solve_stats &st,
@@ -324,6 +328,9 @@ void ctx_free(Ctx *c) {
         if (g) hipGraphExecDestroy(g);
     for (void *p : c->owned) hipFree(p);
     if (c->K32) hipFree(c->K32);
+    if (c->nlm_F) hipFree(c->nlm_F);
+    if (c->nlm_a) hipFree(c->nlm_a);
+    if (c->dofs_tmp) hipFree(c->dofs_tmp);
     if (c->K32s) hipFree(c->K32s);
     if (c->K64s) hipFree(c->K64s);
     if (c->Mk) hipFree(c->Mk);
@@ -363,6 +370,8 @@ int set_error(int rc, const std::string &msg) {
 static int fail(int rc, const std::string &msg) { return set_error(rc, msg); }
 
 extern "C" {
+
+static int explicit_driver(pdx_ctx *pc, int nsteps, int prec, int use_graph, std::string &err);
 
 const char *pdx_last_error(void) { return g_err.c_str(); }
 const char *pdx_version(void) { return "pdx 0.1 (gfx950)"; }
@@ -609,7 +618,7 @@ int pdx_lumped_mass(pdx_ctx *pc, double *mout) {
 int pdx_explicit_run(pdx_ctx *pc, int nsteps, int precision, double *u_out, double *u_prev) {
     if (!pc || nsteps < 0) return fail(PDX_ERR_ARG, "bad argument");
     std::string err;
-    int rc = dev_explicit_steps(*pc->c, nsteps, precision, 1, err);
+    int rc = explicit_driver(pc, nsteps, precision, 1, err);
     if (rc) return fail(rc, err);
     rc = dev_explicit_state(*pc->c, u_out, u_prev, err);
     return rc ? fail(rc, err) : PDX_OK;
@@ -618,8 +627,79 @@ int pdx_explicit_run(pdx_ctx *pc, int nsteps, int precision, double *u_out, doub
 int pdx_explicit_steps(pdx_ctx *pc, int nsteps, int precision, int use_graph) {
     if (!pc || nsteps < 0) return fail(PDX_ERR_ARG, "bad argument");
     std::string err;
-    int rc = dev_explicit_steps(*pc->c, nsteps, precision, use_graph, err);
+    int rc = explicit_driver(pc, nsteps, precision, use_graph, err);
     return rc ? fail(rc, err) : PDX_OK;
+}
+
+int pdx_step_log(const pdx_ctx *pc, double *info, int *n) {
+    if (!pc || !n) return fail(PDX_ERR_ARG, "null argument");
+    *n = (int)(pc->step_log.size() / 3);
+    if (info) std::copy(pc->step_log.begin(), pc->step_log.end(), info);
+    return PDX_OK;
+}
+
+int pdx_model_settings(const pdx_model *pm, pdx_settings *out) {
+    if (!pm || !out) return fail(PDX_ERR_ARG, "null argument");
+    const Model &m = pm->m;
+    pdx_settings s{};
+    s.solver_flag = m.solver_flag;
+    s.newmark = m.newmark;
+    s.lumped_mass = m.lumped;
+    s.num_steps = m.num_steps;
+    s.save_freq = m.save_freq;
+    s.vtk_binary = m.vtk_binary;
+    s.fc = m.fc;
+    s.topk = m.topk;
+    s.n_rf = (int)m.rf_ids.size();
+    for (int i = 0; i < s.n_rf && i < 16; i++) s.rf_ids[i] = m.rf_ids[i];
+    s.dt = m.dt;
+    s.beta = m.beta;
+    s.gamma = m.gamma;
+    s.horizon_factor = m.fac;
+    s.nlf = m.nlf;
+    s.sigma_ult = m.sig_ult;
+    s.kic = m.kic;
+    *out = s;
+    return PDX_OK;
+}
+
+int pdx_model_mass_csr(const pdx_model *pm, long long *ia, long long *ja, long long *nnz) {
+    if (!pm || !nnz) return fail(PDX_ERR_ARG, "null argument");
+    std::vector<int64_t> hia, hja;
+    mass_pattern(pm->m, hia, hja);
+    *nnz = (long long)hja.size();
+    if (ia) std::copy(hia.begin(), hia.end(), ia);
+    if (ja) std::copy(hja.begin(), hja.end(), ja);
+    return PDX_OK;
+}
+
+int pdx_mass_csr(pdx_ctx *pc, double *Mv) {
+    if (!pc || !Mv) return fail(PDX_ERR_ARG, "null argument");
+    Ctx &c = *pc->c;
+    const Model &m = pc->model->m;
+    std::string err;
+    hipSetDevice(c.device);
+    int rc = dev_mass_on_K(c, err);
+    if (rc) return fail(rc, err);
+    std::vector<double> mk(std::max<int64_t>(m.nnz, 1));
+    double *dk = nullptr;
+    if (hipMalloc((void **)&dk, mk.size() * 8) != hipSuccess) return fail(PDX_ERR_HIP, "alloc");
+    rc = dev_extract_csr(c, c.Mk, dk);
+    if (!rc && hipMemcpyAsync(mk.data(), dk, m.nnz * 8, hipMemcpyDeviceToHost, c.stream) != hipSuccess) rc = PDX_ERR_HIP;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) rc = PDX_ERR_HIP;
+    hipFree(dk);
+    if (rc) return fail(rc, "HIP error in the mass assembly");
+    // the mass pattern lies inside K's pattern (family self / FE neighbours of every node)
+    std::vector<int64_t> hia, hja;
+    mass_pattern(m, hia, hja);
+    for (int r = 0; r < m.neq; r++)
+        for (int64_t q = hia[r]; q < hia[r + 1]; q++) {
+            auto b = m.ja.begin() + m.ia[r], e = m.ja.begin() + m.ia[r + 1];
+            auto it = std::lower_bound(b, e, hja[q]);
+            if (it == e || *it != hja[q]) return fail(PDX_ERR_PATTERN, "mass entry outside the stiffness pattern");
+            Mv[q] = mk[it - m.ja.begin()];
+        }
+    return PDX_OK;
 }
 
 int pdx_last_timing(pdx_ctx *pc, double *total_ms, double *per_step_ms) {
@@ -863,6 +943,84 @@ static int fc2_step(pdx_ctx *pc, const double *sig_dev, const std::vector<double
         c.K_valid = false;  // the next assembly sees the new bond states
     }
     return PDX_OK;
+}
+
+// calGlobalNodeStresses + failureProcess after a stored step of a dynamic driver
+// (pdsolve.cpp:6060-6064 explicit, :6258-6261 Newmark): nodal stresses of the DOF values, then
+// the FC 2 update (flag 2, non-static solver: the TopK bonds above the strength break, with the
+// crack segments of Finally_TopK_andUpdate_bondStaus); broken bonds invalidate K, which the
+// driver assembles again before the next step's force.  Logs (kept, broken, largest s1).
+static int dynamic_failure(pdx_ctx *pc, const double *uall_dev, std::string &err) {
+    Ctx &c = *pc->c;
+    const Model &m = pc->model->m;
+    double *dsig = nullptr;
+    PDX_HIP(hipMalloc((void **)&dsig, (size_t)c.nn * 6 * 8));
+    std::vector<double> sig((size_t)c.nn * 6);
+    int rc = dev_node_stresses(c, uall_dev, dsig, err);
+    if (!rc && hipMemcpyAsync(sig.data(), dsig, sig.size() * 8, hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+        rc = PDX_ERR_HIP;
+    if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) rc = PDX_ERR_HIP;
+    double smax = 0.0;
+    int nkept = 0, nbroke = 0;
+    if (!rc) rc = fc2_step(pc, dsig, sig, m.sig_ult, m.topk, smax, nkept, nbroke, err);
+    hipFree(dsig);
+    if (rc) return rc;
+    pc->step_log.push_back(nkept);
+    pc->step_log.push_back(nbroke);
+    pc->step_log.push_back(smax);
+    return PDX_OK;
+}
+
+// failure criteria of the dynamic drivers: FC 0 none, FC 2 in 2D (failureProcess returns at once
+// in 3D, pdsolve.cpp:6594-6597); FC 1 (the J-integral crack growth) is not built
+static int dynamic_failure_mode(const Model &m, bool &fail2, std::string &err) {
+    fail2 = false;
+    if (m.fc != 0 && m.fc != 2) {
+        err = "failure criterion FC " + std::to_string(m.fc) +
+              " is not supported by the dynamic drivers (FC 0 and FC 2 are)";
+        return PDX_ERR_ARG;
+    }
+    fail2 = m.fc == 2 && m.dim == 2;
+    if (fail2 && !(m.sig_ult > 0)) {
+        err = "FC 2 needs the material strength (sigma_ult > 0)";
+        return PDX_ERR_ARG;
+    }
+    return PDX_OK;
+}
+
+// pdfemDynamicSolver_CSRformat (pdsolve.cpp:5842-6090): lumped or consistent mass as the model's
+// LUMPMASS says, FC 2 bond failure after every stored step
+static int explicit_driver(pdx_ctx *pc, int nsteps, int prec, int use_graph, std::string &err) {
+    if (prec != PDX_FP64 && prec != PDX_FP32) { err = "precision must be PDX_FP64 or PDX_FP32"; return PDX_ERR_ARG; }
+    const Model &m = pc->model->m;
+    bool fail2;
+    if (int rc = dynamic_failure_mode(m, fail2, err)) return rc;
+    hipSetDevice(pc->c->device);
+    std::function<int(std::string &)> cb;
+    if (fail2)
+        cb = [pc](std::string &e) -> int {
+            const double *u = nullptr;
+            int rc = dev_explicit_dofvals(*pc->c, &u, e);
+            return rc ? rc : dynamic_failure(pc, u, e);
+        };
+    return dev_explicit_steps(*pc->c, nsteps, prec, use_graph, m.lumped, cb, err);
+}
+
+int pdx_newmark_run(pdx_ctx *pc, int nsteps, const pdx_solve_opts *op, double *u, double *v, double *a,
+                    pdx_solve_stats *st) {
+    if (!pc || nsteps < 0) return fail(PDX_ERR_ARG, "bad argument");
+    pdx_solve_opts o = op ? *op : default_opts();
+    if (!op) o.warm_start = 1;
+    std::string err;
+    bool fail2;
+    if (int rc = dynamic_failure_mode(pc->model->m, fail2, err)) return fail(rc, err);
+    hipSetDevice(pc->c->device);
+    std::function<int(const double *, std::string &)> cb;
+    if (fail2) cb = [pc](const double *uall, std::string &e) -> int { return dynamic_failure(pc, uall, e); };
+    pdx_solve_stats s{};
+    int rc = dev_newmark(*pc->c, nsteps, o, pc->dofval, u, v, a, s, cb, err);
+    if (st) *st = s;
+    return rc ? fail(rc, err) : PDX_OK;
 }
 
 int pdx_quasi_static_run(pdx_ctx *pc, int nsteps, const pdx_solve_opts *op, int fc, int topk, double sigma_ult,

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <functional>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -130,7 +131,11 @@ struct Ctx {
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
     int tune_explicit[2] = {-1, -1};               // chosen step-kernel variant per precision
     float tune_ms[2][PDX_EXPLICIT_FORMS] = {{0}};  // measured ms per variant (-1: not run)
-    int graph_phase[2] = {-1, -1};
+    // consistent-mass explicit steps (LUMPMASS OFF): F = F_ext - K u_n and the acceleration of
+    // M a = F, full DOF vectors; the acceleration warm-starts the next step's solve
+    double *nlm_F = nullptr, *nlm_a = nullptr;
+    long long nlm_iters = 0;  // CG iterations of the mass solves so far
+    double *dofs_tmp = nullptr;  // [nn*d] DOF values of the last stored step (stress input)
 
     // scratch for Krylov
     std::vector<DevBuf> scratch;
@@ -203,6 +208,8 @@ void free_krylov_ws(Ctx &c);
 void free_gnn_work(Ctx &c);
 // nodal stresses of u_dev; check: read the singular-Jacobian flag (synchronises the stream)
 int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err, bool check = true);
+// out[t] = free_src[t] at free DOFs, presc_src[t] at prescribed ones (full DOF vectors, solve.hip)
+int dev_merge_dofvals(Ctx &c, const double *free_src, const double *presc_src, double *out);
 int stress_status_check(Ctx &c, std::string &err);
 int dev_bond_sigma1(Ctx &c, const int *bond_dev, const double *sigma_dev, double *s1_dev, std::string &err);
 double two_level_setup_ms(Ctx &c);
@@ -219,6 +226,8 @@ struct pdx_ctx {
     const pdx_model *model = nullptr;
     std::vector<double> dofval;  // DOF values used by the assembly modes, [nn*dim]
     std::vector<int> bond;       // bond states of this context (the model's, then broken by failure)
+    // per stored step of the dynamic drivers with FC 2: bonds kept, bonds broken, largest s1
+    std::vector<double> step_log;
 };
 
 #define PDX_HIP(call)                                                                 \

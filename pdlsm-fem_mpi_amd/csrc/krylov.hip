@@ -1062,15 +1062,21 @@ static int run_chunk(Ctx &c, KrylovWs<V> &w, const GraphSig &sig, bool use_graph
     }
     hipGraphExec_t &g = it == 0 ? w.g_first : w.g_next;
     if (!g) {
+        // every exit after hipStreamBeginCapture ends the capture, so the stream never stays
+        // in capture mode; a launch error inside the capture fails the chunk
         hipGraph_t gr = nullptr;
-        PDX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < n; k++) body(it + k);
-        PDX_HIP(hipStreamEndCapture(c.stream, &gr));
-        hipError_t e = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
-        hipGraphDestroy(gr);
+        hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (int k = 0; k < n; k++) body(it + k);
+            const hipError_t launch = hipGetLastError();
+            e = hipStreamEndCapture(c.stream, &gr);
+            if (e == hipSuccess) e = launch;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
+        if (gr) hipGraphDestroy(gr);
         if (e != hipSuccess) {
             g = nullptr;
-            err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+            err = std::string("Krylov chunk graph capture: ") + hipGetErrorString(e);
             return PDX_ERR_HIP;
         }
     }

@@ -583,6 +583,57 @@ int Model::setup(std::string &err) {
     return PDX_OK;
 }
 
+// setCSRIndexes_gloMassMat (pdsolve.cpp:5674-5817): per node the family's own node and the nodes
+// of its FE elements (algo 2, every element in element order), deduplicated and sorted; rows of
+// pure PD nodes (type 2) hold the diagonal only, other rows every free column of those nodes
+void mass_pattern(const Model &m, std::vector<int64_t> &ia, std::vector<int64_t> &ja) {
+    const int nn = m.nn, dim = m.dim;
+    std::vector<std::vector<int>> nbr(nn);
+    for (int f = 0; f < m.nfam; f++) nbr[m.fam_node[f]].push_back(m.fam_node[f]);
+    for (int e = 0; e < m.ne; e++) {
+        if (m.algo[e] != 2) continue;
+        for (int a = 0; a < m.nen[e]; a++) {
+            auto &v = nbr[m.econ[size_t(e) * 8 + a]];
+            for (int b = 0; b < m.nen[e]; b++) v.push_back(m.econ[size_t(e) * 8 + b]);
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int k = 0; k < nn; k++) {
+        std::sort(nbr[k].begin(), nbr[k].end());
+        nbr[k].erase(std::unique(nbr[k].begin(), nbr[k].end()), nbr[k].end());
+    }
+    ia.assign(size_t(m.neq) + 1, 0);
+    for (int k = 0; k < nn; k++)
+        for (int i = 0; i < dim; i++) {
+            const int r = m.eq[size_t(k) * 3 + i];
+            if (r < 0) continue;
+            int64_t len = 0;
+            if (m.ntype[k] == 2) len = 1;
+            else
+                for (int q : nbr[k])
+                    for (int j = 0; j < dim; j++) len += m.eq[size_t(q) * 3 + j] >= 0;
+            ia[r + 1] = len;
+        }
+    for (int r = 0; r < m.neq; r++) ia[r + 1] += ia[r];
+    ja.resize(ia[m.neq]);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int k = 0; k < nn; k++)
+        for (int i = 0; i < dim; i++) {
+            const int r = m.eq[size_t(k) * 3 + i];
+            if (r < 0) continue;
+            int64_t p = ia[r];
+            if (m.ntype[k] == 2) {
+                ja[p] = r;
+                continue;
+            }
+            for (int q : nbr[k])
+                for (int j = 0; j < dim; j++) {
+                    const int c = m.eq[size_t(q) * 3 + j];
+                    if (c >= 0) ja[p++] = c;
+                }
+        }
+}
+
 int model_from_desc(const pdx_mesh_desc &d, Model &m, std::string &err) {
     if ((d.dim != 2 && d.dim != 3) || d.n_node <= 0 || d.n_ele <= 0 || !d.xyz || !d.ele_nodes || !d.ele_nen ||
         !d.ele_algo) {
@@ -640,6 +691,10 @@ int model_from_desc(const pdx_mesh_desc &d, Model &m, std::string &err) {
     m.kic = d.kic;
     m.fc = d.fc;
     m.topk = d.topk;
+    m.lumped = d.lumped_mass != 0;
+    m.newmark = d.newmark != 0;
+    m.save_freq = d.save_freq > 0 ? d.save_freq : 1;
+    m.vtk_binary = d.vtk_ascii == 0;
     return m.setup(err);
 }
 
@@ -763,6 +818,7 @@ int model_read_files(const std::string &cmd, Model &m, int &solver_flag, int &nu
         } else if (tk[0] == "SETSOLVING" && tk.size() >= 6) {
             m.dt = solver_flag == 2 ? 1.0 : std::atof(tk[1].c_str());
             num_steps = std::atoi(tk[2].c_str());
+            m.save_freq = std::atoi(tk[3].c_str());
             m.fac = std::atof(tk[4].c_str());
             m.nlf = std::atof(tk[5].c_str());
             if (tk.size() > 6) m.me = std::atof(tk[6].c_str());
@@ -772,9 +828,25 @@ int model_read_files(const std::string &cmd, Model &m, int &solver_flag, int &nu
             if (m.fc == 2 && tk.size() > 2) m.topk = std::atoi(tk[2].c_str());
         } else if (tk[0] == "TOPK" && tk.size() > 1) {
             m.topk = std::atoi(tk[1].c_str());
-        } else if (tk[0] == "NEWMARK" && tk.size() > 3) {
-            m.gamma = std::atof(tk[2].c_str());
-            m.beta = std::atof(tk[3].c_str());
+        } else if (tk[0] == "NEWMARK" && tk.size() > 1) {
+            // fioFiles::excuteCMD (fioFile.cpp:151-181): ON / OFF, then optional gamma and beta
+            if (tk[1] == "ON") m.newmark = true;
+            else if (tk[1] == "OFF") m.newmark = false;
+            if (tk.size() > 3) {
+                m.gamma = std::atof(tk[2].c_str());
+                m.beta = std::atof(tk[3].c_str());
+            }
+        } else if (tk[0] == "LUMPMASS" && tk.size() > 1) {
+            // fioFile.cpp:208-230; anything but ON / OFF is ignored
+            if (tk[1] == "ON") m.lumped = true;
+            else if (tk[1] == "OFF") m.lumped = false;
+        } else if (tk[0] == "VTKFORMAT" && tk.size() > 1) {
+            // fioFile.cpp:422-444
+            if (tk[1] == "ASCII") m.vtk_binary = false;
+            else if (tk[1] == "BINARY") m.vtk_binary = true;
+        } else if (tk[0] == "RF" && tk.size() > 1) {
+            // fioFile.cpp:402-421: EBC set ids whose reaction force the reference reports
+            for (size_t i = 1; i < tk.size(); i++) m.rf_ids.push_back(std::atoi(tk[i].c_str()));
         } else if (!have_mesh) {
             continue;
         } else if ((tk[0] == "EBC" || tk[0] == "VEBC") && tk.size() > 2) {
@@ -790,6 +862,9 @@ int model_read_files(const std::string &cmd, Model &m, int &solver_flag, int &nu
         }
     }
     if (!have_mesh) { err = "the command file names no MSHFILE"; return PDX_ERR_IO; }
+    m.solver_flag = solver_flag;
+    m.num_steps = num_steps;
+    if (m.save_freq <= 0) m.save_freq = 1;
     for (auto &s : m.ebc)
         for (int k : s.nid)
             if (k < 0 || k >= m.nn) { err = "EBC node out of range"; return PDX_ERR_MESH; }

@@ -42,6 +42,13 @@ struct Model {
     std::string title = "pdlsm";  // first token of the mesh file (the VTK title)
     double sig_ult = 0, kic = 0;  // material strength and toughness of the mesh file
     int fc = 0, topk = 0;         // failure criterion (FC) and bonds per step (TOPK) of the command file
+    // run settings of the command file (datModel defaults, datModel.cpp:17-56)
+    int solver_flag = -1;          // SOLVER: 0 dynamic, 1 static, 2 quasi-static
+    int num_steps = 1, save_freq = 1;  // SETSOLVING tokens 2 and 3
+    bool newmark = false;          // NEWMARK ON|OFF (cb_Newmark)
+    bool lumped = false;           // LUMPMASS ON|OFF (cb_lumpedMass)
+    bool vtk_binary = true;        // VTKFORMAT BINARY|ASCII (cb_vtkBinary)
+    std::vector<int> rf_ids;       // RF: EBC set ids of the reaction-force output
 
     // ---- DOFs ----
     std::vector<int> eq;       // [nn*3], -1 prescribed / unused
@@ -88,6 +95,10 @@ struct Model {
 
 int model_from_desc(const pdx_mesh_desc &d, Model &m, std::string &err);
 int model_read_files(const std::string &cmd, Model &m, int &solver_flag, int &num_steps, std::string &err);
+
+// consistent-mass pattern of the explicit driver (setCSRIndexes_gloMassMat, pdsolve.cpp:5674-5817):
+// zero-based int64 CSR over the free DOFs, columns sorted per row
+void mass_pattern(const Model &m, std::vector<int64_t> &ia, std::vector<int64_t> &ja);
 
 // small numerics shared with the device set-up
 double det33(const double m[3][3]);

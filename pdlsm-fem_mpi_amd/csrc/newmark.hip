@@ -8,6 +8,8 @@
 // started from a_n), then the displacement / velocity update.
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -221,8 +223,13 @@ __global__ void k_extrapolate(int n, const double *a, const double *ap, double *
     if (t < n) x[t] = 2.0 * a[t] - ap[t];
 }
 
+// pdfemDynamicNewmarkSolver_CSRformat (pdsolve.cpp:6092-6290).  after_step (may be null) runs
+// after every step with the DOF values of that step (free: u_{n+1}; prescribed: the values
+// advanced after the step, as calGlobalNodeStresses sees them, :6248-6261); when it invalidates
+// K (broken bonds), K and K_eff are assembled again before the next step.
 int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double> &dofval, double *u_h, double *v_h,
-                double *a_h, pdx_solve_stats &stats, std::string &err) {
+                double *a_h, pdx_solve_stats &stats, const std::function<int(const double *, std::string &)> &after_step,
+                std::string &err) {
     int rc;
     if (!c.K_valid && (rc = dev_assemble_K(c, err))) return rc;
     if (!c.Mk_valid && (rc = dev_mass_on_K(c, err))) return rc;
@@ -345,6 +352,17 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
         }
         for (int s = 0; s < c.n_nbc; s++)
             if (c.nbc_vary[s]) c.nbc_val[s] = c.nbc_val[s] + c.dt * c.nbc_vel[s];
+        if (after_step) {
+            if ((rc = dev_merge_dofvals(c, u, val, UN))) { err = "merge of the DOF values failed"; break; }
+            if ((rc = after_step(UN, err))) break;
+            if (!c.K_valid) {
+                // bonds broke: K, K_eff and its copies for the next step
+                if ((rc = dev_assemble_K(c, err))) break;
+                k_keff<<<4096, 256, 0, c.stream>>>(nk, c.K, c.Mk, c.beta, c.dt, Keff);
+                if (o.precision == PDX_FP32 && (rc = to_f32(c, Keff, Keff32, err))) break;
+                if (mir.K64 && (rc = gather_own(c, Keff, own64, own32, err))) break;
+            }
+        }
     }
     // store the displacement of the last completed step in the DOF values
     // (storeDisplacementResult, pdsolve.cpp:5656-5672)
@@ -389,22 +407,6 @@ int pdx_mass_on_K(pdx_ctx *pc, double *Mv) {
     hipStreamSynchronize(c.stream);
     hipFree(dk);
     return hipGetLastError() == hipSuccess ? PDX_OK : set_error(PDX_ERR_HIP, "HIP error in mass assembly");
-}
-
-int pdx_newmark_run(pdx_ctx *pc, int nsteps, const pdx_solve_opts *op, double *u, double *v, double *a,
-                    pdx_solve_stats *st) {
-    if (!pc || nsteps < 0) return set_error(PDX_ERR_ARG, "bad argument");
-    pdx_solve_opts o{};
-    if (op) o = *op;
-    else {
-        o.krylov = PDX_KRYLOV_BICGSTAB; o.precond = PDX_PREC_JACOBI; o.precision = PDX_FP64;
-        o.tol = 1e-8; o.max_iter = 20000; o.inner_tol = 1e-4; o.warm_start = 1;
-    }
-    std::string err;
-    pdx_solve_stats s{};
-    int rc = dev_newmark(*pc->c, nsteps, o, pc->dofval, u, v, a, s, err);
-    if (st) *st = s;
-    return rc ? set_error(rc, err) : PDX_OK;
 }
 
 }  // extern "C"

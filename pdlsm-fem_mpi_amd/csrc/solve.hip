@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 #include "device.hpp"
 #include "reduce.hpp"
@@ -35,7 +36,9 @@ struct ColClasses {
     const int *ksrc;  // mirrored forms: own index of every block (~partner's for a mirrored one)
 };
 
-template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL, bool SYM>
+// FO (force only, consistent mass): the row's F = F_ext - K u_n is stored to Unp1 (0 at prescribed
+// DOFs) instead of the update; the acceleration then comes from M a = F.
+template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL, bool SYM, bool FO = false>
 __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int nbs, const int *rows_l, int nl,
                                                   int nbt, const int *adj_off, const int *adj, ColClasses cc, const T *K,
                                                   const double *Un, const double *Unm1, double *Unp1,
@@ -50,7 +53,8 @@ __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int
         lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
         if (lb >= nbt) return;
     }
-    if (lb == 0 && threadIdx.x < n_nbc) nbc_next[threadIdx.x] = nbc_cur[threadIdx.x] + nbc_inc[threadIdx.x];
+    if (lb == 0)
+        for (int s = threadIdx.x; s < n_nbc; s += blockDim.x) nbc_next[s] = nbc_cur[s] + nbc_inc[s];
     T acc[D];
     int r, lane, grp;
     if (lb < nbs) {
@@ -87,7 +91,9 @@ __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int
     else row_matvec<D, T, double, 64, DIFF, NT, SOA, REL, SYM>(r, lane, adj_off, adj, K, Un, acc, rel, cc.ksrc);
     if (act) {
         const T v = pick<D, T>(acc, lane);
-        if (fr) {
+        if (FO) {
+            Unp1[o] = fr ? fext - double(v) : 0.0;
+        } else if (fr) {
             const double F = fext - double(v);
             if (startup) {
                 const double a = F / mo;
@@ -100,6 +106,38 @@ __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int
             Unp1[o] = un + inc;
         }
     }
+}
+
+// Consistent-mass update (timeIntegration_NLM, pdsolve.cpp:5134-5152, and the start-up step
+// :6008-6016): u_{n+1} = 2 u_n + dt^2 a - u_{n-1}, or u_1 = a dt^2/2 + u_0; prescribed DOFs
+// advance by their increment, as in the lumped kernel.
+__global__ void k_nlm_update(int n, int D, const int *__restrict__ eq, const int *__restrict__ presc_set,
+                             const double *__restrict__ Un, const double *__restrict__ Unm1,
+                             const double *__restrict__ acc, const double *__restrict__ ebc_inc, double dt,
+                             int startup, double *__restrict__ Unp1) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int code = (t / D) * 3 + t % D;
+    if (eq[code] >= 0) {
+        if (startup) {
+            Unp1[t] = acc[t] * (0.5 * dt * dt) + Un[t];
+        } else {
+            const double tA = dt * dt * acc[t];
+            Unp1[t] = Un[t] * 2.0 + tA - Unm1[t];
+        }
+    } else {
+        Unp1[t] = Un[t] + ebc_inc[presc_set[code]];
+    }
+}
+
+// DOF values of the last stored step: free entries of u_{n+1}, prescribed entries of the
+// previous buffer (the values that step's force used; the current buffer already holds the
+// next step's prescribed values)
+__global__ void k_merge_dofvals(int n, int D, const int *__restrict__ eq, const double *__restrict__ cur,
+                                const double *__restrict__ prev, double *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    out[t] = eq[(t / D) * 3 + t % D] >= 0 ? cur[t] : prev[t];
 }
 
 __global__ void k_axpy_d(size_t n, double a, const double *x, double *y) {
@@ -252,14 +290,14 @@ static int ensure_Ks(Ctx &c, int prec, std::string &err) {
     return PDX_OK;
 }
 
-template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL, bool SYM>
+template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL, bool SYM, bool FO = false>
 static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double *Unm1, double *Unp1,
                               const double *nbc_cur, double *nbc_next, int startup) {
     const int nbs = (c.n_short * GS + 255) / 256, nbl = (c.n_long * 64 + 255) / 256;
     const int nbt = std::max(nbs + nbl, 1);
     const int grid = XM ? (nbt + 7) / 8 * 8 : nbt;
     const ColClasses cc{c.cls, c.cls_off, c.cls_rel, c.ksrc};
-    k_explicit<D, T, GS, NT, XM, SOA, REL, SYM><<<grid, 256, 0, c.stream>>>(
+    k_explicit<D, T, GS, NT, XM, SOA, REL, SYM, FO><<<grid, 256, 0, c.stream>>>(
         c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, nbt, c.adj_off, c.adj, cc, K, Un, Unm1, Unp1, c.mlump,
         c.f_unit, c.n_nbc, c.nn, nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup);
 }
@@ -349,6 +387,8 @@ static int tune_explicit(Ctx &c, int prec, std::string &err) {
     return PDX_OK;
 }
 
+constexpr int kExplicitPeriod = 6;  // state rotation (3) x NBC buffer parity (2)
+
 // one step on the device state: phase p = index of u_n buffer, s = step index
 static void explicit_step(Ctx &c, int prec, int startup) {
     const int p = c.step_phase % 3;
@@ -371,7 +411,58 @@ static void host_advance_bc(Ctx &c) {
         if (c.nbc_vary[s]) c.nbc_val[s] = c.nbc_val[s] + c.dt * c.nbc_vel[s];
 }
 
-int explicit_prepare(Ctx &c, int prec, std::string &err) {
+// relative residual of the consistent-mass solves M a = F (the reference factorises M with
+// PARDISO every step, calMatrixOperations.cpp:378-498); M is SPD and well conditioned
+constexpr double kMassTol = 1e-13;
+
+// Consistent-mass step (pdfemDynamicSolver_CSRformat with cb_lumpedMass off, pdsolve.cpp:6003-6057):
+// F = F_ext - K u_n (force-only kernel of form 0), M a = F by PCG with node-block Jacobi
+// warm-started from the previous acceleration, then the update.
+static int explicit_step_nlm(Ctx &c, int prec, int startup, std::string &err) {
+    const int p = c.step_phase % 3;
+    double *Un = c.U[p], *Unm1 = c.U[(p + 2) % 3], *Unp1 = c.U[(p + 1) % 3];
+    const long long s = startup ? 0 : c.steps_done + 1;
+    double *nbc_cur = c.nbcv + (s % 2) * std::max(c.n_nbc, 1);
+    double *nbc_next = c.nbcv + ((s + 1) % 2) * std::max(c.n_nbc, 1);
+    if (prec == PDX_FP32) {
+        if (c.dim == 3)
+            launch_explicit_v<3, float, 8, false, false, false, false, false, true>(c, c.K32, Un, Unm1, c.nlm_F,
+                                                                                    nbc_cur, nbc_next, startup);
+        else
+            launch_explicit_v<2, float, 8, false, false, false, false, false, true>(c, c.K32, Un, Unm1, c.nlm_F,
+                                                                                    nbc_cur, nbc_next, startup);
+    } else {
+        if (c.dim == 3)
+            launch_explicit_v<3, double, 8, false, false, false, false, false, true>(c, c.K, Un, Unm1, c.nlm_F,
+                                                                                     nbc_cur, nbc_next, startup);
+        else
+            launch_explicit_v<2, double, 8, false, false, false, false, false, true>(c, c.K, Un, Unm1, c.nlm_F,
+                                                                                     nbc_cur, nbc_next, startup);
+    }
+    PDX_HIP(hipGetLastError());
+    pdx_solve_opts o{};
+    o.krylov = PDX_KRYLOV_CG;
+    o.precond = PDX_PREC_BLOCK_JACOBI;
+    o.precision = PDX_FP64;
+    o.tol = kMassTol;
+    o.max_iter = 5000;
+    o.warm_start = 1;
+    pdx_solve_stats st{};
+    int rc = dev_solve_mat(c, c.Mk, nullptr, o, c.nlm_F, c.nlm_a, st, err);
+    c.nlm_iters += st.iterations;
+    if (rc) return rc;
+    const int n = c.nn * c.dim;
+    k_nlm_update<<<(n + 255) / 256, 256, 0, c.stream>>>(n, c.dim, c.eq, c.presc_set, Un, Unm1, c.nlm_a,
+                                                         c.d_ebc_vel, c.dt, startup, Unp1);
+    PDX_HIP(hipGetLastError());
+    c.step_phase = (p + 1) % 3;
+    c.steps_done = s;
+    return PDX_OK;
+}
+
+// K (and the FP32 / row-local / mirrored copies the step reads) for the current bond states;
+// the consistent mass once
+static int explicit_operators(Ctx &c, int prec, bool lumped, std::string &err) {
     if (!c.K_valid) {
         int e = dev_assemble_K(c, err);
         if (e) return e;
@@ -382,10 +473,27 @@ int explicit_prepare(Ctx &c, int prec, std::string &err) {
     }
     if (int e = ensure_Ks(c, prec, err)) return e;
     if (int e = ensure_Kown(c, prec, err)) return e;
-    if (!c.mlump_valid) {
+    if (lumped && !c.mlump_valid) {
         int e = dev_lumped_mass(c, err);
         if (e) return e;
     }
+    if (!lumped) {
+        if (!c.Mk_valid) {
+            int e = dev_mass_on_K(c, err);
+            if (e) return e;
+        }
+        const size_t n = (size_t)c.nn * c.dim;
+        if (!c.nlm_F) {
+            PDX_HIP(hipMalloc((void **)&c.nlm_F, n * 8));
+            PDX_HIP(hipMalloc((void **)&c.nlm_a, n * 8));
+            PDX_HIP(hipMemsetAsync(c.nlm_a, 0, n * 8, c.stream));
+        }
+    }
+    return PDX_OK;
+}
+
+int explicit_prepare(Ctx &c, int prec, bool lumped, std::string &err) {
+    if (int e = explicit_operators(c, prec, lumped, err)) return e;
     if (c.steps_done >= 0) return PDX_OK;
     // start-up: u_0 from DOF values, BCs advanced once before the first force (pdsolve.cpp:5973-5992)
     const Model &m = *c.host;
@@ -408,9 +516,32 @@ int explicit_prepare(Ctx &c, int prec, std::string &err) {
     if (c.n_nbc)
         PDX_HIP(hipMemcpyAsync(c.nbcv, c.nbc_val.data(), sizeof(double) * c.n_nbc, hipMemcpyHostToDevice, c.stream));
     c.step_phase = 0;
-    explicit_step(c, prec, 1);
+    if (lumped) {
+        explicit_step(c, prec, 1);
+    } else if (int e = explicit_step_nlm(c, prec, 1, err)) {
+        return e;
+    }
     host_advance_bc(c);
     PDX_HIP(hipGetLastError());
+    return PDX_OK;
+}
+
+int dev_merge_dofvals(Ctx &c, const double *free_src, const double *presc_src, double *out) {
+    const int n = c.nn * c.dim;
+    if (n == 0) return PDX_OK;
+    k_merge_dofvals<<<(n + 255) / 256, 256, 0, c.stream>>>(n, c.dim, c.eq, free_src, presc_src, out);
+    return hipGetLastError() == hipSuccess ? PDX_OK : PDX_ERR_HIP;
+}
+
+// DOF values of the last stored step into c.dofs_tmp (device), the input of the nodal stresses
+int dev_explicit_dofvals(Ctx &c, const double **out, std::string &err) {
+    if (c.steps_done < 0) { err = "no explicit state yet"; return PDX_ERR_ARG; }
+    const int n = c.nn * c.dim;
+    if (!c.dofs_tmp) PDX_HIP(hipMalloc((void **)&c.dofs_tmp, (size_t)std::max(n, 1) * 8));
+    const int p = c.step_phase % 3;
+    k_merge_dofvals<<<(n + 255) / 256, 256, 0, c.stream>>>(n, c.dim, c.eq, c.U[p], c.U[(p + 2) % 3], c.dofs_tmp);
+    PDX_HIP(hipGetLastError());
+    *out = c.dofs_tmp;
     return PDX_OK;
 }
 
@@ -423,44 +554,80 @@ int explicit_tune_if_needed(Ctx &c, int prec, std::string &err) {
     return rc;
 }
 
-int dev_explicit_steps(Ctx &c, int nsteps, int prec, int use_graph, std::string &err) {
-    int e = explicit_prepare(c, prec, err);
+// Capture the 6-step graph of precision prec (three rotating state vectors, two NBC buffers:
+// period 6) for replays that start at phase 0 with an even step index.  Runs before any timed
+// region; the capture only records launches, so the state is not advanced.  Every exit after
+// hipStreamBeginCapture ends the capture, so the stream never stays in capture mode.
+static int capture_explicit_graph(Ctx &c, int prec, std::string &err) {
+    const int gi = prec == PDX_FP32 ? 1 : 0;
+    if (c.graph[gi]) return PDX_OK;
+    const int sp = c.step_phase;
+    const long long sd = c.steps_done;
+    c.step_phase = 0;
+    c.steps_done = 1;  // the first captured step has index 2 (even)
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int i = 0; i < kExplicitPeriod; i++) explicit_step(c, prec, 0);
+        const hipError_t launch = hipGetLastError();
+        e = hipStreamEndCapture(c.stream, &g);
+        if (e == hipSuccess) e = launch;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&c.graph[gi], g, nullptr, nullptr, 0);
+    if (g) hipGraphDestroy(g);
+    c.step_phase = sp;
+    c.steps_done = sd;
+    if (e != hipSuccess) {
+        c.graph[gi] = nullptr;
+        err = std::string("explicit step graph capture: ") + hipGetErrorString(e);
+        return PDX_ERR_HIP;
+    }
+    return PDX_OK;
+}
+
+// nsteps explicit steps (after the start-up step on the first call).  Lumped mass without a
+// per-step callback: graph replays of the captured 6-step period; otherwise one step at a time,
+// after_step (may be null) running after every stored step, the start-up step included, and K
+// re-assembled before a step when a callback has invalidated it (broken bonds).
+int dev_explicit_steps(Ctx &c, int nsteps, int prec, int use_graph, bool lumped,
+                       const std::function<int(std::string &)> &after_step, std::string &err) {
+    const bool startup = c.steps_done < 0;
+    int e = explicit_prepare(c, prec, lumped, err);
     if (e) return e;
+    if (startup && after_step && (e = after_step(err))) return e;
     if ((e = explicit_tune_if_needed(c, prec, err))) return e;
+    const int period = kExplicitPeriod;
+    const bool replay = lumped && !after_step && use_graph && nsteps >= period;
+    // the graph is captured and instantiated before the timed region
+    if (replay && (e = capture_explicit_graph(c, prec, err))) return e;
     hipEvent_t e0, e1;
     PDX_HIP(hipEventCreate(&e0));
     PDX_HIP(hipEventCreate(&e1));
-    const int period = 6;
     int done = 0;
     PDX_HIP(hipEventRecord(e0, c.stream));
-    if (use_graph && nsteps >= period) {
-        // align to the buffer / NBC rotation period, then replay a captured 6-step graph
+    if (replay) {
+        // align to the buffer / NBC rotation period, then replay the captured 6-step graph
         while ((c.step_phase % 3 != 0 || (c.steps_done + 1) % 2 != 0) && done < nsteps) {
             explicit_step(c, prec, 0);
             done++;
         }
-        int gi = prec == PDX_FP32 ? 1 : 0;
-        if (!c.graph[gi]) {
-            hipGraph_t g;
-            const int sp = c.step_phase;
-            const long long sd = c.steps_done;
-            PDX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < period; i++) explicit_step(c, prec, 0);
-            PDX_HIP(hipStreamEndCapture(c.stream, &g));
-            PDX_HIP(hipGraphInstantiate(&c.graph[gi], g, nullptr, nullptr, 0));
-            PDX_HIP(hipGraphDestroy(g));
-            c.step_phase = sp;
-            c.steps_done = sd;
-        }
+        const int gi = prec == PDX_FP32 ? 1 : 0;
         while (nsteps - done >= period) {
             PDX_HIP(hipGraphLaunch(c.graph[gi], c.stream));
             c.steps_done += period;
             done += period;
         }
     }
+    int rc = PDX_OK;
     while (done < nsteps) {
-        explicit_step(c, prec, 0);
+        if (after_step && (rc = explicit_operators(c, prec, lumped, err))) break;
+        if (lumped) explicit_step(c, prec, 0);
+        else if ((rc = explicit_step_nlm(c, prec, 0, err))) break;
         done++;
+        if (after_step) {
+            host_advance_bc(c);
+            if ((rc = after_step(err))) break;
+        }
     }
     PDX_HIP(hipEventRecord(e1, c.stream));
     PDX_HIP(hipEventSynchronize(e1));
@@ -469,8 +636,10 @@ int dev_explicit_steps(Ctx &c, int nsteps, int prec, int use_graph, std::string 
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     c.last_total_ms = ms;
-    c.last_step_ms = nsteps ? ms / nsteps : 0;
-    for (int i = 0; i < nsteps; i++) host_advance_bc(c);
+    c.last_step_ms = done ? ms / done : 0;
+    if (!after_step)
+        for (int i = 0; i < done; i++) host_advance_bc(c);
+    if (rc) return rc;
     PDX_HIP(hipGetLastError());
     return PDX_OK;
 }

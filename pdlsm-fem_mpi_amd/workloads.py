@@ -8,7 +8,8 @@ EBC / NBC / VEBC / VNBC command values (fioFile.cpp:69-462):
   nen[ne], algo[ne] (1 = PD element, 2 = FE element),
   ebc: [{nid, dof, val, vel, vary}], nbc: [{con[nele, 2|4], val, vel, vary}],
   cracks[nc,3,3], fac (horizon factor m), nlf (non-local factor a), me, dt,
-  beta, gamma.
+  beta, gamma; optional run settings lumped_mass (LUMPMASS ON, default
+  off as in the reference), newmark, fc, topk, sigma_ult.
 
 Workloads (sizes as named):
   plate2d   2D plane-stress plate 100 x 50 mm, dx = 0.5 mm (201 x 101 nodes,
@@ -19,7 +20,8 @@ Workloads (sizes as named):
             interior nodes jittered U(-0.2, 0.2) dx, PD elements whose nodes lie
             in the central 20^3-node block, m = 3.015, E = 70 GPa, nu = 0.33,
             rho = 2700; bottom face UZ = 0 plus a corner pin, 1 MPa traction on
-            the top face (quad4 faces).  Explicit central difference.
+            the top face (quad4 faces).  Explicit central difference, lumped
+            mass (LUMPMASS ON).
   crack2d   2D edge-cracked plate 501 x 251 nodes (dx = 0.2 mm), crack from
             the left edge at mid-height, length U[0.1, 0.4] x width, angle
             U[-30, 30] deg per seed; PD band of elements within 3 horizons of
@@ -99,7 +101,7 @@ def box3d(n=60, dx=1e-3, jitter=0.2, seed=0, pd_block=20, m=3.015, nlf=0.33, E=7
     nbc = [dict(con=top, val=traction)]
     return dict(dim=3, proType=-1, E=E, nu=nu, rho=rho, x=x, econ=econ, nen=np.full(len(econ), 8, np.int32),
                 algo=algo, ebc=ebc, nbc=nbc, cracks=np.zeros((0, 3, 3)), fac=m, nlf=nlf, me=0.0, dt=dt,
-                beta=0.25, gamma=0.5)
+                beta=0.25, gamma=0.5, lumped_mass=True)
 
 
 def crack2d(seed=0, nx=501, ny=251, dx=2e-4, m=3.015, nlf=0.33, uy_top=1e-5, E=200e9, nu=0.3, rho=7850.0,

@@ -618,3 +618,252 @@ def test_threaded_explicit_force_matches_serial():
         finally:
             O.lib().om_set_threads(1)
         assert np.linalg.norm(up - u1) <= 1e-13 * np.linalg.norm(u1)
+
+
+# ---------------------------------------------------------------------------
+# consistent mass of the explicit driver (LUMPMASS OFF, the reference's default)
+# ---------------------------------------------------------------------------
+
+def test_consistent_mass_textbook_quad4_and_hex8():
+    """Rectangle a x b (unit thickness) and box a x b x c: the exactly integrated bilinear /
+    trilinear consistent mass is rho V / 36 (quad4) or rho V / 216 (hex8) times 2^s, s = number
+    of natural coordinates two corner nodes share (4 2 1 / 8 4 2 1), on each displacement
+    component, with no coupling between components."""
+    a, b, c, rho = 2e-3, 1e-3, 3e-3, 2700.0
+    q = one_quad(rho=rho)
+    q["x"] = np.array([[0, 0, 0], [a, 0, 0], [a, b, 0], [0, b, 0]], float)
+    Me = O.Model(q).element_M(0)
+    nat = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]])
+    want = np.zeros((8, 8))
+    for i in range(4):
+        for j in range(4):
+            s = int((nat[i] == nat[j]).sum())
+            for d in range(2):
+                want[2 * i + d, 2 * j + d] = rho * a * b / 36 * 2 ** s
+    assert np.allclose(Me, want, rtol=1e-13, atol=0)
+    # hex8 via a one-element box
+    sgn = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1],
+                    [-1, 1, 1]])
+    x = (sgn + 1) / 2 * np.array([a, b, c])
+    h = dict(dim=3, proType=-1, E=70e9, nu=0.33, rho=rho, x=x, econ=np.arange(1, 9, dtype=np.int32)[None, :],
+             nen=np.array([8], np.int32), algo=np.array([2], np.int32),
+             ebc=[dict(nid=np.array([1], np.int32), dof=d, val=0.0) for d in range(3)], nbc=[],
+             cracks=np.zeros((0, 3, 3)), fac=3.0, nlf=0.33, me=0.0, dt=1.0, beta=0.25, gamma=0.5)
+    Me = O.Model(h).element_M(0)
+    want = np.zeros((24, 24))
+    for i in range(8):
+        for j in range(8):
+            s = int((sgn[i] == sgn[j]).sum())
+            for d in range(3):
+                want[3 * i + d, 3 * j + d] = rho * a * b * c / 216 * 2 ** s
+    assert np.allclose(Me, want, rtol=1e-13, atol=0)
+
+
+def _mass_pattern_brute(M):
+    """setCSRIndexes_gloMassMat by definition, with sets: a free row of node k holds the free
+    DOFs of k itself when k has a family, and of every node sharing an FE element with k;
+    rows of pure PD nodes (type 2) only their diagonal."""
+    nd = M.nodes()
+    eq, ntype, algo = nd["eq"], nd["ntype"], nd["algo"]
+    d = M.dim
+    fam_nodes = set(M.families()["node"].tolist())
+    econ, nen = M.mesh["econ"], M.mesh["nen"]
+    near = [set() for _ in range(M.nn)]
+    for k in fam_nodes:
+        near[k].add(k)
+    for e in range(len(econ)):
+        if algo[e] == 2:
+            ns = [int(v) - 1 for v in econ[e, : nen[e]]]
+            for p in ns:
+                near[p].update(ns)
+    rows = {}
+    for k in range(M.nn):
+        for i in range(d):
+            r = eq[k, i]
+            if r < 0:
+                continue
+            if ntype[k] == 2:
+                rows[r] = [r]
+            else:
+                rows[r] = sorted(eq[q, j] for q in near[k] for j in range(d) if eq[q, j] >= 0)
+    ia = np.zeros(M.neq + 1, np.int64)
+    for r in range(M.neq):
+        ia[r + 1] = ia[r] + len(rows[r])
+    return ia, np.concatenate([np.array(rows[r], np.int64) for r in range(M.neq)])
+
+
+@pytest.mark.parametrize("case", ["plate_pd_strip", "crack2d", "box3d"])
+def test_mass_pattern_brute_force(case):
+    """The explicit driver's consistent-mass pattern against its definition (sets over a tiny
+    mesh); its values hold rho cd_v on the diagonal of pure PD rows and sum to the FE mass plus
+    the PD share over free rows and columns of a mesh without prescribed DOFs."""
+    if case == "plate_pd_strip":
+        mesh = W.plate2d(nx=11, ny=6, dx=1e-3, pd_x=(0.003, 0.006))
+    elif case == "crack2d":
+        mesh = W.crack2d(seed=3, nx=21, ny=11, dx=1e-3)
+    else:
+        mesh = W.box3d(n=7, pd_block=4)
+    M = O.Model(mesh)
+    ia, ja, v = M.mass_csr()
+    ib, jb = _mass_pattern_brute(M)
+    assert np.array_equal(ia, ib) and np.array_equal(ja, jb)
+    nd = M.nodes()
+    fam = M.families()
+    rho = mesh["rho"]
+    for f, k in enumerate(fam["node"]):
+        if nd["ntype"][k] != 2:
+            continue
+        for i in range(M.dim):
+            r = nd["eq"][k, i]
+            if r >= 0:
+                assert ia[r + 1] - ia[r] == 1 and v[ia[r]] == rho * fam["cd_v"][f]
+    free = dict(mesh, ebc=[])
+    F = O.Model(free)
+    ia, ja, v = F.mass_csr()
+    ev, algo = F.nodes()["ev"], F.nodes()["algo"]
+    want = F.dim * rho * (ev[algo == 2].sum() + F.families()["cd_v"].sum())
+    assert np.isclose(v.sum(), want, rtol=1e-12)
+
+
+def test_explicit_consistent_mass_two_dof_closed_form():
+    """Two free DOFs (nodes 2 and 3, UX) of the unit quad with consistent mass: the central
+    difference with the start-up step u1 = u0 + dt^2/2 a0 and M a = F is, mode by mode of
+    M^-1 K = V diag(lam) V^-1, u_n = K^-1 f - V diag(cos n theta) V^-1 K^-1 f with
+    2 - 2 cos theta = dt^2 lam (from rest).  The lumped mass gives a measurably different answer."""
+    mesh = dict(one_quad(), ebc=one_quad()["ebc"][:2], lumped_mass=False)
+    M = O.Model(mesh)
+    assert M.neq == 2
+    K, F = M.assemble(1)
+    Kd = M.sparse(K).toarray()
+    ia, ja, mv = M.mass_csr()
+    import scipy.sparse as sp
+    Md = sp.csr_matrix((mv, ja, ia), shape=(2, 2)).toarray()
+    assert Md[0, 1] != 0  # coupled: nodes 2 and 3 share an edge
+    dt = mesh["dt"]
+    lam, V = np.linalg.eig(np.linalg.solve(Md, Kd))
+    th = np.arccos(1 - dt * dt * lam.real / 2)
+    us = np.linalg.solve(Kd, F)
+    c0 = np.linalg.solve(V.real, us)
+    for n in (0, 3, 40):
+        u, _, _ = O.Model(mesh).dynamic_run(n)
+        want = us - V.real @ (np.cos((n + 1) * th) * c0)
+        assert np.allclose(u, want, rtol=1e-9, atol=1e-12 * np.abs(us).max()), (n, u, want)
+    ul, _, _ = O.Model(dict(mesh, lumped_mass=True)).dynamic_run(40)
+    assert np.linalg.norm(ul - u) > 1e-3 * np.linalg.norm(u)
+
+
+def test_dynamic_run_lumped_equals_c_driver():
+    """The Python dynamic driver with lumped mass and FC 0 is the C driver's loop bit for bit."""
+    mesh = W.box3d(n=8, pd_block=4)
+    mesh["dt"] = 2e-8
+    mesh["nbc"][0].update(vel=1e9, vary=True)
+    u0, up0 = O.Model(mesh).explicit_run(7)
+    u1, up1, _ = O.Model(mesh).dynamic_run(7, lumped=True, fc=0)
+    assert np.array_equal(u0, u1) and np.array_equal(up0, up1)
+
+
+# ---------------------------------------------------------------------------
+# Newmark with ramped essential and natural BCs
+# ---------------------------------------------------------------------------
+
+def test_newmark_ramped_ebc_and_nbc_order():
+    """One free DOF u (node 3 UX) coupled to a ramped prescribed DOF p (node 2 UX, velocity w)
+    and loaded by a ramped traction t (velocity s): step n (1-based) solves
+    (m + beta dt^2 k) a' = f1 t_{n-1} - k U_N - c p_n, U_N = u + dt v + dt^2 (1/2 - beta) a, with
+    the predictor's prescribed value already advanced (p_n = n dt w, pdsolve.cpp:6350-6366) and
+    the traction not yet advanced (t_{n-1} = t0 + (n-1) dt s: setVaryNaturalBC runs after the
+    update, :6252).  Either other order gives a measurably different answer."""
+    t0, s, w = 1e5, 4e9, 0.5
+    base = one_quad(dt=2e-4)
+    M0 = O.Model(base)
+    K, F = M0.assemble(1)
+    k, f1 = K[0], F[0] / t0
+    m = M0.mass_on_K()[0]
+    Mq = O.Model(dict(one_quad(), ebc=one_quad()["ebc"][:2]))
+    Kq, _ = Mq.assemble(1)
+    eqq = Mq.nodes()["eq"]
+    c = Mq.sparse(Kq).toarray()[eqq[2, 0], eqq[1, 0]]
+    mesh = one_quad(vebc=True, dt=2e-4)
+    mesh["ebc"][2]["vel"] = w
+    mesh["nbc"][0].update(val=t0, vel=s, vary=True)
+    dt, beta, gamma = mesh["dt"], mesh["beta"], mesh["gamma"]
+
+    def run(nsteps, p_of, t_of):
+        u = v = a = 0.0
+        for n in range(1, nsteps + 1):
+            UN = u + dt * v + dt * dt * (0.5 - beta) * a
+            a1 = (f1 * t_of(n) - k * UN - c * p_of(n)) / (m + beta * dt * dt * k)
+            u, v, a = (u + dt * v + dt * dt * ((0.5 - beta) * a + beta * a1),
+                       v + dt * ((1 - gamma) * a + gamma * a1), a1)
+        return u, v, a
+
+    N = 25
+    want = run(N, lambda n: n * dt * w, lambda n: t0 + (n - 1) * dt * s)
+    u, v, a = O.newmark_run(O.Model(mesh), N)
+    assert np.allclose([u[0], v[0], a[0]], want, rtol=1e-9)
+    for alt in (run(N, lambda n: (n - 1) * dt * w, lambda n: t0 + (n - 1) * dt * s),
+                run(N, lambda n: n * dt * w, lambda n: t0 + n * dt * s)):
+        assert abs(alt[0] - u[0]) > 1e-4 * abs(u[0])
+
+
+# ---------------------------------------------------------------------------
+# quasi-static load ratio (setVaryEssentialBC with rati, failureProcess's 1.001 sigma_ult / s1)
+# ---------------------------------------------------------------------------
+
+def test_quasi_static_load_ratio_rule():
+    """A linear model loaded by one ramped EBC (velocity w, dt = 1): the largest principal bond
+    stress s1 is proportional to the EBC value.  With sigma_ult = 2.6 s1(w): step 1 has
+    (rati - 1) w > w, so the value grows by w (s1 doubles at step 2); step 2 has
+    (rati - 1) 2w < w, so the value becomes rati * 2w and step 3's s1 is 1.001 sigma_ult
+    (pdsolve.cpp:5274-5296, :6616-6627), where bonds are then selected."""
+    m = _quasi_case()
+    _, info0 = O.Model(m).quasi_static_run(2, fc=2, topk=2, sigma_ult=1e30)
+    s1 = info0[1, 3]
+    assert s1 > 0 and info0[1, 0] == 0
+    sig = 2.6 * s1
+    _, info = O.Model(m).quasi_static_run(4, fc=2, topk=2, sigma_ult=sig)
+    assert info[0, 3] == 0 and info[0, 2] == 1.0
+    assert np.isclose(info[1, 3], s1, rtol=1e-12) and info[1, 0] == 0
+    assert np.isclose(info[1, 2], 1.001 * sig / s1, rtol=1e-12)
+    assert np.isclose(info[2, 3], 2 * s1, rtol=1e-10) and info[2, 0] == 0
+    assert np.isclose(info[3, 3], 1.001 * sig, rtol=1e-9)
+    assert info[3, 0] > 0
+
+
+# ---------------------------------------------------------------------------
+# bond failure inside the dynamic drivers (FC 2, 2D)
+# ---------------------------------------------------------------------------
+
+def _dyn_fail_case(lumped):
+    m = W.crack2d(seed=1, nx=31, ny=16, dx=1e-3)
+    m["ebc"][2] = dict(nid=m["ebc"][2]["nid"], dof=1, val=0.0, vary=True, vel=0.2)
+    m.update(lumped_mass=lumped, dt=2e-8, fc=2, topk=0)
+    return m
+
+
+@pytest.mark.parametrize("lumped", [True, False])
+def test_dynamic_fc2_after_each_stored_step(lumped):
+    """failureProcess runs after every stored step on that step's DOF values (free u_{n+1},
+    prescribed values of the step's force): with a strength between the largest bond stresses
+    of steps j-1 and j the FC 2 trajectory equals the FC 0 one through step j, the bonds broken at
+    step j are those one FC 2 update breaks on the FC 0 state of step j, and the next step
+    differs."""
+    m = _dyn_fail_case(lumped)
+    j = 6  # info row j belongs to the stored step u_{j+1}
+    _, _, info_ref = O.Model(m).dynamic_run(j + 1, sigma_ult=1e30)
+    s = info_ref[:, 2]
+    assert np.all(np.diff(s[: j + 1]) > 0) and np.all(info_ref[:, 1] == 0)
+    sig_ult = 0.5 * (s[j - 1] + s[j])
+    Mj = O.Model(m)
+    uj, _, _ = Mj.dynamic_run(j, sigma_ult=1e30)
+    S = O.Model(m)
+    kept, broke, smax = S.fc2_step(S.node_stresses(Mj.values()), sig_ult, 0)
+    assert kept > 0 and broke > 0 and smax == s[j]
+    F = O.Model(m)
+    uf, _, info = F.dynamic_run(j, sigma_ult=sig_ult)
+    assert np.array_equal(uf, uj) and np.all(info[:j, 1] == 0)
+    assert info[j, 0] == kept and info[j, 1] == broke
+    assert np.array_equal(F.families()["bond"], S.families()["bond"])
+    ug, _, _ = O.Model(m).dynamic_run(j + 1, sigma_ult=sig_ult)
+    uh, _, _ = O.Model(m).dynamic_run(j + 1, sigma_ult=1e30)
+    assert not np.array_equal(ug, uh)

@@ -198,3 +198,50 @@ def test_vtk_ascii_layout(tmp_path):
     s0 = sig[0].astype(np.float64)
     full = np.array([[s0[0], s0[3], s0[5]], [s0[3], s0[1], s0[4]], [s0[5], s0[4], s0[2]]])
     assert np.allclose(t, full, rtol=1e-4, atol=1e-30) and lines[i + 4] == ""
+
+
+def test_command_settings_are_parsed(tmp_path):
+    """NEWMARK ON|OFF (2 tokens, optional gamma beta), LUMPMASS, VTKFORMAT, SETSOLVING's save
+    frequency, FC 2 TopK, TOPK and RF reach the model's settings (fioFile.cpp:69-462), the
+    reference's defaults otherwise (datModel.cpp:17-56: no Newmark, consistent mass, binary)."""
+    mesh = W.crack2d(seed=2, nx=21, ny=11, dx=1e-3)
+    write_lis(tmp_path / "m.lis", mesh)
+    (tmp_path / "a.in").write_text("MSHFILE m.lis\nSOLVER DYNAMIC\nSETSOLVING 5e-7 110 20 4 1.0\nNEWMARK ON\n"
+                                   "LUMPMASS ON\nFC 2 3\nRF 1 0\nVTKFORMAT ASCII\n")
+    (tmp_path / "b.in").write_text("MSHFILE m.lis\nSOLVER DYNAMIC\nSETSOLVING 5e-7 7 3 4 1.0\n"
+                                   "NEWMARK OFF 0.6 0.3\nLUMPMASS MAYBE\nTOPK 5\nVTKFORMAT BINARY\n")
+    (tmp_path / "c.in").write_text("MSHFILE m.lis\nSOLVER QUASI-STATIC\nSETSOLVING 5e-7 2 0 4 1.0\n")
+    a = pdx.Model(cmd_file=str(tmp_path / "a.in")).settings()
+    assert (a["solver_flag"], a["newmark"], a["lumped_mass"], a["num_steps"], a["save_freq"]) == (0, 1, 1, 110, 20)
+    assert (a["fc"], a["topk"], a["vtk_binary"], a["rf_ids"]) == (2, 3, 0, [1, 0])
+    assert a["dt"] == 5e-7 and a["horizon_factor"] == 4.0 and a["nlf"] == 1.0 and a["sigma_ult"] == 1e6
+    b = pdx.Model(cmd_file=str(tmp_path / "b.in")).settings()
+    assert (b["newmark"], b["lumped_mass"], b["topk"], b["vtk_binary"], b["save_freq"]) == (0, 0, 5, 1, 3)
+    assert (b["gamma"], b["beta"]) == (0.6, 0.3)
+    c = pdx.Model(cmd_file=str(tmp_path / "c.in")).settings()
+    assert (c["solver_flag"], c["dt"], c["save_freq"], c["newmark"], c["lumped_mass"]) == (2, 1.0, 1, 0, 0)
+    d = pdx.Model(dict(mesh, lumped_mass=True)).settings()
+    assert (d["solver_flag"], d["lumped_mass"], d["vtk_binary"]) == (-1, 1, 1)
+    o = O.read_case(str(tmp_path / "a.in"))
+    assert o["lumped_mass"] and o["newmark"] and o["fc"] == 2 and o["topk"] == 3 and not o["vtk_binary"]
+
+
+@pytest.mark.parametrize("name", ["plate2d", "crack2d", "box3d", "tri_tet"])
+def test_mass_pattern_matches_oracle(name):
+    """setCSRIndexes_gloMassMat: the library's consistent-mass pattern equals the oracle's bit for
+    bit, and lies inside the stiffness pattern."""
+    if name == "plate2d":
+        mesh = W.plate2d(nx=61, ny=31, dx=1e-3, pd_x=(0.02, 0.035))
+    elif name == "crack2d":
+        mesh = W.crack2d(seed=5, nx=61, ny=31, dx=1e-3)
+    elif name == "box3d":
+        mesh = W.box3d(n=12, pd_block=6)
+    else:
+        mesh = tri_tet_mesh()
+    mo, mp = O.Model(mesh), pdx.Model(mesh)
+    ia0, ja0, _ = mo.mass_csr()
+    ia1, ja1 = mp.mass_csr()
+    assert np.array_equal(ia0, ia1) and np.array_equal(ja0, ja1)
+    kia, kja = mp.csr()
+    for r in range(0, mp.neq, max(1, mp.neq // 200)):
+        assert set(ja1[ia1[r]: ia1[r + 1]]) <= set(kja[kia[r]: kia[r + 1]])
