@@ -283,15 +283,16 @@ def hbm_probe(mib=2048, reps=10):
     print(json.dumps({"hbm_probe": out}), flush=True)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, sfc=False):
     """HBM bytes per launch of the explicit kernel on the 60^3 box, from the
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
     profiles/k_explicit_box3d_60_traffic.md (copy-calibrated); None if that
-    kernel form was not profiled."""
+    kernel form was not profiled.  Forms visiting the rows in Morton order launch
+    the same kernel with another row list: their figures are keyed '<kernel> sfc'."""
     path = os.path.join(ROOT, "profiles", "k_explicit_box3d_60_traffic.json")
     try:
         with open(path) as f:
-            k = json.load(f)["kernels"].get(kernel)
+            k = json.load(f)["kernels"].get(kernel + (" sfc" if sfc else ""))
     except (OSError, ValueError, KeyError):
         return None
     return k["traffic_bytes"] if k else None
@@ -477,15 +478,19 @@ def family_batch_bench(pdx, W):
     flops = 45 * 3 * (nb - N) + (P ** 3 // 3 + 2 * P ** 3) * N + 2 * P * 9 * (nb - N)
     fb.run(0)
     block_ms = min(fb.run(0) for _ in range(3))
-    fb.run(-1)  # times every persistent form once, keeps the fastest
+    fb.run(-1)  # times every launch form (one family per block and the persistent shapes), keeps the fastest
     best, tune = fb.tuning()
-    pers_ms = min(fb.run(-1) for _ in range(3))
+    tuned_ms = min(fb.run(-1) for _ in range(3))
+    pers = [(float(t), k) for k, t in enumerate(tune) if k > 0 and t > 0]
+    best_pers = min(pers)[1] if pers else None
+    pers_ms = min(fb.run(best_pers) for _ in range(3)) if best_pers is not None else None
     return {"workload": "126^3 jittered node cloud (2,000,376 nodes), horizons U(2.5,3.5) dx, OR-rule families",
             "n_family": N, "n_bond": nb, "mean_family": nb / N, "max_family": fb.max_family,
-            "host_setup_s": setup_s, "one_family_per_block_ms": block_ms, "persistent_autotuned_ms": pers_ms,
-            "persistent_variant": best, "tuning_ms": [float(t) for t in tune],
-            "speedup_persistent": block_ms / pers_ms, "families_per_s": N / (pers_ms * 1e-3),
-            "algorithmic_GBps": alg_bytes / (pers_ms * 1e-3) / 1e9, "fp64_TFLOPs": flops / (pers_ms * 1e-3) / 1e12,
+            "host_setup_s": setup_s, "one_family_per_block_ms": block_ms, "autotuned_ms": tuned_ms,
+            "autotuned_variant": best, "best_persistent_ms": pers_ms, "best_persistent_variant": best_pers,
+            "tuning_ms": [float(t) for t in tune],
+            "speedup_autotuned_vs_one_per_block": block_ms / tuned_ms, "families_per_s": N / (tuned_ms * 1e-3),
+            "algorithmic_GBps": alg_bytes / (tuned_ms * 1e-3) / 1e9, "fp64_TFLOPs": flops / (tuned_ms * 1e-3) / 1e12,
             "alg_bytes": alg_bytes, "flops": flops}
 
 
@@ -587,16 +592,18 @@ def main():
     tf = lambda b: "true" if b else "false"
     roof = {}
     for prec in (0, 1):
-        lanes, nt, xm, soa, rel, sym, tune_ms = ctxs[prec].explicit_tuning(prec)
+        lanes, nt, xm, soa, rel, sym, tune_ms, sfc, form = ctxs[prec].explicit_tuning_full(prec)
         bytes_step = step_bytes(model, nblk, prec, classes if rel else None, n_own if sym else None)
         achieved = bytes_step / (res[prec]["step_ms"] * 1e-3)
         ty = "float" if prec else "double"
         kname = (f"pdx::k_explicit<{model.dim}, {ty}, {lanes}, {tf(nt)}, {tf(xm)}, {tf(soa)}, {tf(rel)}, "
                  f"{tf(sym)}>")
         roof[prec] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                      "frac": achieved / HBM_PEAK, "traffic": measured_traffic(kname) if args.n == 60 else None,
+                      "frac": achieved / HBM_PEAK,
+                      "traffic": measured_traffic(kname, sfc) if args.n == 60 else None,
                       "kernel": kname + f" (node-block SpMV + central-difference update, {'FP32' if prec else 'FP64'} "
-                                        "values)",
+                                        f"values{', rows in Morton order' if sfc else ''})",
+                      "form": form,
                       "tuner_ms_per_form": [float(t) for t in tune_ms],
                       "bytes_per_launch": bytes_step,
                       "column_classes": {"classes": classes[0], "offsets": classes[1]},

@@ -339,9 +339,9 @@ int pdx_fb_create(int device, int dim, long long n, const double *xyz /*[n*3]*/,
 void pdx_fb_destroy(pdx_fbatch *b);
 int pdx_fb_sizes(const pdx_fbatch *b, long long *n_fam, long long *n_bond, int *max_family);
 int pdx_fb_families(const pdx_fbatch *b, long long *off /*[n+1]*/, int *mem /*[n_bond]*/);
-/* variant -1: the fastest persistent launch (timed on the first call); 0: one
- * family per 64-thread block; 1..6: persistent, (waves per block, families per
- * chunk) = (4, 4), (2, 4), (1, 4), (4, 16), (2, 16), (1, 16).  ms: kernel time */
+/* variant -1: the fastest launch form (every form timed on the first call, one family per block
+ * included); 0: one family per 64-thread block; 1..6: persistent, (waves per block, families
+ * per chunk) = (4, 4), (2, 4), (1, 4), (4, 16), (2, 16), (1, 16).  ms: kernel time */
 #define PDX_FB_VARIANTS 7
 int pdx_fb_run(pdx_fbatch *b, int variant, float *ms);
 int pdx_fb_tuning(const pdx_fbatch *b, int *best_variant, float *ms /*[PDX_FB_VARIANTS]*/);
@@ -358,11 +358,13 @@ int pdx_last_timing(pdx_ctx *c, double *total_ms, double *per_step_ms);
  * XCD-contiguous block order, row-local value order of K, column pattern classes: a row's block
  * columns read as its index plus the offsets of its class, shared by the rows of one stencil;
  * mirrored blocks: a lower block that is the exact transpose of its upper partner (both summed
- * from element matrices only) is read from the partner's storage; all 0 before the first step)
+ * from element matrices only) is read from the partner's storage; forms 20-23 visit the rows
+ * in Morton order of the node coordinates; all 0 before the first step)
  * and its timings of the PDX_EXPLICIT_FORMS forms (ms per step) */
-#define PDX_EXPLICIT_FORMS 20
+#define PDX_EXPLICIT_FORMS 24
 int pdx_explicit_tuning(pdx_ctx *c, int precision, int *lanes, int *nontemporal, int *xcd_order, int *row_soa,
-                        int *col_classes, int *mirrored, float *ms /*[PDX_EXPLICIT_FORMS]*/);
+                        int *col_classes, int *mirrored, int *sfc_order, int *form,
+                        float *ms /*[PDX_EXPLICIT_FORMS]*/);
 /* use explicit-step form `form` (0 .. PDX_EXPLICIT_FORMS-1) from now on instead of the tuner's
  * choice, or -1 to tune again at the next step; forms with the same lanes per row compute the
  * same bits, the lanes per row change the order of the row sum */

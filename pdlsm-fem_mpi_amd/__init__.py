@@ -27,7 +27,7 @@ PDX_KRYLOV_BICGSTAB, PDX_KRYLOV_CG = 0, 1
 PDX_PREC_NONE, PDX_PREC_JACOBI, PDX_PREC_BLOCK_JACOBI, PDX_PREC_LEARNED, PDX_PREC_TWO_LEVEL = 0, 1, 2, 3, 4
 PDX_PREC_ILU0 = 5
 FB_VARIANTS = 7  # family-batch launch forms (PDX_FB_VARIANTS)
-EXPLICIT_FORMS = 20  # explicit-step kernel forms (PDX_EXPLICIT_FORMS)
+EXPLICIT_FORMS = 24  # explicit-step kernel forms (PDX_EXPLICIT_FORMS)
 
 _P, _I, _D, _LL = C.c_void_p, C.c_int, C.c_double, C.c_longlong
 
@@ -108,7 +108,7 @@ EXPORTS = {
     "pdx_newmark_run": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "pdx_solve_csr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
     "pdx_last_timing": (_I, [_P, _P, _P]),
-    "pdx_explicit_tuning": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "pdx_explicit_tuning": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "pdx_explicit_set_form": (_I, [_P, _I, _I]),
     "pdx_node_stresses": (_I, [_P, _P, _P]),
     "pdx_node_stresses_timed": (_I, [_P, _P, _P]),
@@ -414,12 +414,16 @@ class Context:
         """The explicit-step form the tuner chose: (lanes per short row, non-temporal K
         loads, XCD-contiguous block order, row-local K value order, column pattern
         classes, mirrored lower blocks, ms per form)."""
-        lanes, nt, xm, soa, rel, sym = (C.c_int(0) for _ in range(6))
+        return self.explicit_tuning_full(precision)[:7]
+
+    def explicit_tuning_full(self, precision):
+        """explicit_tuning plus (Morton row order, form index)."""
+        lanes, nt, xm, soa, rel, sym, sfc, form = (C.c_int(0) for _ in range(8))
         ms = np.zeros(EXPLICIT_FORMS, np.float32)
         _chk(lib().pdx_explicit_tuning(self.h, precision, C.byref(lanes), C.byref(nt), C.byref(xm), C.byref(soa),
-                                       C.byref(rel), C.byref(sym), _p(ms)))
+                                       C.byref(rel), C.byref(sym), C.byref(sfc), C.byref(form), _p(ms)))
         return (lanes.value, bool(nt.value), bool(xm.value), bool(soa.value), bool(rel.value), bool(sym.value),
-                ms)
+                ms, bool(sfc.value), form.value)
 
     def max_eig(self, iters=100):
         v = C.c_double(0)

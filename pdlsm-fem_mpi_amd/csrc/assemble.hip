@@ -299,39 +299,80 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
         svm[m] = A.nv[j];
     }
     __syncthreads();
-    // shape tensor, one entry per lane, members in reference order
-    for (int e = lane; e < P * P; e += 64) {
-        const ATermExp q = a_term_exp<D>(e);
-        double acc = 0.0;
-        for (int m = 1; m < n; m++) acc = a_term_sel<D>(e, q, sx[m], sy[m], sz[m], som[m], svm[m]) + acc;
-        sA[e] = acc;
+    // shape tensor, members in reference order.  3D: A[r][c] = half_c * U_rc with U symmetric
+    // (the term's half factor is the last, exact, power-of-two scaling, which commutes with every
+    // rounding of the sum), so one lane sums each of the 45 entries r <= c and writes both
+    // orientations: the same bits as 81 separate sums.  2D: one lane per entry.
+    if constexpr (D == 3) {
+        if (lane < 45) {
+            int r = 0, rem = lane;
+            while (rem >= P - r) {
+                rem -= P - r;
+                r++;
+            }
+            const int c = r + rem;
+            const int e = r * P + c;
+            ATermExp q = a_term_exp<D>(e);
+            q.half = 1.0;
+            double acc = 0.0;
+            for (int m = 1; m < n; m++) acc = a_term_sel<D>(e, q, sx[m], sy[m], sz[m], som[m], svm[m]) + acc;
+            const double hc = (c >= 3 && c < 6) ? 0.5 : 1.0, hr = (r >= 3 && r < 6) ? 0.5 : 1.0;
+            sA[r * P + c] = acc * hc;
+            sA[c * P + r] = acc * hr;
+        }
+    } else {
+        for (int e = lane; e < P * P; e += 64) {
+            const ATermExp q = a_term_exp<D>(e);
+            double acc = 0.0;
+            for (int m = 1; m < n; m++) acc = a_term_sel<D>(e, q, sx[m], sy[m], sz[m], som[m], svm[m]) + acc;
+            sA[e] = acc;
+        }
     }
     __syncthreads();
-    // partial-pivot LU (first maximum pivot), one lane, in LDS
-    if (lane == 0) {
+    // partial-pivot LU (first maximum pivot) in LDS, lane-parallel with the operations of the
+    // sequential loop on every entry (so the same bits): per column the pivot row is an argmax
+    // over lanes with ties to the lower row (the sequential scan replaces only on '>'), then the
+    // row swap, the multipliers l = a_ik / a_kk, and the update a_ic - l a_kc
+    {
         int sing = 0;
         for (int kk = 0; kk < P; kk++) {
-            int p = kk;
-            double mx = fabs(sA[kk * P + kk]);
-            for (int i = kk + 1; i < P; i++) {
-                double v = fabs(sA[i * P + kk]);
-                if (v > mx) { mx = v; p = i; }
+            double v = -1.0;
+            int idx = P;
+            if (lane >= kk && lane < P) {
+                v = fabs(sA[lane * P + kk]);
+                idx = lane;
             }
-            spiv[kk] = p;
-            if (sA[p * P + kk] == 0.0) { sing = 1; break; }
-            if (p != kk)
-                for (int c = 0; c < P; c++) {
-                    double t = sA[kk * P + c];
-                    sA[kk * P + c] = sA[p * P + c];
-                    sA[p * P + c] = t;
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) {
+                const double ov = __shfl_down(v, o, 16);
+                const int oi = __shfl_down(idx, o, 16);
+                if (ov > v || (ov == v && oi < idx)) {
+                    v = ov;
+                    idx = oi;
                 }
-            for (int i = kk + 1; i < P; i++) {
-                double l = sA[i * P + kk] / sA[kk * P + kk];
-                sA[i * P + kk] = l;
-                for (int c = kk + 1; c < P; c++) sA[i * P + c] = sA[i * P + c] - l * sA[kk * P + c];
             }
+            const int p = __shfl(idx, 0);
+            if (lane == 0) spiv[kk] = p;
+            if (sA[p * P + kk] == 0.0) {
+                sing = 1;
+                break;
+            }
+            if (p != kk && lane < P) {
+                const double t = sA[kk * P + lane];
+                sA[kk * P + lane] = sA[p * P + lane];
+                sA[p * P + lane] = t;
+            }
+            __syncthreads();
+            if (lane > kk && lane < P) sA[lane * P + kk] = sA[lane * P + kk] / sA[kk * P + kk];
+            __syncthreads();
+            const int mm = P - kk - 1;
+            if (lane < mm * mm) {
+                const int i = kk + 1 + lane / mm, c = kk + 1 + lane % mm;
+                sA[i * P + c] = sA[i * P + c] - sA[i * P + kk] * sA[kk * P + c];
+            }
+            __syncthreads();
         }
-        spiv[P] = sing;
+        if (lane == 0) spiv[P] = sing;
     }
     __syncthreads();
     if (spiv[P]) {

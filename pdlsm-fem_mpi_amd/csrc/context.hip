@@ -23,6 +23,7 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
                 std::string &err);
 int dev_explicit_state(Ctx &c, double *u_out, double *u_prev, std::string &err);
 int dev_max_eig(Ctx &c, int iters, double *lmax, std::string &err);
+
 int dev_solve(Ctx &c, const pdx_solve_opts &o, const double *b_dev, double *x_dev, pdx_solve_stats &st,
               std::string &err);
 
@@ -253,6 +254,33 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
         c->n_long = int(rl.size());
         UP(c->rows_short, rs);
         UP(c->rows_long, rl);
+        // Morton keys of the node coordinates on a 2^10 grid per axis over the bounding box
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int k = 0; k < m.nn; k++)
+            for (int i = 0; i < 3; i++) {
+                lo[i] = std::min(lo[i], m.x[(size_t)k * 3 + i]);
+                hi[i] = std::max(hi[i], m.x[(size_t)k * 3 + i]);
+            }
+        auto key = [&](int k) {
+            uint64_t q[3] = {0, 0, 0};
+            for (int i = 0; i < D; i++) {
+                const double w = hi[i] > lo[i] ? (m.x[(size_t)k * 3 + i] - lo[i]) / (hi[i] - lo[i]) : 0.0;
+                q[i] = (uint64_t)std::min(1023.0, std::max(0.0, std::floor(w * 1024.0)));
+            }
+            uint64_t z = 0;
+            for (int b = 9; b >= 0; b--)
+                for (int i = D - 1; i >= 0; i--) z = (z << 1) | ((q[i] >> b) & 1);
+            return z;
+        };
+        for (auto *v : {&rs, &rl}) {
+            std::vector<std::pair<uint64_t, int>> kv;
+            kv.reserve(v->size());
+            for (int k : *v) kv.push_back({key(k), k});
+            std::sort(kv.begin(), kv.end());
+            for (size_t i = 0; i < kv.size(); i++) (*v)[i] = kv[i].second;
+        }
+        UP(c->rows_short_sfc, rs);
+        UP(c->rows_long_sfc, rl);
     }
     // column pattern classes: rows whose block columns have the same offsets from the row index
     // (the stencil of a structured mesh, jittered or not) share one list of offsets
@@ -317,6 +345,11 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
     PDX_HIP(hipDeviceSynchronize());
     if ((rc = dev_nbc_unit(*c, err))) return rc;
     PDX_HIP(hipStreamSynchronize(c->stream));
+    // node -> family id of the stress recovery
+    {
+        std::vector<int> fid(m.famid.begin(), m.famid.end());
+        UP(c->stress_famid, fid);
+    }
     return PDX_OK;
 }
 
@@ -341,8 +374,6 @@ void ctx_free(Ctx *c) {
     if (c->Kown32) hipFree(c->Kown32);
     if (c->Kown64) hipFree(c->Kown64);
     if (c->stress_stage) hipFree(c->stress_stage);
-    if (c->stress_famid) hipFree(c->stress_famid);
-    if (c->stress_status) hipFree(c->stress_status);
     free_two_level(*c);
     free_ilu0(*c);
     free_krylov_ws(*c);
@@ -772,17 +803,19 @@ int pdx_node_stresses_timed(pdx_ctx *pc, const double *u_all, double *ms) {
 }
 
 int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal, int *xcd_order, int *row_soa,
-                        int *col_classes, int *mirrored, float *ms) {
+                        int *col_classes, int *mirrored, int *sfc_order, int *form, float *ms) {
     if (!pc || (precision != PDX_FP64 && precision != PDX_FP32)) return fail(PDX_ERR_ARG, "bad argument");
     const Ctx &c = *pc->c;
     const int v = c.tune_explicit[precision];
-    const ExplicitForm f = v >= 0 ? kExplicitForms[v] : ExplicitForm{0, false, false, false, false, false};
+    const ExplicitForm f = v >= 0 ? kExplicitForms[v] : ExplicitForm{0, false, false, false, false, false, false, 0};
     if (lanes) *lanes = f.lanes;
     if (nontemporal) *nontemporal = f.nt;
     if (xcd_order) *xcd_order = f.xm;
     if (row_soa) *row_soa = f.soa;
     if (col_classes) *col_classes = f.rel;
     if (mirrored) *mirrored = f.sym;
+    if (sfc_order) *sfc_order = f.sfc;
+    if (form) *form = v;
     if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + PDX_EXPLICIT_FORMS, ms);
     return PDX_OK;
 }

@@ -31,22 +31,26 @@ struct DevBuf {
 
 // explicit-step kernel forms searched by the tuner: lanes per short row, non-temporal K loads,
 // XCD-contiguous block order, row-local value order of K, column pattern classes, mirrored
-// lower blocks read from their transposed partners
+// lower blocks read from their transposed partners, rows visited in space-filling-curve order
+// (same kernel as form `base`, another row list)
 struct ExplicitForm {
     int lanes;
-    bool nt, xm, soa, rel, sym;
+    bool nt, xm, soa, rel, sym, sfc;
+    int base;
 };
 constexpr ExplicitForm kExplicitForms[PDX_EXPLICIT_FORMS] = {
-    {8, false, false, false, false, false},  {4, false, false, false, false, false},
-    {16, false, false, false, false, false}, {8, true, false, false, false, false},
-    {4, true, false, false, false, false},   {16, true, false, false, false, false},
-    {32, false, false, false, false, false}, {32, true, false, false, false, false},
-    {16, false, true, false, false, false},  {8, false, true, false, false, false},
-    {8, false, true, true, false, false},    {16, false, true, true, false, false},
-    {4, false, true, true, false, false},    {32, false, true, true, false, false},
-    {8, false, true, false, true, false},    {16, false, true, false, true, false},
-    {8, false, true, true, true, false},     {16, false, true, true, true, false},
-    {8, false, true, false, true, true},     {16, false, true, false, true, true}};
+    {8, false, false, false, false, false, false, 0},  {4, false, false, false, false, false, false, 1},
+    {16, false, false, false, false, false, false, 2}, {8, true, false, false, false, false, false, 3},
+    {4, true, false, false, false, false, false, 4},   {16, true, false, false, false, false, false, 5},
+    {32, false, false, false, false, false, false, 6}, {32, true, false, false, false, false, false, 7},
+    {16, false, true, false, false, false, false, 8},  {8, false, true, false, false, false, false, 9},
+    {8, false, true, true, false, false, false, 10},   {16, false, true, true, false, false, false, 11},
+    {4, false, true, true, false, false, false, 12},   {32, false, true, true, false, false, false, 13},
+    {8, false, true, false, true, false, false, 14},   {16, false, true, false, true, false, false, 15},
+    {8, false, true, true, true, false, false, 16},    {16, false, true, true, true, false, false, 17},
+    {8, false, true, false, true, true, false, 18},    {16, false, true, false, true, true, false, 19},
+    {16, false, true, false, true, true, true, 19},    {8, false, true, false, true, true, true, 18},
+    {16, false, true, true, true, false, true, 17},    {16, false, true, false, true, false, true, 15}};
 
 struct Ctx {
     int device = 0;
@@ -81,6 +85,10 @@ struct Ctx {
     // rows of the node graph binned by length for the SpMV (short: one group of 8 lanes, long: a wave)
     int n_short = 0, n_long = 0;
     int *rows_short = nullptr, *rows_long = nullptr;
+    // the same rows in Morton order of the node coordinates (space-filling curve): the lower
+    // neighbours of a row, whose blocks the mirrored forms read transposed, come a few rows
+    // earlier instead of a whole plane earlier
+    int *rows_short_sfc = nullptr, *rows_long_sfc = nullptr;
 
     // natural BCs
     int n_nbc = 0, n_nbc_ele = 0;

@@ -658,7 +658,8 @@ int pdx_fb_run(pdx_fbatch *b, int variant, float *ms) {
         int v = variant;
         if (v < 0) {
             if (b->tuned < 0) {
-                // time every launch form on the batch (the results are identical), keep the fastest
+                // time every launch form on the batch (the results are identical: one family per
+                // block and every persistent shape), best of two runs each, keep the fastest
                 hipEvent_t e0, e1;
                 PDX_HIP(hipEventCreate(&e0));
                 PDX_HIP(hipEventCreate(&e1));
@@ -669,19 +670,23 @@ int pdx_fb_run(pdx_fbatch *b, int variant, float *ms) {
                         (void)hipGetLastError();
                         continue;
                     }
-                    PDX_HIP(hipEventRecord(e0, b->stream));
-                    PDX_HIP(fb_run(*b, k));
-                    PDX_HIP(hipEventRecord(e1, b->stream));
-                    PDX_HIP(hipEventSynchronize(e1));
-                    PDX_HIP(hipEventElapsedTime(&b->tune_ms[k], e0, e1));
-                    if (k > 0 && b->tune_ms[k] < best) {
+                    for (int rep = 0; rep < 2; rep++) {
+                        float t = 0;
+                        PDX_HIP(hipEventRecord(e0, b->stream));
+                        PDX_HIP(fb_run(*b, k));
+                        PDX_HIP(hipEventRecord(e1, b->stream));
+                        PDX_HIP(hipEventSynchronize(e1));
+                        PDX_HIP(hipEventElapsedTime(&t, e0, e1));
+                        if (rep == 0 || t < b->tune_ms[k]) b->tune_ms[k] = t;
+                    }
+                    if (b->tune_ms[k] < best) {
                         best = b->tune_ms[k];
                         b->tuned = k;
                     }
                 }
                 hipEventDestroy(e0);
                 hipEventDestroy(e1);
-                if (b->tuned < 0) { err = "pdx_fb_run: no persistent form fits the families"; return PDX_ERR_ARG; }
+                if (b->tuned < 0) { err = "pdx_fb_run: no launch form fits the families"; return PDX_ERR_ARG; }
             }
             v = b->tuned;
         }

@@ -292,13 +292,14 @@ static int ensure_Ks(Ctx &c, int prec, std::string &err) {
 
 template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL, bool SYM, bool FO = false>
 static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double *Unm1, double *Unp1,
-                              const double *nbc_cur, double *nbc_next, int startup) {
+                              const double *nbc_cur, double *nbc_next, int startup, bool sfc = false) {
     const int nbs = (c.n_short * GS + 255) / 256, nbl = (c.n_long * 64 + 255) / 256;
     const int nbt = std::max(nbs + nbl, 1);
     const int grid = XM ? (nbt + 7) / 8 * 8 : nbt;
     const ColClasses cc{c.cls, c.cls_off, c.cls_rel, c.ksrc};
+    const int *rs = sfc ? c.rows_short_sfc : c.rows_short, *rl = sfc ? c.rows_long_sfc : c.rows_long;
     k_explicit<D, T, GS, NT, XM, SOA, REL, SYM, FO><<<grid, 256, 0, c.stream>>>(
-        c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, nbt, c.adj_off, c.adj, cc, K, Un, Unm1, Unp1, c.mlump,
+        rs, c.n_short, nbs, rl, c.n_long, nbt, c.adj_off, c.adj, cc, K, Un, Unm1, Unp1, c.mlump,
         c.f_unit, c.n_nbc, c.nn, nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup);
 }
 
@@ -306,13 +307,15 @@ static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double
 template <int D, typename T>
 static void launch_explicit(Ctx &c, int variant, const T *K, const double *Un, const double *Unm1, double *Unp1,
                             const double *nbc_cur, double *nbc_next, int startup) {
+    const bool sfc = kExplicitForms[variant].sfc;
+    variant = kExplicitForms[variant].base;
 #define PDX_EXPLICIT_FORM(V, GS, NT, XM, SOA, REL, SYM)                                                         \
     case V:                                                                                                     \
         static_assert(kExplicitForms[V].lanes == GS && kExplicitForms[V].nt == NT && kExplicitForms[V].xm == XM && \
                           kExplicitForms[V].soa == SOA && kExplicitForms[V].rel == REL &&                       \
                           kExplicitForms[V].sym == SYM,                                                         \
                       "form table");                                                                            \
-        launch_explicit_v<D, T, GS, NT, XM, SOA, REL, SYM>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);       \
+        launch_explicit_v<D, T, GS, NT, XM, SOA, REL, SYM>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup, sfc);  \
         break;
     switch (variant) {
         PDX_EXPLICIT_FORM(1, 4, false, false, false, false, false)

@@ -486,27 +486,24 @@ int stress_status_check(Ctx &c, std::string &err) {
     return PDX_OK;
 }
 
-int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err, bool check) {
-    if (!c.K_valid) {
-        int rc = dev_assemble_K(c, err);  // the family kernel leaves the bond strain weights
-        if (rc) return rc;
-    }
+// doubles of scratch one recovery uses: FE corner stresses (8 corners x 6 per element), family
+// stresses (6 per family) and one slot for the singular-Jacobian flag
+static size_t stress_scratch_doubles(const Ctx &c) {
+    return (size_t)std::max(c.nfe, 1) * 48 + (size_t)std::max(c.nfam, 1) * 6 + 1;
+}
+
+__global__ void k_clear_flag(int *flag) { *flag = 0; }
+
+// The launches of one recovery (no allocation, no synchronisation) with the scratch at base
+// (stress_scratch_doubles(c) doubles); the flag is cleared by a kernel, not a memset, so the
+// launches may be captured in a graph.
+static int stress_launch(Ctx &c, const double *u_dev, double *sigma_dev, double *base, std::string &err) {
     const Model &m = *c.host;
     const int D = c.dim;
-    if (!c.stress_stage)
-        PDX_HIP(hipMalloc((void **)&c.stress_stage, ((size_t)std::max(c.nfe, 1) * 48 + (size_t)std::max(c.nfam, 1) * 6) *
-                                                        sizeof(double)));
-    double *stage = c.stress_stage, *sig_fam = c.stress_stage + (size_t)std::max(c.nfe, 1) * 48;
-    if (!c.stress_famid) {
-        // node -> family id, and the singular-Jacobian flag, kept on the device for per-step use
-        PDX_HIP(hipMalloc((void **)&c.stress_famid, (size_t)std::max(c.nn, 1) * sizeof(int)));
-        PDX_HIP(hipMalloc((void **)&c.stress_status, sizeof(int)));
-        std::vector<int> h(c.nn);
-        for (int k = 0; k < c.nn; k++) h[k] = m.famid[k];
-        PDX_HIP(hipMemcpy(c.stress_famid, h.data(), (size_t)c.nn * sizeof(int), hipMemcpyHostToDevice));
-    }
-    int *status = c.stress_status, *famid = c.stress_famid;
-    PDX_HIP(hipMemsetAsync(status, 0, sizeof(int), c.stream));
+    double *stage = base, *sig_fam = base + (size_t)std::max(c.nfe, 1) * 48;
+    int *status = (int *)(sig_fam + (size_t)std::max(c.nfam, 1) * 6);
+    const int *famid = c.stress_famid;
+    k_clear_flag<<<1, 1, 0, c.stream>>>(status);
     if (c.nfam > 0) {
         if (D == 3) k_stress_pd<3><<<(c.nfam + 3) / 4, 256, 0, c.stream>>>(c.nfam, c.fam_off, c.fam_mem, c.fam_c,
                                                                            u_dev, c.Dm, sig_fam);
@@ -565,6 +562,17 @@ int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::strin
                                                          m.pro_type == 2, sigma_dev);
         PDX_HIP(hipGetLastError());
     }
+    return PDX_OK;
+}
+
+int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err, bool check) {
+    if (!c.K_valid) {
+        int rc = dev_assemble_K(c, err);  // the family kernel leaves the bond strain weights
+        if (rc) return rc;
+    }
+    if (!c.stress_stage) PDX_HIP(hipMalloc((void **)&c.stress_stage, stress_scratch_doubles(c) * sizeof(double)));
+    c.stress_status = (int *)(c.stress_stage + stress_scratch_doubles(c) - 1);
+    if (int rc = stress_launch(c, u_dev, sigma_dev, c.stress_stage, err)) return rc;
     return check ? stress_status_check(c, err) : PDX_OK;
 }
 

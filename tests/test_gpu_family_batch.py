@@ -79,12 +79,14 @@ def test_family_batch_variants_agree():
             assert np.array_equal(a, b), v
     ms = fb.run(-1)
     best, times = fb.tuning()
-    assert 1 <= best < pdx.FB_VARIANTS and ms > 0 and times[best] > 0
+    # the tuner picks the fastest of every form it could run, one family per block included
+    assert 0 <= best < pdx.FB_VARIANTS and ms > 0 and times[best] > 0
+    assert times[best] == min(t for t in times if t > 0)
 
 
 def test_family_batch_full_cloud_sampled():
     """The 2,000,376-node cloud (126^3, jitter 0.3 dx, horizons U(2.5, 3.5) dx):
-    sampled families against the oracle after the autotuned persistent run."""
+    sampled families against the oracle after the autotuned run."""
     c = W.family_batch()
     fb = pdx.FamilyBatch(c["x"], c["vol"], c["delta"], nlf=0.33, lam=LAM, mu=MU)
     assert fb.n_fam == 126 ** 3
