@@ -282,8 +282,10 @@ int pdx_local_damage(pdx_ctx *c, double *phi);
 /* Nodal stresses, pdsolve::calGlobalNodeStresses (pdsolve.cpp:9540-9604) with
  * calPDNodeStresses (:9081-9166) and calFEMNodeStresses_EXP (:9168-9335), as the
  * drivers call it before every result output (e.g. :6062).  u_all: DOF values
- * [n_node*dim] (the reference's pdDof values), or NULL for the current explicit
- * state u_{n+1}.  sigma [n_node*6]: xx, yy, zz, xy, yz, zx (2D: xx, yy, zz, xy). */
+ * [n_node*dim] (the reference's pdDof values), or NULL for the DOF values of the last stored
+ * explicit step (free DOFs u_{n+1}, prescribed DOFs at the values that step's force used, as
+ * storeDisplacementResult leaves them).  sigma [n_node*6]: xx, yy, zz, xy, yz, zx (2D: xx, yy,
+ * zz, xy). */
 int pdx_node_stresses(pdx_ctx *c, const double *u_all, double *sigma);
 /* the same on the device only, timed (ms of one recovery, inputs resident) */
 int pdx_node_stresses_timed(pdx_ctx *c, const double *u_all, double *ms);

@@ -813,16 +813,28 @@ __global__ void k_fe_gather(long long nblk, const int *__restrict__ fec_off, con
     double acc[D];
 #pragma unroll
     for (int jj = 0; jj < D; jj++) acc[jj] = pd ? kb[jj] : 0.0;
-    for (int c = c0; c < c1; c++) {
-        const int f = fec[c];
-        const double *blk = S + (size_t)(f >> 1) * DD;
-        if (f & 1) {
+    // contributions in chunks of 8: the indices, then the block rows, are loaded before any is
+    // added, so up to 8 gathers are in flight per thread; the sum stays in element order
+    for (int cb = c0; cb < c1; cb += 8) {
+        const int cnt = min(8, c1 - cb);
+        int f[8];
 #pragma unroll
-            for (int jj = 0; jj < D; jj++) acc[jj] = acc[jj] + blk[jj * D + ii];
-        } else {
+        for (int k = 0; k < 8; k++) f[k] = k < cnt ? fec[cb + k] : 0;
+        double v[8][D];
 #pragma unroll
-            for (int jj = 0; jj < D; jj++) acc[jj] = acc[jj] + blk[ii * D + jj];
+        for (int k = 0; k < 8; k++) {
+            if (k < cnt) {
+                const double *blk = S + (size_t)(f[k] >> 1) * DD;
+                const bool tr = f[k] & 1;
+#pragma unroll
+                for (int jj = 0; jj < D; jj++) v[k][jj] = blk[tr ? jj * D + ii : ii * D + jj];
+            }
         }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (k < cnt)
+#pragma unroll
+                for (int jj = 0; jj < D; jj++) acc[jj] = acc[jj] + v[k][jj];
     }
 #pragma unroll
     for (int jj = 0; jj < D; jj++) kb[jj] = acc[jj];

@@ -465,7 +465,10 @@ int pdx_model_csr(const pdx_model *pm, long long *ia, long long *ja) {
     if (!pm) return fail(PDX_ERR_ARG, "null argument");
     const Model &m = pm->m;
     if (ia) std::copy(m.ia.begin(), m.ia.end(), ia);
-    if (ja) std::copy(m.ja.begin(), m.ja.end(), ja);
+    if (ja) {
+        const std::vector<int64_t> &mj = m.csr_ja();
+        std::copy(mj.begin(), mj.end(), ja);
+    }
     return PDX_OK;
 }
 
@@ -723,12 +726,13 @@ int pdx_mass_csr(pdx_ctx *pc, double *Mv) {
     // the mass pattern lies inside K's pattern (family self / FE neighbours of every node)
     std::vector<int64_t> hia, hja;
     mass_pattern(m, hia, hja);
+    const std::vector<int64_t> &kja = m.csr_ja();
     for (int r = 0; r < m.neq; r++)
         for (int64_t q = hia[r]; q < hia[r + 1]; q++) {
-            auto b = m.ja.begin() + m.ia[r], e = m.ja.begin() + m.ia[r + 1];
+            auto b = kja.begin() + m.ia[r], e = kja.begin() + m.ia[r + 1];
             auto it = std::lower_bound(b, e, hja[q]);
             if (it == e || *it != hja[q]) return fail(PDX_ERR_PATTERN, "mass entry outside the stiffness pattern");
-            Mv[q] = mk[it - m.ja.begin()];
+            Mv[q] = mk[it - kja.begin()];
         }
     return PDX_OK;
 }
@@ -754,8 +758,11 @@ int pdx_node_stresses(pdx_ctx *pc, const double *u_all, double *sigma) {
             PDX_HIP(hipMemcpyAsync(du, u_all, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
             src = du;
         } else {
+            // the DOF values of the last stored step: free u_{n+1} with the prescribed values
+            // that step's force used (the state buffer already holds the next step's)
             if (c.steps_done < 0) { err = "no explicit state yet: pass the DOF values"; return PDX_ERR_ARG; }
-            src = c.U[c.step_phase % 3];
+            int rc = dev_explicit_dofvals(c, &src, err);
+            if (rc) return rc;
         }
         int rc = dev_node_stresses(c, src, ds, err);
         if (rc) return rc;

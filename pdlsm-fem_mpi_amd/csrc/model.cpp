@@ -329,22 +329,29 @@ void boundary_entities(Model &m) {
 int break_bonds_across(const Model &m, const double *crack, std::vector<int> &bond) {
     int lo[3], hi[3], nb = 0;
     crack_box(m, crack, lo, hi);
+    // the families of the nodes in the cells around the crack; each bond is tested on its own,
+    // so the families are independent
+    std::vector<int> fams;
     for_cells(m, lo, hi, [&](int b) {
         for (int t = m.cell_node_off[b]; t < m.cell_node_off[b + 1]; t++) {
             const int f = m.famid[m.cell_node[t]];
-            if (f < 0) continue;
-            const double *xk = &m.x[size_t(m.fam_node[f]) * 3];
-            for (int q = m.fam_off[f] + 1; q < m.fam_off[f + 1]; q++) {
-                if (bond[q] != 1) continue;
-                const double *xm = &m.x[size_t(m.fam_mem[q]) * 3];
-                const bool cut = m.dim == 3 ? seg_tri_plane(xk, xm, crack) : seg_seg(xk, xm, crack, crack + 3);
-                if (cut) {
-                    bond[q] = 0;
-                    nb++;
-                }
-            }
+            if (f >= 0) fams.push_back(f);
         }
     });
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nb)
+    for (size_t i = 0; i < fams.size(); i++) {
+        const int f = fams[i];
+        const double *xk = &m.x[size_t(m.fam_node[f]) * 3];
+        for (int q = m.fam_off[f] + 1; q < m.fam_off[f + 1]; q++) {
+            if (bond[q] != 1) continue;
+            const double *xm = &m.x[size_t(m.fam_mem[q]) * 3];
+            const bool cut = m.dim == 3 ? seg_tri_plane(xk, xm, crack) : seg_seg(xk, xm, crack, crack + 3);
+            if (cut) {
+                bond[q] = 0;
+                nb++;
+            }
+        }
+    }
     return nb;
 }
 
@@ -534,29 +541,44 @@ int Model::setup(std::string &err) {
     for (size_t c = 0; c < ncr; c++) break_bonds_across(*this, &crack[c * 9], bond);
 
     // ---- node graph and CSR (pdsolve.cpp:959-1121) ----
-    std::vector<std::vector<int>> nbr(nn);
-    for (int f = 0; f < nfam; f++)
-        nbr[fam_node[f]].assign(fam_mem.begin() + fam_off[f], fam_mem.begin() + fam_off[f + 1]);
-    for (int e : fe)
-        for (int a = 0; a < nen[e]; a++) {
-            auto &v = nbr[econ[size_t(e) * 8 + a]];
-            for (int b = 0; b < nen[e]; b++) v.push_back(econ[size_t(e) * 8 + b]);
+    // per node: the family members and the nodes of its FE elements, sorted and unique; built
+    // node by node in parallel from the node -> FE element incidence
+    {
+        std::vector<int> ne_off(nn + 1, 0), ne_item;
+        for (int e : fe)
+            for (int a = 0; a < nen[e]; a++) ne_off[econ[size_t(e) * 8 + a] + 1]++;
+        for (int k = 0; k < nn; k++) ne_off[k + 1] += ne_off[k];
+        ne_item.resize(ne_off[nn]);
+        {
+            std::vector<int> fill(ne_off.begin(), ne_off.end() - 1);
+            for (int e : fe)
+                for (int a = 0; a < nen[e]; a++) ne_item[fill[econ[size_t(e) * 8 + a]]++] = e;
         }
+        std::vector<std::vector<int>> nbr(nn);
 #pragma omp parallel for schedule(dynamic, 256)
-    for (int k = 0; k < nn; k++) {
-        std::sort(nbr[k].begin(), nbr[k].end());
-        nbr[k].erase(std::unique(nbr[k].begin(), nbr[k].end()), nbr[k].end());
+        for (int k = 0; k < nn; k++) {
+            std::vector<int> &v = nbr[k];
+            const int f = famid[k];
+            if (f >= 0) v.assign(fam_mem.begin() + fam_off[f], fam_mem.begin() + fam_off[f + 1]);
+            for (int t = ne_off[k]; t < ne_off[k + 1]; t++) {
+                const int e = ne_item[t];
+                for (int b = 0; b < nen[e]; b++) v.push_back(econ[size_t(e) * 8 + b]);
+            }
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+        }
+        adj_off.assign(nn + 1, 0);
+        for (int k = 0; k < nn; k++) adj_off[k + 1] = adj_off[k] + int(nbr[k].size());
+        adj.resize(adj_off[nn]);
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < nn; k++) std::copy(nbr[k].begin(), nbr[k].end(), adj.begin() + adj_off[k]);
     }
-    adj_off.assign(nn + 1, 0);
-    for (int k = 0; k < nn; k++) adj_off[k + 1] = adj_off[k] + int(nbr[k].size());
-    adj.resize(adj_off[nn]);
-    for (int k = 0; k < nn; k++) std::copy(nbr[k].begin(), nbr[k].end(), adj.begin() + adj_off[k]);
-    nbr.clear();
     // free-DOF count per node -> row length of every row of that node
     std::vector<int> nfree(nn, 0);
     for (int k = 0; k < nn; k++)
         for (int i = 0; i < dim; i++) nfree[k] += eq[size_t(k) * 3 + i] >= 0;
     ia.assign(size_t(neq) + 1, 0);
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < nn; k++) {
         int64_t len = 0;
         for (int t = adj_off[k]; t < adj_off[k + 1]; t++) len += nfree[adj[t]];
@@ -567,7 +589,16 @@ int Model::setup(std::string &err) {
     }
     for (int r = 0; r < neq; r++) ia[r + 1] += ia[r];
     nnz = ia[neq];
-    ja.resize(nnz);
+    // the column array (8 bytes per entry: 2 GB for the 100^3 box) is only built when asked for
+    ja_built = false;
+    ja_store.clear();
+    return PDX_OK;
+}
+
+const std::vector<int64_t> &Model::csr_ja() const {
+    std::lock_guard<std::mutex> lk(ja_mu);
+    if (ja_built) return ja_store;
+    ja_store.resize(nnz);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int k = 0; k < nn; k++)
         for (int i = 0; i < dim; i++) {
@@ -577,10 +608,11 @@ int Model::setup(std::string &err) {
             for (int t = adj_off[k]; t < adj_off[k + 1]; t++)
                 for (int j = 0; j < dim; j++) {
                     int c = eq[size_t(adj[t]) * 3 + j];
-                    if (c >= 0) ja[p++] = c;
+                    if (c >= 0) ja_store[p++] = c;
                 }
         }
-    return PDX_OK;
+    ja_built = true;
+    return ja_store;
 }
 
 // setCSRIndexes_gloMassMat (pdsolve.cpp:5674-5817): per node the family's own node and the nodes

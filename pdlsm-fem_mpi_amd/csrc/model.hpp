@@ -6,6 +6,7 @@
 // built from it.
 #pragma once
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -83,8 +84,13 @@ struct Model {
 
     // ---- node graph (pattern at node level) and CSR ----
     std::vector<int> adj_off, adj;   // sorted unique neighbour nodes per node
-    std::vector<int64_t> ia, ja;     // K_ff CSR, zero based
+    std::vector<int64_t> ia;         // K_ff CSR row offsets, zero based
     int64_t nnz = 0;
+    // K_ff CSR column indices (sorted per row), built on first use and kept
+    const std::vector<int64_t> &csr_ja() const;
+    mutable std::vector<int64_t> ja_store;
+    mutable bool ja_built = false;
+    mutable std::mutex ja_mu;
 
     // ---- material ----
     double D[36] = {0};

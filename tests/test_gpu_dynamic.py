@@ -66,8 +66,16 @@ def test_explicit_consistent_mass_trajectory(name, prec, tol):
 
 def _fc2_case(lumped):
     """Edge-cracked 41 x 21 plate pulled apart at the top and bottom (the shape of the reference's
-    plateCrack dynamic examples: VEBC -1 / +1 on the two loaded edges, FC 2 0)."""
+    plateCrack dynamic examples: VEBC -1 / +1 on the two loaded edges, FC 2 0).  Interior nodes
+    are jittered by up to 0.15 dx: on an exact grid an FC 2 crack segment (through a bond's
+    midpoint, normal to the principal direction) runs almost along a row of nodes and bonds, where
+    the reference's segment test decides on the last bits of the stresses, which the GPU and the
+    oracle share only to 1e-12."""
     m = W.crack2d(seed=1, nx=41, ny=21, dx=1e-3)
+    x = m["x"].copy()
+    inner = (x[:, 0] > 0) & (x[:, 0] < 40e-3) & (x[:, 1] > 0) & (x[:, 1] < 20e-3)
+    x[inner, :2] += np.random.default_rng(11).uniform(-0.15e-3, 0.15e-3, (int(inner.sum()), 2))
+    m["x"] = x
     m["ebc"][0].update(val=0.0, vel=-0.5, vary=True)
     m["ebc"][2].update(val=0.0, vel=0.5, vary=True)
     m.update(lumped_mass=lumped, fc=2, topk=0, dt=5e-8)

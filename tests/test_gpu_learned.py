@@ -6,10 +6,12 @@
                (relative, per parameter block)
   zero head    untrained network (zero output weights) == two-level block Jacobi
   training     Adam on the GPU lowers the loss
-  held-out     trained on cracked plates of other seeds, the learned smoother needs
-               no more BiCGStab iterations than two-level block Jacobi on a held-out
-               seed and the solve reaches the 1e-8 true residual (measured: equal
-               counts -- the node-local smoother has no headroom in this scheme)
+  held-out     trained on cracked plates of other seeds, the learned smoother reaches the
+               1e-8 true residual on a held-out seed in a fifth of plain Jacobi's
+               iterations; in FP32 with FP64 refinement it needs fewer iterations than
+               the same solve without the network (two-level block Jacobi) over
+               held-out 501 x 251 plates (in FP64 the two are about even: the node-local
+               smoother has little headroom once the coarse space is there)
 """
 import numpy as np
 import pytest
@@ -103,6 +105,29 @@ def test_learned_gain_on_held_out_seed():
     # a bound on the loss is asserted there
     assert s_ln["iterations"] * 5 <= s_pl["iterations"], (s_ln, s_pl)
     assert s_ln["iterations"] <= 1.5 * s_tl["iterations"], (s_ln, s_tl)
+
+
+def test_learned_gain_fp32_refined_held_out():
+    """What the network gains over the method without it: trained on the GPU on eight 121 x 61
+    plates (seeds 0-7, 4 epochs of 10 Adam steps, as the bench does), the learned smoother in the
+    FP32-refined two-level solve (inner BiCGStab to 1e-5, FP64 refinement to a 1e-8 true
+    residual) needs fewer iterations in total than the same solve with the block-Jacobi smoother
+    on held-out 501 x 251 plates (seeds 448-451)."""
+    train = [_ctx(_crack(s)) for s in range(8)]
+    net = pdx.GNN(2, seed=3)
+    for epoch in range(4):
+        for k, cx in enumerate(train):
+            net.train(cx, steps=10, n_probes=8, probe_seed=100 * epoch + k, lr=3e-3, coarse_agg=8)
+    its = {"two_level": 0, "learned": 0}
+    for seed in (448, 449, 450, 451):
+        cx = _ctx(W.crack2d(seed=seed))
+        net.infer(cx, bf16=True)
+        for name, pc in (("two_level", pdx.PDX_PREC_TWO_LEVEL), ("learned", pdx.PDX_PREC_LEARNED)):
+            _, st = cx.static_solve(pdx.solve_opts(precision=1, precond=pc, tol=1e-8, inner_tol=1e-5))
+            assert st["converged"] and st["true_relres"] <= 1e-8
+            its[name] += st["iterations"]
+    print("held-out FP32-refined iterations", its)
+    assert its["learned"] < its["two_level"], its
 
 
 @pytest.mark.parametrize("dim,bf16", [(2, False), (2, True), (3, False), (3, True)])
