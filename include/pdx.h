@@ -184,6 +184,15 @@ int pdx_model_mass_csr(const pdx_model *m, long long *ia, long long *ja, long lo
  * nodal damage; a NULL array writes zeros.  title NULL: the mesh file's title. */
 int pdx_write_vtk(const pdx_model *m, const char *path, const char *title, const double *u, const double *sigma,
                   const double *phi, int binary);
+/* The ASCII files of one part of the mesh that the static driver writes beside the whole-mesh
+ * file (fioFiles::writeResults with its default flag 3, fioFile.cpp:464-482): part 0
+ * writeResultsPD_vtk (PD nodes and PD elements, with phi), part 1 writeResultsFEM_vtk (nodes that
+ * are not pure PD, FE elements), fioFile.cpp:771-1001.  Arrays as for pdx_write_vtk. */
+int pdx_write_vtk_part(const pdx_model *m, const char *path, const char *title, const double *u, const double *sigma,
+                       const double *phi, int part);
+/* The node tables an ASCII whole-mesh result comes with (fioFile.cpp:486-538): kind 0 writeUofNode
+ * (".disp": id, x, y, z, Ux, Uy, Uz), kind 1 writeSigofNode (".sig": id, x, y, z, six stresses). */
+int pdx_write_node_table(const pdx_model *m, const char *path, const double *u, const double *sigma, int kind);
 /* out[0..15]: n_node, n_ele, dim, neq, nnz, nfam, n_members, npdbe, npde, nfe,
  *             n_adj (node-graph entries), n_presc, nb0, nb1, nb2, max_family */
 int pdx_model_sizes(const pdx_model *m, long long *out);

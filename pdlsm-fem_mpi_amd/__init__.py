@@ -77,6 +77,8 @@ EXPORTS = {
     "pdx_model_create": (_I, [_P, _P]),
     "pdx_model_read": (_I, [C.c_char_p, _P, _P, _P]),
     "pdx_write_vtk": (_I, [_P, C.c_char_p, C.c_char_p, _P, _P, _P, _I]),
+    "pdx_write_vtk_part": (_I, [_P, C.c_char_p, C.c_char_p, _P, _P, _P, _I]),
+    "pdx_write_node_table": (_I, [_P, C.c_char_p, _P, _P, _I]),
     "pdx_quasi_static_run": (_I, [_P, _I, _P, _I, _I, _D, _P, _P, _P]),
     "pdx_ctx_bonds": (_I, [_P, _P]),
     "pdx_local_damage": (_I, [_P, _P]),
@@ -251,6 +253,31 @@ class Model:
         _chk(lib().pdx_write_vtk(self.h, str(path).encode(), None if title is None else title.encode(),
                                  None if u3 is None else _p(u3), None if sg is None else _p(sg),
                                  None if ph is None else _p(ph), int(bool(binary))))
+
+    def write_vtk_part(self, path, part, u=None, sigma=None, phi=None, title=None):
+        """The reference's PD-only (part 0) or FE-only (part 1) ASCII result file
+        (writeResultsPD_vtk / writeResultsFEM_vtk, fioFile.cpp:771-1001)."""
+        u3, sg, ph = self._result_arrays(u, sigma, phi)
+        _chk(lib().pdx_write_vtk_part(self.h, str(path).encode(), None if title is None else title.encode(),
+                                      None if u3 is None else _p(u3), None if sg is None else _p(sg),
+                                      None if ph is None else _p(ph), int(part)))
+
+    def write_node_table(self, path, kind, u=None, sigma=None):
+        """The .disp (kind 0) or .sig (kind 1) node table of an ASCII result (fioFile.cpp:486-538)."""
+        u3, sg, _ = self._result_arrays(u, sigma, None)
+        _chk(lib().pdx_write_node_table(self.h, str(path).encode(), None if u3 is None else _p(u3),
+                                        None if sg is None else _p(sg), int(kind)))
+
+    def _result_arrays(self, u, sigma, phi):
+        nn = self.nn
+        u3 = None
+        if u is not None:
+            u = np.asarray(u, np.float64).reshape(nn, -1)
+            u3 = np.zeros((nn, 3))
+            u3[:, : u.shape[1]] = u
+        sg = None if sigma is None else np.ascontiguousarray(np.asarray(sigma, np.float64).reshape(nn, 6))
+        ph = None if phi is None else np.ascontiguousarray(np.asarray(phi, np.float64).reshape(nn))
+        return u3, sg, ph
 
     def adjacency(self):
         off = np.zeros(self.nn + 1, np.int32)

@@ -440,6 +440,22 @@ int pdx_write_vtk(const pdx_model *pm, const char *path, const char *title, cons
     return rc ? fail(rc, err) : PDX_OK;
 }
 
+int pdx_write_vtk_part(const pdx_model *pm, const char *path, const char *title, const double *u, const double *sigma,
+                       const double *phi, int part) {
+    if (!pm || !path) return fail(PDX_ERR_ARG, "pdx_write_vtk_part: null argument");
+    std::string err;
+    const int rc =
+        write_vtk_part(pm->m, path, title ? std::string(title) : pm->m.title, u, sigma, phi, part, err);
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
+int pdx_write_node_table(const pdx_model *pm, const char *path, const double *u, const double *sigma, int kind) {
+    if (!pm || !path) return fail(PDX_ERR_ARG, "pdx_write_node_table: null argument");
+    std::string err;
+    const int rc = write_node_table(pm->m, path, u, sigma, kind, err);
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
 int pdx_model_sizes(const pdx_model *pm, long long *out) {
     if (!pm || !out) return fail(PDX_ERR_ARG, "null argument");
     const Model &m = pm->m;

@@ -116,5 +116,11 @@ int break_bonds_across(const Model &m, const double *crack, std::vector<int> &bo
 // result file in the reference's VTK layout (vtk.cpp); null arrays write zeros
 int write_vtk(const Model &m, const std::string &path, const std::string &title, const double *u,
               const double *sigma, const double *phi, bool binary, std::string &err);
+// the PD-only (part 0) or FE-only (part 1) ASCII result file
+int write_vtk_part(const Model &m, const std::string &path, const std::string &title, const double *u,
+                   const double *sigma, const double *phi, int part, std::string &err);
+// the .disp (kind 0) or .sig (kind 1) node table
+int write_node_table(const Model &m, const std::string &path, const double *u, const double *sigma, int kind,
+                     std::string &err);
 
 }  // namespace pdx

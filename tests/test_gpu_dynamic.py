@@ -66,15 +66,14 @@ def test_explicit_consistent_mass_trajectory(name, prec, tol):
 
 def _fc2_case(lumped):
     """Edge-cracked 41 x 21 plate pulled apart at the top and bottom (the shape of the reference's
-    plateCrack dynamic examples: VEBC -1 / +1 on the two loaded edges, FC 2 0).  Interior nodes
-    are jittered by up to 0.15 dx: on an exact grid an FC 2 crack segment (through a bond's
-    midpoint, normal to the principal direction) runs almost along a row of nodes and bonds, where
-    the reference's segment test decides on the last bits of the stresses, which the GPU and the
+    plateCrack dynamic examples: VEBC -1 / +1 on the two loaded edges, FC 2 0).  Every node is
+    jittered by up to 0.15 dx: on an exact grid an FC 2 crack segment (through a bond's midpoint,
+    normal to the principal direction) runs almost along a row of nodes and bonds, where the
+    reference's segment test decides on the last bits of the stresses, which the GPU and the
     oracle share only to 1e-12."""
     m = W.crack2d(seed=1, nx=41, ny=21, dx=1e-3)
     x = m["x"].copy()
-    inner = (x[:, 0] > 0) & (x[:, 0] < 40e-3) & (x[:, 1] > 0) & (x[:, 1] < 20e-3)
-    x[inner, :2] += np.random.default_rng(11).uniform(-0.15e-3, 0.15e-3, (int(inner.sum()), 2))
+    x[:, :2] += np.random.default_rng(11).uniform(-0.15e-3, 0.15e-3, (len(x), 2))
     m["x"] = x
     m["ebc"][0].update(val=0.0, vel=-0.5, vary=True)
     m["ebc"][2].update(val=0.0, vel=0.5, vary=True)
@@ -85,9 +84,12 @@ def _fc2_case(lumped):
 @pytest.mark.parametrize("lumped", [True, False])
 def test_explicit_fc2_matches_oracle(lumped):
     """Bond failure after every stored explicit step: the bonds selected and broken per step and
-    the final bond states equal the oracle's; u within 1e-8."""
+    the final bond states equal the oracle's; u within 1e-8.  The run covers the first breaks up
+    to the onset of the cascade, where hundreds of bonds sit near the strength every step and a
+    stress difference in the last bits (the GPU and the oracle agree to 1e-12) can decide one of
+    them either way."""
     m = _fc2_case(lumped)
-    N = 40
+    N = 32
     _, _, probe = O.Model(m).dynamic_run(N, sigma_ult=1e30)
     s = probe[:, 2]
     m["sigma_ult"] = float(0.5 * (s[N // 2 - 1] + s[N // 2]))  # bonds start to break half-way
@@ -109,7 +111,7 @@ def test_newmark_fc2_matches_oracle():
     final bond states exact, u within 1e-8."""
     m = _fc2_case(False)
     m["dt"] = 2e-7
-    N = 16
+    N = 12
     probe = []
     O.newmark_run(O.Model(m), N, fc=2, topk=0, sigma_ult=1e30, info=probe)
     s = np.array(probe)[:, 2]

@@ -245,3 +245,79 @@ def test_mass_pattern_matches_oracle(name):
     kia, kja = mp.csr()
     for r in range(0, mp.neq, max(1, mp.neq // 200)):
         assert set(ja1[ia1[r]: ia1[r + 1]]) <= set(kja[kia[r]: kia[r + 1]])
+
+
+def test_vtk_ascii_quads_share_one_line(tmp_path):
+    """pdfemEleQuad4N::print_vtk ends no line (pdfemEleQuad4N.cpp:201-215): in the ASCII result of
+    a 2D quad mesh every cell follows on one line; triangles end theirs (pdfemEleTri3N.cpp:105-121)."""
+    mesh = tri_tet_mesh()
+    m = pdx.Model(mesh)
+    path = tmp_path / "q.vtk"
+    m.write_vtk(path, binary=False, title="q")
+    lines = path.read_text().split("\n")
+    i = next(k for k, l in enumerate(lines) if l.startswith("CELLS "))
+    nq = int((np.asarray(mesh["nen"]) == 4).sum())
+    want = []
+    for e in range(len(mesh["econ"])):
+        n = int(mesh["nen"][e])
+        want.append(f"{n} " + "".join(f"{int(v) - 1} " for v in mesh["econ"][e][:n]))
+    body = "\n".join(lines[i + 1: lines.index("", i + 1) + 1])
+    expect = ""
+    for e, w in enumerate(want):
+        expect += w + ("" if mesh["nen"][e] == 4 else "\n")
+    assert body.rstrip("\n") == expect.rstrip("\n") and nq > 0
+
+
+def test_vtk_pd_and_fem_parts(tmp_path):
+    """writeResultsPD_vtk / writeResultsFEM_vtk (fioFile.cpp:771-1001): the PD file holds the
+    family nodes and the PD elements with U, sigma and phi; the FEM file the nodes that are not pure
+    PD and the FE elements with U and sigma; cells index the part's own points."""
+    mesh = W.crack2d(seed=2, nx=21, ny=11, dx=1e-3)
+    m = pdx.Model(mesh)
+    nd = m.nodes()
+    rng = np.random.default_rng(1)
+    U = rng.normal(size=(m.nn, 3))
+    S = rng.normal(size=(m.nn, 6))
+    phi = rng.uniform(size=m.nn)
+    for part, keep_n, algo in ((0, nd["ntype"] != 0, 1), (1, nd["ntype"] != 2, 2)):
+        path = tmp_path / f"p{part}.vtk"
+        m.write_vtk_part(path, part, u=U, sigma=S, phi=phi, title="t")
+        lines = path.read_text().split("\n")
+        np_ = int(keep_n.sum())
+        assert lines[:5] == ["# vtk DataFile Version 3.0", "t", "ASCII", "DATASET UNSTRUCTURED_GRID",
+                             f"POINTS {np_} float"]
+        pts = np.array([[float(v) for v in l.split()] for l in lines[5: 5 + np_]])
+        assert np.allclose(pts, mesh["x"][keep_n], rtol=1e-4)
+        ce = np.nonzero(nd["algo"] == algo)[0]
+        i = lines.index(f"CELLS {len(ce)} {5 * len(ce)}")
+        ids = [int(v) for v in lines[i + 1].split()]  # quads share one line
+        mp = -np.ones(m.nn, int)
+        mp[keep_n] = np.arange(np_)
+        want = []
+        for e in ce:
+            want += [4] + [int(mp[v - 1]) for v in mesh["econ"][e][:4]]
+        assert ids == want and min(want) >= 0
+        assert lines[i + 2] == f"CELL_TYPES {len(ce)}"  # the endl after the cells ends their one line
+        i = lines.index("VECTORS U float")
+        u = np.array([[float(v) for v in l.split()] for l in lines[i + 1: i + 1 + np_]])
+        assert np.allclose(u, U[keep_n], rtol=1e-4)
+        assert ("SCALARS phi float 1" in lines) == (part == 0)
+
+
+def test_node_tables(tmp_path):
+    """writeUofNode / writeSigofNode (fioFile.cpp:486-538): header lines, then per node the id,
+    the coordinates and the values, tab-separated, scientific with 4 (.disp) or 5 (.sig) digits."""
+    mesh = W.box3d(n=5, pd_block=2)
+    m = pdx.Model(mesh)
+    rng = np.random.default_rng(2)
+    U, S = rng.normal(size=(m.nn, 3)), rng.normal(size=(m.nn, 6))
+    m.write_node_table(tmp_path / "a.disp", 0, u=U)
+    m.write_node_table(tmp_path / "a.sig", 1, sigma=S)
+    d = (tmp_path / "a.disp").read_text().split("\n")
+    assert d[:2] == ["*****The final displacement of  point*****", "ID\tx\ty\tz\tUx\tUy\tUz"]
+    row = d[2].split("\t")
+    assert row[0] == "1" and row[4] == f"{U[0, 0]:.4e}" and len(row) == 7
+    s = (tmp_path / "a.sig").read_text().split("\n")
+    assert s[1] == "NID\tx\ty\tz\tSxx\tSyy\tSzz\tSxy\tSyz\tSzx"
+    row = s[3].split("\t")
+    assert row[0] == "2" and row[1] == f"{mesh['x'][1, 0]:.5e}" and row[9] == f"{S[1, 5]:.5e}"
