@@ -12,8 +12,9 @@ at 0.8 x the critical dt (power iteration on M^-1 K).
 A step is one explicit central-difference step (internal force K u_n over all
 rows, external force, update of the free and prescribed DOFs) over the whole
 mesh; state is resident in HBM when the timed region starts and the steps run
-as replays of a captured 6-step hipGraph.  Value = DOF-steps/s of the FP32
-internal-force / FP64-state variant; the FP64 variant is reported beside it.
+as replays of a captured 6-step hipGraph, captured before the timed region.
+Value = DOF-steps/s in FP64 (the reference's precision); the FP32 internal-force /
+FP64-state variant and its drift from the FP64 trajectory are reported beside it.
 
 Also reported: K assembly time (family + element kernels), the static solve
 time to a 1e-8 true relative residual on the edge-cracked plate (501 x 251
@@ -283,16 +284,16 @@ def hbm_probe(mib=2048, reps=10):
     print(json.dumps({"hbm_probe": out}), flush=True)
 
 
-def measured_traffic(kernel, sfc=False):
+def measured_traffic(kernel, alt=False):
     """HBM bytes per launch of the explicit kernel on the 60^3 box, from the
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
     profiles/k_explicit_box3d_60_traffic.md (copy-calibrated); None if that
-    kernel form was not profiled.  Forms visiting the rows in Morton order launch
-    the same kernel with another row list: their figures are keyed '<kernel> sfc'."""
+    kernel form was not profiled.  Forms that reverse the sweep every other step launch
+    the same kernel: their figures are keyed '<kernel> alternating'."""
     path = os.path.join(ROOT, "profiles", "k_explicit_box3d_60_traffic.json")
     try:
         with open(path) as f:
-            k = json.load(f)["kernels"].get(kernel + (" sfc" if sfc else ""))
+            k = json.load(f)["kernels"].get(kernel + (" alternating" if alt else ""))
     except (OSError, ValueError, KeyError):
         return None
     return k["traffic_bytes"] if k else None
@@ -431,17 +432,21 @@ def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
     the two block-Jacobi runs again with the SpMVs on mirrored storage (same
     results bit for bit).  The other runs' displacements are compared with the
     plain run's."""
-    t0 = time.time()
     m0 = W.penny3d(n=n)
     cx0 = pdx.Context(pdx.Model(m0))
     lmax = cx0.max_eig(200)
     dt = 10.0 * 2.0 / np.sqrt(lmax)
     del cx0
     m = W.penny3d(n=n, dt=dt, ramp_steps=steps)
-    mp = pdx.Model(m)
+    t0 = time.time()
+    mp = pdx.Model(m)  # the reference's set-up: volumes, cells, PD/FE split, families, node graph, CSR rows
     setup_s = time.time() - t0
+    t0 = time.time()
+    cx = pdx.Context(mp)  # device structures: uploads, block maps, column classes, mirrored index
+    ctx_s = time.time() - t0
+    del cx
     out = {"workload": f"penny3d {n}^3 nodes, PD 30^3, dt = 10 dt_crit = {dt:.3e} s, {steps} Newmark steps",
-           "neq": mp.neq, "host_setup_s": setup_s}
+           "neq": mp.neq, "host_setup_s": setup_s, "context_setup_s": ctx_s}
     runs = {"plain_fp64_blockjacobi": solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
                                                      max_iter=100000),
             "fp32_ir_blockjacobi": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
@@ -611,7 +616,7 @@ def main():
     tf = lambda b: "true" if b else "false"
     roof = {}
     for prec in (0, 1):
-        lanes, nt, xm, soa, rel, sym, tune_ms, sfc, form = ctxs[prec].explicit_tuning_full(prec)
+        lanes, nt, xm, soa, rel, sym, tune_ms, alt, form = ctxs[prec].explicit_tuning_full(prec)
         bytes_step = step_bytes(model, nblk, prec, classes if rel else None, n_own if sym else None)
         achieved = bytes_step / (res[prec]["step_ms"] * 1e-3)
         ty = "float" if prec else "double"
@@ -619,9 +624,9 @@ def main():
                  f"{tf(sym)}>")
         roof[prec] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                       "frac": achieved / HBM_PEAK,
-                      "traffic": measured_traffic(kname, sfc) if args.n == 60 else None,
+                      "traffic": measured_traffic(kname, alt) if args.n == 60 else None,
                       "kernel": kname + f" (node-block SpMV + central-difference update, {'FP32' if prec else 'FP64'} "
-                                        f"values{', rows in Morton order' if sfc else ''})",
+                                        f"values{', sweep reversed every other step' if alt else ''})",
                       "form": form,
                       "tuner_ms_per_form": [float(t) for t in tune_ms],
                       "bytes_per_launch": bytes_step,

@@ -369,12 +369,13 @@ int pdx_last_timing(pdx_ctx *c, double *total_ms, double *per_step_ms);
  * XCD-contiguous block order, row-local value order of K, column pattern classes: a row's block
  * columns read as its index plus the offsets of its class, shared by the rows of one stencil;
  * mirrored blocks: a lower block that is the exact transpose of its upper partner (both summed
- * from element matrices only) is read from the partner's storage; forms 20-23 visit the rows
- * in Morton order of the node coordinates; all 0 before the first step)
+ * from element matrices only) is read from the partner's storage; alternating: every other step
+ * visits the rows last to first, so a step begins on the data the previous one read last (forms
+ * 20-23); all 0 before the first step)
  * and its timings of the PDX_EXPLICIT_FORMS forms (ms per step) */
 #define PDX_EXPLICIT_FORMS 24
 int pdx_explicit_tuning(pdx_ctx *c, int precision, int *lanes, int *nontemporal, int *xcd_order, int *row_soa,
-                        int *col_classes, int *mirrored, int *sfc_order, int *form,
+                        int *col_classes, int *mirrored, int *alternating, int *form,
                         float *ms /*[PDX_EXPLICIT_FORMS]*/);
 /* use explicit-step form `form` (0 .. PDX_EXPLICIT_FORMS-1) from now on instead of the tuner's
  * choice, or -1 to tune again at the next step; forms with the same lanes per row compute the

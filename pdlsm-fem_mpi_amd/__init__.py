@@ -444,13 +444,13 @@ class Context:
         return self.explicit_tuning_full(precision)[:7]
 
     def explicit_tuning_full(self, precision):
-        """explicit_tuning plus (Morton row order, form index)."""
-        lanes, nt, xm, soa, rel, sym, sfc, form = (C.c_int(0) for _ in range(8))
+        """explicit_tuning plus (alternating sweep direction, form index)."""
+        lanes, nt, xm, soa, rel, sym, alt, form = (C.c_int(0) for _ in range(8))
         ms = np.zeros(EXPLICIT_FORMS, np.float32)
         _chk(lib().pdx_explicit_tuning(self.h, precision, C.byref(lanes), C.byref(nt), C.byref(xm), C.byref(soa),
-                                       C.byref(rel), C.byref(sym), C.byref(sfc), C.byref(form), _p(ms)))
+                                       C.byref(rel), C.byref(sym), C.byref(alt), C.byref(form), _p(ms)))
         return (lanes.value, bool(nt.value), bool(xm.value), bool(soa.value), bool(rel.value), bool(sym.value),
-                ms, bool(sfc.value), form.value)
+                ms, bool(alt.value), form.value)
 
     def max_eig(self, iters=100):
         v = C.c_double(0)

@@ -44,10 +44,15 @@ static int alloc(Ctx &c, T *&dst, size_t n, std::string &err) {
     return PDX_OK;
 }
 
-#define UP(dst, src) \
-    if ((rc = up(*c, dst, src, err))) return rc
-#define AL(dst, n) \
-    if ((rc = alloc(*c, dst, n, err))) return rc
+// statement macros (do/while, so an else after one binds to the caller's if)
+#define UP(dst, src)                                       \
+    do {                                                   \
+        if ((rc = up(*c, dst, src, err))) return rc;       \
+    } while (0)
+#define AL(dst, n)                                         \
+    do {                                                   \
+        if ((rc = alloc(*c, dst, n, err))) return rc;      \
+    } while (0)
 
 static int find_block(const Model &m, int row_node, int col_node) {
     auto b = m.adj.begin() + m.adj_off[row_node], e = m.adj.begin() + m.adj_off[row_node + 1];
@@ -254,33 +259,6 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
         c->n_long = int(rl.size());
         UP(c->rows_short, rs);
         UP(c->rows_long, rl);
-        // Morton keys of the node coordinates on a 2^10 grid per axis over the bounding box
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (int k = 0; k < m.nn; k++)
-            for (int i = 0; i < 3; i++) {
-                lo[i] = std::min(lo[i], m.x[(size_t)k * 3 + i]);
-                hi[i] = std::max(hi[i], m.x[(size_t)k * 3 + i]);
-            }
-        auto key = [&](int k) {
-            uint64_t q[3] = {0, 0, 0};
-            for (int i = 0; i < D; i++) {
-                const double w = hi[i] > lo[i] ? (m.x[(size_t)k * 3 + i] - lo[i]) / (hi[i] - lo[i]) : 0.0;
-                q[i] = (uint64_t)std::min(1023.0, std::max(0.0, std::floor(w * 1024.0)));
-            }
-            uint64_t z = 0;
-            for (int b = 9; b >= 0; b--)
-                for (int i = D - 1; i >= 0; i--) z = (z << 1) | ((q[i] >> b) & 1);
-            return z;
-        };
-        for (auto *v : {&rs, &rl}) {
-            std::vector<std::pair<uint64_t, int>> kv;
-            kv.reserve(v->size());
-            for (int k : *v) kv.push_back({key(k), k});
-            std::sort(kv.begin(), kv.end());
-            for (size_t i = 0; i < kv.size(); i++) (*v)[i] = kv[i].second;
-        }
-        UP(c->rows_short_sfc, rs);
-        UP(c->rows_long_sfc, rl);
     }
     // column pattern classes: rows whose block columns have the same offsets from the row index
     // (the stencil of a structured mesh, jittered or not) share one list of offsets
@@ -826,7 +804,7 @@ int pdx_node_stresses_timed(pdx_ctx *pc, const double *u_all, double *ms) {
 }
 
 int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal, int *xcd_order, int *row_soa,
-                        int *col_classes, int *mirrored, int *sfc_order, int *form, float *ms) {
+                        int *col_classes, int *mirrored, int *alternating, int *form, float *ms) {
     if (!pc || (precision != PDX_FP64 && precision != PDX_FP32)) return fail(PDX_ERR_ARG, "bad argument");
     const Ctx &c = *pc->c;
     const int v = c.tune_explicit[precision];
@@ -837,7 +815,7 @@ int pdx_explicit_tuning(pdx_ctx *pc, int precision, int *lanes, int *nontemporal
     if (row_soa) *row_soa = f.soa;
     if (col_classes) *col_classes = f.rel;
     if (mirrored) *mirrored = f.sym;
-    if (sfc_order) *sfc_order = f.sfc;
+    if (alternating) *alternating = f.alt;
     if (form) *form = v;
     if (ms) std::copy(c.tune_ms[precision], c.tune_ms[precision] + PDX_EXPLICIT_FORMS, ms);
     return PDX_OK;

@@ -31,11 +31,13 @@ struct DevBuf {
 
 // explicit-step kernel forms searched by the tuner: lanes per short row, non-temporal K loads,
 // XCD-contiguous block order, row-local value order of K, column pattern classes, mirrored
-// lower blocks read from their transposed partners, rows visited in space-filling-curve order
-// (same kernel as form `base`, another row list)
+// lower blocks read from their transposed partners, and the sweep direction (alt: odd steps
+// visit the blocks in reverse order, so a step starts on the rows the previous step read last,
+// which are still in the memory-side cache when the step's data exceeds it; the same kernel as
+// form `base`, same bits)
 struct ExplicitForm {
     int lanes;
-    bool nt, xm, soa, rel, sym, sfc;
+    bool nt, xm, soa, rel, sym, alt;
     int base;
 };
 constexpr ExplicitForm kExplicitForms[PDX_EXPLICIT_FORMS] = {
@@ -85,10 +87,6 @@ struct Ctx {
     // rows of the node graph binned by length for the SpMV (short: one group of 8 lanes, long: a wave)
     int n_short = 0, n_long = 0;
     int *rows_short = nullptr, *rows_long = nullptr;
-    // the same rows in Morton order of the node coordinates (space-filling curve): the lower
-    // neighbours of a row, whose blocks the mirrored forms read transposed, come a few rows
-    // earlier instead of a whole plane earlier
-    int *rows_short_sfc = nullptr, *rows_long_sfc = nullptr;
 
     // natural BCs
     int n_nbc = 0, n_nbc_ele = 0;

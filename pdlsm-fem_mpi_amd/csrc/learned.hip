@@ -432,8 +432,13 @@ static int gnn_work(Ctx &c, GnnWork *&out, std::string &err) {
     g->nblk = c.nblk;
     int rc;
     auto &own = g->owned;
-#define GA(p, count) \
-    if ((rc = walloc(own, p, (count), err))) { delete g; return rc; }
+#define GA(p, count)                                     \
+    do {                                                 \
+        if ((rc = walloc(own, p, (count), err))) {       \
+            delete g;                                    \
+            return rc;                                   \
+        }                                                \
+    } while (0)
     GA(g->X, (size_t)nn * GIN);
     GA(g->cw, (size_t)c.nblk);
     GA(g->Aaa, (size_t)nn * DD);

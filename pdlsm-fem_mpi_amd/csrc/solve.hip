@@ -29,7 +29,9 @@ namespace pdx {
 // order (chosen by the tuner).  The hardware deals workgroups to the 8 XCDs
 // round-robin; with XM the launch is padded to a multiple of 8 and block b works
 // on logical block (b % 8) * per + b / 8, so each XCD (and its L2) owns one
-// contiguous eighth of the rows and the gathers of u_n stay local to it.
+// contiguous eighth of the rows and the gathers of u_n stay local to it.  With rev
+// the blocks of each range are visited last to first (alternating forms reverse
+// every other step), which changes which rows run when, not what a row computes.
 // column pattern classes (REL forms): class of each row, offsets of each class's relative columns
 struct ColClasses {
     const int *cls, *cls_off, *cls_rel;
@@ -45,13 +47,15 @@ __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int
                                                   const double *m, const double *f_unit, int n_nbc, int nn,
                                                   const double *nbc_cur, double *nbc_next, const double *nbc_inc,
                                                   const int *eq, const int *presc_set, const double *ebc_inc,
-                                                  double dt, int startup) {
+                                                  double dt, int startup, int rev) {
     constexpr bool DIFF = sizeof(T) == 4;
     int lb = blockIdx.x;
     if (XM) {
-        const int per = gridDim.x / 8;
-        lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+        const int per = gridDim.x / 8, q = blockIdx.x / 8;
+        lb = (blockIdx.x % 8) * per + (rev ? per - 1 - q : q);
         if (lb >= nbt) return;
+    } else if (rev) {
+        lb = gridDim.x - 1 - blockIdx.x;
     }
     if (lb == 0)
         for (int s = threadIdx.x; s < n_nbc; s += blockDim.x) nbc_next[s] = nbc_cur[s] + nbc_inc[s];
@@ -292,22 +296,22 @@ static int ensure_Ks(Ctx &c, int prec, std::string &err) {
 
 template <int D, typename T, int GS, bool NT, bool XM, bool SOA, bool REL, bool SYM, bool FO = false>
 static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double *Unm1, double *Unp1,
-                              const double *nbc_cur, double *nbc_next, int startup, bool sfc = false) {
+                              const double *nbc_cur, double *nbc_next, int startup, int rev = 0) {
     const int nbs = (c.n_short * GS + 255) / 256, nbl = (c.n_long * 64 + 255) / 256;
     const int nbt = std::max(nbs + nbl, 1);
     const int grid = XM ? (nbt + 7) / 8 * 8 : nbt;
     const ColClasses cc{c.cls, c.cls_off, c.cls_rel, c.ksrc};
-    const int *rs = sfc ? c.rows_short_sfc : c.rows_short, *rl = sfc ? c.rows_long_sfc : c.rows_long;
     k_explicit<D, T, GS, NT, XM, SOA, REL, SYM, FO><<<grid, 256, 0, c.stream>>>(
-        rs, c.n_short, nbs, rl, c.n_long, nbt, c.adj_off, c.adj, cc, K, Un, Unm1, Unp1, c.mlump,
-        c.f_unit, c.n_nbc, c.nn, nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup);
+        c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, nbt, c.adj_off, c.adj, cc, K, Un, Unm1, Unp1, c.mlump,
+        c.f_unit, c.n_nbc, c.nn, nbc_cur, nbc_next, c.d_nbc_vel, c.eq, c.presc_set, c.d_ebc_vel, c.dt, startup, rev);
 }
 
 // one launch of form `variant` of kExplicitForms (K: the plain or the row-local copy, as the form reads)
+// for step `step` (its parity sets the sweep direction of an alternating form)
 template <int D, typename T>
 static void launch_explicit(Ctx &c, int variant, const T *K, const double *Un, const double *Unm1, double *Unp1,
-                            const double *nbc_cur, double *nbc_next, int startup) {
-    const bool sfc = kExplicitForms[variant].sfc;
+                            const double *nbc_cur, double *nbc_next, int startup, long long step) {
+    const int rev = kExplicitForms[variant].alt ? int(step & 1) : 0;
     variant = kExplicitForms[variant].base;
 #define PDX_EXPLICIT_FORM(V, GS, NT, XM, SOA, REL, SYM)                                                         \
     case V:                                                                                                     \
@@ -315,7 +319,7 @@ static void launch_explicit(Ctx &c, int variant, const T *K, const double *Un, c
                           kExplicitForms[V].soa == SOA && kExplicitForms[V].rel == REL &&                       \
                           kExplicitForms[V].sym == SYM,                                                         \
                       "form table");                                                                            \
-        launch_explicit_v<D, T, GS, NT, XM, SOA, REL, SYM>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup, sfc);  \
+        launch_explicit_v<D, T, GS, NT, XM, SOA, REL, SYM>(c, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup, rev); \
         break;
     switch (variant) {
         PDX_EXPLICIT_FORM(1, 4, false, false, false, false, false)
@@ -344,16 +348,16 @@ static void launch_explicit(Ctx &c, int variant, const T *K, const double *Un, c
 }
 
 static void launch_explicit_any(Ctx &c, int prec, int variant, const double *Un, const double *Unm1, double *Unp1,
-                                const double *nbc_cur, double *nbc_next, int startup) {
+                                const double *nbc_cur, double *nbc_next, int startup, long long step) {
     const bool soa = kExplicitForms[variant].soa, sym = kExplicitForms[variant].sym;
     if (prec == PDX_FP32) {
         const float *K = sym ? c.Kown32 : soa ? c.K32s : c.K32;
-        if (c.dim == 3) launch_explicit<3, float>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
-        else launch_explicit<2, float>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
+        if (c.dim == 3) launch_explicit<3, float>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup, step);
+        else launch_explicit<2, float>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup, step);
     } else {
         const double *K = sym ? c.Kown64 : soa ? c.K64s : c.K;
-        if (c.dim == 3) launch_explicit<3, double>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
-        else launch_explicit<2, double>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup);
+        if (c.dim == 3) launch_explicit<3, double>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup, step);
+        else launch_explicit<2, double>(c, variant, K, Un, Unm1, Unp1, nbc_cur, nbc_next, startup, step);
     }
 }
 
@@ -371,15 +375,15 @@ static int tune_explicit(Ctx &c, int prec, std::string &err) {
     float best = 1e30f;
     int bestv = 0;
     for (int v = 0; v < PDX_EXPLICIT_FORMS; v++) {
-        launch_explicit_any(c, prec, v, c.U[p], c.U[(p + 2) % 3], scratch, c.nbcv, nscratch, 0);  // warm
+        launch_explicit_any(c, prec, v, c.U[p], c.U[(p + 2) % 3], scratch, c.nbcv, nscratch, 0, 1);  // warm
         PDX_HIP(hipEventRecord(e0, c.stream));
-        for (int r = 0; r < 5; r++)
-            launch_explicit_any(c, prec, v, c.U[p], c.U[(p + 2) % 3], scratch, c.nbcv, nscratch, 0);
+        for (int r = 0; r < 6; r++)
+            launch_explicit_any(c, prec, v, c.U[p], c.U[(p + 2) % 3], scratch, c.nbcv, nscratch, 0, r);
         PDX_HIP(hipEventRecord(e1, c.stream));
         PDX_HIP(hipEventSynchronize(e1));
         float ms = 0;
         PDX_HIP(hipEventElapsedTime(&ms, e0, e1));
-        c.tune_ms[prec][v] = ms / 5;
+        c.tune_ms[prec][v] = ms / 6;
         if (ms < best) { best = ms; bestv = v; }
     }
     hipEventDestroy(e0);
@@ -400,7 +404,7 @@ static void explicit_step(Ctx &c, int prec, int startup) {
     double *nbc_cur = c.nbcv + (s % 2) * std::max(c.n_nbc, 1);
     double *nbc_next = c.nbcv + ((s + 1) % 2) * std::max(c.n_nbc, 1);
     launch_explicit_any(c, prec, std::max(c.tune_explicit[prec == PDX_FP32 ? 1 : 0], 0), Un, Unm1, Unp1, nbc_cur,
-                        nbc_next, startup);
+                        nbc_next, startup, s);
     c.step_phase = (p + 1) % 3;
     c.steps_done = s;
 }
