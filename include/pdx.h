@@ -296,6 +296,17 @@ int pdx_local_damage(pdx_ctx *c, double *phi);
  * storeDisplacementResult leaves them).  sigma [n_node*6]: xx, yy, zz, xy, yz, zx (2D: xx, yy,
  * zz, xy). */
 int pdx_node_stresses(pdx_ctx *c, const double *u_all, double *sigma);
+/* The same recovery without allocating.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  u_dev [n_node*dim] and sigma_dev [n_node*6] are device pointers
+ * owned by the caller; the element corner stresses, family stresses and the status word live in
+ * a fixed 96 MiB device array of the library (one per GPU, held until the process ends):
+ * models needing more (8*6 doubles per FE element + 6 per family + 1) get PDX_ERR_LIMIT.  K must
+ * have been assembled on the context (the recovery reads the bond strain weights the assembly
+ * leaves).  The launches go to the context's stream and are not waited for; no memset, host
+ * copy or synchronisation, so they may be captured.  Calls on one GPU share the array: calls
+ * that overlap in time on one GPU (two streams, two threads) race. */
+int pdx_node_stresses_noalloc(pdx_ctx *c, const double *u_dev, double *sigma_dev);
 /* the same on the device only, timed (ms of one recovery, inputs resident) */
 int pdx_node_stresses_timed(pdx_ctx *c, const double *u_all, double *ms);
 /* Newmark (acceleration form), pdfemDynamicNewmarkSolver_CSRformat (pdsolve.cpp:6092-6290): per

@@ -114,6 +114,7 @@ EXPORTS = {
     "pdx_explicit_set_form": (_I, [_P, _I, _I]),
     "pdx_node_stresses": (_I, [_P, _P, _P]),
     "pdx_node_stresses_timed": (_I, [_P, _P, _P]),
+    "pdx_node_stresses_noalloc": (_I, [_P, _P, _P]),
     "pdx_gnn_create": (_I, [_I, _I, C.c_ulonglong, _P]),
     "pdx_gnn_destroy": (None, [_P]),
     "pdx_gnn_num_params": (_LL, [_P]),

@@ -23,6 +23,8 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
                 std::string &err);
 int dev_explicit_state(Ctx &c, double *u_out, double *u_prev, std::string &err);
 int dev_max_eig(Ctx &c, int iters, double *lmax, std::string &err);
+int stress_scratch_init(int device, std::string &err);
+int dev_node_stresses_noalloc(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err);
 
 int dev_solve(Ctx &c, const pdx_solve_opts &o, const double *b_dev, double *x_dev, pdx_solve_stats &st,
               std::string &err);
@@ -323,11 +325,13 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
     PDX_HIP(hipDeviceSynchronize());
     if ((rc = dev_nbc_unit(*c, err))) return rc;
     PDX_HIP(hipStreamSynchronize(c->stream));
-    // node -> family id of the stress recovery
+    // node -> family id of the stress recovery, and the recovery's allocation-free scratch on
+    // this device (its address is taken once per device, with the device current)
     {
         std::vector<int> fid(m.famid.begin(), m.famid.end());
         UP(c->stress_famid, fid);
     }
+    if ((rc = stress_scratch_init(device, err))) return rc;
     return PDX_OK;
 }
 
@@ -767,6 +771,15 @@ int pdx_node_stresses(pdx_ctx *pc, const double *u_all, double *sigma) {
     int rc = run();
     if (du) hipFree(du);
     if (ds) hipFree(ds);
+    return rc ? fail(rc, err) : PDX_OK;
+}
+
+int pdx_node_stresses_noalloc(pdx_ctx *pc, const double *u_dev, double *sigma_dev) {
+    if (!pc || !u_dev || !sigma_dev) return fail(PDX_ERR_ARG, "pdx_node_stresses_noalloc: null argument");
+    Ctx &c = *pc->c;
+    std::string err;
+    if (hipSetDevice(c.device) != hipSuccess) return fail(PDX_ERR_HIP, "hipSetDevice failed");
+    const int rc = dev_node_stresses_noalloc(c, u_dev, sigma_dev, err);
     return rc ? fail(rc, err) : PDX_OK;
 }
 

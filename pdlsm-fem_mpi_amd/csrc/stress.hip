@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 
 #include "device.hpp"
 #include "element.hpp"
@@ -574,6 +576,56 @@ int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::strin
     c.stress_status = (int *)(c.stress_stage + stress_scratch_doubles(c) - 1);
     if (int rc = stress_launch(c, u_dev, sigma_dev, c.stress_stage, err)) return rc;
     return check ? stress_status_check(c, err) : PDX_OK;
+}
+
+// ---- the recovery without allocation ----------------------------------------------------
+// Scratch of pdx_node_stresses_noalloc in a file-scope device array: every GPU the library runs
+// on holds its own copy until the process ends.  Host code never passes the array's name to a
+// kernel (it is a host-side stand-in there): the address on each device comes from
+// hipGetSymbolAddress, taken once per device at context creation with that device current.
+constexpr size_t kNoAllocStressDoubles = size_t(12) << 20;  // 96 MiB
+static __device__ __align__(256) double scratch[kNoAllocStressDoubles];
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_ptr;  // device -> address of that device's scratch
+
+int stress_scratch_init(int device, std::string &err) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(device)) return PDX_OK;
+    double *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return PDX_ERR_HIP;
+    }
+    g_scratch_ptr[device] = ptr;
+    return PDX_OK;
+}
+
+int dev_node_stresses_noalloc(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err) {
+    if (!c.K_valid) {
+        err = "K is not assembled: the recovery reads the bond strain weights of the last assembly "
+              "(call pdx_assemble or a driver first)";
+        return PDX_ERR_ARG;
+    }
+    const size_t need = stress_scratch_doubles(c);
+    if (need > kNoAllocStressDoubles) {
+        err = "model needs " + std::to_string(need * 8) + " bytes of stress scratch, more than the " +
+              std::to_string(kNoAllocStressDoubles * 8) + " bytes of pdx_node_stresses_noalloc";
+        return PDX_ERR_LIMIT;
+    }
+    double *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        auto it = g_scratch_ptr.find(c.device);
+        if (it != g_scratch_ptr.end()) base = it->second;
+    }
+    if (!base) {
+        err = "stress scratch not set up on this device";
+        return PDX_ERR_ARG;
+    }
+    if (int rc = stress_launch(c, u_dev, sigma_dev, base, err)) return rc;
+    PDX_HIP(hipGetLastError());
+    return PDX_OK;
 }
 
 }  // namespace pdx
