@@ -441,12 +441,25 @@ def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
     t0 = time.time()
     mp = pdx.Model(m)  # the reference's set-up: volumes, cells, PD/FE split, families, node graph, CSR rows
     setup_s = time.time() - t0
+    # the same set-up with the family search and the node graph on the GPU
+    pdx.set_setup_device(0)
+    try:
+        t0 = time.time()
+        mg = pdx.Model(m)
+        gpu_setup_s = time.time() - t0
+    finally:
+        pdx.set_setup_device(-1)
+    fh, fg = mp.families(), mg.families()
+    same = bool(np.array_equal(fh["off"], fg["off"]) and np.array_equal(fh["mem"], fg["mem"])
+                and all(np.array_equal(a, b) for a, b in zip(mp.adjacency(), mg.adjacency())))
+    del mg, fh, fg
     t0 = time.time()
     cx = pdx.Context(mp)  # device structures: uploads, block maps, column classes, mirrored index
     ctx_s = time.time() - t0
     del cx
     out = {"workload": f"penny3d {n}^3 nodes, PD 30^3, dt = 10 dt_crit = {dt:.3e} s, {steps} Newmark steps",
-           "neq": mp.neq, "host_setup_s": setup_s, "context_setup_s": ctx_s}
+           "neq": mp.neq, "host_setup_s": setup_s, "gpu_setup_s": gpu_setup_s, "gpu_setup_identical": same,
+           "context_setup_s": ctx_s}
     runs = {"plain_fp64_blockjacobi": solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
                                                      max_iter=100000),
             "fp32_ir_blockjacobi": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
@@ -598,7 +611,7 @@ def main():
         res[prec] = dict(total_ms=total_ms, step_ms=step_ms, dofs=model.neq * args.steps / (total_ms * 1e-3))
         log(f"precision {prec}: {step_ms * 1e3:.2f} us/step, {res[prec]['dofs']:.4e} DOF-steps/s")
     u64, _ = ctxs[0].explicit_run(0)
-    u32, _ = ctxs[1].explicit_run(0)
+    u32, _ = ctxs[1].explicit_run(0, precision=1)
     drift = float(np.linalg.norm(u32 - u64) / max(np.linalg.norm(u64), 1e-300))
     # per-step nodal stress recovery (the reference recovers stresses after every explicit step,
     # pdsolve.cpp:6062), timed on its own: PD strains from the bond weights K's assembly left,
@@ -621,7 +634,7 @@ def main():
         achieved = bytes_step / (res[prec]["step_ms"] * 1e-3)
         ty = "float" if prec else "double"
         kname = (f"pdx::k_explicit<{model.dim}, {ty}, {lanes}, {tf(nt)}, {tf(xm)}, {tf(soa)}, {tf(rel)}, "
-                 f"{tf(sym)}>")
+                 f"{tf(sym)}, false>")
         roof[prec] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                       "frac": achieved / HBM_PEAK,
                       "traffic": measured_traffic(kname, alt) if args.n == 60 else None,

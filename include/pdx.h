@@ -167,6 +167,12 @@ const char *pdx_last_error(void);
 const char *pdx_version(void);
 
 /* ---- host data model -------------------------------------------------- */
+/* Where later pdx_model_create / pdx_model_read calls run the family search (initialPDN_Fami,
+ * pdsolve.cpp:1442-1568) and the node graph of the stiffness pattern (setCSRIndexes_gloStiffMat,
+ * pdsolve.cpp:959-1121): -1 (the default) on the host, threaded; device >= 0 on that GPU.  Both
+ * give the same families and pattern.  Process-wide; PDX_ERR_ARG for a device that does not
+ * exist. */
+int pdx_set_setup_device(int device);
 int pdx_model_create(const pdx_mesh_desc *desc, pdx_model **out);
 int pdx_model_read(const char *cmd_file, pdx_model **out, int *solver_flag, int *num_steps);
 void pdx_model_destroy(pdx_model *m);

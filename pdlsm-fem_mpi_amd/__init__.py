@@ -115,6 +115,7 @@ EXPORTS = {
     "pdx_node_stresses": (_I, [_P, _P, _P]),
     "pdx_node_stresses_timed": (_I, [_P, _P, _P]),
     "pdx_node_stresses_noalloc": (_I, [_P, _P, _P]),
+    "pdx_set_setup_device": (_I, [_I]),
     "pdx_gnn_create": (_I, [_I, _I, C.c_ulonglong, _P]),
     "pdx_gnn_destroy": (None, [_P]),
     "pdx_gnn_num_params": (_LL, [_P]),
@@ -161,6 +162,12 @@ class PdxError(RuntimeError):
 def _chk(rc):
     if rc != 0:
         raise PdxError(f"pdx error {rc}: {lib().pdx_last_error().decode()}")
+
+
+def set_setup_device(device: int) -> None:
+    """Run the family search and the node graph of later Model(...) set-ups on GPU `device`
+    (-1: on the host, the default); both give the same families and pattern."""
+    _chk(lib().pdx_set_setup_device(int(device)))
 
 
 def mesh_desc(mesh: dict):

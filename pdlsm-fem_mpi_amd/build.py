@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-functi
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-mllvm", "-amdgpu-mfma-vgpr-form=1"]
 SOURCES = ["model.cpp", "assemble.hip", "solve.hip", "krylov.hip", "context.hip", "newmark.hip", "csr_solve.hip",
-           "learned.hip", "family_batch.hip", "stress.hip", "vtk.cpp", "ilu0.hip"]
+           "learned.hip", "family_batch.hip", "stress.hip", "vtk.cpp", "ilu0.hip", "setup.hip"]
 
 
 def _compile(src: str, force: bool = False) -> str:

@@ -579,13 +579,13 @@ struct KeArgs {
     int *status;
 };
 
-// dN/dX of the 8 nodes and detJ at Gauss point g of element list entry idx
-// (pdfemEleBrick8N::eleStiffMatFEM's order: r outer, q, p inner), into g8o / *wdet; the
-// parent derivatives are formed again per node for the solves instead of being kept live
-__device__ inline void hex8_gauss_point(const KeArgs &A, int idx, int g, double (*g8o)[3], double *wdet) {
+// dN/dX of the 8 nodes and detJ at Gauss point g (pdfemEleBrick8N::eleStiffMatFEM's order: r
+// outer, q, p inner) of the element with node coordinates xn, into g8o / *wdet; the parent
+// derivatives are formed again per node for the solves instead of being kept live
+template <class XN>
+__device__ inline void hex8_gp(const XN &xn, int g, double (*g8o)[3], double *wdet, int *status) {
     const double gp = 0.57735026918962573;
     const double gpt[2] = {-gp, gp};
-    const int e = A.fe_elem[A.list[idx]];
     const double p = gpt[g & 1], qq = gpt[(g >> 1) & 1], r = gpt[g >> 2];
     auto dN = [&](int i, double d[3]) {
         d[0] = 1.0 / 8 * c_hp[i] * (1 + qq * c_hq[i]) * (1 + r * c_hr[i]);
@@ -598,11 +598,10 @@ __device__ inline void hex8_gauss_point(const KeArgs &A, int idx, int g, double 
     // every J entry sums over the nodes in node order, as the reference's cc-outer loop does
 #pragma unroll 2
     for (int i = 0; i < 8; i++) {
-        const int j = A.econ[(size_t)e * 8 + i];
         double d[3], xi[3];
         dN(i, d);
 #pragma unroll
-        for (int cc = 0; cc < 3; cc++) xi[cc] = A.x[(size_t)j * 3 + cc];
+        for (int cc = 0; cc < 3; cc++) xi[cc] = xn(i, cc);
 #pragma unroll
         for (int cc = 0; cc < 3; cc++) {
             J[0 * 3 + cc] = d[0] * xi[cc] + J[0 * 3 + cc];
@@ -612,7 +611,7 @@ __device__ inline void hex8_gauss_point(const KeArgs &A, int idx, int g, double 
     }
     const double detJ = det3(J);
     int piv[3];
-    if (!lu3_factor(J, piv)) atomicOr(A.status, 2);
+    if (!lu3_factor(J, piv)) atomicOr(status, 2);
 #pragma unroll 2
     for (int b = 0; b < 8; b++) {
         double d[3], o[3];
@@ -622,6 +621,13 @@ __device__ inline void hex8_gauss_point(const KeArgs &A, int idx, int g, double 
         for (int c = 0; c < 3; c++) g8o[b][c] = o[c];
     }
     *wdet = 1.0 * 1.0 * 1.0 * detJ;
+}
+
+// the same at Gauss point g of element list entry idx, coordinates read from global memory
+__device__ inline void hex8_gauss_point(const KeArgs &A, int idx, int g, double (*g8o)[3], double *wdet) {
+    const int e = A.fe_elem[A.list[idx]];
+    auto xn = [&](int i, int cc) { return A.x[(size_t)A.econ[(size_t)e * 8 + i] * 3 + cc]; };
+    hex8_gp(xn, g, g8o, wdet, A.status);
 }
 
 constexpr int HEX_EPB = 8;  // 8 elements x 24 rows = 192 threads
@@ -651,18 +657,34 @@ __global__ void __launch_bounds__(HEX_EPB * 24) k_ke_hex8(KeArgs A) {
 // points (27 FMA each instead of the 324 of three full B^T D B rows) and writes the block and,
 // Ke being symmetric, its transpose.
 constexpr int HEXI_EPB = 16;  // 16 elements: 128 set-up threads, 16 x 36 = 576 block threads
+// The element's node coordinates are loaded into LDS by the whole block first (one coordinate
+// per thread, so the gathers are in flight together), the Gauss-point set-up reads them there,
+// and the 36 blocks of each element are staged back in LDS (over the gradients) and written
+// with 16-byte stores over the element's contiguous 2,592 bytes.
 __global__ void __launch_bounds__(HEXI_EPB * 36) k_ke_hex8_iso(KeArgs A) {
-    __shared__ double sg[HEXI_EPB][8][8][3];
+    constexpr int KE = 36 * 9;  // doubles of one element's upper blocks
+    __shared__ double sx[HEXI_EPB][8][3];
     __shared__ double sw[HEXI_EPB][8];
+    __shared__ __align__(16) double sbuf[HEXI_EPB * KE];  // gradients [el][g][node][3], then blocks
+    static_assert(HEXI_EPB * 8 * 8 * 3 <= HEXI_EPB * KE, "gradients fit the block buffer");
+    double(*sg)[8][8][3] = reinterpret_cast<double(*)[8][8][3]>(sbuf);
     const int t = threadIdx.x;
     const int e0 = blockIdx.x * HEXI_EPB;
-    if (t < HEXI_EPB * 8) {
-        const int el = t / 8, g = t % 8, idx = e0 + el;
-        if (idx < A.nlist) hex8_gauss_point(A, idx, g, sg[el][g], &sw[el][g]);
+    const int nel = min(HEXI_EPB, A.nlist - e0);
+    for (int q = t; q < nel * 24; q += blockDim.x) {
+        const int el = q / 24, a = (q % 24) / 3, cc = q % 3;
+        const int e = A.fe_elem[A.list[e0 + el]];
+        sx[el][a][cc] = A.x[(size_t)A.econ[(size_t)e * 8 + a] * 3 + cc];
     }
     __syncthreads();
-    const int el = t / 36, idx = e0 + el;
-    if (idx >= A.nlist) return;
+    if (t < nel * 8) {
+        const int el = t / 8, g = t % 8;
+        auto xn = [&](int i, int cc) { return sx[el][i][cc]; };
+        hex8_gp(xn, g, sg[el][g], &sw[el][g], A.status);
+    }
+    __syncthreads();
+    const int el = t / 36;
+    const bool act = el < nel;
     int a = 0, rem = t % 36;
     while (rem >= 8 - a) {
         rem -= 8 - a;
@@ -673,23 +695,33 @@ __global__ void __launch_bounds__(HEXI_EPB * 36) k_ke_hex8_iso(KeArgs A) {
     double k[9];
 #pragma unroll
     for (int e = 0; e < 9; e++) k[e] = 0.0;
+    if (act) {
 #pragma unroll 2
-    for (int g = 0; g < 8; g++) {
-        const double w = sw[el][g];
-        const double wa[3] = {w * sg[el][g][a][0], w * sg[el][g][a][1], w * sg[el][g][a][2]};
-        const double gb[3] = {sg[el][g][b][0], sg[el][g][b][1], sg[el][g][b][2]};
-        const double dot = wa[0] * gb[0] + wa[1] * gb[1] + wa[2] * gb[2];
+        for (int g = 0; g < 8; g++) {
+            const double w = sw[el][g];
+            const double wa[3] = {w * sg[el][g][a][0], w * sg[el][g][a][1], w * sg[el][g][a][2]};
+            const double gb[3] = {sg[el][g][b][0], sg[el][g][b][1], sg[el][g][b][2]};
+            const double dot = wa[0] * gb[0] + wa[1] * gb[1] + wa[2] * gb[2];
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+            for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) k[i * 3 + j] += lam * wa[i] * gb[j] + mu * gb[i] * wa[j];
+                for (int j = 0; j < 3; j++) k[i * 3 + j] += lam * wa[i] * gb[j] + mu * gb[i] * wa[j];
 #pragma unroll
-        for (int i = 0; i < 3; i++) k[i * 4] += mu * dot;
+            for (int i = 0; i < 3; i++) k[i * 4] += mu * dot;
+        }
     }
+    __syncthreads();  // every gradient read
     // pair index t % 36 enumerates a <= b in ke_pair order: one contiguous block per thread
-    double *out = A.S + (size_t)A.list[idx] * (ke_pairs(3, 24) * 9) + (size_t)(t % 36) * 9;
+    if (act) {
 #pragma unroll
-    for (int e = 0; e < 9; e++) out[e] = k[e];
+        for (int e = 0; e < 9; e++) sbuf[el * KE + (t % 36) * 9 + e] = k[e];
+    }
+    __syncthreads();
+    for (int j = t; j < nel * (KE / 2); j += blockDim.x) {
+        const int ej = j / (KE / 2), ch = j % (KE / 2);
+        const double2 v = *reinterpret_cast<const double2 *>(&sbuf[ej * KE + ch * 2]);
+        *reinterpret_cast<double2 *>(A.S + (size_t)A.list[e0 + ej] * KE + ch * 2) = v;
+    }
 }
 
 constexpr int QUAD_EPB = 32;  // 32 elements x 8 rows = 256 threads
@@ -791,53 +823,67 @@ __global__ void k_ke_simplex(KeArgs A) {
     for (int c = D * la; c < NEN * D; c++) out[ke_up(D, ND, row, c)] = Krow[c] * V;
 }
 
-// Block-owned add of the staged element blocks: one thread per (K block, row) starts from the
-// block's PD part and adds the element terms in element order (the reference's summation order
-// of FE terms into K), then writes the row once.  The contributions of block (a, c) are the
-// staged blocks of the elements that hold both nodes, listed as (staged block index << 1 |
-// transposed); the d row threads of a K block read one contiguous run of d*d values per
-// contribution, and K is written once, contiguously.
+// Block-owned add of the staged element blocks, one node pair at a time: one thread per (pair,
+// row) takes an upper block (r, c), r <= c, and its lower partner (c, r), starts each from its PD
+// part, adds the element terms in element order (the reference's summation order of FE terms
+// into K) and writes both rows once.  The contributions of a pair are the staged blocks of the
+// elements that hold both nodes, listed as (staged block index << 1 | transposed relative to
+// (r, c)); the lower block sums the same blocks transposed, so every staged block is read once.
 template <int D>
-__global__ void k_fe_gather(long long nblk, const int *__restrict__ fec_off, const int *__restrict__ fec,
+__global__ void k_fe_gather(int npair, const int *__restrict__ pair_up, const int *__restrict__ pair_lo,
+                            const int *__restrict__ fec_off, const int *__restrict__ fec,
                             const unsigned char *__restrict__ pd_bits, const double *__restrict__ S, double *K) {
     constexpr int DD = D * D;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= nblk * D) return;
-    const long long q = t / D;
-    const int ii = int(t % D);
-    const int c0 = fec_off[q], c1 = fec_off[q + 1];
-    if (c0 == c1) return;
-    double *kb = K + (size_t)q * DD + ii * D;
+    if (t >= (long long)npair * D) return;
+    const int pr = int(t / D), ii = int(t % D);
+    const int bu = pair_up[pr], bl = pair_lo[pr];
+    const int c0 = fec_off[bu], c1 = fec_off[bu + 1];
+    auto pd = [&](int q) { return (pd_bits[q >> 3] >> (q & 7)) & 1; };
+    double *ku = K + (size_t)bu * DD + ii * D;
+    double *kl = bl >= 0 ? K + (size_t)bl * DD + ii * D : nullptr;
     // blocks without a PD part start from zero instead of reading K
-    const bool pd = (pd_bits[q >> 3] >> (q & 7)) & 1;
-    double acc[D];
+    const bool pu = pd(bu), pl = bl >= 0 && pd(bl);
+    double au[D], al[D];
 #pragma unroll
-    for (int jj = 0; jj < D; jj++) acc[jj] = pd ? kb[jj] : 0.0;
-    // contributions in chunks of 8: the indices, then the block rows, are loaded before any is
-    // added, so up to 8 gathers are in flight per thread; the sum stays in element order
-    for (int cb = c0; cb < c1; cb += 8) {
-        const int cnt = min(8, c1 - cb);
-        int f[8];
+    for (int jj = 0; jj < D; jj++) {
+        au[jj] = pu ? ku[jj] : 0.0;
+        al[jj] = pl ? kl[jj] : 0.0;
+    }
+    // contributions in chunks of 4: the indices, then the block rows and columns, are loaded
+    // before any is added; the sums stay in element order
+    for (int cb = c0; cb < c1; cb += 4) {
+        const int cnt = min(4, c1 - cb);
+        int f[4];
 #pragma unroll
-        for (int k = 0; k < 8; k++) f[k] = k < cnt ? fec[cb + k] : 0;
-        double v[8][D];
+        for (int k = 0; k < 4; k++) f[k] = k < cnt ? fec[cb + k] : 0;
+        double vu[4][D], vl[4][D];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < 4; k++) {
             if (k < cnt) {
                 const double *blk = S + (size_t)(f[k] >> 1) * DD;
                 const bool tr = f[k] & 1;
 #pragma unroll
-                for (int jj = 0; jj < D; jj++) v[k][jj] = blk[tr ? jj * D + ii : ii * D + jj];
+                for (int jj = 0; jj < D; jj++) {
+                    vu[k][jj] = blk[tr ? jj * D + ii : ii * D + jj];
+                    vl[k][jj] = blk[tr ? ii * D + jj : jj * D + ii];
+                }
             }
         }
 #pragma unroll
-        for (int k = 0; k < 8; k++)
+        for (int k = 0; k < 4; k++)
             if (k < cnt)
 #pragma unroll
-                for (int jj = 0; jj < D; jj++) acc[jj] = acc[jj] + v[k][jj];
+                for (int jj = 0; jj < D; jj++) {
+                    au[jj] = au[jj] + vu[k][jj];
+                    al[jj] = al[jj] + vl[k][jj];
+                }
     }
 #pragma unroll
-    for (int jj = 0; jj < D; jj++) kb[jj] = acc[jj];
+    for (int jj = 0; jj < D; jj++) ku[jj] = au[jj];
+    if (bl >= 0)
+#pragma unroll
+        for (int jj = 0; jj < D; jj++) kl[jj] = al[jj];
 }
 
 // ------------------------------------------------------------------------
@@ -997,6 +1043,7 @@ int dev_assemble_K(Ctx &c, std::string &err) {
     PDX_HIP(hipMallocAsync((void **)&status, sizeof(int), c.stream));
     PDX_HIP(hipMemsetAsync(status, 0, sizeof(int), c.stream));
     if (c.k_needs_fill && dev_fill(c, c.K, 0.0, (size_t)c.dd * c.nblk)) { err = "fill failed"; return PDX_ERR_HIP; }
+    hipStream_t fs = c.stream;  // stream of the family kernel
     if (c.nfam > 0) {
         FamArgs A;
         A.x = c.x; A.nv = c.nv; A.fam_node = c.fam_node; A.fam_off = c.fam_off; A.fam_mem = c.fam_mem;
@@ -1018,14 +1065,26 @@ int dev_assemble_K(Ctx &c, std::string &err) {
             hipFreeAsync(status, c.stream);
             return PDX_ERR_LIMIT;
         }
+        // with elements to do as well, the families run on the side stream beside the element
+        // kernel (they write disjoint arrays: K's PD blocks and fam_c, the element stage); the
+        // gather joins both
+        if (c.nfe > 0) {
+            if (!c.side) PDX_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+            if (!c.ev_fork) PDX_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+            if (!c.ev_join) PDX_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+            PDX_HIP(hipEventRecord(c.ev_fork, c.stream));
+            PDX_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+            fs = c.side;
+        }
         if (c.dim == 3) {
             PDX_HIP(hipFuncSetAttribute((const void *)k_family<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k_family<3><<<c.nfam, 64, lds, c.stream>>>(A);
+            k_family<3><<<c.nfam, 64, lds, fs>>>(A);
         } else {
             PDX_HIP(hipFuncSetAttribute((const void *)k_family<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k_family<2><<<c.nfam, 64, lds, c.stream>>>(A);
+            k_family<2><<<c.nfam, 64, lds, fs>>>(A);
         }
         PDX_HIP(hipGetLastError());
+        if (fs != c.stream) PDX_HIP(hipEventRecord(c.ev_join, fs));
     }
     if (c.nfe > 0) {
         const int ND = c.dim == 3 ? 24 : 8;
@@ -1061,8 +1120,10 @@ int dev_assemble_K(Ctx &c, std::string &err) {
                 A.list = c.fe_list_b; A.nlist = c.n_fe_b;
                 k_ke_simplex<3><<<(c.n_fe_b * 12 + 127) / 128, 128, 0, c.stream>>>(A);
             }
-            k_fe_gather<3><<<(unsigned)((c.nblk * 3 + 255) / 256), 256, 0, c.stream>>>(c.nblk, c.fec_off, c.fec, c.pd_bits,
-                                                                                      c.Ke_stage, c.K);
+            if (fs != c.stream) PDX_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+            if (c.n_fe_pair)
+                k_fe_gather<3><<<(unsigned)(((long long)c.n_fe_pair * 3 + 255) / 256), 256, 0, c.stream>>>(
+                    c.n_fe_pair, c.fe_pair_up, c.fe_pair_lo, c.fec_off, c.fec, c.pd_bits, c.Ke_stage, c.K);
         } else {
             if (c.n_fe_a) {
                 A.list = c.fe_list_a; A.nlist = c.n_fe_a;
@@ -1072,8 +1133,10 @@ int dev_assemble_K(Ctx &c, std::string &err) {
                 A.list = c.fe_list_b; A.nlist = c.n_fe_b;
                 k_ke_simplex<2><<<(c.n_fe_b * 6 + 127) / 128, 128, 0, c.stream>>>(A);
             }
-            k_fe_gather<2><<<(unsigned)((c.nblk * 2 + 255) / 256), 256, 0, c.stream>>>(c.nblk, c.fec_off, c.fec, c.pd_bits,
-                                                                                      c.Ke_stage, c.K);
+            if (fs != c.stream) PDX_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+            if (c.n_fe_pair)
+                k_fe_gather<2><<<(unsigned)(((long long)c.n_fe_pair * 2 + 255) / 256), 256, 0, c.stream>>>(
+                    c.n_fe_pair, c.fe_pair_up, c.fe_pair_lo, c.fec_off, c.fec, c.pd_bits, c.Ke_stage, c.K);
         }
         PDX_HIP(hipGetLastError());
     }

@@ -2,16 +2,16 @@
 // device.hpp) and the extern "C" entry points of libpdx.so (include/pdx.h).
 #include <hip/hip_runtime.h>
 
-#include <map>
-
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <unordered_map>
 
 #include "../../include/pdx.h"
 #include "device.hpp"
+#include "setup.hpp"
 
 namespace pdx {
 
@@ -104,12 +104,15 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
 
     // families + scatter map (block index of every member in the family node's row)
     std::vector<int> fam_blk(m.fam_mem.size());
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
     for (int f = 0; f < m.nfam; f++)
         for (int q = m.fam_off[f]; q < m.fam_off[f + 1]; q++) {
-            int b = find_block(m, m.fam_node[f], m.fam_mem[q]);
-            if (b < 0) { err = "Error: can not build matrix in CSR format (family member)"; return PDX_ERR_PATTERN; }
-            fam_blk[q] = b;
+            const int b = find_block(m, m.fam_node[f], m.fam_mem[q]);
+            if (b < 0) bad = 1;
+            else fam_blk[q] = b;
         }
+    if (bad) { err = "Error: can not build matrix in CSR format (family member)"; return PDX_ERR_PATTERN; }
     UP(c->fam_node, m.fam_node);
     UP(c->fam_off, m.fam_off);
     UP(c->fam_mem, m.fam_mem);
@@ -125,18 +128,34 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
     // transpose exactly and is read from the partner's storage
     {
         auto pd = [&](long long b) { return (pd_bits[(size_t)b >> 3] >> (b & 7)) & 1; };
-        std::vector<int> ksrc((size_t)std::max<long long>(c->nblk, 1)), own_slot, partner((size_t)c->nblk, -1);
-        own_slot.reserve((size_t)c->nblk);
-        for (int r = 0; r < m.nn; r++)
+        std::vector<int> ksrc((size_t)std::max<long long>(c->nblk, 1)), partner((size_t)c->nblk, -1);
+        std::vector<long long> nown((size_t)m.nn + 1, 0);
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int r = 0; r < m.nn; r++) {
+            long long own = 0;
             for (int b = m.adj_off[r]; b < m.adj_off[r + 1]; b++) {
                 const int col = m.adj[b];
                 if (col < r && !pd(b)) {
                     const int tb = find_block(m, col, r);
                     if (tb >= 0 && !pd(tb)) { partner[b] = tb; continue; }
                 }
-                ksrc[b] = (int)own_slot.size();
-                own_slot.push_back(b);
+                own++;
             }
+            nown[r + 1] = own;
+        }
+        for (int r = 0; r < m.nn; r++) nown[r + 1] += nown[r];
+        // own blocks numbered in block order
+        std::vector<int> own_slot((size_t)nown[m.nn]);
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int r = 0; r < m.nn; r++) {
+            long long s = nown[r];
+            for (int b = m.adj_off[r]; b < m.adj_off[r + 1]; b++)
+                if (partner[b] < 0) {
+                    ksrc[b] = int(s);
+                    own_slot[(size_t)s++] = b;
+                }
+        }
+#pragma omp parallel for schedule(static)
         for (long long b = 0; b < c->nblk; b++)
             if (partner[b] >= 0) ksrc[b] = ~ksrc[partner[b]];  // partners are upper blocks, always own
         c->n_own = (long long)own_slot.size();
@@ -175,16 +194,21 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
     {
         c->nfe = int(m.fe.size());
         std::vector<int> fe_blk((size_t)c->nfe * 64, -1), cnt(m.nn + 1, 0);
+        int bad_fe = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad_fe)
         for (int t = 0; t < c->nfe; t++) {
-            int e = m.fe[t];
-            for (int a = 0; a < m.nen[e]; a++) {
-                cnt[m.econ[(size_t)e * 8 + a] + 1]++;
+            const int e = m.fe[t];
+            for (int a = 0; a < m.nen[e]; a++)
                 for (int b = 0; b < m.nen[e]; b++) {
-                    int bi = find_block(m, m.econ[(size_t)e * 8 + a], m.econ[(size_t)e * 8 + b]);
-                    if (bi < 0) { err = "Error: can not build matrix in CSR format (element)"; return PDX_ERR_PATTERN; }
-                    fe_blk[(size_t)t * 64 + a * 8 + b] = bi;
+                    const int bi = find_block(m, m.econ[(size_t)e * 8 + a], m.econ[(size_t)e * 8 + b]);
+                    if (bi < 0) bad_fe = 1;
+                    else fe_blk[(size_t)t * 64 + a * 8 + b] = bi;
                 }
-            }
+        }
+        if (bad_fe) { err = "Error: can not build matrix in CSR format (element)"; return PDX_ERR_PATTERN; }
+        for (int t = 0; t < c->nfe; t++) {
+            const int e = m.fe[t];
+            for (int a = 0; a < m.nen[e]; a++) cnt[m.econ[(size_t)e * 8 + a] + 1]++;
         }
         for (int k = 0; k < m.nn; k++) cnt[k + 1] += cnt[k];
         std::vector<int> item(cnt[m.nn]), fill(cnt.begin(), cnt.end() - 1);
@@ -204,7 +228,10 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
         UP(c->nfe_item, item);
         // per K block, its FE contributions in element order, for the block-owned gather
         {
+            // the blocks a node's element rows reach lie in that node's row, so the nodes can be
+            // processed in parallel
             std::vector<int> fcnt((size_t)c->nblk + 1, 0);
+#pragma omp parallel for schedule(dynamic, 256)
             for (int k = 0; k < m.nn; k++)
                 for (int q = cnt[k]; q < cnt[k + 1]; q++) {
                     const int t = item[q] >> 3, a = item[q] & 7;
@@ -218,6 +245,7 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
                 return PDX_ERR_LIMIT;
             }
             std::vector<int> fec(std::max(fcnt[c->nblk], 1)), ffill(fcnt.begin(), fcnt.end() - 1);
+#pragma omp parallel for schedule(dynamic, 256)
             for (int k = 0; k < m.nn; k++)
                 for (int q = cnt[k]; q < cnt[k + 1]; q++) {
                     const int t = item[q] >> 3, a = item[q] & 7;
@@ -230,9 +258,39 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
                 }
             UP(c->fec_off, fcnt);
             UP(c->fec, fec);
+            // the gather's node pairs: every upper block with FE terms, with its lower partner
+            {
+                std::vector<int> npr((size_t)m.nn + 1, 0);
+#pragma omp parallel for schedule(dynamic, 256)
+                for (int r = 0; r < m.nn; r++) {
+                    int n = 0;
+                    for (int b = m.adj_off[r]; b < m.adj_off[r + 1]; b++) n += m.adj[b] >= r && fcnt[b + 1] > fcnt[b];
+                    npr[r + 1] = n;
+                }
+                for (int r = 0; r < m.nn; r++) npr[r + 1] += npr[r];
+                std::vector<int> pu((size_t)npr[m.nn]), pl((size_t)npr[m.nn]);
+                int bad_pair = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : bad_pair)
+                for (int r = 0; r < m.nn; r++) {
+                    int q = npr[r];
+                    for (int b = m.adj_off[r]; b < m.adj_off[r + 1]; b++) {
+                        const int col = m.adj[b];
+                        if (col < r || fcnt[b + 1] == fcnt[b]) continue;
+                        const int l = col > r ? find_block(m, col, r) : -1;
+                        if (col > r && l < 0) bad_pair = 1;
+                        pu[q] = b;
+                        pl[q++] = l;
+                    }
+                }
+                if (bad_pair) { err = "Error: can not build matrix in CSR format (element pair)"; return PDX_ERR_PATTERN; }
+                c->n_fe_pair = npr[m.nn];
+                UP(c->fe_pair_up, pu);
+                UP(c->fe_pair_lo, pl);
+            }
             // a block neither stored by a family nor gathered from elements keeps the fill's zero;
             // without such blocks the assembly needs no fill of K
             long long bare = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bare)
             for (long long q = 0; q < c->nblk; q++)
                 if (!((pd_bits[(size_t)q >> 3] >> (q & 7)) & 1) && fcnt[q + 1] == fcnt[q]) bare++;
             c->k_needs_fill = bare > 0;
@@ -265,20 +323,37 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
     // column pattern classes: rows whose block columns have the same offsets from the row index
     // (the stencil of a structured mesh, jittered or not) share one list of offsets
     {
-        std::map<std::vector<int>, int> ids;
-        std::vector<int> cls(m.nn), coff(1, 0), crel, key;
+        // a hash of each row's offsets in parallel, then classes numbered in order of first
+        // appearance, rows with equal hashes compared offset by offset
+        std::vector<uint64_t> hsh(m.nn);
+#pragma omp parallel for schedule(static)
         for (int r = 0; r < m.nn; r++) {
-            key.clear();
-            for (int b = m.adj_off[r]; b < m.adj_off[r + 1]; b++) key.push_back(m.adj[b] - r);
-            auto it = ids.find(key);
-            if (it == ids.end()) {
-                it = ids.emplace(key, int(ids.size())).first;
-                crel.insert(crel.end(), key.begin(), key.end());
+            uint64_t h = 1469598103934665603ull ^ uint64_t(m.adj_off[r + 1] - m.adj_off[r]);
+            for (int b = m.adj_off[r]; b < m.adj_off[r + 1]; b++) h = (h ^ uint64_t(uint32_t(m.adj[b] - r))) * 1099511628211ull;
+            hsh[r] = h;
+        }
+        std::unordered_map<uint64_t, std::vector<int>> ids;
+        std::vector<int> cls(m.nn), coff(1, 0), crel;
+        int ncls = 0;
+        for (int r = 0; r < m.nn; r++) {
+            const int b0 = m.adj_off[r], len = m.adj_off[r + 1] - b0;
+            std::vector<int> &cand = ids[hsh[r]];
+            int found = -1;
+            for (int id : cand) {
+                if (coff[id + 1] - coff[id] != len) continue;
+                bool same = true;
+                for (int j = 0; j < len && same; j++) same = crel[(size_t)coff[id] + j] == m.adj[b0 + j] - r;
+                if (same) { found = id; break; }
+            }
+            if (found < 0) {
+                found = ncls++;
+                cand.push_back(found);
+                for (int j = 0; j < len; j++) crel.push_back(m.adj[b0 + j] - r);
                 coff.push_back(int(crel.size()));
             }
-            cls[r] = it->second;
+            cls[r] = found;
         }
-        c->n_cls = int(ids.size());
+        c->n_cls = ncls;
         UP(c->cls, cls);
         UP(c->cls_off, coff);
         UP(c->cls_rel, crel);
@@ -360,6 +435,12 @@ void ctx_free(Ctx *c) {
     free_ilu0(*c);
     free_krylov_ws(*c);
     free_gnn_work(*c);
+    if (c->side) {
+        hipStreamSynchronize(c->side);
+        hipStreamDestroy(c->side);
+    }
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -388,6 +469,17 @@ static int explicit_driver(pdx_ctx *pc, int nsteps, int prec, int use_graph, std
 
 const char *pdx_last_error(void) { return g_err.c_str(); }
 const char *pdx_version(void) { return "pdx 0.1 (gfx950)"; }
+
+int pdx_set_setup_device(int device) {
+    if (device < -1) return fail(PDX_ERR_ARG, "pdx_set_setup_device: device must be -1 or a GPU index");
+    if (device >= 0) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || device >= n)
+            return fail(PDX_ERR_ARG, "pdx_set_setup_device: no GPU " + std::to_string(device));
+    }
+    set_setup_device(device);
+    return PDX_OK;
+}
 
 int pdx_model_create(const pdx_mesh_desc *desc, pdx_model **out) {
     if (!desc || !out) return fail(PDX_ERR_ARG, "null argument");

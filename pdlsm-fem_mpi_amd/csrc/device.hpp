@@ -57,6 +57,10 @@ constexpr ExplicitForm kExplicitForms[PDX_EXPLICIT_FORMS] = {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of the K assembly (the family kernel beside the element kernel) and its
+    // fork / join events; created on the first assembly
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     const Model *host = nullptr;  // not owned; copied fields below
     int dim = 3, dd = 9, nn = 0, ne = 0, neq = 0, nfam = 0, max_family = 0, npdbe = 0;
     long long nblk = 0, nnz = 0;
@@ -75,6 +79,10 @@ struct Ctx {
     int nfe = 0;
     int *fe_elem = nullptr, *fe_blk = nullptr, *nfe_off = nullptr, *nfe_item = nullptr;
     int *fec_off = nullptr, *fec = nullptr;  // per K block: FE contributions (staged block << 1 | transposed)
+    // node pairs of the gather: upper K blocks (r, c), r <= c, with FE terms, and their lower
+    // partners (c, r) (-1 on the diagonal)
+    int n_fe_pair = 0;
+    int *fe_pair_up = nullptr, *fe_pair_lo = nullptr;
     unsigned char *pd_bits = nullptr;        // per K block: 1 if a family kernel stores it
     bool k_needs_fill = true;                // some K block is written by neither assembly kernel
     int n_fe_a = 0, n_fe_b = 0;  // FE-list positions of hex8/quad4 (a) and tet4/tri3 (b) elements

@@ -36,6 +36,7 @@
 
 #include "../../include/pdx.h"
 #include "device.hpp"
+#include "setup.hpp"
 
 namespace pdx {
 
@@ -402,7 +403,6 @@ struct pdx_fbatch {  // handle behind the opaque C-ABI pointer
     long long n = 0, nf = 0, nbond = 0;
     int mf = 0;
     std::vector<long long> off;
-    std::vector<int> mem;
     double nlf = 0.33, lam = 0, mu = 0;
     double *x = nullptr, *vol = nullptr, *delta = nullptr, *A = nullptr, *LU = nullptr;
     long long *doff = nullptr;
@@ -490,81 +490,58 @@ static hipError_t fb_run(pdx_fbatch &b, int variant) {
 // Cell list of edge max(delta); the 27 (2D: 9) neighbouring cells are scanned
 // x-cell outermost, then y, then z, nodes in node order inside a cell (the
 // reference's block scan order).  Self first.
-static void fb_members(int dim, long long n, const double *x, const double *delta, std::vector<long long> &off,
-                       std::vector<int> &mem, int &mf) {
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, dmax = 0, dmin = 1e300;
+// Members of every node's family (self first, then the 3 x 3 x 3 cells around the node's cell,
+// x outer, z inner, each cell's nodes in node order; tiny < d2 and (d2 < delta_k^2 + tiny or
+// d2 < delta_j^2 + tiny), tiny = 1e-16 delta_min^2) on the GPU.  The cells are delta_max wide
+// and clamp to the grid; the cell lists are built here by a counting sort that keeps node order.
+// The members stay on the device (*dmem); off comes back to the host.
+static int fb_members(int device, int dim, long long n, const double *x, const double *delta,
+                      std::vector<long long> &off, int **dmem, int &mf, std::string &err) {
+    if (n > (long long)INT32_MAX) {
+        err = "pdx_fb_create: more than 2^31 nodes";
+        return PDX_ERR_ARG;
+    }
+    CellGrid g{};
+    double hi[3] = {-1e300, -1e300, -1e300}, dmax = 0, dmin = 1e300;
+    for (int i = 0; i < 3; i++) g.lo[i] = 1e300;
     for (long long k = 0; k < n; k++) {
         for (int i = 0; i < 3; i++) {
-            lo[i] = std::min(lo[i], x[k * 3 + i]);
+            g.lo[i] = std::min(g.lo[i], x[k * 3 + i]);
             hi[i] = std::max(hi[i], x[k * 3 + i]);
         }
         dmax = std::max(dmax, delta[k]);
         dmin = std::min(dmin, delta[k]);
     }
-    const double md2 = dmin * dmin * 1.0e-16;
-    const double cs = dmax > 0 ? dmax : 1.0;
-    int nb[3] = {1, 1, 1};
-    for (int i = 0; i < dim; i++) nb[i] = std::max(1, (int)std::floor((hi[i] - lo[i]) / cs) + 1);
-    auto cell_of = [&](const double *p, int ci[3]) {
-        for (int i = 0; i < 3; i++) {
-            ci[i] = i < dim ? std::min(nb[i] - 1, std::max(0, (int)std::floor((p[i] - lo[i]) / cs))) : 0;
-        }
-    };
-    const long long ncell = (long long)nb[0] * nb[1] * nb[2];
-    std::vector<long long> coff(ncell + 1, 0);
-    std::vector<long long> cnode(n);
-    for (long long k = 0; k < n; k++) {
-        int ci[3];
-        cell_of(&x[k * 3], ci);
-        coff[ci[0] + (long long)nb[0] * (ci[1] + (long long)nb[1] * ci[2]) + 1]++;
+    const double tiny = dmin * dmin * 1.0e-16;
+    g.cs = dmax > 0 ? dmax : 1.0;
+    g.dim = dim;
+    g.clamp = true;
+    for (int i = 0; i < 3; i++) g.nb[i] = i < dim ? std::max(1, (int)std::floor((hi[i] - g.lo[i]) / g.cs) + 1) : 1;
+    const long long ncell = (long long)g.nb[0] * g.nb[1] * g.nb[2];
+    if (ncell > 400000000LL) {
+        err = "pdx_fb_create: cell grid too large";
+        return PDX_ERR_ARG;
     }
-    for (long long c = 0; c < ncell; c++) coff[c + 1] += coff[c];
-    {
-        std::vector<long long> fill(coff.begin(), coff.end() - 1);
-        for (long long k = 0; k < n; k++) {
-            int ci[3];
-            cell_of(&x[k * 3], ci);
-            cnode[fill[ci[0] + (long long)nb[0] * (ci[1] + (long long)nb[1] * ci[2])]++] = k;
+    auto cell_of = [&](const double *p) {
+        long long c = 0, stride = 1;
+        for (int i = 0; i < dim; i++) {
+            const int ci = std::min(g.nb[i] - 1, std::max(0, (int)std::floor((p[i] - g.lo[i]) / g.cs)));
+            c += ci * stride;
+            stride *= g.nb[i];
         }
-    }
-    std::vector<int> cnt(n);
-    auto scan = [&](long long k, int *out) {
-        int c = 0;
-        if (out) out[c] = (int)k;
-        c++;
-        int ci[3];
-        cell_of(&x[k * 3], ci);
-        const double *p = &x[k * 3];
-        const int r2 = dim == 3 ? 1 : 0;
-        for (int xi = ci[0] - 1; xi <= ci[0] + 1; xi++)
-            for (int yi = ci[1] - 1; yi <= ci[1] + 1; yi++)
-                for (int zi = ci[2] - r2; zi <= ci[2] + r2; zi++) {
-                    if (xi < 0 || yi < 0 || zi < 0 || xi >= nb[0] || yi >= nb[1] || zi >= nb[2]) continue;
-                    const long long cc = xi + (long long)nb[0] * (yi + (long long)nb[1] * zi);
-                    for (long long t = coff[cc]; t < coff[cc + 1]; t++) {
-                        const long long j = cnode[t];
-                        const double *q = &x[j * 3];
-                        const double dist = (q[0] - p[0]) * (q[0] - p[0]) + (q[1] - p[1]) * (q[1] - p[1]) +
-                                            (q[2] - p[2]) * (q[2] - p[2]);
-                        if (dist > md2 && (dist < delta[k] * delta[k] + md2 || dist < delta[j] * delta[j] + md2)) {
-                            if (out) out[c] = (int)j;
-                            c++;
-                        }
-                    }
-                }
         return c;
     };
-#pragma omp parallel for schedule(dynamic, 4096)
-    for (long long k = 0; k < n; k++) cnt[k] = scan(k, nullptr);
-    off.assign(n + 1, 0);
-    mf = 0;
-    for (long long k = 0; k < n; k++) {
-        off[k + 1] = off[k] + cnt[k];
-        mf = std::max(mf, cnt[k]);
+    std::vector<long long> cell((size_t)n);
+    std::vector<int> coff((size_t)ncell + 1, 0), cnode((size_t)n);
+#pragma omp parallel for schedule(static)
+    for (long long k = 0; k < n; k++) cell[k] = cell_of(&x[k * 3]);
+    for (long long k = 0; k < n; k++) coff[cell[k] + 1]++;
+    for (long long c = 0; c < ncell; c++) coff[c + 1] += coff[c];
+    {
+        std::vector<int> fill(coff.begin(), coff.end() - 1);
+        for (long long k = 0; k < n; k++) cnode[fill[cell[k]]++] = int(k);
     }
-    mem.resize(off[n]);
-#pragma omp parallel for schedule(dynamic, 4096)
-    for (long long k = 0; k < n; k++) scan(k, mem.data() + off[k]);
+    return dev_cell_search(device, int(n), x, delta, coff, cnode, g, n, nullptr, tiny, off, nullptr, mf, dmem, err);
 }
 
 }  // namespace pdx
@@ -597,7 +574,7 @@ int pdx_fb_create(int device, int dim, long long n, const double *xyz, const dou
         b->nlf = nlf;
         b->lam = lambda;
         b->mu = mu;
-        fb_members(dim, n, xyz, delta, b->off, b->mem, b->mf);
+        if (int rc = fb_members(device, dim, n, xyz, delta, b->off, &b->dmem, b->mf, err)) return rc;
         b->nf = n;
         b->nbond = b->off[n];
         if (b->nbond > (long long)INT32_MAX) { err = "pdx_fb_create: more than 2^31 bonds"; return PDX_ERR_ARG; }
@@ -612,7 +589,6 @@ int pdx_fb_create(int device, int dim, long long n, const double *xyz, const dou
         PDX_HIP(hipMalloc((void **)&b->vol, n * 8));
         PDX_HIP(hipMalloc((void **)&b->delta, n * 8));
         PDX_HIP(hipMalloc((void **)&b->doff, (n + 1) * 8));
-        PDX_HIP(hipMalloc((void **)&b->dmem, std::max<long long>(b->nbond, 1) * 4));
         PDX_HIP(hipMalloc((void **)&b->A, n * P * P * 8));
         PDX_HIP(hipMalloc((void **)&b->LU, n * P * P * 8));
         PDX_HIP(hipMalloc((void **)&b->H, std::max<long long>(b->nbond, 1) * DD * 4));
@@ -622,7 +598,6 @@ int pdx_fb_create(int device, int dim, long long n, const double *xyz, const dou
         PDX_HIP(hipMemcpy(b->vol, vol, n * 8, hipMemcpyHostToDevice));
         PDX_HIP(hipMemcpy(b->delta, delta, n * 8, hipMemcpyHostToDevice));
         PDX_HIP(hipMemcpy(b->doff, b->off.data(), (n + 1) * 8, hipMemcpyHostToDevice));
-        PDX_HIP(hipMemcpy(b->dmem, b->mem.data(), b->nbond * 4, hipMemcpyHostToDevice));
         PDX_HIP(hipMemset(b->status, 0, 4));
         return PDX_OK;
     };
@@ -646,7 +621,15 @@ int pdx_fb_sizes(const pdx_fbatch *b, long long *n_fam, long long *n_bond, int *
 int pdx_fb_families(const pdx_fbatch *b, long long *off, int *mem) {
     if (!b) return set_error(PDX_ERR_ARG, "null argument");
     if (off) std::copy(b->off.begin(), b->off.end(), off);
-    if (mem) std::copy(b->mem.begin(), b->mem.end(), mem);
+    if (mem && b->nbond) {
+        std::string err;
+        auto run = [&]() -> int {
+            PDX_HIP(hipSetDevice(b->device));
+            PDX_HIP(hipMemcpy(mem, b->dmem, (size_t)b->nbond * 4, hipMemcpyDeviceToHost));
+            return PDX_OK;
+        };
+        if (int rc = run()) return set_error(rc, err);
+    }
     return PDX_OK;
 }
 

@@ -4,6 +4,7 @@
 // orientation, CSR column order), restated with cell lists built by counting
 // sort and OpenMP-parallel family / pattern loops.
 #include "model.hpp"
+#include "setup.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -355,6 +356,69 @@ int break_bonds_across(const Model &m, const double *crack, std::vector<int> &bo
     return nb;
 }
 
+// initialPDN_Fami's search on the host: per family its node, then the nodes of the 3 x 3 x 3 cells
+// around the node's cell (for_cells order, each cell in node order) within either horizon
+static void host_families(Model &m, const std::vector<double> &hz, double tiny) {
+    const int nfam = m.nfam;
+    std::vector<std::vector<int>> mem(nfam);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int f = 0; f < nfam; f++) {
+        int k = m.fam_node[f];
+        const double *cx = &m.x[size_t(k) * 3];
+        double dk = m.fam_delta[f];
+        std::vector<int> &out = mem[f];
+        out.push_back(k);
+        int lo[3], hi[3];
+        for (int i = 0; i < 3; i++) {
+            int c = int((cx[i] - m.lbc[i]) / m.bsize);
+            lo[i] = c - 1;
+            hi[i] = c + 1;
+        }
+        for_cells(m, lo, hi, [&](int b) {
+            for (int t = m.cell_node_off[b]; t < m.cell_node_off[b + 1]; t++) {
+                int j = m.cell_node[t];
+                const double *mx = &m.x[size_t(j) * 3];
+                double dist = ((mx[0] - cx[0]) * (mx[0] - cx[0]) + (mx[1] - cx[1]) * (mx[1] - cx[1]) +
+                               (mx[2] - cx[2]) * (mx[2] - cx[2]));
+                double dj = hz[j];
+                if (dist > tiny && (dist < (dk * dk + tiny) || dist < (dj * dj + tiny))) out.push_back(j);
+            }
+        });
+    }
+    m.fam_off.assign(nfam + 1, 0);
+    m.max_family = 0;
+    for (int f = 0; f < nfam; f++) {
+        m.fam_off[f + 1] = m.fam_off[f] + int(mem[f].size());
+        m.max_family = std::max(m.max_family, int(mem[f].size()));
+    }
+    m.fam_mem.resize(m.fam_off[nfam]);
+    for (int f = 0; f < nfam; f++) std::copy(mem[f].begin(), mem[f].end(), m.fam_mem.begin() + m.fam_off[f]);
+}
+
+// setCSRIndexes_gloStiffMat's node graph on the host: per node the sorted, unique union of its
+// family's members and the nodes of its FE elements (ne_off / ne_item: FE elements per node)
+static void host_node_graph(Model &m, const std::vector<int> &ne_off, const std::vector<int> &ne_item) {
+    const int nn = m.nn;
+    std::vector<std::vector<int>> nbr(nn);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int k = 0; k < nn; k++) {
+        std::vector<int> &v = nbr[k];
+        const int f = m.famid[k];
+        if (f >= 0) v.assign(m.fam_mem.begin() + m.fam_off[f], m.fam_mem.begin() + m.fam_off[f + 1]);
+        for (int t = ne_off[k]; t < ne_off[k + 1]; t++) {
+            const int e = ne_item[t];
+            for (int b = 0; b < m.nen[e]; b++) v.push_back(m.econ[size_t(e) * 8 + b]);
+        }
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    m.adj_off.assign(nn + 1, 0);
+    for (int k = 0; k < nn; k++) m.adj_off[k + 1] = m.adj_off[k] + int(nbr[k].size());
+    m.adj.resize(m.adj_off[nn]);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nn; k++) std::copy(nbr[k].begin(), nbr[k].end(), m.adj.begin() + m.adj_off[k]);
+}
+
 int Model::setup(std::string &err) {
     // ---- D matrix and Lame constants (pdsolve.cpp:37-94) ----
     std::fill(D, D + 36, 0.0);
@@ -501,40 +565,30 @@ int Model::setup(std::string &err) {
             fam_v[f] = fam_v[f] + ev[e] / nen[e];
         }
     const double tiny = minDelta * minDelta * 1.0e-16;
-    std::vector<std::vector<int>> mem(nfam);
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int f = 0; f < nfam; f++) {
-        int k = fam_node[f];
-        const double *cx = &x[size_t(k) * 3];
-        double dk = fam_delta[f];
-        std::vector<int> &out = mem[f];
-        out.push_back(k);
-        int lo[3], hi[3];
+    const int sdev = setup_device();
+    if (sdev >= 0) {
+        // the same search on the GPU (setup.hip), lists identical to the host's
+        CellGrid g{};
         for (int i = 0; i < 3; i++) {
-            int c = int((cx[i] - lbc[i]) / bsize);
-            lo[i] = c - 1;
-            hi[i] = c + 1;
+            g.lo[i] = lbc[i];
+            g.nb[i] = nb[i];
         }
-        for_cells(*this, lo, hi, [&](int b) {
-            for (int t = cell_node_off[b]; t < cell_node_off[b + 1]; t++) {
-                int j = cell_node[t];
-                const double *mx = &x[size_t(j) * 3];
-                double dist = ((mx[0] - cx[0]) * (mx[0] - cx[0]) + (mx[1] - cx[1]) * (mx[1] - cx[1]) +
-                               (mx[2] - cx[2]) * (mx[2] - cx[2]));
-                double dj = hz[j];
-                if (dist > tiny && (dist < (dk * dk + tiny) || dist < (dj * dj + tiny))) out.push_back(j);
-            }
-        });
+        g.cs = bsize;
+        g.dim = dim;
+        g.clamp = false;
+        std::vector<long long> off;
+        int rc = dev_cell_search(sdev, nn, x.data(), hz.data(), cell_node_off, cell_node, g, nfam, fam_node.data(),
+                                 tiny, off, &fam_mem, max_family, nullptr, err);
+        if (rc) return rc;
+        if (off[nfam] > (long long)INT32_MAX) {
+            err = "more than 2^31 family members";
+            return PDX_ERR_LIMIT;
+        }
+        fam_off.assign(nfam + 1, 0);
+        for (int f = 0; f <= nfam; f++) fam_off[f] = int(off[f]);
+    } else {
+        host_families(*this, hz, tiny);
     }
-    fam_off.assign(nfam + 1, 0);
-    max_family = 0;
-    for (int f = 0; f < nfam; f++) {
-        fam_off[f + 1] = fam_off[f] + int(mem[f].size());
-        max_family = std::max(max_family, int(mem[f].size()));
-    }
-    fam_mem.resize(fam_off[nfam]);
-    for (int f = 0; f < nfam; f++) std::copy(mem[f].begin(), mem[f].end(), fam_mem.begin() + fam_off[f]);
-    mem.clear();
     bond.assign(fam_mem.size(), 1);
 
     // ---- bond states across pre-existing cracks (pdsolve.cpp:1823-1915, :8910-9004) ----
@@ -554,24 +608,11 @@ int Model::setup(std::string &err) {
             for (int e : fe)
                 for (int a = 0; a < nen[e]; a++) ne_item[fill[econ[size_t(e) * 8 + a]]++] = e;
         }
-        std::vector<std::vector<int>> nbr(nn);
-#pragma omp parallel for schedule(dynamic, 256)
-        for (int k = 0; k < nn; k++) {
-            std::vector<int> &v = nbr[k];
-            const int f = famid[k];
-            if (f >= 0) v.assign(fam_mem.begin() + fam_off[f], fam_mem.begin() + fam_off[f + 1]);
-            for (int t = ne_off[k]; t < ne_off[k + 1]; t++) {
-                const int e = ne_item[t];
-                for (int b = 0; b < nen[e]; b++) v.push_back(econ[size_t(e) * 8 + b]);
-            }
-            std::sort(v.begin(), v.end());
-            v.erase(std::unique(v.begin(), v.end()), v.end());
+        if (sdev >= 0) {
+            if (int rc = dev_node_graph(sdev, *this, ne_off, ne_item, adj_off, adj, err)) return rc;
+        } else {
+            host_node_graph(*this, ne_off, ne_item);
         }
-        adj_off.assign(nn + 1, 0);
-        for (int k = 0; k < nn; k++) adj_off[k + 1] = adj_off[k] + int(nbr[k].size());
-        adj.resize(adj_off[nn]);
-#pragma omp parallel for schedule(static)
-        for (int k = 0; k < nn; k++) std::copy(nbr[k].begin(), nbr[k].end(), adj.begin() + adj_off[k]);
     }
     // free-DOF count per node -> row length of every row of that node
     std::vector<int> nfree(nn, 0);

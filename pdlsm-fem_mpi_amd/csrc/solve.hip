@@ -599,9 +599,11 @@ static int capture_explicit_graph(Ctx &c, int prec, std::string &err) {
 int dev_explicit_steps(Ctx &c, int nsteps, int prec, int use_graph, bool lumped,
                        const std::function<int(std::string &)> &after_step, std::string &err) {
     const bool startup = c.steps_done < 0;
+    if (nsteps == 0 && !startup) return PDX_OK;  // nothing to run: the state is read as it is
     int e = explicit_prepare(c, prec, lumped, err);
     if (e) return e;
     if (startup && after_step && (e = after_step(err))) return e;
+    if (nsteps == 0) return PDX_OK;  // the start-up step only
     if ((e = explicit_tune_if_needed(c, prec, err))) return e;
     const int period = kExplicitPeriod;
     const bool replay = lumped && !after_step && use_graph && nsteps >= period;
