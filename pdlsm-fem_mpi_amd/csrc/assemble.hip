@@ -559,15 +559,12 @@ __device__ inline void btdb_row2(const double (*g)[2], int nnod, int la, int ii,
 // Jacobian, its partial-pivot LU and dN/dX of all nodes into LDS, then one
 // thread per (element, Ke row) accumulates B^T D B w detJ over the Gauss
 // points in the reference loop order.
-// Staged element matrices keep the upper triangle of node blocks only (Ke is symmetric): the
-// d x d block of node pair a <= b of an element with at most NB = ND / d nodes is contiguous
-// at (pair * d + i) * d + j, pair = a NB - a (a - 1) / 2 + b - a, NP = NB (NB + 1) / 2 blocks
-// per element.  The block-owned gather reads a lower block as the transpose of its partner.
-__host__ __device__ constexpr int ke_pairs(int D, int ND) { return (ND / D) * (ND / D + 1) / 2; }
+// Staged element matrices keep the upper triangle of node blocks only (Ke is symmetric): node
+// pair a <= b of an element with at most NB nodes is pair = a NB - a (a - 1) / 2 + b - a of
+// NP = NB (NB + 1) / 2, and its d x d block goes to the staging slot the context assigned it:
+// the slots of one K node pair (r, c) follow each other in element order, every block in the
+// (r, c) orientation, so the gather reads each pair's run front to back.
 __host__ __device__ inline int ke_pair(int NB, int a, int b) { return a * NB - a * (a - 1) / 2 + (b - a); }
-__device__ inline size_t ke_up(int D, int ND, int row, int col) {
-    return (size_t)ke_pair(ND / D, row / D, col / D) * (D * D) + (row % D) * D + col % D;
-}
 
 struct KeArgs {
     const double *x, *Dm;
@@ -575,9 +572,19 @@ struct KeArgs {
     const int *econ, *nen, *fe_elem;
     const int *list;  // FE-list positions of the elements of this type
     int nlist;
-    double *S;
+    double *S;        // staging: one d x d block per slot
+    const int *slot;  // (slot << 1 | transposed) of every (FE-list position, upper node pair)
     int *status;
 };
+
+// entry (i, j) of node-pair block (a, b), a <= b, of FE element t into its staging slot, in the
+// orientation of the K block the slot feeds
+template <int D>
+__device__ inline void ke_put(const KeArgs &A, int t, int a, int b, int i, int j, double v) {
+    constexpr int NB = D == 3 ? 8 : 4, NP = NB * (NB + 1) / 2;
+    const int code = A.slot[(size_t)t * NP + ke_pair(NB, a, b)];
+    A.S[(size_t)(code >> 1) * (D * D) + ((code & 1) ? j * D + i : i * D + j)] = v;
+}
 
 // dN/dX of the 8 nodes and detJ at Gauss point g (pdfemEleBrick8N::eleStiffMatFEM's order: r
 // outer, q, p inner) of the element with node coordinates xn, into g8o / *wdet; the parent
@@ -647,8 +654,7 @@ __global__ void __launch_bounds__(HEX_EPB * 24) k_ke_hex8(KeArgs A) {
     double Krow[24];
     for (int c = 0; c < 24; c++) Krow[c] = 0.0;
     for (int g = 0; g < 8; g++) btdb_row3(sg[el][g], 8, la, ii, A.Dm, sw[el][g], Krow);
-    double *out = A.S + (size_t)A.list[idx] * (ke_pairs(3, 24) * 9);
-    for (int c = 3 * la; c < 24; c++) out[ke_up(3, 24, row, c)] = Krow[c];
+    for (int c = 3 * la; c < 24; c++) ke_put<3>(A, A.list[idx], la, c / 3, ii, c % 3, Krow[c]);
 }
 
 // Hex8 with an isotropic D (the reference's D, pdsolve.cpp:37-94, always is): per Gauss point
@@ -656,18 +662,16 @@ __global__ void __launch_bounds__(HEX_EPB * 24) k_ke_hex8(KeArgs A) {
 // so one thread per (element, node pair a <= b) accumulates its 3 x 3 block over the 8 Gauss
 // points (27 FMA each instead of the 324 of three full B^T D B rows) and writes the block and,
 // Ke being symmetric, its transpose.
-constexpr int HEXI_EPB = 16;  // 16 elements: 128 set-up threads, 16 x 36 = 576 block threads
+constexpr int HEXI_EPB = 4;  // 4 elements: 32 set-up threads, 4 x 36 = 144 block threads
 // The element's node coordinates are loaded into LDS by the whole block first (one coordinate
-// per thread, so the gathers are in flight together), the Gauss-point set-up reads them there,
-// and the 36 blocks of each element are staged back in LDS (over the gradients) and written
-// with 16-byte stores over the element's contiguous 2,592 bytes.
+// per thread, so the gathers are in flight together) and the Gauss-point set-up reads them there.
+// Each block goes to its staging slot, transposed when the slot's K block is the other
+// orientation.  Four elements per 144-thread block: the set-up runs on 32 threads while the
+// rest wait, and small blocks let more of them share a CU.
 __global__ void __launch_bounds__(HEXI_EPB * 36) k_ke_hex8_iso(KeArgs A) {
-    constexpr int KE = 36 * 9;  // doubles of one element's upper blocks
     __shared__ double sx[HEXI_EPB][8][3];
     __shared__ double sw[HEXI_EPB][8];
-    __shared__ __align__(16) double sbuf[HEXI_EPB * KE];  // gradients [el][g][node][3], then blocks
-    static_assert(HEXI_EPB * 8 * 8 * 3 <= HEXI_EPB * KE, "gradients fit the block buffer");
-    double(*sg)[8][8][3] = reinterpret_cast<double(*)[8][8][3]>(sbuf);
+    __shared__ double sg[HEXI_EPB][8][8][3];
     const int t = threadIdx.x;
     const int e0 = blockIdx.x * HEXI_EPB;
     const int nel = min(HEXI_EPB, A.nlist - e0);
@@ -710,17 +714,19 @@ __global__ void __launch_bounds__(HEXI_EPB * 36) k_ke_hex8_iso(KeArgs A) {
             for (int i = 0; i < 3; i++) k[i * 4] += mu * dot;
         }
     }
-    __syncthreads();  // every gradient read
-    // pair index t % 36 enumerates a <= b in ke_pair order: one contiguous block per thread
     if (act) {
+        // pair index t % 36 enumerates a <= b in ke_pair order
+        const int code = A.slot[(size_t)A.list[e0 + el] * 36 + t % 36];
+        double *out = A.S + (size_t)(code >> 1) * 9;
+        if (code & 1) {
 #pragma unroll
-        for (int e = 0; e < 9; e++) sbuf[el * KE + (t % 36) * 9 + e] = k[e];
-    }
-    __syncthreads();
-    for (int j = t; j < nel * (KE / 2); j += blockDim.x) {
-        const int ej = j / (KE / 2), ch = j % (KE / 2);
-        const double2 v = *reinterpret_cast<const double2 *>(&sbuf[ej * KE + ch * 2]);
-        *reinterpret_cast<double2 *>(A.S + (size_t)A.list[e0 + ej] * KE + ch * 2) = v;
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) out[j * 3 + i] = k[i * 3 + j];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 9; e++) out[e] = k[e];
+        }
     }
 }
 
@@ -766,8 +772,7 @@ __global__ void __launch_bounds__(QUAD_EPB * 8) k_ke_quad4(KeArgs A) {
     double Krow[8];
     for (int c = 0; c < 8; c++) Krow[c] = 0.0;
     for (int g = 0; g < 4; g++) btdb_row2(sg[el][g], 4, la, ii, A.Dm, sw[el][g], Krow);
-    double *out = A.S + (size_t)A.list[idx] * (ke_pairs(2, 8) * 4);
-    for (int c = 2 * la; c < 8; c++) out[ke_up(2, 8, row, c)] = Krow[c];
+    for (int c = 2 * la; c < 8; c++) ke_put<2>(A, A.list[idx], la, c / 2, ii, c % 2, Krow[c]);
 }
 
 // tri3 / tet4: constant B, Ke = (Bt D B) V (pdfemEleTri3N.cpp:33-47, :185-210;
@@ -819,26 +824,25 @@ __global__ void k_ke_simplex(KeArgs A) {
         for (int i = 0; i < 3; i++) { g[i][0] = bb[i] / V2; g[i][1] = cc[i] / V2; }
         btdb_row2(g, 3, la, ii, A.Dm, 1.0, Krow);
     }
-    double *out = A.S + (size_t)A.list[idx] * (ke_pairs(D, ND) * D * D);
-    for (int c = D * la; c < NEN * D; c++) out[ke_up(D, ND, row, c)] = Krow[c] * V;
+    for (int c = D * la; c < NEN * D; c++) ke_put<D>(A, A.list[idx], la, c / D, ii, c % D, Krow[c] * V);
 }
 
 // Block-owned add of the staged element blocks, one node pair at a time: one thread per (pair,
 // row) takes an upper block (r, c), r <= c, and its lower partner (c, r), starts each from its PD
-// part, adds the element terms in element order (the reference's summation order of FE terms
-// into K) and writes both rows once.  The contributions of a pair are the staged blocks of the
-// elements that hold both nodes, listed as (staged block index << 1 | transposed relative to
-// (r, c)); the lower block sums the same blocks transposed, so every staged block is read once.
+// part and adds the element terms in element order (the reference's summation order of FE terms
+// into K), then writes both rows once.  The element blocks of a pair sit consecutively in the
+// staging, in the (r, c) orientation, so a pair's run is read once, front to back: row ii for
+// the upper block, column ii for the lower one.
 template <int D>
 __global__ void k_fe_gather(int npair, const int *__restrict__ pair_up, const int *__restrict__ pair_lo,
-                            const int *__restrict__ fec_off, const int *__restrict__ fec,
-                            const unsigned char *__restrict__ pd_bits, const double *__restrict__ S, double *K) {
+                            const int *__restrict__ pair_off, const unsigned char *__restrict__ pd_bits,
+                            const double *__restrict__ S, double *K) {
     constexpr int DD = D * D;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= (long long)npair * D) return;
     const int pr = int(t / D), ii = int(t % D);
     const int bu = pair_up[pr], bl = pair_lo[pr];
-    const int c0 = fec_off[bu], c1 = fec_off[bu + 1];
+    const int c0 = pair_off[pr], c1 = pair_off[pr + 1];
     auto pd = [&](int q) { return (pd_bits[q >> 3] >> (q & 7)) & 1; };
     double *ku = K + (size_t)bu * DD + ii * D;
     double *kl = bl >= 0 ? K + (size_t)bl * DD + ii * D : nullptr;
@@ -850,23 +854,19 @@ __global__ void k_fe_gather(int npair, const int *__restrict__ pair_up, const in
         au[jj] = pu ? ku[jj] : 0.0;
         al[jj] = pl ? kl[jj] : 0.0;
     }
-    // contributions in chunks of 4: the indices, then the block rows and columns, are loaded
-    // before any is added; the sums stay in element order
+    // in chunks of 4 blocks: the rows and columns are loaded before any is added; the sums stay
+    // in element order
     for (int cb = c0; cb < c1; cb += 4) {
         const int cnt = min(4, c1 - cb);
-        int f[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) f[k] = k < cnt ? fec[cb + k] : 0;
         double vu[4][D], vl[4][D];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (k < cnt) {
-                const double *blk = S + (size_t)(f[k] >> 1) * DD;
-                const bool tr = f[k] & 1;
+                const double *blk = S + (size_t)(cb + k) * DD;
 #pragma unroll
                 for (int jj = 0; jj < D; jj++) {
-                    vu[k][jj] = blk[tr ? jj * D + ii : ii * D + jj];
-                    vl[k][jj] = blk[tr ? ii * D + jj : jj * D + ii];
+                    vu[k][jj] = blk[ii * D + jj];
+                    vl[k][jj] = blk[jj * D + ii];
                 }
             }
         }
@@ -1038,9 +1038,11 @@ static size_t family_lds_bytes(int D, int mf, int maxpb) {
            (size_t)P * P * sizeof(double) + (P + 2) * sizeof(int);
 }
 
-int dev_assemble_K(Ctx &c, std::string &err) {
-    int *status = nullptr;
-    PDX_HIP(hipMallocAsync((void **)&status, sizeof(int), c.stream));
+// Enqueue one assembly of K on the context's stream (the status word is read by
+// dev_assemble_K_finish).
+int dev_assemble_K_launch(Ctx &c, std::string &err) {
+    if (!c.asm_status) PDX_HIP(hipMalloc((void **)&c.asm_status, sizeof(int)));
+    int *status = c.asm_status;
     PDX_HIP(hipMemsetAsync(status, 0, sizeof(int), c.stream));
     if (c.k_needs_fill && dev_fill(c, c.K, 0.0, (size_t)c.dd * c.nblk)) { err = "fill failed"; return PDX_ERR_HIP; }
     hipStream_t fs = c.stream;  // stream of the family kernel
@@ -1062,7 +1064,6 @@ int dev_assemble_K(Ctx &c, std::string &err) {
         size_t lds = family_lds_bytes(c.dim, A.mf, A.maxpb);
         if (lds > 160 * 1024) {
             err = "family too large for LDS (" + std::to_string(c.max_family) + " members)";
-            hipFreeAsync(status, c.stream);
             return PDX_ERR_LIMIT;
         }
         // with elements to do as well, the families run on the side stream beside the element
@@ -1087,11 +1088,11 @@ int dev_assemble_K(Ctx &c, std::string &err) {
         if (fs != c.stream) PDX_HIP(hipEventRecord(c.ev_join, fs));
     }
     if (c.nfe > 0) {
-        const int ND = c.dim == 3 ? 24 : 8;
         if (!c.Ke_stage)
-            PDX_HIP(hipMalloc((void **)&c.Ke_stage, (size_t)c.nfe * ke_pairs(c.dim, ND) * c.dd * sizeof(double)));
+            PDX_HIP(hipMalloc((void **)&c.Ke_stage, (size_t)std::max<long long>(c.n_stage, 1) * c.dd * sizeof(double)));
         KeArgs A;
         A.x = c.x; A.Dm = c.Dm; A.econ = c.econ; A.nen = c.nen; A.fe_elem = c.fe_elem; A.S = c.Ke_stage;
+        A.slot = c.fe_slot;
         A.status = status;
         // isotropic 3D D (lam + 2 mu on the normal diagonal, lam off it, mu on the shear diagonal)
         bool iso = false;
@@ -1123,7 +1124,7 @@ int dev_assemble_K(Ctx &c, std::string &err) {
             if (fs != c.stream) PDX_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
             if (c.n_fe_pair)
                 k_fe_gather<3><<<(unsigned)(((long long)c.n_fe_pair * 3 + 255) / 256), 256, 0, c.stream>>>(
-                    c.n_fe_pair, c.fe_pair_up, c.fe_pair_lo, c.fec_off, c.fec, c.pd_bits, c.Ke_stage, c.K);
+                    c.n_fe_pair, c.fe_pair_up, c.fe_pair_lo, c.pair_off, c.pd_bits, c.Ke_stage, c.K);
         } else {
             if (c.n_fe_a) {
                 A.list = c.fe_list_a; A.nlist = c.n_fe_a;
@@ -1136,14 +1137,18 @@ int dev_assemble_K(Ctx &c, std::string &err) {
             if (fs != c.stream) PDX_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
             if (c.n_fe_pair)
                 k_fe_gather<2><<<(unsigned)(((long long)c.n_fe_pair * 2 + 255) / 256), 256, 0, c.stream>>>(
-                    c.n_fe_pair, c.fe_pair_up, c.fe_pair_lo, c.fec_off, c.fec, c.pd_bits, c.Ke_stage, c.K);
+                    c.n_fe_pair, c.fe_pair_up, c.fe_pair_lo, c.pair_off, c.pd_bits, c.Ke_stage, c.K);
         }
         PDX_HIP(hipGetLastError());
     }
+    return PDX_OK;
+}
+
+// Wait for the assembly and check what its kernels reported.
+int dev_assemble_K_finish(Ctx &c, std::string &err) {
     int hs = 0;
-    PDX_HIP(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PDX_HIP(hipMemcpyAsync(&hs, c.asm_status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     PDX_HIP(hipStreamSynchronize(c.stream));
-    PDX_HIP(hipFree(status));
     if (hs & 1) { err = "singular shape tensor (family with too few non-collinear members)"; return PDX_ERR_SINGULAR; }
     if (hs & 2) { err = "singular element Jacobian"; return PDX_ERR_SINGULAR; }
     if (hs & 4) { err = "Area is zero (PD boundary face)"; return PDX_ERR_MESH; }
@@ -1152,6 +1157,11 @@ int dev_assemble_K(Ctx &c, std::string &err) {
     c.K_gen++;
     c.K32_valid = false;
     return PDX_OK;
+}
+
+int dev_assemble_K(Ctx &c, std::string &err) {
+    if (int rc = dev_assemble_K_launch(c, err)) return rc;
+    return dev_assemble_K_finish(c, err);
 }
 
 int dev_lumped_mass(Ctx &c, std::string &err) {

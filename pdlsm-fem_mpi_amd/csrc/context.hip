@@ -194,18 +194,20 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
     {
         c->nfe = int(m.fe.size());
         std::vector<int> fe_blk((size_t)c->nfe * 64, -1), cnt(m.nn + 1, 0);
-        int bad_fe = 0;
-#pragma omp parallel for schedule(static) reduction(| : bad_fe)
+        int bad_fe = 0, rep_fe = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad_fe, rep_fe)
         for (int t = 0; t < c->nfe; t++) {
             const int e = m.fe[t];
             for (int a = 0; a < m.nen[e]; a++)
                 for (int b = 0; b < m.nen[e]; b++) {
+                    if (a != b && m.econ[(size_t)e * 8 + a] == m.econ[(size_t)e * 8 + b]) rep_fe = 1;
                     const int bi = find_block(m, m.econ[(size_t)e * 8 + a], m.econ[(size_t)e * 8 + b]);
                     if (bi < 0) bad_fe = 1;
                     else fe_blk[(size_t)t * 64 + a * 8 + b] = bi;
                 }
         }
         if (bad_fe) { err = "Error: can not build matrix in CSR format (element)"; return PDX_ERR_PATTERN; }
+        if (rep_fe) { err = "an FE element lists one node twice"; return PDX_ERR_MESH; }
         for (int t = 0; t < c->nfe; t++) {
             const int e = m.fe[t];
             for (int a = 0; a < m.nen[e]; a++) cnt[m.econ[(size_t)e * 8 + a] + 1]++;
@@ -256,9 +258,9 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
                         fec[ffill[fe_blk[(size_t)t * 64 + a * 8 + b]]++] = ((t * NP + pair) << 1) | (a > b ? 1 : 0);
                     }
                 }
-            UP(c->fec_off, fcnt);
-            UP(c->fec, fec);
-            // the gather's node pairs: every upper block with FE terms, with its lower partner
+            // the gather's node pairs (every upper block with FE terms, with its lower partner) and
+            // the staging slots: the element blocks of one pair are staged consecutively, in
+            // element order, each in the pair's (r, c) orientation
             {
                 std::vector<int> npr((size_t)m.nn + 1, 0);
 #pragma omp parallel for schedule(dynamic, 256)
@@ -283,9 +285,26 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
                     }
                 }
                 if (bad_pair) { err = "Error: can not build matrix in CSR format (element pair)"; return PDX_ERR_PATTERN; }
-                c->n_fe_pair = npr[m.nn];
+                const int ntask = npr[m.nn];
+                std::vector<long long> poff((size_t)ntask + 1, 0);
+                for (int q = 0; q < ntask; q++) poff[q + 1] = poff[q] + (fcnt[pu[q] + 1] - fcnt[pu[q]]);
+                if (poff[ntask] >= (1LL << 30)) {
+                    err = "too many FE element blocks for 32-bit staging slots";
+                    return PDX_ERR_LIMIT;
+                }
+                std::vector<int> slot((size_t)c->nfe * NP, -1), po((size_t)ntask + 1);
+                for (int q = 0; q <= ntask; q++) po[q] = int(poff[q]);
+                // every element pair belongs to one upper block (the element's nodes are distinct)
+#pragma omp parallel for schedule(dynamic, 1024)
+                for (int q = 0; q < ntask; q++)
+                    for (int k = fcnt[pu[q]]; k < fcnt[pu[q] + 1]; k++)
+                        slot[(size_t)(fec[k] >> 1)] = ((po[q] + (k - fcnt[pu[q]])) << 1) | (fec[k] & 1);
+                c->n_fe_pair = ntask;
+                c->n_stage = poff[ntask];
                 UP(c->fe_pair_up, pu);
                 UP(c->fe_pair_lo, pl);
+                UP(c->pair_off, po);
+                UP(c->fe_slot, slot);
             }
             // a block neither stored by a family nor gathered from elements keeps the fill's zero;
             // without such blocks the assembly needs no fill of K
@@ -427,6 +446,7 @@ void ctx_free(Ctx *c) {
     if (c->Ke_stage) hipFree(c->Ke_stage);
     if (c->red) hipFree(c->red);
     if (c->learned_Minv) hipFree(c->learned_Minv);
+    if (c->gnn_keff) hipFree(c->gnn_keff);
     if (c->fam_c) hipFree(c->fam_c);
     if (c->Kown32) hipFree(c->Kown32);
     if (c->Kown64) hipFree(c->Kown64);
@@ -441,6 +461,7 @@ void ctx_free(Ctx *c) {
     }
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->asm_status) hipFree(c->asm_status);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -709,9 +730,11 @@ int pdx_assemble_timed(pdx_ctx *pc, double *ms) {
     std::string err;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail(PDX_ERR_HIP, "event");
+    // the events bracket the kernels; the status check after them is not timed
     hipEventRecord(e0, c.stream);
-    int rc = dev_assemble_K(c, err);
+    int rc = dev_assemble_K_launch(c, err);
     hipEventRecord(e1, c.stream);
+    if (!rc) rc = dev_assemble_K_finish(c, err);
     hipEventSynchronize(e1);
     float t = 0;
     hipEventElapsedTime(&t, e0, e1);

@@ -60,6 +60,7 @@ struct Ctx {
     // second stream of the K assembly (the family kernel beside the element kernel) and its
     // fork / join events; created on the first assembly
     hipStream_t side = nullptr;
+    int *asm_status = nullptr;  // status word of the assembly kernels
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     const Model *host = nullptr;  // not owned; copied fields below
     int dim = 3, dd = 9, nn = 0, ne = 0, neq = 0, nfam = 0, max_family = 0, npdbe = 0;
@@ -78,7 +79,11 @@ struct Ctx {
     int max_pdbe_per_family = 0;  // most PD-boundary entities incident to one family
     int nfe = 0;
     int *fe_elem = nullptr, *fe_blk = nullptr, *nfe_off = nullptr, *nfe_item = nullptr;
-    int *fec_off = nullptr, *fec = nullptr;  // per K block: FE contributions (staged block << 1 | transposed)
+    // element block staging: fe_slot[t * NP + pair] = (staging slot << 1 | transposed) of the
+    // upper node-pair block `pair` of FE element t (-1 for pairs the element does not have); the
+    // slots of one K node pair are consecutive, in element order (pair_off)
+    int *fe_slot = nullptr, *pair_off = nullptr;
+    long long n_stage = 0;
     // node pairs of the gather: upper K blocks (r, c), r <= c, with FE terms, and their lower
     // partners (c, r) (-1 on the diagonal)
     int n_fe_pair = 0;
@@ -163,6 +168,14 @@ struct Ctx {
     float *solve_b32 = nullptr;  // FP32 right-hand side of the refined inner solves [nn*d]
     double *learned_Minv = nullptr;  // [nn*dd] node-block smoother predicted by the graph network
     long long learned_gen = -1;      // K_gen the smoother was predicted for
+    const double *learned_for = nullptr;  // the operator it was predicted for (K, or a Newmark K_eff)
+    // operator the network's inputs, training probes and pdx_gnn_apply use: 0 K, 1 the Newmark
+    // operator M + beta dt^2 K (formed in gnn_keff for the current K); the network a learned
+    // Newmark solve predicts its smoother with
+    int gnn_op = 0;
+    double *gnn_keff = nullptr;
+    long long gnn_keff_gen = -1;
+    const void *gnn_net = nullptr;
     void *gnn_work = nullptr;        // graph-network inputs and activations (learned.hip)
     double *stress_stage = nullptr;  // [nfe*8*6 + nfam*6] element corner stresses, family stresses
     int *stress_famid = nullptr;     // [nn] family id of every node (-1: none)
@@ -186,6 +199,8 @@ void ctx_free(Ctx *c);
 
 // assembly (assemble.hip)
 int dev_assemble_K(Ctx &c, std::string &err);
+int dev_assemble_K_launch(Ctx &c, std::string &err);  // enqueue only
+int dev_assemble_K_finish(Ctx &c, std::string &err);  // wait, check the status word
 int dev_lumped_mass(Ctx &c, std::string &err);
 int dev_nbc_unit(Ctx &c, std::string &err);
 int dev_mass_on_K(Ctx &c, std::string &err);
@@ -220,6 +235,11 @@ double ilu0_setup_ms(const void *h);
 void free_ilu0(Ctx &c);
 void free_krylov_ws(Ctx &c);
 void free_gnn_work(Ctx &c);
+// smoother of the context's attached network for operator A (learned.hip)
+int dev_learned_smoother(Ctx &c, const double *A, std::string &err);
+void forget_learned_operator(Ctx &c, const double *A);
+// K_eff = beta dt^2 K + M (newmark.hip)
+int dev_form_keff(Ctx &c, double *Keff, std::string &err);
 // nodal stresses of u_dev; check: read the singular-Jacobian flag (synchronises the stream)
 int dev_node_stresses(Ctx &c, const double *u_dev, double *sigma_dev, std::string &err, bool check = true);
 // out[t] = free_src[t] at free DOFs, presc_src[t] at prescribed ones (full DOF vectors, solve.hip)

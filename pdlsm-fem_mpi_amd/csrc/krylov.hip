@@ -1322,7 +1322,7 @@ static int build_precond(Ctx &c, const double *K64, int kind, V *Minv, int *stat
         PDX_HIP(hipMemcpyAsync(Minv, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice, c.stream));
         PDX_HIP(hipStreamSynchronize(c.stream));
     } else if (kind == PDX_PREC_LEARNED) {
-        if (!c.learned_Minv || c.learned_gen != c.K_gen) {
+        if (!c.learned_Minv || c.learned_gen != c.K_gen || c.learned_for != K64) {
             err = "no learned smoother for the current operator (run pdx_gnn_apply first)";
             return PDX_ERR_ARG;
         }
@@ -1551,7 +1551,7 @@ int dev_solve_mat(Ctx &c, const double *K64, const float *K32, const pdx_solve_o
         st.attempts = attempt;
         // a learned preconditioner with no smoother predicted for this operator also falls back
         const bool learned_missing = rc == PDX_ERR_ARG && a.precond == PDX_PREC_LEARNED &&
-                                     (!c.learned_Minv || c.learned_gen != c.K_gen);
+                                     (!c.learned_Minv || c.learned_gen != c.K_gen || c.learned_for != K64);
         if (rc == PDX_OK || !o.fallback || (rc != PDX_ERR_NOCONV && rc != PDX_ERR_SINGULAR && !learned_missing))
             return rc;
         // restart from the iterate the failed attempt left, unless it is no better than zero

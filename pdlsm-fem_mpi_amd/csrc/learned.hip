@@ -382,6 +382,7 @@ __global__ void k_d2f(size_t n, const double *a, float *b) {
 // ------------------------------------------------------------------------
 struct GnnWork {
     long long gen = -1;
+    const double *op = nullptr;  // the operator the inputs were formed from
     int nn = 0, D = 2, P = 0, agg = 0;
     long long nblk = 0;
     float *X = nullptr, *cw = nullptr, *Aaa = nullptr, *B = nullptr;
@@ -411,15 +412,29 @@ void free_gnn_work(Ctx &c) {
     c.gnn_work = nullptr;
 }
 
-// features, block Jacobi inverse and activations for the context's current K
-static int gnn_work(Ctx &c, GnnWork *&out, std::string &err) {
-    if (!c.K_valid) {
-        int rc = dev_assemble_K(c, err);
-        if (rc) return rc;
+// the operator of the context's network work (c.gnn_op): K, or the Newmark operator formed for
+// the current K
+static int gnn_operator(Ctx &c, const double *&A, std::string &err) {
+    if (!c.K_valid)
+        if (int rc = dev_assemble_K(c, err)) return rc;
+    if (c.gnn_op == 0) {
+        A = c.K;
+        return PDX_OK;
     }
+    if (!c.gnn_keff) PDX_HIP(hipMalloc((void **)&c.gnn_keff, (size_t)c.dd * c.nblk * sizeof(double)));
+    if (c.gnn_keff_gen != c.K_gen) {
+        if (int rc = dev_form_keff(c, c.gnn_keff, err)) return rc;
+        c.gnn_keff_gen = c.K_gen;
+    }
+    A = c.gnn_keff;
+    return PDX_OK;
+}
+
+// features, block Jacobi inverse and activations for operator A (K or a K_eff of the current K)
+static int gnn_work_for(Ctx &c, const double *A, GnnWork *&out, std::string &err) {
     if (c.dim != 2 && c.dim != 3) { err = "the learned smoother needs a 2D or 3D model"; return PDX_ERR_ARG; }
     GnnWork *g = (GnnWork *)c.gnn_work;
-    if (g && g->gen == c.K_gen) {
+    if (g && g->gen == c.K_gen && g->op == A) {
         out = g;
         return PDX_OK;
     }
@@ -487,20 +502,28 @@ static int gnn_work(Ctx &c, GnnWork *&out, std::string &err) {
     PDX_HIP(hipMemcpy(ntype, m.ntype.data(), (size_t)nn * 4, hipMemcpyHostToDevice));
     const int nb = (nn + 255) / 256;
     if (D == 3) {
-        k_gnn_scale<3><<<nb, 256, 0, c.stream>>>(nn, c.adj_off, c.adj, c.K, c.eq, S);
-        k_gnn_inputs<3><<<nb, 256, 0, c.stream>>>(nn, c.adj_off, c.adj, c.K, c.eq, ntype, S, g->X, g->cw, g->Aaa);
+        k_gnn_scale<3><<<nb, 256, 0, c.stream>>>(nn, c.adj_off, c.adj, A, c.eq, S);
+        k_gnn_inputs<3><<<nb, 256, 0, c.stream>>>(nn, c.adj_off, c.adj, A, c.eq, ntype, S, g->X, g->cw, g->Aaa);
     } else {
-        k_gnn_scale<2><<<nb, 256, 0, c.stream>>>(nn, c.adj_off, c.adj, c.K, c.eq, S);
-        k_gnn_inputs<2><<<nb, 256, 0, c.stream>>>(nn, c.adj_off, c.adj, c.K, c.eq, ntype, S, g->X, g->cw, g->Aaa);
+        k_gnn_scale<2><<<nb, 256, 0, c.stream>>>(nn, c.adj_off, c.adj, A, c.eq, S);
+        k_gnn_inputs<2><<<nb, 256, 0, c.stream>>>(nn, c.adj_off, c.adj, A, c.eq, ntype, S, g->X, g->cw, g->Aaa);
     }
-    if ((rc = block_jacobi_inverse(c, c.K, Bd, err))) { delete g; return rc; }
+    if ((rc = block_jacobi_inverse(c, A, Bd, err))) { delete g; return rc; }
     k_d2f<<<2048, 256, 0, c.stream>>>((size_t)nn * DD, Bd, g->B);
     PDX_HIP(hipGetLastError());
     PDX_HIP(hipStreamSynchronize(c.stream));
     g->gen = c.K_gen;
+    g->op = A;
     c.gnn_work = g;
     out = g;
     return PDX_OK;
+}
+
+// the same for the context's network operator
+static int gnn_work(Ctx &c, GnnWork *&out, std::string &err) {
+    const double *A = nullptr;
+    if (int rc = gnn_operator(c, A, err)) return rc;
+    return gnn_work_for(c, A, out, err);
 }
 
 // Gaussian probes on the free DOFs (seeded, reproducible) -> f, y (FP32)
@@ -530,7 +553,7 @@ static int gnn_probes(Ctx &c, GnnWork &g, int P, unsigned long long seed, int ag
     PDX_HIP(hipMalloc((void **)&fd, P * n * 8));
     PDX_HIP(hipMalloc((void **)&yd, P * n * 8));
     PDX_HIP(hipMemcpy(zd, z.data(), P * n * 8, hipMemcpyHostToDevice));
-    int rc = two_level_probes(c, c.K, agg, zd, fd, yd, P, err);
+    int rc = two_level_probes(c, g.op, agg, zd, fd, yd, P, err);
     if (!rc) {
         k_dot<double><<<RED_PARTS, 256, 0, c.stream>>>(fd, fd, P * n, g.lpart);
         g.fnorm2 = host_sum(c, g.lpart, RED_PARTS, err);
@@ -715,6 +738,33 @@ static int check_pair(const pdx_ctx *pc, const pdx_gnn *g, std::string &err) {
     return PDX_OK;
 }
 
+// drop what was computed for operator A (about to be freed): the smoother and the network's inputs
+void forget_learned_operator(Ctx &c, const double *A) {
+    if (c.learned_for == A) c.learned_for = nullptr;
+    if (c.gnn_work && ((GnnWork *)c.gnn_work)->op == A) free_gnn_work(c);
+}
+
+// The smoother of the attached network for operator A (a Newmark K_eff of the current K): the
+// same inference as pdx_gnn_apply, on A's inputs.
+int dev_learned_smoother(Ctx &c, const double *A, std::string &err) {
+    if (!c.gnn_net) {
+        err = "a learned Newmark solve needs a network attached to the context (pdx_gnn_attach)";
+        return PDX_ERR_ARG;
+    }
+    const pdx_gnn &net = *(const pdx_gnn *)c.gnn_net;
+    GnnWork *w;
+    if (int rc = gnn_work_for(c, A, w, err)) return rc;
+    gnn_forward(c, *w, net);
+    if (!c.learned_Minv) PDX_HIP(hipMalloc((void **)&c.learned_Minv, (size_t)c.nn * c.dd * 8));
+    const int nb = (c.nn + 255) / 256;
+    if (c.dim == 3) k_smoother<3><<<nb, 256, 0, c.stream>>>(c.nn, w->O, w->B, c.eq, c.learned_Minv);
+    else k_smoother<2><<<nb, 256, 0, c.stream>>>(c.nn, w->O, w->B, c.eq, c.learned_Minv);
+    PDX_HIP(hipGetLastError());
+    c.learned_gen = c.K_gen;
+    c.learned_for = A;
+    return PDX_OK;
+}
+
 }  // namespace pdx
 
 using namespace pdx;
@@ -871,6 +921,25 @@ int pdx_gnn_apply(pdx_ctx *pc, const pdx_gnn *g) {
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c.stream) != hipSuccess)
         return gfail(PDX_ERR_HIP, "pdx_gnn_apply: kernel failed");
     c.learned_gen = c.K_gen;
+    c.learned_for = w->op;
+    return PDX_OK;
+}
+
+int pdx_gnn_operator(pdx_ctx *pc, int mode) {
+    if (!pc || (mode != 0 && mode != 1)) return gfail(PDX_ERR_ARG, "pdx_gnn_operator: mode is 0 (K) or 1 (Newmark)");
+    Ctx &c = *pc->c;
+    hipSetDevice(c.device);
+    c.gnn_op = mode;
+    return PDX_OK;
+}
+
+int pdx_gnn_attach(pdx_ctx *pc, const pdx_gnn *g) {
+    if (!pc) return gfail(PDX_ERR_ARG, "null argument");
+    if (g) {
+        std::string err;
+        if (int rc = check_pair(pc, g, err)) return gfail(rc, err);
+    }
+    pc->c->gnn_net = g;
     return PDX_OK;
 }
 
@@ -901,6 +970,7 @@ int pdx_gnn_infer(pdx_ctx *pc, const pdx_gnn *g, int bf16, double *M, double *ms
     hipEventRecord(e1, c.stream);
     if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
         return gfail(PDX_ERR_HIP, "pdx_gnn_infer: kernel failed");
+    c.learned_for = w->op;
     float t = 0.f;
     hipEventElapsedTime(&t, e0, e1);
     hipEventDestroy(e0);

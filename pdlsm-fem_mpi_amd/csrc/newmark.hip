@@ -147,6 +147,17 @@ __global__ void k_keff(size_t n, const double *K, const double *M, double beta, 
         Ke[i] = K[i] * beta * dt * dt + M[i];
 }
 
+// K_eff = beta dt^2 K + M of the context's K and consistent mass into Keff [dd * nblk]
+int dev_form_keff(Ctx &c, double *Keff, std::string &err) {
+    if (!c.K_valid)
+        if (int rc = dev_assemble_K(c, err)) return rc;
+    if (!c.Mk_valid)
+        if (int rc = dev_mass_on_K(c, err)) return rc;
+    k_keff<<<4096, 256, 0, c.stream>>>((size_t)c.dd * c.nblk, c.K, c.Mk, c.beta, c.dt, Keff);
+    PDX_HIP(hipGetLastError());
+    return PDX_OK;
+}
+
 __global__ void k_fill_n(double *p, double v, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -262,6 +273,20 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
     PDX_HIP(dalloc((void **)&val, n * 8));
     PDX_HIP(dalloc((void **)&dU, std::max<size_t>(m.ebc.size(), 1) * 8));
     k_keff<<<4096, 256, 0, c.stream>>>(nk, c.K, c.Mk, c.beta, c.dt, Keff);
+    // a learned solve predicts its smoother for this K_eff with the attached network (with
+    // fallback on, a missing network leaves the solve to fall back); the smoother and the
+    // network's inputs of this K_eff are forgotten when it is freed
+    const bool learned = o.precond == PDX_PREC_LEARNED;
+    struct ForgetKeff {
+        Ctx &c;
+        const double *A;
+        ~ForgetKeff() { forget_learned_operator(c, A); }
+    } forget{c, Keff};
+    if (learned) {
+        rc = dev_learned_smoother(c, Keff, err);
+        if (rc && !o.fallback) return rc;
+        rc = PDX_OK;
+    }
     if (o.precision == PDX_FP32) {
         rc = to_f32(c, Keff, Keff32, err);
         if (Keff32) owned.p.push_back(Keff32);
@@ -359,6 +384,11 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
                 // bonds broke: K, K_eff and its copies for the next step
                 if ((rc = dev_assemble_K(c, err))) break;
                 k_keff<<<4096, 256, 0, c.stream>>>(nk, c.K, c.Mk, c.beta, c.dt, Keff);
+                if (learned) {
+                    rc = dev_learned_smoother(c, Keff, err);
+                    if (rc && !o.fallback) break;
+                    rc = PDX_OK;
+                }
                 if (o.precision == PDX_FP32 && (rc = to_f32(c, Keff, Keff32, err))) break;
                 if (mir.K64 && (rc = gather_own(c, Keff, own64, own32, err))) break;
             }
