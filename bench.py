@@ -428,7 +428,9 @@ def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
     warm-started; each step solves (M + beta dt^2 K) a = F - K U_N to a 1e-8 true
     residual: plain (FP64 node-block-Jacobi BiCGStab), FP32 BiCGStab with FP64
     refinement and node-block Jacobi, the same with the two-level hybrid
-    preconditioner (aggregates of coarse_agg^3 nodes, built once per run), and
+    preconditioner (aggregates of coarse_agg^3 nodes, built once per run), the
+    same with the learned smoother (a 3D network trained on the Newmark operator
+    of smaller penny boxes, predicting the smoother of this run's K_eff), and
     the two block-Jacobi runs again with the SpMVs on mirrored storage (same
     results bit for bit).  The other runs' displacements are compared with the
     plain run's."""
@@ -466,13 +468,30 @@ def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
                                                   max_iter=100000),
             "fp32_ir_two_level": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8,
                                                 max_iter=100000, coarse_agg=coarse_agg),
+            "fp32_ir_learned": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_LEARNED, tol=1e-8,
+                                              max_iter=100000, coarse_agg=coarse_agg),
             "plain_fp64_blockjacobi_mirrored": solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_BLOCK_JACOBI,
                                                               tol=1e-8, max_iter=100000, mirrored=True),
             "fp32_ir_blockjacobi_mirrored": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_BLOCK_JACOBI,
                                                            tol=1e-8, max_iter=100000, mirrored=True)}
+    # the learned run's network: trained on the Newmark operator K_eff of smaller penny boxes at the
+    # same spacing and time step, then attached to the run's context, which predicts the smoother
+    # of its K_eff with it (inference inside the timed run)
+    t1 = time.time()
+    net = pdx.GNN(3, seed=5)
+    for epoch in range(3):
+        for k, nt in enumerate((24, 32, 40)):
+            tr = pdx.Context(pdx.Model(W.penny3d(n=nt, pd_block=nt // 3, dt=dt, ramp_steps=steps)))
+            tr.gnn_operator(1)
+            net.train(tr, steps=10, n_probes=4, probe_seed=10 * epoch + k, lr=3e-3, coarse_agg=coarse_agg)
+            del tr
+    out["learned_train"] = {"models": "penny3d 24^3, 32^3, 40^3 (PD third), Newmark operator", "epochs": 3,
+                            "adam_steps_per_model": 10, "seconds": time.time() - t1}
     us = {}
     for name, opts in runs.items():
         cx = pdx.Context(mp)
+        if name == "fp32_ir_learned":
+            net.attach(cx)
         t1 = time.time()
         try:
             u, v, a, st = cx.newmark_run(steps, opts)

@@ -319,7 +319,10 @@ int pdx_node_stresses_timed(pdx_ctx *c, const double *u_all, double *ms);
  * step the predictor U_N (ramped prescribed values advanced by dt * velocity), F = F_ext - K U_N,
  * (M + beta dt^2 K) a_{n+1} = F by the Krylov solve of o (warm-started from a_n), the update, then
  * the BCs advance; with FC 2 in 2D the nodal stresses and the bond failure follow (:6258-6261,
- * pdx_step_log) and K is assembled again when bonds broke.  u, v, a: [neq] (may be NULL). */
+ * pdx_step_log) and K is assembled again when bonds broke.  u, v, a: [neq] (may be NULL).
+ * PDX_PREC_LEARNED predicts the smoother of each K_eff with the network of pdx_gnn_attach.  st:
+ * iterations and refinements summed over the steps, the largest true residual, the preconditioner
+ * and precision of the last step's solve, the most attempts any step took. */
 int pdx_newmark_run(pdx_ctx *c, int nsteps, const pdx_solve_opts *o, double *u, double *v, double *a,
                     pdx_solve_stats *st);
 
@@ -345,8 +348,19 @@ int pdx_gnn_loss_grad(pdx_ctx *c, pdx_gnn *g, int n_probes, unsigned long long p
 /* the network inputs: X [nn*8], c [nblk], A_aa [nn*d*d], B [nn*d*d], f, y [n_probes*nn*d]; any may be NULL */
 int pdx_gnn_inputs(pdx_ctx *c, int n_probes, unsigned long long probe_seed, int coarse_agg, float *X, float *cw,
                    float *Aaa, float *B, float *f, float *y);
-/* predict the smoother for the context's current K (used by PDX_PREC_LEARNED) */
+/* predict the smoother for the context's current operator (pdx_gnn_operator; used by
+ * PDX_PREC_LEARNED on that operator) */
 int pdx_gnn_apply(pdx_ctx *c, const pdx_gnn *g);
+/* The operator the network's inputs, training probes and pdx_gnn_apply / pdx_gnn_infer use:
+ * mode 0 (default) the stiffness K of the static and quasi-static solves, mode 1 the Newmark
+ * operator K_eff = M + beta dt^2 K of pdfemDynamicNewmarkSolver_CSRformat (pdsolve.cpp:6092-6290),
+ * formed on the device for the current K.  A smoother serves only the operator it was predicted
+ * for; a solve on another operator reports that none is there (or falls back). */
+int pdx_gnn_operator(pdx_ctx *c, int mode);
+/* Network a learned Newmark run (pdx_newmark_run with opts.precond = PDX_PREC_LEARNED) predicts
+ * its smoother with, for each K_eff it forms (again after bonds break); NULL detaches.  The
+ * caller keeps the network alive while it is attached.  No counterpart in the reference. */
+int pdx_gnn_attach(pdx_ctx *c, const pdx_gnn *g);
 /* The same prediction with FP32 (bf16 = 0) or BF16-activation MFMA (bf16 != 0,
  * v_mfma_f32_16x16x32_bf16, FP32 accumulation) inference; M [n_node*d*d] (or NULL) receives the
  * smoother, ms (or NULL) the inference time.  Also makes it the context's learned smoother. */

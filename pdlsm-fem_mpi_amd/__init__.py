@@ -116,6 +116,8 @@ EXPORTS = {
     "pdx_node_stresses_timed": (_I, [_P, _P, _P]),
     "pdx_node_stresses_noalloc": (_I, [_P, _P, _P]),
     "pdx_set_setup_device": (_I, [_I]),
+    "pdx_gnn_operator": (_I, [_P, _I]),
+    "pdx_gnn_attach": (_I, [_P, _P]),
     "pdx_gnn_create": (_I, [_I, _I, C.c_ulonglong, _P]),
     "pdx_gnn_destroy": (None, [_P]),
     "pdx_gnn_num_params": (_LL, [_P]),
@@ -441,6 +443,10 @@ class Context:
         _chk(lib().pdx_node_stresses_timed(self.h, _p(u), C.byref(ms)))
         return ms.value
 
+    def gnn_operator(self, mode: int):
+        """Operator of the network's inputs, training and apply: 0 K, 1 the Newmark K_eff."""
+        _chk(lib().pdx_gnn_operator(self.h, int(mode)))
+
     def explicit_set_form(self, precision, form):
         """Step with kernel form `form` (-1: tune again at the next step)."""
         _chk(lib().pdx_explicit_set_form(self.h, precision, form))
@@ -585,6 +591,12 @@ class GNN:
 
     def apply(self, cx: "Context"):
         _chk(lib().pdx_gnn_apply(cx.h, self.h))
+
+    def attach(self, cx: "Context"):
+        """Make this the network cx's learned Newmark runs predict their smoothers with (the
+        context keeps a reference to it)."""
+        _chk(lib().pdx_gnn_attach(cx.h, self.h))
+        cx._gnn = self
 
     def infer(self, cx: "Context", bf16: bool = False):
         """Smoother [nn, d, d] predicted for cx's K (FP32, or BF16 MFMA inference) and the

@@ -359,6 +359,10 @@ int dev_newmark(Ctx &c, int nsteps, const pdx_solve_opts &o, std::vector<double>
         stats.refinements += st.refinements;
         stats.true_relres = std::max(stats.true_relres, st.true_relres);
         stats.mirrored_used = st.mirrored_used;
+        // the preconditioner and precision of the last step's solve; attempts: the most any step took
+        stats.precond_used = st.precond_used;
+        stats.precision_used = st.precision_used;
+        stats.attempts = std::max(stats.attempts, st.attempts);
         if (rc) break;
         k_newmark_update<<<(th + 255) / 256, 256, 0, c.stream>>>(th, c.eq, D, u, v, a, a1, c.dt, c.beta, c.gamma);
         std::swap(ap, a);  // ap <- a_n

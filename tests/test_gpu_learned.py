@@ -12,11 +12,15 @@
                the same solve without the network (two-level block Jacobi) over
                held-out 501 x 251 plates (in FP64 the two are about even: the node-local
                smoother has little headroom once the coarse space is there)
+  Newmark      a 3D network trained on the Newmark operator K_eff of a small penny box and
+               attached to the context predicts the smoother of each K_eff: FP32-refined
+               learned Newmark matches the oracle's trajectory to 1e-8; a K smoother does not
+               serve K_eff, and without a network the run fails or falls back
 """
 import numpy as np
 import pytest
 
-from helpers import load_gnn_oracle, load_oracle, load_pdx, workloads
+from helpers import load_gnn_oracle, load_oracle, load_pdx, rel, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -172,3 +176,46 @@ def test_bf16_inference_solve():
         assert st["converged"] and np.linalg.norm(F - A @ u) / np.linalg.norm(F) <= 1.2e-8
         its.append(st["iterations"])
     assert abs(its[1] - its[0]) <= 2, its
+
+
+def _newmark_net():
+    """A 3D network trained on the Newmark operator K_eff of a small penny-crack box."""
+    net = pdx.GNN(3, seed=5)
+    tr = pdx.Context(pdx.Model(W.penny3d(n=14, pd_block=6, ramp_steps=6, dt=1e-6)))
+    tr.gnn_operator(1)
+    loss = net.train(tr, steps=10, n_probes=4, probe_seed=1, lr=3e-3, coarse_agg=4)
+    assert np.all(np.isfinite(loss)) and loss[-1] < loss[0], loss
+    return net
+
+
+def test_learned_newmark_penny3d_matches_oracle():
+    """Learned FP32-refined Newmark on the 3D penny crack (20^3 nodes, pre-broken bonds, ramped
+    traction): the attached network predicts the smoother of each K_eff, the two-level BiCGStab
+    solves every step to the tolerance, and the trajectory matches the oracle's direct solves to
+    1e-8, as the block-Jacobi runs do (test_gpu_dynamic.py)."""
+    net = _newmark_net()
+    mesh = W.penny3d(n=20, pd_block=10, ramp_steps=6, dt=1e-6)
+    mo = O.Model(mesh)
+    u0, v0, _ = O.newmark_run(mo, 6)
+    cx = pdx.Context(pdx.Model(mesh))
+    net.attach(cx)
+    u1, v1, _, st = cx.newmark_run(6, pdx.solve_opts(precision=1, precond=pdx.PDX_PREC_LEARNED, tol=1e-12,
+                                                     max_iter=200000, inner_tol=1e-5, coarse_agg=4))
+    assert st["precond_used"] == pdx.PDX_PREC_LEARNED and st["attempts"] == 1, st
+    assert np.abs(u0).max() > 0
+    assert rel(u1, u0) <= 1e-8 and rel(v1, v0) <= 1e-7, (rel(u1, u0), rel(v1, v0))
+
+
+def test_learned_newmark_needs_its_own_smoother():
+    """A learned Newmark run needs an attached network; a smoother predicted for K does not
+    serve K_eff; with fallback on, the run falls back to another preconditioner instead."""
+    mesh = W.penny3d(n=12, pd_block=4, ramp_steps=2, dt=1e-6)
+    cx = pdx.Context(pdx.Model(mesh))
+    net = pdx.GNN(3, seed=1)
+    net.apply(cx)  # a smoother for K (operator 0)
+    opts = dict(precision=0, precond=pdx.PDX_PREC_LEARNED, tol=1e-10, max_iter=100000)
+    with pytest.raises(pdx.PdxError):
+        cx.newmark_run(2, pdx.solve_opts(**opts))
+    cx = pdx.Context(pdx.Model(mesh))
+    u, _, _, st = cx.newmark_run(2, pdx.solve_opts(fallback=True, **opts))
+    assert st["precond_used"] != pdx.PDX_PREC_LEARNED and np.abs(u).max() > 0, st
