@@ -332,66 +332,81 @@ def gnn_bench(pdx, cx, model):
     return out
 
 
-def learned_bench(pdx, W, train_seeds=range(8), held_seeds=tuple(range(448, 464)), epochs=4, steps=10,
+def learned_bench(pdx, W, warm_seeds=range(8), train_seeds=range(8, 40), held_seeds=tuple(range(448, 512)),
                   plain_seeds=4):
-    """The metric's learned-vs-plain comparison on held-out cracked plates: a network trained
-    on the GPU (the library's forward, backward and Adam kernels) on 121 x 61 plates of the
-    training seeds, applied with BF16 MFMA inference to 501 x 251 plates of held-out seeds,
-    and solved to a 1e-8 true residual in FP64 and in FP32 with FP64 refinement (inner
-    solves to 1e-5: on these plates fewer refinement sweeps pay for the tighter inner
-    tolerance; the Newmark runs keep the default 1e-4), beside two-level without the network
-    and plain FP64 Jacobi (on the first plain_seeds held-out plates).  The summary gives the
-    mean iterations and solve times over the held-out plates and what the network gains over the
-    same two-level solve without it."""
+    """The metric's learned-vs-plain comparison on the edge-cracked plate variants (501 x 251
+    nodes, crack length and angle drawn per seed): a network trained on the GPU (the library's
+    forward, backward and Adam kernels) -- 4 epochs x 10 Adam steps on 121 x 61 plates of
+    warm_seeds, then one pass of 5 steps over 501 x 251 plates of train_seeds -- is applied with
+    BF16 MFMA inference to the 64 held-out seeds 448-511 and solved to a 1e-8 true residual in FP64
+    and in FP32 with FP64 refinement (inner solves to 1e-5: on these plates fewer refinement
+    sweeps pay for the tighter inner tolerance; the Newmark runs keep the default 1e-4), beside
+    two-level without the network and plain FP64 Jacobi (on the first plain_seeds held-out
+    plates).  Each solve is timed on its second call (the first builds the coarse space).  The
+    summary gives the mean iterations and solve times over the held-out plates and what the
+    network gains over the same two-level solve without it."""
     t0 = time.time()
     net = pdx.GNN(2, seed=3)
-    ctxs = []
-    for s in train_seeds:
-        cx = pdx.Context(pdx.Model(W.crack2d(seed=s, nx=121, ny=61, dx=0.1 / 120)))
+    for epoch in range(4):
+        for k, s in enumerate(warm_seeds):
+            cx = pdx.Context(pdx.Model(W.crack2d(seed=s, nx=121, ny=61, dx=0.1 / 120)))
+            cx.assemble(pdx.PDX_MODE_STATIC)
+            net.train(cx, steps=10, n_probes=8, probe_seed=100 * epoch + k, lr=3e-3, coarse_agg=8)
+    t_warm = time.time() - t0
+    for k, s in enumerate(train_seeds):
+        cx = pdx.Context(pdx.Model(W.crack2d(seed=s)))
         cx.assemble(pdx.PDX_MODE_STATIC)
-        ctxs.append(cx)
-    for epoch in range(epochs):
-        for k, cx in enumerate(ctxs):
-            net.train(cx, steps=steps, n_probes=8, probe_seed=100 * epoch + k, lr=3e-3, coarse_agg=8)
-    out = {"train": f"{len(ctxs)} seeds ({min(train_seeds)}-{max(train_seeds)}) at 121x61, {epochs} epochs x "
-                    f"{steps} Adam steps", "train_s": time.time() - t0, "held_out": {}}
+        net.train(cx, steps=5, n_probes=4, probe_seed=1000 + k, lr=1e-3)
+    out = {"train": f"4 epochs x 10 Adam steps on {len(warm_seeds)} seeds at 121x61 ({t_warm:.1f} s), then 5 steps on "
+                    f"each of {len(train_seeds)} seeds ({min(train_seeds)}-{max(train_seeds)}) at 501x251",
+           "train_s": time.time() - t0, "held_out_seeds": f"{min(held_seeds)}-{max(held_seeds)}"}
     cfgs = (("plain_fp64_jacobi", dict(precision=0, precond=1, tol=1e-8, max_iter=100000)),
             ("fp64_two_level", dict(precision=0, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8)),
             ("fp32_ir_two_level", dict(precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8, inner_tol=1e-5)),
             ("fp64_learned", dict(precision=0, precond=pdx.PDX_PREC_LEARNED, tol=1e-8)),
             ("fp32_ir_learned", dict(precision=1, precond=pdx.PDX_PREC_LEARNED, tol=1e-8, inner_tol=1e-5)))
+    per = {name: {"iterations": [], "ms": []} for name, _ in cfgs}
+    per["bf16_inference_ms"] = []
+    errors = {}
     for i, s in enumerate(held_seeds):
         cx = pdx.Context(pdx.Model(W.crack2d(seed=s)))
         cx.assemble(pdx.PDX_MODE_STATIC)
         _, infer_ms = net.infer(cx, bf16=True)
-        row = {"bf16_inference_ms": infer_ms}
+        per["bf16_inference_ms"].append(infer_ms)
         for name, kw in cfgs:
             if name == "plain_fp64_jacobi" and i >= plain_seeds:
                 continue
             try:
                 cx.static_solve(solve_opts(pdx, **kw))
                 _, st = cx.static_solve(solve_opts(pdx, **kw))
-                row[name] = {"seconds": st["seconds"], "iterations": st["iterations"], "true_relres": st["true_relres"]}
+                if st["true_relres"] > 1e-8 * 1.2:
+                    raise RuntimeError(f"true residual {st['true_relres']:.2e}")
+                per[name]["iterations"].append(int(st["iterations"]))
+                per[name]["ms"].append(st["seconds"] * 1e3)
             except Exception as e:  # reported, not hidden
-                row[name] = {"error": str(e)}
-        if all("seconds" in row.get(k, {}) for k in ("plain_fp64_jacobi", "fp32_ir_learned", "fp64_learned")):
-            row["speedup_fp32_ir_learned_vs_plain"] = row["plain_fp64_jacobi"]["seconds"] / row["fp32_ir_learned"]["seconds"]
-            row["speedup_fp64_learned_vs_plain"] = row["plain_fp64_jacobi"]["seconds"] / row["fp64_learned"]["seconds"]
-        out["held_out"][f"seed {s}"] = row
+                errors[f"seed {s} {name}"] = str(e)
     summ = {}
     for name, _ in cfgs:
-        rows = [r[name] for r in out["held_out"].values() if "seconds" in r.get(name, {})]
-        if rows:
-            summ[name] = {"plates": len(rows), "mean_iterations": float(np.mean([r["iterations"] for r in rows])),
-                          "mean_ms": float(np.mean([r["seconds"] for r in rows]) * 1e3)}
+        if per[name]["ms"]:
+            summ[name] = {"plates": len(per[name]["ms"]), "mean_iterations": float(np.mean(per[name]["iterations"])),
+                          "mean_ms": float(np.mean(per[name]["ms"]))}
     for prec in ("fp64", "fp32_ir"):
         a, b = summ.get(f"{prec}_two_level"), summ.get(f"{prec}_learned")
         if a and b:
             summ[f"{prec}_network_gain"] = {"iterations": a["mean_iterations"] / b["mean_iterations"],
                                             "time": a["mean_ms"] / b["mean_ms"]}
+            ia, ib = np.array(per[f"{prec}_two_level"]["iterations"]), np.array(per[f"{prec}_learned"]["iterations"])
+            if len(ia) == len(ib):
+                summ[f"{prec}_network_gain"]["plates_fewer_iterations"] = int((ib < ia).sum())
+                summ[f"{prec}_network_gain"]["plates_more_iterations"] = int((ib > ia).sum())
     if "plain_fp64_jacobi" in summ and "fp32_ir_learned" in summ:
         summ["fp32_ir_learned_vs_plain_time"] = summ["plain_fp64_jacobi"]["mean_ms"] / summ["fp32_ir_learned"]["mean_ms"]
+        summ["fp64_learned_vs_plain_time"] = summ["plain_fp64_jacobi"]["mean_ms"] / summ["fp64_learned"]["mean_ms"]
+    summ["mean_bf16_inference_ms"] = float(np.mean(per["bf16_inference_ms"]))
     out["summary"] = summ
+    out["per_plate_iterations"] = {name: per[name]["iterations"] for name, _ in cfgs}
+    if errors:
+        out["errors"] = errors
     return out
 
 
