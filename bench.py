@@ -611,12 +611,13 @@ def main():
     if args.stress_only:
         model = pdx.Model(W.box3d(n=args.n))
         cx = pdx.Context(model)
-        cx.assemble_timed()
+        asm = [cx.assemble_timed() for _ in range(6)]  # the first also allocates the staging
         uall = np.random.default_rng(0).uniform(-1e-6, 1e-6, model.nn * model.dim)
         ms = [cx.node_stresses_timed(uall) for _ in range(max(args.steps, 1))]
         sb = stress_bytes(model)
-        print(json.dumps({"stress_recovery": {"ms": min(ms), "alg_bytes": sb,
-                                              "GBps": sb / (min(ms) * 1e-3) / 1e9}}), flush=True)
+        print(json.dumps({"assembly_ms": min(asm[1:]), "stress_recovery": {"ms": min(ms), "alg_bytes": sb,
+                                                                           "GBps": sb / (min(ms) * 1e-3) / 1e9}}),
+              flush=True)
         return
     if args.fb_only:
         print(json.dumps({"family_batch": family_batch_bench(pdx, W)}), flush=True)

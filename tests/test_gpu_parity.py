@@ -131,6 +131,28 @@ def test_explicit_trajectory_matches_oracle(case, prec, tol):
     assert rel(u1, u0) <= tol and rel(up1, up0) <= tol
 
 
+def test_explicit_many_nbc_sets():
+    """More natural-BC sets than one workgroup has threads (324 ramped traction sets, one top face
+    each, each with its own rate): every set advances every step, eager and in graph replay, as
+    in the oracle (FP64 trajectory <= 1e-10)."""
+    mesh = W.box3d(n=19, pd_block=5)
+    faces = mesh["nbc"][0]["con"]
+    assert len(faces) > 256
+    mesh["nbc"] = [dict(con=faces[i:i + 1], val=1e5, vel=1e9 * (1 + i % 7), vary=True) for i in range(len(faces))]
+    mp = pdx.Model(mesh)
+    cx = pdx.Context(mp)
+    mesh["dt"] = 0.8 * 2.0 / np.sqrt(cx.max_eig(200))
+    mo = O.Model(mesh)
+    u0, up0 = mo.explicit_run(25)
+    for graph in (False, True):
+        cx = pdx.Context(pdx.Model(mesh))
+        cx.explicit_steps(1)
+        cx.explicit_steps(24, use_graph=graph)
+        u1, up1 = cx.explicit_run(0)
+        assert np.abs(u0).max() > 0
+        assert rel(u1, u0) <= 1e-10 and rel(up1, up0) <= 1e-10, (graph, rel(u1, u0))
+
+
 def test_explicit_graph_replay_matches_eager():
     mesh = _explicit_case()
     mesh["dt"] = 2e-8
