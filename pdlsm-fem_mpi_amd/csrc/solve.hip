@@ -27,9 +27,11 @@ namespace pdx {
 // FP32 values use the difference form (see rowblock.hpp), FP64 the plain form.
 // GS = lanes per short row, NT = non-temporal loads of K, XM = XCD-aware block
 // order (chosen by the tuner).  The hardware deals workgroups to the 8 XCDs
-// round-robin; with XM the launch is padded to a multiple of 8 and block b works
-// on logical block (b % 8) * per + b / 8, so each XCD (and its L2) owns one
-// contiguous eighth of the rows and the gathers of u_n stay local to it.  With rev
+// round-robin; with XM the launch has 8 x (an eighth of the short-row blocks + an
+// eighth of the long-row blocks) workgroups and the ones XCD x runs take the x-th
+// contiguous eighth of each, so every XCD (and its L2) owns one contiguous range of
+// rows of each kind, the gathers of u_n stay local to it, and the wavefront-wide
+// long rows (the PD nodes) do not all land on one XCD.  With rev
 // the blocks of each range are visited last to first (alternating forms reverse
 // every other step), which changes which rows run when, not what a row computes.
 // column pattern classes (REL forms): class of each row, offsets of each class's relative columns
@@ -51,9 +53,18 @@ __global__ void __launch_bounds__(256) k_explicit(const int *rows_s, int ns, int
     constexpr bool DIFF = sizeof(T) == 4;
     int lb = blockIdx.x;
     if (XM) {
-        const int per = gridDim.x / 8, q = blockIdx.x / 8;
-        lb = (blockIdx.x % 8) * per + (rev ? per - 1 - q : q);
-        if (lb >= nbt) return;
+        // each XCD takes one contiguous eighth of the short-row blocks and one of the long-row
+        // blocks (short ones first), so the long rows' bytes are spread over all eight L2s
+        const int per = gridDim.x / 8, q0 = blockIdx.x / 8, x = blockIdx.x % 8;
+        const int q = rev ? per - 1 - q0 : q0;
+        const int ps = (nbs + 7) / 8;
+        if (q < ps) {
+            lb = x * ps + q;
+            if (lb >= nbs) return;
+        } else {
+            lb = nbs + x * (per - ps) + (q - ps);
+            if (lb >= nbt) return;
+        }
     } else if (rev) {
         lb = gridDim.x - 1 - blockIdx.x;
     }
@@ -299,7 +310,8 @@ static void launch_explicit_v(Ctx &c, const T *K, const double *Un, const double
                               const double *nbc_cur, double *nbc_next, int startup, int rev = 0) {
     const int nbs = (c.n_short * GS + 255) / 256, nbl = (c.n_long * 64 + 255) / 256;
     const int nbt = std::max(nbs + nbl, 1);
-    const int grid = XM ? (nbt + 7) / 8 * 8 : nbt;
+    // XM: 8 XCDs x (an eighth of the short blocks + an eighth of the long ones)
+    const int grid = XM ? 8 * ((nbs + 7) / 8 + (nbt - nbs + 7) / 8) : nbt;
     const ColClasses cc{c.cls, c.cls_off, c.cls_rel, c.ksrc};
     k_explicit<D, T, GS, NT, XM, SOA, REL, SYM, FO><<<grid, 256, 0, c.stream>>>(
         c.rows_short, c.n_short, nbs, c.rows_long, c.n_long, nbt, c.adj_off, c.adj, cc, K, Un, Unm1, Unp1, c.mlump,
