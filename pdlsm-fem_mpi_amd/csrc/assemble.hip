@@ -586,27 +586,28 @@ __device__ inline void ke_put(const KeArgs &A, int t, int a, int b, int i, int j
     A.S[(size_t)(code >> 1) * (D * D) + ((code & 1) ? j * D + i : i * D + j)] = v;
 }
 
-// dN/dX of the 8 nodes and detJ at Gauss point g (pdfemEleBrick8N::eleStiffMatFEM's order: r
-// outer, q, p inner) of the element with node coordinates xn, into g8o / *wdet; the parent
-// derivatives are formed again per node for the solves instead of being kept live
-template <class XN>
-__device__ inline void hex8_gp(const XN &xn, int g, double (*g8o)[3], double *wdet, int *status) {
+// Parent derivatives of hex8 node i at Gauss point g (pdfemEleBrick8N::eleStiffMatFEM's order:
+// r outer, q, p inner)
+__device__ inline void hex8_dN(int g, int i, double d[3]) {
     const double gp = 0.57735026918962573;
     const double gpt[2] = {-gp, gp};
     const double p = gpt[g & 1], qq = gpt[(g >> 1) & 1], r = gpt[g >> 2];
-    auto dN = [&](int i, double d[3]) {
-        d[0] = 1.0 / 8 * c_hp[i] * (1 + qq * c_hq[i]) * (1 + r * c_hr[i]);
-        d[1] = 1.0 / 8 * c_hq[i] * (1 + p * c_hp[i]) * (1 + r * c_hr[i]);
-        d[2] = 1.0 / 8 * c_hr[i] * (1 + p * c_hp[i]) * (1 + qq * c_hq[i]);
-    };
-    double J[9];
+    d[0] = 1.0 / 8 * c_hp[i] * (1 + qq * c_hq[i]) * (1 + r * c_hr[i]);
+    d[1] = 1.0 / 8 * c_hq[i] * (1 + p * c_hp[i]) * (1 + r * c_hr[i]);
+    d[2] = 1.0 / 8 * c_hr[i] * (1 + p * c_hp[i]) * (1 + qq * c_hq[i]);
+}
+
+// Jacobian J at Gauss point g of the element with node coordinates xn, its partial-pivot LU
+// (J, piv) and the weight w detJ
+template <class XN>
+__device__ inline void hex8_gp_lu(const XN &xn, int g, double J[9], int piv[3], double *wdet, int *status) {
 #pragma unroll
     for (int i = 0; i < 9; i++) J[i] = 0.0;
     // every J entry sums over the nodes in node order, as the reference's cc-outer loop does
 #pragma unroll 2
     for (int i = 0; i < 8; i++) {
         double d[3], xi[3];
-        dN(i, d);
+        hex8_dN(g, i, d);
 #pragma unroll
         for (int cc = 0; cc < 3; cc++) xi[cc] = xn(i, cc);
 #pragma unroll
@@ -617,17 +618,26 @@ __device__ inline void hex8_gp(const XN &xn, int g, double (*g8o)[3], double *wd
         }
     }
     const double detJ = det3(J);
-    int piv[3];
     if (!lu3_factor(J, piv)) atomicOr(status, 2);
+    *wdet = 1.0 * 1.0 * 1.0 * detJ;
+}
+
+// dN/dX of the 8 nodes and detJ at Gauss point g of the element with node coordinates xn,
+// into g8o / *wdet; the parent derivatives are formed again per node for the solves instead of
+// being kept live
+template <class XN>
+__device__ inline void hex8_gp(const XN &xn, int g, double (*g8o)[3], double *wdet, int *status) {
+    double J[9];
+    int piv[3];
+    hex8_gp_lu(xn, g, J, piv, wdet, status);
 #pragma unroll 2
     for (int b = 0; b < 8; b++) {
         double d[3], o[3];
-        dN(b, d);
+        hex8_dN(g, b, d);
         lu3_solve(J, piv, d, o);
 #pragma unroll
         for (int c = 0; c < 3; c++) g8o[b][c] = o[c];
     }
-    *wdet = 1.0 * 1.0 * 1.0 * detJ;
 }
 
 // the same at Gauss point g of element list entry idx, coordinates read from global memory
@@ -659,15 +669,16 @@ __global__ void __launch_bounds__(HEX_EPB * 24) k_ke_hex8(KeArgs A) {
 
 // Hex8 with an isotropic D (the reference's D, pdsolve.cpp:37-94, always is): per Gauss point
 // the node block B_a^T D B_b w detJ = lam (w g_a) g_b^T + mu g_b (w g_a)^T + mu (w g_a . g_b) I,
-// so one thread per (element, node pair a <= b) accumulates its 3 x 3 block over the 8 Gauss
-// points (27 FMA each instead of the 324 of three full B^T D B rows) and writes the block and,
-// Ke being symmetric, its transpose.
+// so one thread per (element, node pair a <= b) accumulates M = sum_g (w g_a) g_b^T over the 8
+// Gauss points (9 FMA each instead of the 324 of three full B^T D B rows), forms
+// lam M + mu M^T + mu tr(M) I once, and writes the block (Ke being symmetric, only a <= b).
 constexpr int HEXI_EPB = 4;  // 4 elements: 32 set-up threads, 4 x 36 = 144 block threads
 // The element's node coordinates are loaded into LDS by the whole block first (one coordinate
 // per thread, so the gathers are in flight together) and the Gauss-point set-up reads them there.
 // Each block goes to its staging slot, transposed when the slot's K block is the other
 // orientation.  Four elements per 144-thread block: the set-up runs on 32 threads while the
-// rest wait, and small blocks let more of them share a CU.
+// rest wait, and small blocks let more of them share a CU (one thread per Gauss point for J and
+// its LU and one per Gauss point and node for the solves, in 256-thread blocks, was slower).
 __global__ void __launch_bounds__(HEXI_EPB * 36) k_ke_hex8_iso(KeArgs A) {
     __shared__ double sx[HEXI_EPB][8][3];
     __shared__ double sw[HEXI_EPB][8];
@@ -696,24 +707,29 @@ __global__ void __launch_bounds__(HEXI_EPB * 36) k_ke_hex8_iso(KeArgs A) {
     }
     const int b = a + rem;
     const double lam = A.lam, mu = A.mu;
-    double k[9];
+    // M = sum_g w_g g_a g_b^T over the Gauss points (9 FMA each), then the block
+    // lam M + mu M^T + mu tr(M) I
+    double M[9];
 #pragma unroll
-    for (int e = 0; e < 9; e++) k[e] = 0.0;
+    for (int e = 0; e < 9; e++) M[e] = 0.0;
     if (act) {
 #pragma unroll 2
         for (int g = 0; g < 8; g++) {
             const double w = sw[el][g];
             const double wa[3] = {w * sg[el][g][a][0], w * sg[el][g][a][1], w * sg[el][g][a][2]};
             const double gb[3] = {sg[el][g][b][0], sg[el][g][b][1], sg[el][g][b][2]};
-            const double dot = wa[0] * gb[0] + wa[1] * gb[1] + wa[2] * gb[2];
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) k[i * 3 + j] += lam * wa[i] * gb[j] + mu * gb[i] * wa[j];
-#pragma unroll
-            for (int i = 0; i < 3; i++) k[i * 4] += mu * dot;
+                for (int j = 0; j < 3; j++) M[i * 3 + j] = wa[i] * gb[j] + M[i * 3 + j];
         }
     }
+    const double tr = M[0] + M[4] + M[8];
+    double k[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) k[i * 3 + j] = lam * M[i * 3 + j] + mu * M[j * 3 + i] + (i == j ? mu * tr : 0.0);
     if (act) {
         // pair index t % 36 enumerates a <= b in ke_pair order
         const int code = A.slot[(size_t)A.list[e0 + el] * 36 + t % 36];
@@ -828,62 +844,42 @@ __global__ void k_ke_simplex(KeArgs A) {
 }
 
 // Block-owned add of the staged element blocks, one node pair at a time: one thread per (pair,
-// row) takes an upper block (r, c), r <= c, and its lower partner (c, r), starts each from its PD
-// part and adds the element terms in element order (the reference's summation order of FE terms
-// into K), then writes both rows once.  The element blocks of a pair sit consecutively in the
-// staging, in the (r, c) orientation, so a pair's run is read once, front to back: row ii for
-// the upper block, column ii for the lower one.
+// value e) takes value e of an upper block (r, c), r <= c, and the transposed value of its lower
+// partner (c, r), starts each from its PD part and adds value e of the pair's element blocks in
+// element order (the reference's summation order of FE terms into K), then writes both values
+// once.  The element blocks of a pair sit consecutively in the staging, in the (r, c)
+// orientation, so the d x d threads of consecutive pairs read the staging front to back, each
+// staged value once.
 template <int D>
 __global__ void k_fe_gather(int npair, const int *__restrict__ pair_up, const int *__restrict__ pair_lo,
                             const int *__restrict__ pair_off, const unsigned char *__restrict__ pd_bits,
                             const double *__restrict__ S, double *K) {
     constexpr int DD = D * D;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= (long long)npair * D) return;
-    const int pr = int(t / D), ii = int(t % D);
+    if (t >= (long long)npair * DD) return;
+    const int pr = int(t / DD), e = int(t % DD), et = (e % D) * D + e / D;
     const int bu = pair_up[pr], bl = pair_lo[pr];
     const int c0 = pair_off[pr], c1 = pair_off[pr + 1];
     auto pd = [&](int q) { return (pd_bits[q >> 3] >> (q & 7)) & 1; };
-    double *ku = K + (size_t)bu * DD + ii * D;
-    double *kl = bl >= 0 ? K + (size_t)bl * DD + ii * D : nullptr;
     // blocks without a PD part start from zero instead of reading K
-    const bool pu = pd(bu), pl = bl >= 0 && pd(bl);
-    double au[D], al[D];
-#pragma unroll
-    for (int jj = 0; jj < D; jj++) {
-        au[jj] = pu ? ku[jj] : 0.0;
-        al[jj] = pl ? kl[jj] : 0.0;
-    }
-    // in chunks of 4 blocks: the rows and columns are loaded before any is added; the sums stay
-    // in element order
+    double au = pd(bu) ? K[(size_t)bu * DD + e] : 0.0;
+    double al = bl >= 0 && pd(bl) ? K[(size_t)bl * DD + et] : 0.0;
+    // in chunks of 4 blocks: the values are loaded before any is added; the sums stay in element order
     for (int cb = c0; cb < c1; cb += 4) {
         const int cnt = min(4, c1 - cb);
-        double vu[4][D], vl[4][D];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (k < cnt) {
-                const double *blk = S + (size_t)(cb + k) * DD;
-#pragma unroll
-                for (int jj = 0; jj < D; jj++) {
-                    vu[k][jj] = blk[ii * D + jj];
-                    vl[k][jj] = blk[jj * D + ii];
-                }
-            }
-        }
+        double v[4];
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (k < cnt)
+            if (k < cnt) v[k] = S[(size_t)(cb + k) * DD + e];
 #pragma unroll
-                for (int jj = 0; jj < D; jj++) {
-                    au[jj] = au[jj] + vu[k][jj];
-                    al[jj] = al[jj] + vl[k][jj];
-                }
+        for (int k = 0; k < 4; k++)
+            if (k < cnt) {
+                au = au + v[k];
+                al = al + v[k];
+            }
     }
-#pragma unroll
-    for (int jj = 0; jj < D; jj++) ku[jj] = au[jj];
-    if (bl >= 0)
-#pragma unroll
-        for (int jj = 0; jj < D; jj++) kl[jj] = al[jj];
+    K[(size_t)bu * DD + e] = au;
+    if (bl >= 0) K[(size_t)bl * DD + et] = al;
 }
 
 // ------------------------------------------------------------------------
@@ -1038,6 +1034,27 @@ static size_t family_lds_bytes(int D, int mf, int maxpb) {
            (size_t)P * P * sizeof(double) + (P + 2) * sizeof(int);
 }
 
+// The gather on the context's stream: the pairs without a PD part right away, the others after
+// the family kernel on stream fs (when that is the side stream) has finished.
+template <int D>
+static int gather_pairs(Ctx &c, hipStream_t fs, std::string &err) {
+    constexpr int DD = D * D;
+    auto launch = [&](int p0, int p1) {
+        if (p1 > p0)
+            k_fe_gather<D><<<(unsigned)(((long long)(p1 - p0) * DD + 255) / 256), 256, 0, c.stream>>>(
+                p1 - p0, c.fe_pair_up + p0, c.fe_pair_lo + p0, c.pair_off + p0, c.pd_bits, c.Ke_stage, c.K);
+    };
+    if (fs != c.stream) {
+        launch(0, c.n_fe_pair_fe);
+        PDX_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+        launch(c.n_fe_pair_fe, c.n_fe_pair);
+    } else {
+        launch(0, c.n_fe_pair);
+    }
+    PDX_HIP(hipGetLastError());
+    return PDX_OK;
+}
+
 // Enqueue one assembly of K on the context's stream (the status word is read by
 // dev_assemble_K_finish).
 int dev_assemble_K_launch(Ctx &c, std::string &err) {
@@ -1121,10 +1138,7 @@ int dev_assemble_K_launch(Ctx &c, std::string &err) {
                 A.list = c.fe_list_b; A.nlist = c.n_fe_b;
                 k_ke_simplex<3><<<(c.n_fe_b * 12 + 127) / 128, 128, 0, c.stream>>>(A);
             }
-            if (fs != c.stream) PDX_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
-            if (c.n_fe_pair)
-                k_fe_gather<3><<<(unsigned)(((long long)c.n_fe_pair * 3 + 255) / 256), 256, 0, c.stream>>>(
-                    c.n_fe_pair, c.fe_pair_up, c.fe_pair_lo, c.pair_off, c.pd_bits, c.Ke_stage, c.K);
+            if (int rc = gather_pairs<3>(c, fs, err)) return rc;
         } else {
             if (c.n_fe_a) {
                 A.list = c.fe_list_a; A.nlist = c.n_fe_a;
@@ -1134,10 +1148,7 @@ int dev_assemble_K_launch(Ctx &c, std::string &err) {
                 A.list = c.fe_list_b; A.nlist = c.n_fe_b;
                 k_ke_simplex<2><<<(c.n_fe_b * 6 + 127) / 128, 128, 0, c.stream>>>(A);
             }
-            if (fs != c.stream) PDX_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
-            if (c.n_fe_pair)
-                k_fe_gather<2><<<(unsigned)(((long long)c.n_fe_pair * 2 + 255) / 256), 256, 0, c.stream>>>(
-                    c.n_fe_pair, c.fe_pair_up, c.fe_pair_lo, c.pair_off, c.pd_bits, c.Ke_stage, c.K);
+            if (int rc = gather_pairs<2>(c, fs, err)) return rc;
         }
         PDX_HIP(hipGetLastError());
     }

@@ -286,6 +286,25 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
                 }
                 if (bad_pair) { err = "Error: can not build matrix in CSR format (element pair)"; return PDX_ERR_PATTERN; }
                 const int ntask = npr[m.nn];
+                // pairs without a PD part in either block first (the gather adds them while the
+                // family kernel still runs), then the others, each part in row order
+                {
+                    auto pdb = [&](int b) { return (pd_bits[(size_t)b >> 3] >> (b & 7)) & 1; };
+                    std::vector<int> pu2, pl2;
+                    pu2.reserve(ntask);
+                    pl2.reserve(ntask);
+                    for (int pass = 0; pass < 2; pass++)
+                        for (int q = 0; q < ntask; q++) {
+                            const bool with_pd = pdb(pu[q]) || (pl[q] >= 0 && pdb(pl[q]));
+                            if (with_pd == (pass == 1)) {
+                                pu2.push_back(pu[q]);
+                                pl2.push_back(pl[q]);
+                            }
+                            if (pass == 0 && !with_pd) c->n_fe_pair_fe = int(pu2.size());
+                        }
+                    pu.swap(pu2);
+                    pl.swap(pl2);
+                }
                 std::vector<long long> poff((size_t)ntask + 1, 0);
                 for (int q = 0; q < ntask; q++) poff[q + 1] = poff[q] + (fcnt[pu[q] + 1] - fcnt[pu[q]]);
                 if (poff[ntask] >= (1LL << 30)) {

@@ -85,8 +85,9 @@ struct Ctx {
     int *fe_slot = nullptr, *pair_off = nullptr;
     long long n_stage = 0;
     // node pairs of the gather: upper K blocks (r, c), r <= c, with FE terms, and their lower
-    // partners (c, r) (-1 on the diagonal)
-    int n_fe_pair = 0;
+    // partners (c, r) (-1 on the diagonal); the first n_fe_pair_fe pairs have no PD part in
+    // either block
+    int n_fe_pair = 0, n_fe_pair_fe = 0;
     int *fe_pair_up = nullptr, *fe_pair_lo = nullptr;
     unsigned char *pd_bits = nullptr;        // per K block: 1 if a family kernel stores it
     bool k_needs_fill = true;                // some K block is written by neither assembly kernel
