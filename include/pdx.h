@@ -291,6 +291,12 @@ int pdx_quasi_static_run(pdx_ctx *c, int nsteps, const pdx_solve_opts *o, int fc
                          double *u_all, double *step_info, pdx_solve_stats *st);
 /* the context's bond states, laid out like pdx_model_families' bond array */
 int pdx_ctx_bonds(pdx_ctx *c, int *bond);
+/* How K follows bond failure (FC 2 in the quasi-static, explicit and Newmark drivers): on (the
+ * default), the rows of K of the families whose bonds changed are recomputed on the last whole
+ * assembly (the family kernel for those families, then their node pairs with element terms),
+ * which gives the same K bit for bit; off, every change assembles the whole K again, as the
+ * reference rebuilds its system every step.  No counterpart in the reference's interface. */
+int pdx_set_incremental_bonds(pdx_ctx *c, int on);
 /* calLocalDamage (pdsolve.cpp:7338-7373): phi [n_node] = 1 - intact / all bonds of the node's
  * family (0 for nodes without a family), from the context's bond states */
 int pdx_local_damage(pdx_ctx *c, double *phi);

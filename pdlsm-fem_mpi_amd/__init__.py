@@ -81,6 +81,7 @@ EXPORTS = {
     "pdx_write_node_table": (_I, [_P, C.c_char_p, _P, _P, _I]),
     "pdx_quasi_static_run": (_I, [_P, _I, _P, _I, _I, _D, _P, _P, _P]),
     "pdx_ctx_bonds": (_I, [_P, _P]),
+    "pdx_set_incremental_bonds": (_I, [_P, _I]),
     "pdx_local_damage": (_I, [_P, _P]),
     "pdx_model_destroy": (None, [_P]),
     "pdx_model_settings": (_I, [_P, _P]),
@@ -516,6 +517,11 @@ class Context:
         _chk(lib().pdx_quasi_static_run(self.h, nsteps, C.byref(opts), fc, topk, sigma_ult, _p(u), _p(info),
                                         C.byref(st)))
         return u, info[:nsteps], st.as_dict()
+
+    def set_incremental_bonds(self, on=True):
+        """On (the default): bond failure recomputes only the changed families' rows of K (the same
+        K bit for bit); off: every change assembles the whole K again."""
+        _chk(lib().pdx_set_incremental_bonds(self.h, int(bool(on))))
 
     def bonds(self):
         """Bond states of this context (1 intact, 0 broken), laid out like Model.families()['bond']."""

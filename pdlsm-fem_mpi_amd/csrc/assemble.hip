@@ -135,6 +135,7 @@ struct FamArgs {
     int *status;
     int mf;     // LDS capacity in members
     int maxpb;  // LDS capacity in PD-boundary entities per family (3D N.D cache)
+    const int *flist = nullptr;  // families of the blocks (null: block f is family f)
 };
 
 // normMatArea (pdsolve.cpp:2917-2967): area-weighted face normal matrix 3x6
@@ -269,7 +270,7 @@ __global__ void __launch_bounds__(64) k_family(FamArgs A) {
 #pragma clang fp contract(off)
     constexpr int P = Dm<D>::P, DD = Dm<D>::DD;
     extern __shared__ double lds[];
-    const int f = blockIdx.x, lane = threadIdx.x;
+    const int f = A.flist ? A.flist[blockIdx.x] : (int)blockIdx.x, lane = threadIdx.x;
     const int off = A.fam_off[f], n = A.fam_off[f + 1] - off;
     const int mf = A.mf;
     double *sx = lds;               // [mf]
@@ -882,6 +883,43 @@ __global__ void k_fe_gather(int npair, const int *__restrict__ pair_up, const in
     if (bl >= 0) K[(size_t)bl * DD + et] = al;
 }
 
+// The same sums for the node pairs with a PD part whose row r (upper block) or column c (lower
+// block) belongs to a family recomputed after bond changes (rowflag), writing only those blocks:
+// their PD parts are the family kernel's new values, the FE terms the staged element blocks of
+// the last whole assembly, added in the same order, so K equals a whole assembly bit for bit.
+template <int D>
+__global__ void k_fe_gather_rows(int npair, const int *__restrict__ pair_up, const int *__restrict__ pair_lo,
+                                 const int *__restrict__ pair_off, const int *__restrict__ pair_row,
+                                 const int *__restrict__ adj, const unsigned char *__restrict__ rowflag,
+                                 const unsigned char *__restrict__ pd_bits, const double *__restrict__ S, double *K) {
+    constexpr int DD = D * D;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)npair * DD) return;
+    const int pr = int(t / DD), e = int(t % DD), et = (e % D) * D + e / D;
+    const int bu = pair_up[pr], bl = pair_lo[pr];
+    const bool du = rowflag[pair_row[pr]], dl = bl >= 0 && rowflag[adj[bu]];
+    if (!du && !dl) return;
+    const int c0 = pair_off[pr], c1 = pair_off[pr + 1];
+    auto pd = [&](int q) { return (pd_bits[q >> 3] >> (q & 7)) & 1; };
+    double au = pd(bu) ? K[(size_t)bu * DD + e] : 0.0;
+    double al = bl >= 0 && pd(bl) ? K[(size_t)bl * DD + et] : 0.0;
+    for (int cb = c0; cb < c1; cb += 4) {
+        const int cnt = min(4, c1 - cb);
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k < cnt) v[k] = S[(size_t)(cb + k) * DD + e];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k < cnt) {
+                au = au + v[k];
+                al = al + v[k];
+            }
+    }
+    if (du) K[(size_t)bu * DD + e] = au;
+    if (dl) K[(size_t)bl * DD + et] = al;
+}
+
 // ------------------------------------------------------------------------
 // lumped mass (pdsolve.cpp:4618-4669), node-centric over all elements
 // ------------------------------------------------------------------------
@@ -1034,6 +1072,78 @@ static size_t family_lds_bytes(int D, int mf, int maxpb) {
            (size_t)P * P * sizeof(double) + (P + 2) * sizeof(int);
 }
 
+// arguments and LDS bytes of the family kernel over all families (A.flist null)
+static int family_args(Ctx &c, int *status, FamArgs &A, size_t &lds, std::string &err) {
+    A.x = c.x; A.nv = c.nv; A.fam_node = c.fam_node; A.fam_off = c.fam_off; A.fam_mem = c.fam_mem;
+    A.bond = c.bond; A.fam_blk = c.fam_blk; A.fam_v = c.fam_v; A.fam_delta = c.fam_delta;
+    A.fp_off = c.fp_off; A.fp_list = c.fp_list; A.pdbe = c.pdbe; A.Dm = c.Dm;
+    A.lam = c.lambda; A.mu = c.mu; A.nlf = c.nlf; A.K = c.K; A.nblk = c.nblk; A.status = status;
+    A.mf = c.max_family;
+    A.maxpb = c.max_pdbe_per_family;
+    A.flist = nullptr;
+    if (!c.fam_c) {
+        int nmem = 0;
+        PDX_HIP(hipMemcpy(&nmem, c.fam_off + c.nfam, sizeof(int), hipMemcpyDeviceToHost));
+        PDX_HIP(hipMalloc((void **)&c.fam_c, std::max<size_t>((size_t)nmem * c.dim, 1) * sizeof(double)));
+        PDX_HIP(hipMemsetAsync(c.fam_c, 0, std::max<size_t>((size_t)nmem * c.dim, 1) * sizeof(double), c.stream));
+    }
+    A.fam_c = c.fam_c;
+    lds = family_lds_bytes(c.dim, A.mf, A.maxpb);
+    if (lds > 160 * 1024) {
+        err = "family too large for LDS (" + std::to_string(c.max_family) + " members)";
+        return PDX_ERR_LIMIT;
+    }
+    return PDX_OK;
+}
+
+template <int D>
+static int launch_family(const FamArgs &A, int nblocks, size_t lds, hipStream_t s, std::string &err) {
+    PDX_HIP(hipFuncSetAttribute((const void *)k_family<D>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_family<D><<<nblocks, 64, lds, s>>>(A);
+    PDX_HIP(hipGetLastError());
+    return PDX_OK;
+}
+
+// Bond changes applied to a whole assembly of K: the family kernel recomputes the rows of the
+// changed families (c.dirty_fams), then the node pairs with a PD part in those rows add their
+// staged element blocks again (the staging of the last whole assembly; elements do not change).
+static int update_K_bonds(Ctx &c, std::string &err) {
+    std::vector<int> fams(c.dirty_fams);
+    std::sort(fams.begin(), fams.end());
+    fams.erase(std::unique(fams.begin(), fams.end()), fams.end());
+    const int nf = (int)fams.size();
+    int *status = c.asm_status;
+    PDX_HIP(hipMemsetAsync(status, 0, sizeof(int), c.stream));
+    if (!c.d_flist) PDX_HIP(hipMalloc((void **)&c.d_flist, std::max(c.nfam, 1) * sizeof(int)));
+    if (!c.d_rowflag) PDX_HIP(hipMalloc((void **)&c.d_rowflag, std::max(c.nn, 1)));
+    const Model &m = *c.host;
+    std::vector<unsigned char> flag(c.nn, 0);
+    for (int f : fams) flag[m.fam_node[f]] = 1;
+    PDX_HIP(hipMemcpyAsync(c.d_flist, fams.data(), nf * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    PDX_HIP(hipMemcpyAsync(c.d_rowflag, flag.data(), c.nn, hipMemcpyHostToDevice, c.stream));
+    FamArgs A;
+    size_t lds = 0;
+    if (int rc = family_args(c, status, A, lds, err)) return rc;
+    A.flist = c.d_flist;
+    if (int rc = c.dim == 3 ? launch_family<3>(A, nf, lds, c.stream, err) : launch_family<2>(A, nf, lds, c.stream, err))
+        return rc;
+    const int p0 = c.n_fe_pair_fe, np = c.n_fe_pair - p0;
+    if (np > 0) {
+        if (c.dim == 3)
+            k_fe_gather_rows<3><<<(unsigned)(((long long)np * 9 + 255) / 256), 256, 0, c.stream>>>(
+                np, c.fe_pair_up + p0, c.fe_pair_lo + p0, c.pair_off + p0, c.fe_pair_row, c.adj, c.d_rowflag,
+                c.pd_bits, c.Ke_stage, c.K);
+        else
+            k_fe_gather_rows<2><<<(unsigned)(((long long)np * 4 + 255) / 256), 256, 0, c.stream>>>(
+                np, c.fe_pair_up + p0, c.fe_pair_lo + p0, c.pair_off + p0, c.fe_pair_row, c.adj, c.d_rowflag,
+                c.pd_bits, c.Ke_stage, c.K);
+        PDX_HIP(hipGetLastError());
+    }
+    // the host copies of fams / flag must outlive the copies above
+    PDX_HIP(hipStreamSynchronize(c.stream));
+    return PDX_OK;
+}
+
 // The gather on the context's stream: the pairs without a PD part right away, the others after
 // the family kernel on stream fs (when that is the side stream) has finished.
 template <int D>
@@ -1065,24 +1175,8 @@ int dev_assemble_K_launch(Ctx &c, std::string &err) {
     hipStream_t fs = c.stream;  // stream of the family kernel
     if (c.nfam > 0) {
         FamArgs A;
-        A.x = c.x; A.nv = c.nv; A.fam_node = c.fam_node; A.fam_off = c.fam_off; A.fam_mem = c.fam_mem;
-        A.bond = c.bond; A.fam_blk = c.fam_blk; A.fam_v = c.fam_v; A.fam_delta = c.fam_delta;
-        A.fp_off = c.fp_off; A.fp_list = c.fp_list; A.pdbe = c.pdbe; A.Dm = c.Dm;
-        A.lam = c.lambda; A.mu = c.mu; A.nlf = c.nlf; A.K = c.K; A.nblk = c.nblk; A.status = status;
-        A.mf = c.max_family;
-        A.maxpb = c.max_pdbe_per_family;
-        if (!c.fam_c) {
-            int nmem = 0;
-            PDX_HIP(hipMemcpy(&nmem, c.fam_off + c.nfam, sizeof(int), hipMemcpyDeviceToHost));
-            PDX_HIP(hipMalloc((void **)&c.fam_c, std::max<size_t>((size_t)nmem * c.dim, 1) * sizeof(double)));
-            PDX_HIP(hipMemsetAsync(c.fam_c, 0, std::max<size_t>((size_t)nmem * c.dim, 1) * sizeof(double), c.stream));
-        }
-        A.fam_c = c.fam_c;
-        size_t lds = family_lds_bytes(c.dim, A.mf, A.maxpb);
-        if (lds > 160 * 1024) {
-            err = "family too large for LDS (" + std::to_string(c.max_family) + " members)";
-            return PDX_ERR_LIMIT;
-        }
+        size_t lds = 0;
+        if (int rc = family_args(c, status, A, lds, err)) return rc;
         // with elements to do as well, the families run on the side stream beside the element
         // kernel (they write disjoint arrays: K's PD blocks and fam_c, the element stage); the
         // gather joins both
@@ -1094,14 +1188,8 @@ int dev_assemble_K_launch(Ctx &c, std::string &err) {
             PDX_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
             fs = c.side;
         }
-        if (c.dim == 3) {
-            PDX_HIP(hipFuncSetAttribute((const void *)k_family<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k_family<3><<<c.nfam, 64, lds, fs>>>(A);
-        } else {
-            PDX_HIP(hipFuncSetAttribute((const void *)k_family<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k_family<2><<<c.nfam, 64, lds, fs>>>(A);
-        }
-        PDX_HIP(hipGetLastError());
+        if (int rc = c.dim == 3 ? launch_family<3>(A, c.nfam, lds, fs, err) : launch_family<2>(A, c.nfam, lds, fs, err))
+            return rc;
         if (fs != c.stream) PDX_HIP(hipEventRecord(c.ev_join, fs));
     }
     if (c.nfe > 0) {
@@ -1165,12 +1253,20 @@ int dev_assemble_K_finish(Ctx &c, std::string &err) {
     if (hs & 4) { err = "Area is zero (PD boundary face)"; return PDX_ERR_MESH; }
     if (hs & 8) { err = "family exceeds the LDS capacity"; return PDX_ERR_LIMIT; }
     c.K_valid = true;
+    c.K_full = true;
+    c.dirty_fams.clear();
     c.K_gen++;
     c.K32_valid = false;
     return PDX_OK;
 }
 
 int dev_assemble_K(Ctx &c, std::string &err) {
+    // bond changes on a whole assembly: only the changed families' rows are recomputed
+    if (!c.K_valid && c.K_full && c.incremental_bonds && !c.dirty_fams.empty() && c.asm_status) {
+        c.K_full = false;  // until the update below has finished
+        if (int rc = update_K_bonds(c, err)) return rc;
+        return dev_assemble_K_finish(c, err);
+    }
     if (int rc = dev_assemble_K_launch(c, err)) return rc;
     return dev_assemble_K_finish(c, err);
 }

@@ -270,7 +270,7 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
                     npr[r + 1] = n;
                 }
                 for (int r = 0; r < m.nn; r++) npr[r + 1] += npr[r];
-                std::vector<int> pu((size_t)npr[m.nn]), pl((size_t)npr[m.nn]);
+                std::vector<int> pu((size_t)npr[m.nn]), pl((size_t)npr[m.nn]), prow((size_t)npr[m.nn]);
                 int bad_pair = 0;
 #pragma omp parallel for schedule(dynamic, 256) reduction(| : bad_pair)
                 for (int r = 0; r < m.nn; r++) {
@@ -281,6 +281,7 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
                         const int l = col > r ? find_block(m, col, r) : -1;
                         if (col > r && l < 0) bad_pair = 1;
                         pu[q] = b;
+                        prow[q] = r;
                         pl[q++] = l;
                     }
                 }
@@ -290,20 +291,23 @@ int ctx_build(const Model &m, int device, Ctx *&out, std::string &err) {
                 // family kernel still runs), then the others, each part in row order
                 {
                     auto pdb = [&](int b) { return (pd_bits[(size_t)b >> 3] >> (b & 7)) & 1; };
-                    std::vector<int> pu2, pl2;
+                    std::vector<int> pu2, pl2, prow_pd;
                     pu2.reserve(ntask);
                     pl2.reserve(ntask);
-                    for (int pass = 0; pass < 2; pass++)
+                    for (int pass = 0; pass < 2; pass++) {
                         for (int q = 0; q < ntask; q++) {
                             const bool with_pd = pdb(pu[q]) || (pl[q] >= 0 && pdb(pl[q]));
                             if (with_pd == (pass == 1)) {
                                 pu2.push_back(pu[q]);
                                 pl2.push_back(pl[q]);
+                                if (pass == 1) prow_pd.push_back(prow[q]);
                             }
-                            if (pass == 0 && !with_pd) c->n_fe_pair_fe = int(pu2.size());
                         }
+                        if (pass == 0) c->n_fe_pair_fe = int(pu2.size());
+                    }
                     pu.swap(pu2);
                     pl.swap(pl2);
+                    UP(c->fe_pair_row, prow_pd);
                 }
                 std::vector<long long> poff((size_t)ntask + 1, 0);
                 for (int q = 0; q < ntask; q++) poff[q + 1] = poff[q] + (fcnt[pu[q] + 1] - fcnt[pu[q]]);
@@ -463,6 +467,8 @@ void ctx_free(Ctx *c) {
     if (c->K64s) hipFree(c->K64s);
     if (c->Mk) hipFree(c->Mk);
     if (c->Ke_stage) hipFree(c->Ke_stage);
+    if (c->d_flist) hipFree(c->d_flist);
+    if (c->d_rowflag) hipFree(c->d_rowflag);
     if (c->red) hipFree(c->red);
     if (c->learned_Minv) hipFree(c->learned_Minv);
     if (c->gnn_keff) hipFree(c->gnn_keff);
@@ -1091,6 +1097,7 @@ static int fc2_step(pdx_ctx *pc, const double *sig_dev, const std::vector<double
     nkept = (int)cand.size();
     nbroke = 0;
     const double minD = 0.5 * m.minDelta;
+    const std::vector<int> bond_before = cand.empty() ? std::vector<int>() : pc->bond;
     for (const Cand &k : cand) {
         const int nk = m.fam_node[k.f], nm = m.fam_mem[k.q];
         const double *xk = &m.x[(size_t)nk * 3], *xm = &m.x[(size_t)nm * 3];
@@ -1121,7 +1128,15 @@ static int fc2_step(pdx_ctx *pc, const double *sig_dev, const std::vector<double
     }
     if (nbroke) {
         PDX_HIP(hipMemcpy(c.bond, pc->bond.data(), nb * sizeof(int), hipMemcpyHostToDevice));
-        c.K_valid = false;  // the next assembly sees the new bond states
+        // the next assembly sees the new bond states: the families with a changed bond get their
+        // rows of K recomputed
+        for (int f = 0; f < m.nfam; f++)
+            for (int q = m.fam_off[f] + 1; q < m.fam_off[f + 1]; q++)
+                if (pc->bond[q] != bond_before[q]) {
+                    c.dirty_fams.push_back(f);
+                    break;
+                }
+        c.K_valid = false;
     }
     return PDX_OK;
 }
@@ -1316,6 +1331,12 @@ int pdx_quasi_static_run(pdx_ctx *pc, int nsteps, const pdx_solve_opts *op, int 
 int pdx_ctx_bonds(pdx_ctx *pc, int *bond) {
     if (!pc || !bond) return fail(PDX_ERR_ARG, "null argument");
     std::copy(pc->bond.begin(), pc->bond.end(), bond);
+    return PDX_OK;
+}
+
+int pdx_set_incremental_bonds(pdx_ctx *pc, int on) {
+    if (!pc) return fail(PDX_ERR_ARG, "null argument");
+    pc->c->incremental_bonds = on != 0;
     return PDX_OK;
 }
 

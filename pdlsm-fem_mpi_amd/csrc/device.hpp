@@ -89,6 +89,7 @@ struct Ctx {
     // either block
     int n_fe_pair = 0, n_fe_pair_fe = 0;
     int *fe_pair_up = nullptr, *fe_pair_lo = nullptr;
+    int *fe_pair_row = nullptr;  // row node r of each pair with a PD part (pairs n_fe_pair_fe ...)
     unsigned char *pd_bits = nullptr;        // per K block: 1 if a family kernel stores it
     bool k_needs_fill = true;                // some K block is written by neither assembly kernel
     int n_fe_a = 0, n_fe_b = 0;  // FE-list positions of hex8/quad4 (a) and tet4/tri3 (b) elements
@@ -140,6 +141,13 @@ struct Ctx {
     double *Mk = nullptr;  // consistent mass on K blocks (Newmark)
     bool K_valid = false, K32_valid = false, Mk_valid = false;
     long long K_gen = 0;  // incremented by every assembly of K
+    // bond changes since K was assembled: the families whose bond states changed, applied to K
+    // by recomputing those families' rows (K_full: K holds a whole assembly; incremental_bonds:
+    // off, every change assembles K again from scratch)
+    std::vector<int> dirty_fams;
+    bool K_full = false, incremental_bonds = true;
+    int *d_flist = nullptr;
+    unsigned char *d_rowflag = nullptr;
     double *mlump = nullptr;  // [nn*d] lumped mass (1 for prescribed)
     bool mlump_valid = false;
 

@@ -168,3 +168,45 @@ def test_dynamic_drivers_reject_unsupported_failure_criteria():
         cx.explicit_run(2)
     with pytest.raises(pdx.PdxError, match="FC 1"):
         cx.newmark_run(2)
+
+
+def _run_fc2(driver, m, incremental, n):
+    cx = pdx.Context(pdx.Model(m))
+    cx.set_incremental_bonds(incremental)
+    if driver == "explicit":
+        u, up = cx.explicit_run(n)
+        out, info = (u, up), cx.step_log()
+    elif driver == "newmark":
+        u, v, a, st = cx.newmark_run(n, pdx.solve_opts(precision=0, precond=2, tol=1e-12, max_iter=100000))
+        out, info = (u, v, a), cx.step_log()
+    else:
+        u, info, st = cx.quasi_static_run(n, pdx.solve_opts(precision=0, precond=2, tol=1e-12, max_iter=100000),
+                                          fc=2, topk=2, sigma_ult=3.56e7)
+        out = (u,)
+    return out, np.asarray(info), cx.bonds()
+
+
+@pytest.mark.parametrize("driver", ["explicit", "newmark", "quasi_static"])
+def test_fc2_incremental_update_equals_whole_assembly(driver):
+    """Bond failure applied to K by recomputing only the changed families' rows (the default)
+    gives, bit for bit, the run that assembles the whole K again after every change: the DOF
+    values, the bonds selected and broken per step and the final bond states, into the cascade
+    where bonds break in most steps."""
+    if driver == "quasi_static":
+        m = W.crack2d(seed=1, nx=41, ny=21, dx=1e-3)
+        m["ebc"][2] = dict(nid=m["ebc"][2]["nid"], dof=1, val=0.0, vary=True, vel=2e-6)
+        n = 10
+    else:
+        m = _fc2_case(driver == "explicit")
+        n = 40 if driver == "explicit" else 12
+        if driver == "newmark":
+            m["dt"] = 2e-7
+        m["sigma_ult"] = 1e30
+        _, probe, _ = _run_fc2(driver, m, True, n)
+        m["sigma_ult"] = float(probe[n // 4, 2])  # bonds break from a quarter of the run on
+    a_out, a_info, a_bond = _run_fc2(driver, m, True, n)
+    b_out, b_info, b_bond = _run_fc2(driver, m, False, n)
+    assert a_info[:, 1].sum() > 0 and (a_info[:, 1] > 0).sum() >= 2
+    for x, y in zip(a_out, b_out):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a_info, b_info) and np.array_equal(a_bond, b_bond)
