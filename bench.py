@@ -112,7 +112,7 @@ def own_blocks(m):
     return int(len(adj) - mirrored.sum())
 
 
-def survey_step_bytes(m, prec):
+def scalar_csr_step_bytes(m, prec):
     """The scalar-CSR figure: nnz (s_v + 4) + (N_eq + 1) 4 + N_dof s_x + N_eq (8*4 + 8)."""
     sv = 4 if prec == 1 else 8
     return m.nnz * (sv + 4) + (m.neq + 1) * 4 + m.nn * m.dim * sv + m.neq * 40
@@ -680,7 +680,7 @@ def main():
                       "bytes_per_launch": bytes_step,
                       "column_classes": {"classes": classes[0], "offsets": classes[1]},
                       "mirrored_storage": {"own_blocks": n_own, "blocks": nblk},
-                      "scalar_csr_bytes_per_step": survey_step_bytes(model, prec)}
+                      "scalar_csr_bytes_per_step": scalar_csr_step_bytes(model, prec)}
     # the headline is the reference's precision (FP64 values, FP64 accumulation, FP64 state);
     # the FP32-force / FP64-state figure and its drift against FP64 follow as secondary keys
     head = {"metric": METRIC, "value": res[0]["dofs"], "unit": "DOF-steps/s", "n_gpus": 1, "steps": args.steps,
