@@ -601,7 +601,7 @@ def main():
         hbm_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: the whole mesh lives in one HBM and this build has no "
+        print("bench.py runs on one GPU only: the whole mesh lives in one HBM and the library has no "
               "multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     pdx = load_pdx()

@@ -1,7 +1,7 @@
 """CPU oracle of the learned smoother's network and loss -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline may use this
-module, and only as the checker of the HIP library (libpdx.so).
+module, and only as the reference the HIP library (libpdx.so) is tested against.
 
 The learned smoother has no counterpart in the reference: QibangLiu/PDLSM-FEM_mpi
 solves its systems with MKL PARDISO (SourceFiles/Solver/calMatrixOperations.cpp:378-498)

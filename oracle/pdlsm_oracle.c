@@ -1,8 +1,8 @@
 /*
  * pdlsm_oracle.c -- CPU restatement of the PDLSM-FEM hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
- * pdlsm-fem_mpi_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * TEST INFRASTRUCTURE ONLY.  The HIP library in pdlsm-fem_mpi_amd/ is tested against
+ * this file: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.  Nothing in the product links or calls it.
  *
  * It restates, in plain single-threaded C, the algorithms of
