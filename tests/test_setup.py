@@ -137,6 +137,37 @@ def test_c_abi_exports_every_declared_symbol():
     assert pdx.lib().pdx_version().decode().startswith("pdx")
 
 
+def test_context_entry_points_reject_a_null_context():
+    """Every context entry point returns PDX_ERR_ARG for a null context (before any device
+    call) and leaves a message in pdx_last_error."""
+    L = pdx.lib()
+    null = None
+    buf = (ctypes.c_double * 8)()
+    ibuf = (ctypes.c_int * 8)()
+    calls = {
+        "pdx_explicit_steps": (null, 1, 0, 1),
+        "pdx_explicit_run": (null, 1, 0, buf, buf),
+        "pdx_newmark_run": (null, 1, null, buf, buf, buf, null),
+        "pdx_node_stresses": (null, buf, buf),
+        "pdx_node_stresses_noalloc": (null, buf, buf),
+        "pdx_assemble": (null, 0, null, buf, buf, 1),
+        "pdx_static_solve": (null, null, buf, null),
+        "pdx_quasi_static_run": (null, 1, null, 2, 1, 1.0, buf, buf, null),
+        "pdx_last_timing": (null, buf, buf),
+        "pdx_step_log": (null, buf, ibuf),
+        "pdx_set_incremental_bonds": (null, 1),
+        "pdx_ctx_bonds": (null, ibuf),
+        "pdx_local_damage": (null, buf),
+        "pdx_explicit_set_form": (null, 0, 0),
+        "pdx_max_eig": (null, 10, buf),
+        "pdx_set_dt": (null, 1e-7),
+    }
+    for name, args in calls.items():
+        rc = getattr(L, name)(*args)
+        assert rc == -1, (name, rc)
+        assert L.pdx_last_error(), name
+
+
 def test_python_constants_match_the_header():
     hdr = open(os.path.join(ROOT, "include", "pdx.h")).read()
     defs = dict(re.findall(r"#define\s+(PDX_[A-Z0-9_]+)\s+(-?\d+)", hdr))
