@@ -68,13 +68,17 @@ __device__ inline void row_matvec(int r, int lane, const int *__restrict__ adj_o
         for (int i = 0; i < D; i++) acc[i] += __shfl_xor(acc[i], s, G);
 }
 
+// acc[lane] for lane < D (0 for the other lanes), as an OR of the selected values' bit patterns:
+// a chain of `if (lane == i) v = acc[i]` is folded by the compiler into one read at a lane-dependent
+// offset, which keeps acc in memory (LDS or scratch) instead of registers
 template <int D, typename T>
 __device__ inline T pick(const T acc[D], int lane) {
-    T v = acc[0];
+    using U = std::conditional_t<sizeof(T) == 8, unsigned long long, unsigned int>;
+    static_assert(sizeof(T) == sizeof(U), "pick: 4- or 8-byte values");
+    U bits = 0;
 #pragma unroll
-    for (int i = 1; i < D; i++)
-        if (lane == i) v = acc[i];
-    return v;
+    for (int i = 0; i < D; i++) bits |= lane == i ? __builtin_bit_cast(U, acc[i]) : U(0);
+    return __builtin_bit_cast(T, bits);
 }
 
 }  // namespace pdx
