@@ -174,6 +174,9 @@ def _run_fc2(driver, m, incremental, n):
     cx = pdx.Context(pdx.Model(m))
     cx.set_incremental_bonds(incremental)
     if driver == "explicit":
+        # one form (one order of the row sums) for both runs: the tuner picks by timing, and forms
+        # with different lane counts round the row sums differently
+        cx.explicit_set_form(0, 0)
         u, up = cx.explicit_run(n)
         out, info = (u, up), cx.step_log()
     elif driver == "newmark":
