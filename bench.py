@@ -447,7 +447,8 @@ def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
     same with the learned smoother (a 3D network trained on the Newmark operator
     of smaller penny boxes, predicting the smoother of this run's K_eff), and
     the two block-Jacobi runs again with the SpMVs on mirrored storage (same
-    results bit for bit).  The other runs' displacements are compared with the
+    results bit for bit).  The two-level and learned runs read mirrored storage
+    (the faster of the two for the 3-DOF SpMV, same bits).  The other runs' displacements are compared with the
     plain run's."""
     m0 = W.penny3d(n=n)
     cx0 = pdx.Context(pdx.Model(m0))
@@ -482,9 +483,9 @@ def newmark_bench(pdx, W, n=100, steps=50, coarse_agg=10):
             "fp32_ir_blockjacobi": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_BLOCK_JACOBI, tol=1e-8,
                                                   max_iter=100000),
             "fp32_ir_two_level": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_TWO_LEVEL, tol=1e-8,
-                                                max_iter=100000, coarse_agg=coarse_agg),
+                                                max_iter=100000, coarse_agg=coarse_agg, mirrored=True),
             "fp32_ir_learned": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_LEARNED, tol=1e-8,
-                                              max_iter=100000, coarse_agg=coarse_agg),
+                                              max_iter=100000, coarse_agg=coarse_agg, mirrored=True),
             "plain_fp64_blockjacobi_mirrored": solve_opts(pdx, precision=0, precond=pdx.PDX_PREC_BLOCK_JACOBI,
                                                               tol=1e-8, max_iter=100000, mirrored=True),
             "fp32_ir_blockjacobi_mirrored": solve_opts(pdx, precision=1, precond=pdx.PDX_PREC_BLOCK_JACOBI,

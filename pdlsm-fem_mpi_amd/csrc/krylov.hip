@@ -45,13 +45,120 @@ int ensure_K32(Ctx &c, std::string &err);
 // algebraically identical; the block rows of a stiffness sum to ~0 (rigid
 // translations), so in FP32 this avoids the cancellation of the diagonal
 // against the neighbours for smooth vectors.
-// Rows [lb * per, (lb + 1) * per) of the list go to logical block lb; the G-lane groups of the
-// block stride through them.  With REL the block columns come from the row's column pattern
-// class (cls, cls_off, cls_rel) instead of adj.  With SYM, K holds only the own blocks of the
-// mirrored storage and src[b] is block b's own index, or ~(its partner's) for a lower block read
-// as its partner's transpose (the same values, so the same products in the same order).
+// With REL the block columns come from the row's column pattern class (cls, cls_off, cls_rel)
+// instead of adj.  With SYM, K holds only the own blocks of the mirrored storage and src[b] is
+// block b's own index, or ~(its partner's) for a lower block read as its partner's transpose (the
+// same values, so the same products in the same order).
+// One row per G-lane group: lane < D of the group writes y and adds its share of the dots.
 template <int D, typename T, typename V, int G, bool DIFF, bool REL, bool SYM>
-__device__ inline void spmv_dot_rows(const int *rows, int nrows, int lb, int nblocks, const int *adj_off,
+__device__ inline void spmv_dot_row(int r, const int *adj_off, const int *adj, const int *cls, const int *cls_off,
+                                    const int *cls_rel, const int *src, const T *K, const T *RS, const V *x, V *y,
+                                    const int *eq, const V *w0, const V *w1, double &d0, double &d1) {
+    const int lane = threadIdx.x % G;
+    T acc[D];
+    const int *rel = REL ? cls_rel + cls_off[cls[r]] : nullptr;
+    row_matvec<D, T, V, G, DIFF, false, false, REL, SYM>(r, lane, adj_off, adj, K, x, acc, rel, src);
+    if (lane < D) {
+        T v = pick<D, T>(acc, lane);
+        if (DIFF) {
+#pragma unroll
+            for (int j = 0; j < D; j++) v += RS[(size_t)r * D * D + lane * D + j] * T(x[(size_t)r * D + j]);
+        }
+        const size_t o = (size_t)r * D + lane;
+        const V yv = eq[r * 3 + lane] >= 0 ? V(v) : V(0);
+        y[o] = yv;
+        if (w0) d0 += double(w0[o]) * double(yv);
+        if (w1) d1 += double(w1[o]) * double(yv);
+    }
+}
+
+// Launch shape of k_spmv_dot: 256-thread blocks, the short rows 32 to a block (8 lanes each), then
+// the long rows 4 to a block (a wavefront each); nbs short-row blocks of nbt, the grid a multiple
+// of 8 (one partial per block)
+struct SpmvShape {
+    int nbs = 0, nbt = 1, grid = 8;
+};
+inline SpmvShape spmv_shape(const Ctx &c) {
+    SpmvShape s;
+    s.nbs = (c.n_short * 8 + 255) / 256;
+    const int nbl = (c.n_long * 64 + 255) / 256;
+    s.nbt = std::max(s.nbs + nbl, 1);
+    s.grid = 8 * ((s.nbs + 7) / 8 + (s.nbt - s.nbs + 7) / 8);
+    return s;
+}
+
+// The hardware deals blocks to the 8 XCDs round-robin.  Block b works on logical block
+// x * ceil(nbs / 8) + q (short rows) or nbs + x * ceil((nbt - nbs) / 8) + q - ceil(nbs / 8) (long
+// rows), x = b % 8, q = b / 8: each XCD (and its L2) gets one contiguous eighth of the short-row
+// blocks and one of the long-row blocks, so its gathers of x stay local and the long rows are
+// spread over all XCDs.  Each block writes its dot partials to bp0 / bp1[b] (0 for a block with no
+// rows); k_fold_parts sums them in block order into the partials the vector kernels read.
+template <int D, typename T, typename V, bool REL, bool SYM>
+__global__ void __launch_bounds__(256) k_spmv_dot(const int *rows_s, int ns, int nbs, const int *rows_l, int nl,
+                                                  int nbt, const int *adj_off, const int *adj, const int *cls,
+                                                  const int *cls_off, const int *cls_rel, const int *src,
+                                                  const T *K, const T *RS, const V *x, V *y, const int *eq,
+                                                  const V *w0, double *bp0, const V *w1, double *bp1) {
+    __shared__ double sm[17];
+    double d0 = 0.0, d1 = 0.0;
+    const int per = gridDim.x / 8, q = blockIdx.x / 8, xcd = blockIdx.x % 8, ps = (nbs + 7) / 8;
+    int lb = -1;
+    if (q < ps) {
+        if (xcd * ps + q < nbs) lb = xcd * ps + q;
+    } else {
+        const int l = nbs + xcd * (per - ps) + (q - ps);
+        if (l < nbt) lb = l;
+    }
+    if (lb >= 0 && lb < nbs) {
+        const int gid = lb * 32 + (int)threadIdx.x / 8;
+        if (gid < ns) {
+            if (RS)
+                spmv_dot_row<D, T, V, 8, true, REL, SYM>(rows_s[gid], adj_off, adj, cls, cls_off, cls_rel, src, K, RS,
+                                                         x, y, eq, w0, w1, d0, d1);
+            else
+                spmv_dot_row<D, T, V, 8, false, REL, SYM>(rows_s[gid], adj_off, adj, cls, cls_off, cls_rel, src, K, RS,
+                                                          x, y, eq, w0, w1, d0, d1);
+        }
+    } else if (lb >= nbs) {
+        const int gid = (lb - nbs) * 4 + (int)threadIdx.x / 64;
+        if (gid < nl) {
+            if (RS)
+                spmv_dot_row<D, T, V, 64, true, REL, SYM>(rows_l[gid], adj_off, adj, cls, cls_off, cls_rel, src, K,
+                                                          RS, x, y, eq, w0, w1, d0, d1);
+            else
+                spmv_dot_row<D, T, V, 64, false, REL, SYM>(rows_l[gid], adj_off, adj, cls, cls_off, cls_rel, src, K,
+                                                           RS, x, y, eq, w0, w1, d0, d1);
+        }
+    }
+    if (bp0) {
+        d0 = block_sum(d0, sm);
+        if (threadIdx.x == 0) bp0[blockIdx.x] = d0;
+    }
+    if (bp1) {
+        d1 = block_sum(d1, sm);
+        if (threadIdx.x == 0) bp1[blockIdx.x] = d1;
+    }
+}
+
+// part[i] = the sum, in order, of bp[j] over the i-th of NPART contiguous ranges of [0, nbp)
+__global__ void __launch_bounds__(256) k_fold_parts(int nbp, const double *bp0, double *p0, const double *bp1,
+                                                    double *p1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPART) return;
+    const int per = (nbp + NPART - 1) / NPART, j0 = min(nbp, i * per), j1 = min(nbp, j0 + per);
+    double s0 = 0.0, s1 = 0.0;
+    for (int j = j0; j < j1; j++) {
+        if (bp0) s0 += bp0[j];
+        if (bp1) s1 += bp1[j];
+    }
+    if (p0) p0[i] = s0;
+    if (p1) p1[i] = s1;
+}
+
+// Persistent form: rows [lb * per, (lb + 1) * per) of the list go to logical block lb, and the
+// G-lane groups of the block stride through them (REL, SYM and RS as in spmv_dot_row).
+template <int D, typename T, typename V, int G, bool DIFF, bool REL, bool SYM>
+__device__ inline void spmv_dot_rows_p(const int *rows, int nrows, int lb, int nblocks, const int *adj_off,
                                      const int *adj, const int *cls, const int *cls_off, const int *cls_rel,
                                      const int *src, const T *K, const T *RS, const V *x, V *y, const int *eq,
                                      const V *w0, const V *w1, double &d0, double &d1) {
@@ -77,10 +184,12 @@ __device__ inline void spmv_dot_rows(const int *rows, int nrows, int lb, int nbl
     }
 }
 
-// The hardware deals blocks to the 8 XCDs round-robin: logical block (b % 8) * (grid / 8) + b / 8
-// gives each XCD (and its L2) one contiguous eighth of the rows, so its gathers of x stay local.
+// Persistent form for the 1- and 2-DOF operators: RED_PARTS blocks of SPMV_THREADS threads, logical
+// block (b % 8) * (grid / 8) + b / 8 (each XCD, and its L2, gets one contiguous eighth of the rows)
+// walking its share of the short, then the long rows, one partial per block straight into the NPART
+// slots (no fold: their iterations are short enough that a second launch per product shows).
 template <int D, typename T, typename V, bool REL, bool SYM>
-__global__ void __launch_bounds__(SPMV_THREADS) k_spmv_dot(const int *rows_s, int ns, const int *rows_l, int nl,
+__global__ void __launch_bounds__(SPMV_THREADS) k_spmv_dot_p(const int *rows_s, int ns, const int *rows_l, int nl,
                                                   const int *adj_off, const int *adj, const int *cls,
                                                   const int *cls_off, const int *cls_rel, const int *src,
                                                   const T *K, const T *RS, const V *x, V *y, const int *eq,
@@ -90,14 +199,14 @@ __global__ void __launch_bounds__(SPMV_THREADS) k_spmv_dot(const int *rows_s, in
     const int nb = gridDim.x;
     const int lb = nb % 8 == 0 ? (int)(blockIdx.x % 8) * (nb / 8) + (int)blockIdx.x / 8 : (int)blockIdx.x;
     if (RS) {
-        spmv_dot_rows<D, T, V, 8, true, REL, SYM>(rows_s, ns, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
+        spmv_dot_rows_p<D, T, V, 8, true, REL, SYM>(rows_s, ns, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
                                                   RS, x, y, eq, w0, w1, d0, d1);
-        spmv_dot_rows<D, T, V, 64, true, REL, SYM>(rows_l, nl, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
+        spmv_dot_rows_p<D, T, V, 64, true, REL, SYM>(rows_l, nl, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
                                                    RS, x, y, eq, w0, w1, d0, d1);
     } else {
-        spmv_dot_rows<D, T, V, 8, false, REL, SYM>(rows_s, ns, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
+        spmv_dot_rows_p<D, T, V, 8, false, REL, SYM>(rows_s, ns, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
                                                    RS, x, y, eq, w0, w1, d0, d1);
-        spmv_dot_rows<D, T, V, 64, false, REL, SYM>(rows_l, nl, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
+        spmv_dot_rows_p<D, T, V, 64, false, REL, SYM>(rows_l, nl, lb, nb, adj_off, adj, cls, cls_off, cls_rel, src, K,
                                                     RS, x, y, eq, w0, w1, d0, d1);
     }
     if (part0) {
@@ -114,6 +223,7 @@ __global__ void __launch_bounds__(SPMV_THREADS) k_spmv_dot(const int *rows_s, in
             if (part1) part1[i] = 0.0;
         }
 }
+
 
 // block row sums RS_a = sum_b A_ab (FP64), stored as T
 template <int D, typename T>
@@ -1121,23 +1231,47 @@ static GraphSig make_sig(Ctx &c, const void *K, const KrylovWs<V> &w, int krylov
 
 // y = mask(A x) with fused dot partials; A is K (all blocks) or, with src, the own blocks of
 // its mirrored storage (node-block operators only)
+// 1- and 2-DOF operators use the persistent form; the 3-DOF operators the blocked form, whose
+// per-block dot partials live in c.red after the RED_SLOTS partial slots (solve_masked allocates
+// them before any solve that asks for dots) and are folded by k_fold_parts.  Either way mirrored
+// and full storage run the same kernel, so their dots and results are the same bits.
 template <int D, typename T, typename V>
 static void spmv_dot(Ctx &c, const T *K, const int *src, const T *RS, const V *x, V *y, const V *w0, double *p0,
                      const V *w1, double *p1) {
-#define PDX_SPMV_DOT(REL, SYM)                                                                                 \
-    k_spmv_dot<D, T, V, REL, SYM><<<RED_PARTS, SPMV_THREADS, 0, c.stream>>>(                                  \
+    if constexpr (D < 3) {
+#define PDX_SPMV_DOT_P(REL, SYM)                                                                               \
+    k_spmv_dot_p<D, T, V, REL, SYM><<<RED_PARTS, SPMV_THREADS, 0, c.stream>>>(                                \
         c.rows_short, c.n_short, c.rows_long, c.n_long, c.adj_off, c.adj, REL ? c.cls : nullptr,              \
         REL ? c.cls_off : nullptr, REL ? c.cls_rel : nullptr, src, K, RS, x, y, c.eq, w0, p0, w1, p1)
-    if constexpr (D > 1) {
+        if constexpr (D > 1) {
+            if (src) {
+                if (c.cls) PDX_SPMV_DOT_P(true, true);
+                else PDX_SPMV_DOT_P(false, true);
+                return;
+            }
+        }
+        if (c.cls) PDX_SPMV_DOT_P(true, false);
+        else PDX_SPMV_DOT_P(false, false);
+#undef PDX_SPMV_DOT_P
+    } else {
+        const SpmvShape sh = spmv_shape(c);
+        double *bp = c.red ? c.red + (size_t)RED_SLOTS * NPART + 16 : nullptr;
+        double *bp0 = p0 ? bp : nullptr, *bp1 = p1 ? bp + sh.grid : nullptr;
+#define PDX_SPMV_DOT(REL, SYM)                                                                                  \
+    k_spmv_dot<D, T, V, REL, SYM><<<sh.grid, 256, 0, c.stream>>>(                                             \
+        c.rows_short, c.n_short, sh.nbs, c.rows_long, c.n_long, sh.nbt, c.adj_off, c.adj, REL ? c.cls : nullptr, \
+        REL ? c.cls_off : nullptr, REL ? c.cls_rel : nullptr, src, K, RS, x, y, c.eq, w0, bp0, w1, bp1)
         if (src) {
             if (c.cls) PDX_SPMV_DOT(true, true);
             else PDX_SPMV_DOT(false, true);
-            return;
+        } else if (c.cls) {
+            PDX_SPMV_DOT(true, false);
+        } else {
+            PDX_SPMV_DOT(false, false);
         }
-    }
-    if (c.cls) PDX_SPMV_DOT(true, false);
-    else PDX_SPMV_DOT(false, false);
 #undef PDX_SPMV_DOT
+        if (p0 || p1) k_fold_parts<<<(NPART + 255) / 256, 256, 0, c.stream>>>(sh.grid, bp0, p0, bp1, p1);
+    }
 }
 
 // one BiCGStab iteration (it = iteration index, parity selects the rho slot)
@@ -1389,8 +1523,10 @@ static int solve_masked(Ctx &c, const double *K64, const float *K32, const pdx_s
     auto t0 = std::chrono::steady_clock::now();
     const size_t n = (size_t)c.nn * c.dim;
     if (!c.red) {
-        PDX_HIP(hipMalloc((void **)&c.red, sizeof(double) * ((size_t)RED_SLOTS * NPART + 16)));
-        PDX_HIP(hipMemsetAsync(c.red, 0, sizeof(double) * ((size_t)RED_SLOTS * NPART + 16), c.stream));
+        // the partial slots, 16 scalars, then the SpMV's per-block partials of two dots
+        const size_t nred = (size_t)RED_SLOTS * NPART + 16 + 2 * (size_t)spmv_shape(c).grid;
+        PDX_HIP(hipMalloc((void **)&c.red, sizeof(double) * nred));
+        PDX_HIP(hipMemsetAsync(c.red, 0, sizeof(double) * nred, c.stream));
     }
     double *part = c.red + (size_t)P_BB * NPART;
     k_dot<double><<<RED_PARTS, 256, 0, c.stream>>>(b_dev, b_dev, n, part);

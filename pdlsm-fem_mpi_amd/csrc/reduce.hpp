@@ -1,7 +1,8 @@
 // Deterministic reductions shared by the explicit, eigenvalue, Krylov and
-// graph-network code: every reducing kernel runs RED_PARTS blocks of 256
-// threads and writes one partial sum per block; consumers re-reduce the
-// partials in a fixed order, so results are bitwise reproducible.
+// graph-network code: every reducing kernel writes NPART partial sums (one per
+// block of RED_PARTS blocks, or, for the 3-DOF Krylov SpMV, per-block partials
+// folded into NPART in block order); consumers re-reduce the partials in a fixed
+// order, so results are bitwise reproducible.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -14,7 +15,7 @@
 namespace pdx {
 
 constexpr int NPART = RED_PARTS;  // partial sums per reduction (one per block)
-constexpr int SPMV_THREADS = 1024;  // block size of the Krylov SpMV (16 waves: more gathers in flight)
+constexpr int SPMV_THREADS = 1024;  // block size of the persistent Krylov SpMV (16 waves)
 
 // block reduction of one value per thread (blockDim.x a multiple of 64, <= 1024)
 __device__ inline double block_sum(double v, double *sm) {
