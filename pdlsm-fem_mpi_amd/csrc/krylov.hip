@@ -140,19 +140,25 @@ __global__ void __launch_bounds__(256) k_spmv_dot(const int *rows_s, int ns, int
     }
 }
 
-// part[i] = the sum, in order, of bp[j] over the i-th of NPART contiguous ranges of [0, nbp)
-__global__ void __launch_bounds__(256) k_fold_parts(int nbp, const double *bp0, double *p0, const double *bp1,
-                                                    double *p1) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NPART) return;
+// part[i] = the sum of bp[j] over the i-th of NPART contiguous ranges of [0, nbp): one wavefront per
+// range, the lanes' loads in flight together, then a fixed shuffle tree (the same order every call)
+__global__ void __launch_bounds__(64) k_fold_parts(int nbp, const double *bp0, double *p0, const double *bp1,
+                                                   double *p1) {
+    const int i = blockIdx.x;
     const int per = (nbp + NPART - 1) / NPART, j0 = min(nbp, i * per), j1 = min(nbp, j0 + per);
     double s0 = 0.0, s1 = 0.0;
-    for (int j = j0; j < j1; j++) {
+    for (int j = j0 + (int)threadIdx.x; j < j1; j += 64) {
         if (bp0) s0 += bp0[j];
         if (bp1) s1 += bp1[j];
     }
-    if (p0) p0[i] = s0;
-    if (p1) p1[i] = s1;
+    for (int s = 32; s > 0; s >>= 1) {
+        s0 += __shfl_xor(s0, s, 64);
+        s1 += __shfl_xor(s1, s, 64);
+    }
+    if (threadIdx.x == 0) {
+        if (p0) p0[i] = s0;
+        if (p1) p1[i] = s1;
+    }
 }
 
 // Persistent form: rows [lb * per, (lb + 1) * per) of the list go to logical block lb, and the
@@ -1270,7 +1276,7 @@ static void spmv_dot(Ctx &c, const T *K, const int *src, const T *RS, const V *x
             PDX_SPMV_DOT(false, false);
         }
 #undef PDX_SPMV_DOT
-        if (p0 || p1) k_fold_parts<<<(NPART + 255) / 256, 256, 0, c.stream>>>(sh.grid, bp0, p0, bp1, p1);
+        if (p0 || p1) k_fold_parts<<<NPART, 64, 0, c.stream>>>(sh.grid, bp0, p0, bp1, p1);
     }
 }
 
