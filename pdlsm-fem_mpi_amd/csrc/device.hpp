@@ -101,6 +101,7 @@ struct Ctx {
     int *row_node = nullptr;  // [neq] node*3+dof of each free row
     // rows of the node graph binned by length for the SpMV (short: one group of 8 lanes, long: a wave)
     int n_short = 0, n_long = 0;
+    unsigned spmv_calls = 0;  // products of the 3-DOF Krylov SpMV so far (their sweep alternates)
     int *rows_short = nullptr, *rows_long = nullptr;
 
     // natural BCs

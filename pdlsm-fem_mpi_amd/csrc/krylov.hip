@@ -91,17 +91,22 @@ inline SpmvShape spmv_shape(const Ctx &c) {
 // x * ceil(nbs / 8) + q (short rows) or nbs + x * ceil((nbt - nbs) / 8) + q - ceil(nbs / 8) (long
 // rows), x = b % 8, q = b / 8: each XCD (and its L2) gets one contiguous eighth of the short-row
 // blocks and one of the long-row blocks, so its gathers of x stay local and the long rows are
-// spread over all XCDs.  Each block writes its dot partials to bp0 / bp1[b] (0 for a block with no
-// rows); k_fold_parts sums them in block order into the partials the vector kernels read.
+// spread over all XCDs.  With rev, q counts down: the product visits the rows last to first, so
+// it starts on the blocks the previous product read last, which may still be in the memory-side
+// cache.  Each block writes its dot partials to bp0 / bp1 at slot q * 8 + x (0 for a block with
+// no rows), the same slot in either direction; k_fold_parts sums them in slot order into the
+// partials the vector kernels read, so both directions give the same bits.
 template <int D, typename T, typename V, bool REL, bool SYM>
 __global__ void __launch_bounds__(256) k_spmv_dot(const int *rows_s, int ns, int nbs, const int *rows_l, int nl,
                                                   int nbt, const int *adj_off, const int *adj, const int *cls,
                                                   const int *cls_off, const int *cls_rel, const int *src,
                                                   const T *K, const T *RS, const V *x, V *y, const int *eq,
-                                                  const V *w0, double *bp0, const V *w1, double *bp1) {
+                                                  const V *w0, double *bp0, const V *w1, double *bp1, int rev) {
     __shared__ double sm[17];
     double d0 = 0.0, d1 = 0.0;
-    const int per = gridDim.x / 8, q = blockIdx.x / 8, xcd = blockIdx.x % 8, ps = (nbs + 7) / 8;
+    const int per = gridDim.x / 8, q0 = blockIdx.x / 8, xcd = blockIdx.x % 8, ps = (nbs + 7) / 8;
+    const int q = rev ? per - 1 - q0 : q0;
+    const int slot = q * 8 + xcd;  // the partials' slot follows the logical block, not the dispatch
     int lb = -1;
     if (q < ps) {
         if (xcd * ps + q < nbs) lb = xcd * ps + q;
@@ -132,11 +137,11 @@ __global__ void __launch_bounds__(256) k_spmv_dot(const int *rows_s, int ns, int
     }
     if (bp0) {
         d0 = block_sum(d0, sm);
-        if (threadIdx.x == 0) bp0[blockIdx.x] = d0;
+        if (threadIdx.x == 0) bp0[slot] = d0;
     }
     if (bp1) {
         d1 = block_sum(d1, sm);
-        if (threadIdx.x == 0) bp1[blockIdx.x] = d1;
+        if (threadIdx.x == 0) bp1[slot] = d1;
     }
 }
 
@@ -1263,10 +1268,11 @@ static void spmv_dot(Ctx &c, const T *K, const int *src, const T *RS, const V *x
         const SpmvShape sh = spmv_shape(c);
         double *bp = c.red ? c.red + (size_t)RED_SLOTS * NPART + 16 : nullptr;
         double *bp0 = p0 ? bp : nullptr, *bp1 = p1 ? bp + sh.grid : nullptr;
+        const int rev = int(c.spmv_calls++ & 1);  // every other product sweeps the rows in reverse
 #define PDX_SPMV_DOT(REL, SYM)                                                                                  \
     k_spmv_dot<D, T, V, REL, SYM><<<sh.grid, 256, 0, c.stream>>>(                                             \
         c.rows_short, c.n_short, sh.nbs, c.rows_long, c.n_long, sh.nbt, c.adj_off, c.adj, REL ? c.cls : nullptr, \
-        REL ? c.cls_off : nullptr, REL ? c.cls_rel : nullptr, src, K, RS, x, y, c.eq, w0, bp0, w1, bp1)
+        REL ? c.cls_off : nullptr, REL ? c.cls_rel : nullptr, src, K, RS, x, y, c.eq, w0, bp0, w1, bp1, rev)
         if (src) {
             if (c.cls) PDX_SPMV_DOT(true, true);
             else PDX_SPMV_DOT(false, true);
